@@ -1,0 +1,197 @@
+/*
+ * swiftimpute_hip.h -- the C ABI of the MI355X (gfx950) imputation hot path.
+ *
+ * Plain C: device pointers, sizes, an opaque stream handle (a hipStream_t
+ * passed as void*, NULL = the null stream).  No allocation happens inside an
+ * entry point that takes a workspace; every such call is asynchronous on the
+ * given stream and returns a status (0 = SI_OK, negative = error, message in
+ * si_last_error(), which is thread-local).
+ *
+ * The C++ API in include/swiftimpute/imputer.hpp (Imputer, ReferencePanel,
+ * TargetData, ImputationConfig, ImputationResult) is implemented on top of
+ * these entry points.  Each declaration names the reference interface it
+ * replaces (paths relative to the tcBio/SwiftImpute tree).
+ *
+ * Data layouts (all little-endian, row-major):
+ *   cols     [M][W] uint64, W = ceil(H/64): bit (h & 63) of word (h >> 6) of
+ *            row m is 1 iff the panel allele at (m, h) is nonzero.
+ *   miss     [M][W] uint64 or NULL: bit set iff the allele is missing (255).
+ *   prefix   [M][H] uint32 -- PBWT order after marker m.
+ *   div      [M][H] uint32 -- PBWT divergence (every entry of row m is m).
+ *   sa       [M][Lp] uint8  -- state-allele table, Lp = si_padded_states(L):
+ *              bit0 allele(h0)==1, bit1 allele(h1)==1, bit2 h0 missing,
+ *              bit3 h1 missing, bits4-5 emission class min(a0+a1,2),
+ *              bit6 padding state (k >= L).
+ *   mmask    [M] uint8      -- bit c set iff some real state has class c.
+ *   codes    [B][pitch] uint8 -- genotype-likelihood palette index per
+ *            (sample, marker); pitch >= M, multiple of 16.
+ *   palette  [P][3] float   -- log10 genotype likelihoods (ll_00, ll_01, ll_11)
+ *            (GenotypeLikelihoods, src/core/types.hpp:46-52).
+ *   eps      [P][8][4] float -- emission table built by si_emission_table.
+ *   emax     [P][8] float   -- its log offsets.
+ *   ds       [B][M] float   -- posterior dosage, ap [B][M][2] float,
+ *   hap      [B][2][M] uint8 (ImputationResult layout, src/api/imputer.hpp:191),
+ *   prob     [B][M][L] float -- the sampler's normalized state probabilities.
+ */
+#ifndef SWIFTIMPUTE_HIP_H
+#define SWIFTIMPUTE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SI_ABI_VERSION 1
+
+enum si_status {
+    SI_OK = 0,
+    SI_ERR_ARG = -1,         /* invalid argument (reference: ImputationError) */
+    SI_ERR_DEVICE = -2,      /* HIP runtime error (reference: CUDAError)      */
+    SI_ERR_UNSUPPORTED = -3, /* shape or mode outside what this build handles */
+    SI_ERR_WORKSPACE = -4    /* workspace missing or too small                */
+};
+
+/* Thread-local message of the last failing call. */
+const char* si_last_error(void);
+int si_abi_version(void);
+
+/* Padded state count the kernels use: the next of 4, 8, 16, 32, 64 >= L. */
+uint32_t si_padded_states(uint32_t L);
+
+/* ------------------------------------------------------------------ PBWT --
+ * Replaces PBWTBuilder::build / build_parallel / build_marker
+ * (src/pbwt/pbwt_index.cpp:95-247, declared src/pbwt/pbwt_index.hpp:105-135),
+ * which the reference runs on the host.  div may be NULL (not written).
+ */
+size_t si_pbwt_workspace_bytes(uint32_t H, uint32_t M);
+int si_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
+                  uint32_t* div, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------- selection --
+ * Host: the permutation libstdc++ std::sort applies to H equal keys under the
+ * reference comparator (PBWTIndex::select_states, src/pbwt/pbwt_index.cpp:20-44).
+ * perm receives min(L, H) entries.
+ */
+int si_select_perm(uint32_t H, uint32_t L, uint32_t* perm);
+
+/* Device: L-best state selection + pairing + allele lookup, replacing the
+ * reference's per-(sample, marker) CPU selection and pair expansion
+ * (src/api/imputer.cpp:894-951) and the emission kernels' allele reads
+ * (src/kernels/emission.cu:96-103).  The selection is sample-independent, so
+ * one [M][L] table serves every sample.
+ *   mode 0: PBWT selection, states[m][k] = prefix[m][perm[k]] (k < H), k % H
+ *           otherwise; partner (h + 1) % H.
+ *   mode 1: simple selection (use_pbwt_selection = false), pair (2k, 2k+1)
+ *           (src/api/imputer.cpp:933-942); requires 2L <= H.
+ * perm_host: min(L, H) entries from si_select_perm (ignored in mode 1).
+ * states [M][L] uint32 (h0) may be NULL.  sa/mmask as described above.
+ */
+int si_state_table(const uint32_t* prefix, const uint64_t* cols, const uint64_t* miss,
+                   uint32_t H, uint32_t M, uint32_t L, uint32_t mode,
+                   const uint32_t* perm_host, uint32_t* states, uint8_t* sa, uint8_t* mmask,
+                   void* stream);
+
+/* ----------------------------------------------------------- transitions --
+ * Host: Li-Stephens constants per interval t = 0..M-2 from the genetic map in
+ * Morgans (TransitionComputer::compute, src/kernels/transition.cu:83-121,
+ * 242-289, with the map of src/api/imputer.cpp:596-605).  FP64 math, FP32
+ * results: Td (stay) and To (switch), log10 values as the reference stores.
+ */
+int si_genetic_map(const uint64_t* pos, const double* cM, uint32_t M, double* g);
+int si_transition_constants(const double* g, uint32_t M, uint32_t ne, float rho_rate,
+                            float* Td, float* To);
+
+/* Per-step coefficients the kernels consume for one interval:
+ *   t_o = e^To, u = e^Td - e^To (linear space), lto = To, lu = log(e^Td - e^To).
+ * The reference combines its log10 constants with natural exp/log
+ * (src/kernels/logsumexp.cuh:67-70); these coefficients keep that. */
+typedef struct si_step_coef {
+    float t_o;
+    float u;
+    float lto;
+    float lu;
+} si_step_coef;
+int si_step_coefs(const float* Td, const float* To, uint32_t n, si_step_coef* out);
+
+/* ------------------------------------------------------------- emission --
+ * Host: per palette entry p and class-present mask c (1..7), the emission
+ * factors eps[p][c][k] = exp(palette[p][k] - emax[p][c]) for each class k
+ * present, where emax[p][c] = max over present classes (FP64 exp, FP32 out).
+ * Restates compute_single_emission (src/kernels/emission.cu:35-54) in the
+ * max-shifted linear form the forward-backward consumes.
+ */
+int si_emission_table(const float* palette, uint32_t P, float* eps, float* emax);
+
+/* -------------------------------------------------------- forward-backward --
+ * One imputation job: B samples over M markers with L states.  Replaces
+ * EmissionComputer::compute, launch_forward_pass, launch_backward_pass and
+ * HaplotypeSampler::sample (src/kernels/emission.cu:208-257,
+ * src/kernels/forward_backward.cu:321-388, src/kernels/sampling.cu:269-320),
+ * fused; adds the posterior dosage the reference leaves unfilled
+ * (ImputationResult::set_posterior, src/api/imputer.cpp:459-465).
+ */
+enum si_fb_flags {
+    SI_FB_PER_MARKER = 1u << 0, /* steps[] per marker; else `step` for every
+                                   marker, as the reference (interval 0 only,
+                                   forward_backward.cu:108)                   */
+    SI_FB_ARGMAX = 1u << 1,     /* argmax instead of the Philox draw           */
+    SI_FB_LOGSPACE = 1u << 2    /* log-space FP32 kernel instead of the
+                                   rescaled linear-space one                   */
+};
+
+typedef struct si_fb_job {
+    uint32_t n_samples;   /* B */
+    uint32_t n_markers;   /* M */
+    uint32_t n_states;    /* L (1..64) */
+    uint32_t flags;       /* si_fb_flags */
+    uint32_t code_pitch;  /* row pitch of codes, >= M, multiple of 16 */
+    uint32_t n_palette;   /* P (1..256) */
+    uint32_t rng_stream;  /* third Philox counter word */
+    uint32_t sample_base; /* global index of sample 0 (Philox counter word 1) */
+    uint64_t seed;        /* Philox key */
+    si_step_coef step;    /* used when SI_FB_PER_MARKER is clear */
+    const si_step_coef* steps; /* [M] device, entry m = step into marker m */
+    const uint8_t* sa;    /* [M][Lp] device */
+    const uint8_t* mmask; /* [M] device */
+    const uint8_t* codes; /* [B][code_pitch] device */
+    const float* eps;     /* [P][8][4] device */
+    const float* emax;    /* [P][8] device */
+    float* ds;            /* [B][M] device or NULL */
+    float* ap;            /* [B][M][2] device or NULL */
+    uint8_t* hap;         /* [B][2][M] device or NULL */
+    float* prob;          /* [B][M][L] device or NULL */
+    double* loglik;       /* [B] device or NULL */
+} si_fb_job;
+
+/* Workspace (device bytes) si_fb_run needs for these jobs. */
+size_t si_fb_workspace_bytes(const si_fb_job* jobs, uint32_t n_jobs);
+
+/* Plain form: one launch sequence per job, jobs processed in order on the
+ * stream.  jobs is a host array. */
+int si_fb_run(const si_fb_job* jobs, uint32_t n_jobs, void* workspace, size_t workspace_bytes,
+              void* stream);
+
+/* The two launches si_fb_run issues per job, selectable: SI_FB_PASS_CHAIN
+ * (forward and backward chains writing block checkpoints into the workspace)
+ * and SI_FB_PASS_BLOCK (block replay, posteriors, dosage, sampling, reading
+ * them).  Running CHAIN then BLOCK on the same workspace equals si_fb_run;
+ * callers use the split to time the passes or to pipeline batches across
+ * streams.  No counterpart in the reference (its forward and backward are
+ * separate launches, forward_backward.cu:321-388). */
+enum si_fb_pass { SI_FB_PASS_CHAIN = 1u, SI_FB_PASS_BLOCK = 2u };
+int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
+                     size_t workspace_bytes, uint32_t passes, void* stream);
+
+/* Philox4x32-10 uniform the sampler uses for (sample s, marker m): word
+ * (m & 3) of Philox4x32-10(counter = {m >> 2, s, rng_stream, 0} with s the
+ * global sample index (sample_base + job-local index),
+ * key = {seed, seed >> 32}), as (x >> 8) * 2^-24.  Host function. */
+float si_sample_uniform(uint64_t seed, uint32_t s, uint32_t m, uint32_t rng_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWIFTIMPUTE_HIP_H */

@@ -1,0 +1,234 @@
+"""Python driver of the C ABI: device buffers (torch allocations), the resident
+panel index, and forward-backward jobs.
+
+This mirrors what the C++ ``Imputer`` does (include/swiftimpute/imputer.hpp):
+``DeviceIndex`` is ``Imputer::build_index()`` (PBWT sweep + state table on the
+GPU), ``run_job`` is the per-batch ``impute`` step.  torch only provides
+memory and the stream; all compute is in libsi_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _native
+from ._native import FbJob, StepCoef, check, hip
+
+DEV = "cuda"
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def padded_states(L: int) -> int:
+    return int(hip().si_padded_states(L))
+
+
+def select_perm(H: int, L: int) -> np.ndarray:
+    out = np.empty(min(L, H), np.uint32)
+    check(hip().si_select_perm(H, L, out.ctypes.data_as(_native._u32p)), "si_select_perm")
+    return out
+
+
+# ------------------------------------------------------------------ transitions
+def genetic_map(pos: np.ndarray, cM: np.ndarray | None = None) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.uint64)
+    g = np.empty(pos.size, np.float64)
+    cmv = None if cM is None else np.ascontiguousarray(cM, np.float64)
+    check(hip().si_genetic_map(pos.ctypes.data_as(_native._u64p),
+                               cmv.ctypes.data_as(_native._f64p) if cmv is not None else None,
+                               pos.size, g.ctypes.data_as(_native._f64p)), "si_genetic_map")
+    return g
+
+
+def transition_constants(g: np.ndarray, ne: int, rho_rate: float = 4.0):
+    g = np.ascontiguousarray(g, np.float64)
+    n = max(g.size - 1, 0)
+    Td = np.zeros(max(n, 1), np.float32)
+    To = np.zeros(max(n, 1), np.float32)
+    check(hip().si_transition_constants(g.ctypes.data_as(_native._f64p), g.size, ne, rho_rate,
+                                        Td.ctypes.data_as(_native._f32p),
+                                        To.ctypes.data_as(_native._f32p)),
+          "si_transition_constants")
+    return Td[:n], To[:n]
+
+
+def step_coefs(Td: np.ndarray, To: np.ndarray) -> np.ndarray:
+    """[n][4] float32 (t_o, u, lto, lu) per interval."""
+    n = len(Td)
+    arr = (StepCoef * max(n, 1))()
+    Td = np.ascontiguousarray(Td, np.float32)
+    To = np.ascontiguousarray(To, np.float32)
+    check(hip().si_step_coefs(Td.ctypes.data_as(_native._f32p), To.ctypes.data_as(_native._f32p),
+                              n, arr), "si_step_coefs")
+    return np.frombuffer(arr, np.float32).reshape(-1, 4)[:n].copy()
+
+
+def emission_table(palette: np.ndarray):
+    palette = np.ascontiguousarray(palette, np.float32)
+    P = palette.shape[0]
+    eps = np.empty((P, 8, 4), np.float32)
+    emax = np.empty((P, 8), np.float32)
+    check(hip().si_emission_table(palette.ctypes.data_as(_native._f32p), P,
+                                  eps.ctypes.data_as(_native._f32p),
+                                  emax.ctypes.data_as(_native._f32p)), "si_emission_table")
+    return eps, emax
+
+
+@dataclass
+class Transitions:
+    """Per-step coefficients for one job.  per_marker=False reproduces the
+    reference, whose kernels read only the first interval's matrix."""
+    per_marker: bool
+    step: np.ndarray                 # [4] coefficient used for every step
+    steps_dev: torch.Tensor | None   # [M][4] when per_marker
+    Td: np.ndarray
+    To: np.ndarray
+
+    @staticmethod
+    def build(pos, cM, ne: int, rho_rate: float = 4.0, per_marker: bool = False,
+              device=DEV) -> "Transitions":
+        g = genetic_map(pos, cM)
+        Td, To = transition_constants(g, ne, rho_rate)
+        co = step_coefs(Td, To)
+        step = co[0] if len(co) else np.array([1.0, 0.0, 0.0, -np.inf], np.float32)
+        steps_dev = None
+        if per_marker:
+            full = np.zeros((len(g), 4), np.float32)
+            full[1:] = co
+            steps_dev = torch.from_numpy(full).to(device)
+        return Transitions(per_marker, step, steps_dev, Td, To)
+
+
+# --------------------------------------------------------------------- index
+class DeviceIndex:
+    """Resident panel, PBWT index and state table (Imputer::build_index)."""
+
+    def __init__(self, cols: np.ndarray, H: int, L: int, mode: int = 0, miss: np.ndarray | None = None,
+                 keep_prefix: bool = True, with_div: bool = True, stream=None, device=DEV):
+        M = cols.shape[0]
+        self.H, self.M, self.L, self.Lp = H, M, L, padded_states(L)
+        self.cols = torch.from_numpy(np.ascontiguousarray(cols).view(np.int64)).to(device)
+        self.miss = (torch.from_numpy(np.ascontiguousarray(miss).view(np.int64)).to(device)
+                     if miss is not None else None)
+        sh = _stream_handle(stream)
+        lib = hip()
+        self.prefix = torch.empty((M, H), dtype=torch.int32, device=device)
+        self.div = torch.empty((M, H), dtype=torch.int32, device=device) if with_div else None
+        if mode == 0:
+            check(lib.si_pbwt_sweep(_ptr(self.cols), H, M, _ptr(self.prefix), _ptr(self.div), None, 0,
+                                    sh), "si_pbwt_sweep")
+        self.perm = select_perm(H, L)
+        self.states = torch.empty((M, L), dtype=torch.int32, device=device)
+        self.sa = torch.empty((M, self.Lp), dtype=torch.uint8, device=device)
+        self.mmask = torch.empty((M,), dtype=torch.uint8, device=device)
+        check(lib.si_state_table(_ptr(self.prefix), _ptr(self.cols), _ptr(self.miss), H, M, L, mode,
+                                 self.perm.ctypes.data_as(_native._u32p), _ptr(self.states),
+                                 _ptr(self.sa), _ptr(self.mmask), sh), "si_state_table")
+        if not keep_prefix:
+            self.prefix = None
+            self.div = None
+
+    def state_alleles(self):
+        """(a0, a1) [M][L] uint8 raw alleles of the selected state pairs, from
+        the device state table (for the oracle)."""
+        sa = self.sa.cpu().numpy()[:, : self.L]
+        a0 = np.where(sa & 4, 255, sa & 1).astype(np.uint8)
+        a1 = np.where(sa & 8, 255, (sa >> 1) & 1).astype(np.uint8)
+        return a0, a1
+
+
+# ----------------------------------------------------------------------- jobs
+@dataclass
+class JobOutputs:
+    ds: torch.Tensor | None = None
+    ap: torch.Tensor | None = None
+    hap: torch.Tensor | None = None
+    prob: torch.Tensor | None = None
+    loglik: torch.Tensor | None = None
+
+
+@dataclass
+class Job:
+    """One imputation job (B samples x M markers against a DeviceIndex)."""
+    index: DeviceIndex
+    codes: torch.Tensor          # [B][pitch] uint8 on device
+    palette: np.ndarray          # [P][3]
+    trans: Transitions
+    seed: int = 12345
+    rng_stream: int = 0
+    sample_base: int = 0
+    argmax: bool = False
+    out: JobOutputs = field(default_factory=JobOutputs)
+    _keep: list = field(default_factory=list)
+
+    @property
+    def B(self):
+        return self.codes.shape[0]
+
+    def alloc_outputs(self, ds=True, ap=False, hap=True, prob=False, loglik=False):
+        B, M, L = self.B, self.index.M, self.index.L
+        dev = self.codes.device
+        self.out = JobOutputs(
+            ds=torch.empty((B, M), dtype=torch.float32, device=dev) if ds else None,
+            ap=torch.empty((B, M, 2), dtype=torch.float32, device=dev) if ap else None,
+            hap=torch.empty((B, 2, M), dtype=torch.uint8, device=dev) if hap else None,
+            prob=torch.empty((B, M, L), dtype=torch.float32, device=dev) if prob else None,
+            loglik=torch.empty((B,), dtype=torch.float64, device=dev) if loglik else None,
+        )
+        return self.out
+
+    def c_job(self) -> FbJob:
+        eps, emax = emission_table(self.palette)
+        dev = self.codes.device
+        eps_t = torch.from_numpy(eps).to(dev)
+        emax_t = torch.from_numpy(emax).to(dev)
+        self._keep = [eps_t, emax_t]
+        j = FbJob()
+        j.n_samples = self.B
+        j.n_markers = self.index.M
+        j.n_states = self.index.L
+        j.flags = ((_native.SI_FB_PER_MARKER if self.trans.per_marker else 0)
+                   | (_native.SI_FB_ARGMAX if self.argmax else 0))
+        j.code_pitch = self.codes.shape[1]
+        j.n_palette = self.palette.shape[0]
+        j.rng_stream = self.rng_stream
+        j.sample_base = self.sample_base
+        j.seed = self.seed
+        s = self.trans.step
+        j.step = StepCoef(float(s[0]), float(s[1]), float(s[2]), float(s[3]))
+        j.steps = _ptr(self.trans.steps_dev)
+        j.sa = _ptr(self.index.sa)
+        j.mmask = _ptr(self.index.mmask)
+        j.codes = _ptr(self.codes)
+        j.eps = _ptr(eps_t)
+        j.emax = _ptr(emax_t)
+        o = self.out
+        j.ds, j.ap, j.hap, j.prob, j.loglik = (_ptr(o.ds), _ptr(o.ap), _ptr(o.hap), _ptr(o.prob),
+                                               _ptr(o.loglik))
+        return j
+
+
+def workspace_bytes(jobs) -> int:
+    arr = (FbJob * len(jobs))(*[j if isinstance(j, FbJob) else j.c_job() for j in jobs])
+    return int(hip().si_fb_workspace_bytes(arr, len(jobs)))
+
+
+def run_jobs(cjobs, workspace: torch.Tensor | None = None, stream=None):
+    """si_fb_run over prepared FbJob structs (plain form: one job after another)."""
+    arr = (FbJob * len(cjobs))(*cjobs)
+    need = int(hip().si_fb_workspace_bytes(arr, len(cjobs)))
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=DEV)
+    check(hip().si_fb_run(arr, len(cjobs), _ptr(workspace), workspace.numel(),
+                          _stream_handle(stream)), "si_fb_run")
+    return workspace

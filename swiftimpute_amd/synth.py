@@ -1,0 +1,113 @@
+"""Synthetic workloads (numpy front end of ``include/swiftimpute/synth.h``).
+
+The workload shapes are the imputation benchmarks this package is measured
+on; every generator is seeded and thread-count independent.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+
+HARDCALL_PALETTE = np.array([[0.0, -999.0, -999.0], [-999.0, 0.0, -999.0],
+                             [-999.0, -999.0, 0.0], [0.0, 0.0, 0.0]], np.float32)
+CODE_MISSING = 3
+
+
+def _threads(n):
+    return n or min(16, os.cpu_count() or 1)
+
+
+@dataclass
+class Panel:
+    H: int
+    M: int
+    cols: np.ndarray      # [M][W] uint64 bit-packed
+    pos: np.ndarray       # [M] uint64 bp
+    cM: np.ndarray        # [M] centiMorgans (copying map)
+    morgans: np.ndarray   # [M] Morgans (copying map)
+    spacing_mode: int
+
+    @property
+    def W(self):
+        return (self.H + 63) // 64
+
+    def alleles(self) -> np.ndarray:
+        """[M][H] uint8 panel (the reference's ReferencePanel layout)."""
+        b = self.cols.view(np.uint8).reshape(self.M, self.W * 8)
+        return np.unpackbits(b, axis=1, bitorder="little")[:, : self.H].copy()
+
+    def genetic_cM(self):
+        """cM values to hand the imputer: None for the constant-spacing map
+        (positions alone give 1 cM/Mb, as the reference computes), the
+        hotspot-aware map otherwise."""
+        return None if self.spacing_mode == 0 else self.cM
+
+
+def make_panel(H: int, M: int, seed: int = 12345, spacing_mode: int = 0, n_founders: int = 40,
+               mutation: float = 1e-3, threads: int | None = None) -> Panel:
+    lib = _native.api()
+    cfg = _native.SynthPanelCfg()
+    lib.si_synth_panel_defaults(ctypes.byref(cfg), H, M, seed)
+    cfg.spacing_mode = spacing_mode
+    cfg.n_founders = n_founders
+    cfg.mutation = mutation
+    W = (H + 63) // 64
+    cols = np.empty((M, W), np.uint64)
+    pos = np.empty(M, np.uint64)
+    cM = np.empty(M, np.float64)
+    mg = np.empty(M, np.float64)
+    rc = lib.si_synth_panel(ctypes.byref(cfg), cols.ctypes.data_as(_native._u64p),
+                            pos.ctypes.data_as(_native._u64p), cM.ctypes.data_as(_native._f64p),
+                            mg.ctypes.data_as(_native._f64p), _threads(threads))
+    if rc != 0:
+        raise ValueError(f"si_synth_panel failed ({rc})")
+    return Panel(H, M, cols, pos, cM, mg, spacing_mode)
+
+
+@dataclass
+class Targets:
+    B: int
+    M: int
+    pitch: int
+    codes: np.ndarray         # [B][pitch] uint8, palette indices
+    truth: np.ndarray | None  # [B][2][M] uint8
+    typed_every: int
+
+    def untyped_mask(self) -> np.ndarray:
+        return self.codes[:, : self.M] == CODE_MISSING
+
+    def gl(self) -> np.ndarray:
+        """[B][M][3] float32 log10 genotype likelihoods (TargetData layout)."""
+        return HARDCALL_PALETTE[self.codes[:, : self.M]]
+
+
+def make_targets(panel: Panel, B: int, typed_every: int, seed: int = 12346,
+                 geno_error: float = 0.005, with_truth: bool = True,
+                 threads: int | None = None) -> Targets:
+    lib = _native.api()
+    cfg = _native.SynthTargetCfg()
+    lib.si_synth_target_defaults(ctypes.byref(cfg), B, panel.M, typed_every, seed)
+    cfg.geno_error = geno_error
+    codes = np.empty((B, cfg.code_pitch), np.uint8)
+    truth = np.empty((B, 2, panel.M), np.uint8) if with_truth else None
+    rc = lib.si_synth_targets(ctypes.byref(cfg), panel.cols.ctypes.data_as(_native._u64p),
+                              panel.H, panel.M, panel.morgans.ctypes.data_as(_native._f64p),
+                              codes.ctypes.data_as(_native._u8p),
+                              truth.ctypes.data_as(_native._u8p) if truth is not None else None,
+                              _threads(threads))
+    if rc != 0:
+        raise ValueError(f"si_synth_targets failed ({rc})")
+    return Targets(B, panel.M, cfg.code_pitch, codes, truth, typed_every)
+
+
+# Benchmark workloads: (H, M, B, L, Ne, typed_every, spacing_mode)
+WORKLOADS = {
+    "cfg1": dict(H=4432, M=1644, B=444, L=8, ne=10000, typed_every=4, spacing_mode=0),
+    "cfg2": dict(H=20000, M=100000, B=10000, L=16, ne=20000, typed_every=20, spacing_mode=0),
+    "cfg3": dict(H=8000, M=100000, B=2000, L=8, ne=10000, typed_every=20, spacing_mode=1),
+}

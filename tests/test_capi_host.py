@@ -1,0 +1,142 @@
+"""CPU checks of the C ABI: the library loads, exports every declared symbol,
+its host-side functions agree with the oracle bit for bit, and argument
+errors come back as status codes (no GPU call is made by any of these)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from swiftimpute_amd import _native, device, synth
+from conftest import ROOT
+
+
+def declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(si_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_hip_library_exports_every_declared_symbol():
+    lib = _native.hip()
+    names = declared("swiftimpute_hip.h")
+    assert len(names) >= 14
+    for n in names:
+        assert hasattr(lib, n), n
+    assert {n for n, _, _ in _native.HIP_SYMBOLS} == set(names)
+
+
+def test_api_library_exports_synth_symbols():
+    lib = _native.api()
+    for n in declared(os.path.join("swiftimpute", "synth.h")):
+        assert hasattr(lib, n), n
+
+
+def test_abi_version_and_padding():
+    lib = _native.hip()
+    assert lib.si_abi_version() == 1
+    assert [lib.si_padded_states(L) for L in (1, 4, 5, 8, 9, 16, 17, 33, 64)] == \
+        [4, 4, 8, 8, 16, 16, 32, 64, 64]
+
+
+@pytest.mark.parametrize("H,L", [(1, 8), (16, 16), (4432, 8), (20000, 16), (8000, 8), (40000, 8)])
+def test_select_perm_matches_oracle(H, L):
+    assert np.array_equal(device.select_perm(H, L), oracle.select_perm(H, L))
+
+
+@pytest.mark.parametrize("spacing_mode", [0, 1])
+@pytest.mark.parametrize("ne", [1000, 10000, 20000, 100000])
+def test_transition_constants_match_oracle(spacing_mode, ne):
+    p = synth.make_panel(64, 300, seed=3, spacing_mode=spacing_mode)
+    g1 = device.genetic_map(p.pos, p.genetic_cM())
+    g2 = oracle.genetic_map(p.pos, p.genetic_cM())
+    assert np.array_equal(g1, g2)
+    Td1, To1 = device.transition_constants(g1, ne)
+    Td2, To2 = oracle.transitions(g2, ne)
+    assert np.array_equal(Td1, Td2) and np.array_equal(To1, To2)
+
+
+def test_step_coefs():
+    Td = np.array([-0.1, -4.3, 0.0, -2.0], np.float32)
+    To = np.array([-5.0, -4.3, -999.0, -3.0], np.float32)
+    c = device.step_coefs(Td, To)
+    assert np.allclose(c[:, 0], np.exp(To.astype(np.float64)), rtol=1e-7)
+    assert np.allclose(c[:, 1], np.exp(Td.astype(np.float64)) - np.exp(To.astype(np.float64)),
+                       rtol=1e-6)
+    assert c[1, 1] == 0.0 and c[1, 3] == -np.inf  # Td == To: no diagonal excess
+    assert c[2, 0] == 0.0 and c[2, 1] == 1.0
+
+
+def test_emission_table_hardcalls():
+    eps, emax = device.emission_table(synth.HARDCALL_PALETTE)
+    # hom-ref call with every class present: (1, 0, 0), offset 0
+    assert eps[0, 7, :3].tolist() == [1.0, 0.0, 0.0] and emax[0, 7] == 0.0
+    # hom-ref call where only het/hom-alt states exist: all mismatch, offset -999
+    assert eps[0, 6, :3].tolist() == [0.0, 1.0, 1.0] and emax[0, 6] == -999.0
+    # missing: uniform
+    assert eps[3, 7, :3].tolist() == [1.0, 1.0, 1.0]
+
+
+def test_sample_uniform_matches_oracle():
+    lib = _native.hip()
+    for seed in (0, 12345, 2**40 + 7):
+        for s in (0, 3, 9999):
+            for m in (0, 1, 2, 3, 4, 99999):
+                assert lib.si_sample_uniform(seed, s, m, 0) == oracle.sample_uniform(seed, s, m, 0)
+
+
+def test_synth_is_thread_count_independent():
+    a = synth.make_panel(300, 500, seed=9, spacing_mode=1, threads=1)
+    b = synth.make_panel(300, 500, seed=9, spacing_mode=1, threads=8)
+    assert np.array_equal(a.cols, b.cols) and np.array_equal(a.pos, b.pos)
+    ta = synth.make_targets(a, 7, 4, threads=1)
+    tb = synth.make_targets(a, 7, 4, threads=8)
+    assert np.array_equal(ta.codes, tb.codes) and np.array_equal(ta.truth, tb.truth)
+
+
+def test_synth_targets_typed_pattern_and_truth():
+    p = synth.make_panel(200, 400, seed=1)
+    t = synth.make_targets(p, 20, 4, geno_error=0.0)
+    typed = np.arange(400) % 4 == 0
+    assert (t.codes[:, :400][:, ~typed] == 3).all()
+    g = t.truth[:, 0, :] + t.truth[:, 1, :]
+    assert np.array_equal(t.codes[:, :400][:, typed], g[:, typed])
+
+
+def _job(**kw):
+    j = _native.FbJob()
+    j.n_samples, j.n_markers, j.n_states, j.code_pitch, j.n_palette = 1, 16, 8, 16, 4
+    for k, v in kw.items():
+        setattr(j, k, v)
+    return j
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(n_states=0), -1), (dict(n_states=65), -1), (dict(n_palette=0), -3),
+    (dict(n_palette=65), -3), (dict(code_pitch=8), -1), (dict(code_pitch=17), -1),
+    (dict(flags=4), -3),
+])
+def test_fb_argument_errors(kw, code):
+    lib = _native.hip()
+    j = _job(**kw)
+    rc = lib.si_fb_run(ctypes.byref(j), 1, None, 0, None)
+    assert rc == code
+    assert lib.si_last_error()
+
+
+def test_fb_missing_inputs_error():
+    lib = _native.hip()
+    j = _job()
+    assert lib.si_fb_run(ctypes.byref(j), 1, None, 0, None) == -1  # sa/codes missing
+    assert b"required" in lib.si_last_error()
+
+
+def test_state_table_argument_errors():
+    lib = _native.hip()
+    perm = np.zeros(8, np.uint32)
+    p32 = perm.ctypes.data_as(_native._u32p)
+    assert lib.si_state_table(None, None, None, 0, 10, 8, 0, p32, None, None, None, None) == -1
+    assert lib.si_state_table(None, None, None, 10, 10, 8, 1, p32, None, None, None, None) == -1
+    assert lib.si_state_table(None, None, None, 10, 10, 8, 0, None, None, None, None, None) == -1

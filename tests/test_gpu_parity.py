@@ -1,0 +1,268 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: PBWT prefix/divergence, selected states and state alleles bit-exact;
+posterior dosage / allele probabilities within 1e-5 absolute of the FP64
+oracle; sampled haplotypes bit-exact against the host replay of the same
+Philox4x32-10 counters on the device's FP32 probabilities.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from swiftimpute_amd import device, synth
+from test_oracle import fixture_panel
+
+pytestmark = pytest.mark.gpu
+
+DS_TOL = 1e-5
+
+
+def pack_cols(ref):
+    """[M][H] uint8 -> (cols [M][W] uint64 of nonzero alleles, miss or None)."""
+    M, H = ref.shape
+    W = (H + 63) // 64
+    def pack(bits):
+        b = np.zeros((M, W * 64), np.uint8)
+        b[:, :H] = bits
+        return np.packbits(b, axis=1, bitorder="little").view(np.uint64).reshape(M, W)
+    miss = ref == 255
+    return pack(ref != 0), (pack(miss) if miss.any() else None)
+
+
+def gpu_index(ref, L, mode=0):
+    cols, miss = pack_cols(ref)
+    idx = device.DeviceIndex(cols, ref.shape[1], L, mode=mode, miss=miss)
+    torch.cuda.synchronize()
+    return idx
+
+
+def run_fb(idx, codes, palette, trans, seed=12345, argmax=False, sample_base=0, prob=True):
+    B, M = codes.shape[0], idx.M
+    pitch = (M + 15) // 16 * 16
+    c = np.full((B, pitch), palette.shape[0] - 1, np.uint8)
+    c[:, :M] = codes[:, :M]
+    job = device.Job(idx, torch.from_numpy(c).cuda(), palette, trans, seed=seed, argmax=argmax,
+                     sample_base=sample_base)
+    job.alloc_outputs(ds=True, ap=True, hap=True, prob=prob, loglik=True)
+    device.run_jobs([job.c_job()])
+    torch.cuda.synchronize()
+    o = job.out
+    return {k: (getattr(o, k).cpu().numpy() if getattr(o, k) is not None else None)
+            for k in ("ds", "ap", "hap", "prob", "loglik")}
+
+
+# ----------------------------------------------------------------------- PBWT
+def test_pbwt_fixture():
+    _, ref = fixture_panel()
+    idx = gpu_index(ref, 4)
+    assert idx.prefix.cpu().numpy()[:2].tolist() == [[0, 1, 2, 5, 3, 4, 6, 7],
+                                                      [0, 1, 2, 3, 5, 4, 6, 7]]
+    p, d = oracle.pbwt_build(ref)
+    assert np.array_equal(idx.prefix.cpu().numpy().view(np.uint32), p)
+    assert np.array_equal(idx.div.cpu().numpy().view(np.uint32), d)
+
+
+@pytest.mark.parametrize("M,H", [(1, 1), (2, 5), (40, 63), (33, 64), (50, 65), (20, 1000),
+                                 (300, 4432), (7, 20000)])
+def test_pbwt_random(M, H):
+    rng = np.random.default_rng(M * 1000 + H)
+    ref = (rng.random((M, H)) < rng.uniform(0.05, 0.9)).astype(np.uint8)
+    idx = gpu_index(ref, 4)
+    p, d = oracle.pbwt_build(ref)
+    assert np.array_equal(idx.prefix.cpu().numpy().view(np.uint32), p)
+    assert np.array_equal(idx.div.cpu().numpy().view(np.uint32), d)
+
+
+def test_pbwt_config1_panel():
+    pan = synth.make_panel(4432, 1644, seed=12345)
+    ref = pan.alleles()
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    torch.cuda.synchronize()
+    p, d = oracle.pbwt_build(ref)
+    assert np.array_equal(idx.prefix.cpu().numpy().view(np.uint32), p)
+    assert np.array_equal(idx.div.cpu().numpy().view(np.uint32), d)
+
+
+# ---------------------------------------------------------------- state table
+@pytest.mark.parametrize("H,L", [(4432, 8), (300, 16), (100, 64), (5, 8), (3, 3), (1, 4),
+                                 (77, 5), (200, 12)])
+def test_state_table(H, L):
+    rng = np.random.default_rng(H + L)
+    M = 40
+    ref = (rng.random((M, H)) < 0.5).astype(np.uint8)
+    ref[rng.random((M, H)) < 0.02] = 255
+    idx = gpu_index(ref, L)
+    p, _ = oracle.pbwt_build(ref)
+    h0, _, a0, a1 = oracle.state_table(ref, p, L)
+    assert np.array_equal(idx.states.cpu().numpy().view(np.uint32), h0)
+    g0, g1 = idx.state_alleles()
+    assert np.array_equal(g0, a0) and np.array_equal(g1, a1)
+    # class-present mask and padding bytes
+    sa = idx.sa.cpu().numpy()
+    cls = np.minimum(a0.astype(int) + a1.astype(int), 2)
+    assert np.array_equal((sa[:, :L] >> 4) & 3, cls)
+    assert (sa[:, L:] == 0x70).all()
+    mm = idx.mmask.cpu().numpy()
+    for m in range(M):
+        assert mm[m] == sum(1 << c for c in set(cls[m].tolist()))
+
+
+def test_state_table_simple_mode():
+    rng = np.random.default_rng(4)
+    ref = (rng.random((10, 40)) < 0.5).astype(np.uint8)
+    idx = gpu_index(ref, 8, mode=1)
+    h0, _, a0, a1 = oracle.state_table(ref, np.zeros((10, 40), np.uint32), 8, simple_mode=True)
+    assert np.array_equal(idx.states.cpu().numpy().view(np.uint32), h0)
+    g0, g1 = idx.state_alleles()
+    assert np.array_equal(g0, a0) and np.array_equal(g1, a1)
+
+
+# ------------------------------------------------------------ forward-backward
+def check_fb(idx, codes, palette, trans, res, per_marker, fold=True, sample_base=0, seed=12345,
+             argmax=False):
+    a0, a1 = idx.state_alleles()
+    gl = palette[codes[:, : idx.M]]
+    if fold:
+        ref = oracle.fb_f64(a0, a1, gl, trans.Td, trans.To, per_marker=per_marker)
+    else:
+        ref = oracle.fb_f64_fast(a0, a1, gl, trans.Td, trans.To, per_marker=per_marker)
+    err = np.abs(res["ds"] - ref["ds"]).max()
+    assert err <= DS_TOL, f"max |DS - DS_fp64| = {err:.3g}"
+    assert np.abs(res["ap"] - ref["ap"]).max() <= DS_TOL
+    if ref["prob"] is not None and res["prob"] is not None:
+        assert np.abs(res["prob"] - ref["prob"]).max() <= DS_TOL
+    ll_err = np.abs(res["loglik"] - ref["loglik"]) / np.maximum(1.0, np.abs(ref["loglik"]))
+    assert ll_err.max() < 1e-5
+    hap = oracle.sample_replay(res["prob"], a0, a1, seed, mode="argmax" if argmax else "philox",
+                               sample_base=sample_base)
+    assert np.array_equal(res["hap"], hap), "sampled haplotypes differ from the host replay"
+    return err
+
+
+def hardcall_codes(rng, B, M, typed_every=4):
+    codes = np.full((B, M), 3, np.uint8)
+    codes[:, ::typed_every] = rng.integers(0, 3, (B, (M + typed_every - 1) // typed_every))
+    return codes
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 8, 12, 16, 32, 64])
+@pytest.mark.parametrize("per_marker", [False, True])
+def test_fb_states_and_modes(L, per_marker):
+    rng = np.random.default_rng(L * 7 + per_marker)
+    pan = synth.make_panel(max(4 * L, 64), 97, seed=L, spacing_mode=1)
+    idx = device.DeviceIndex(pan.cols, pan.H, L)
+    codes = hardcall_codes(rng, 9, pan.M)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 10000, per_marker=per_marker)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, per_marker)
+
+
+def test_fb_general_genotype_likelihoods():
+    # a palette of soft likelihoods (not hard calls), including mixed -999s
+    rng = np.random.default_rng(21)
+    pal = np.concatenate([synth.HARDCALL_PALETTE[:3],
+                          rng.uniform(-4, 0, (20, 3)).astype(np.float32),
+                          np.array([[-999, -2.0, -999], [0, 0, 0]], np.float32)])
+    pan = synth.make_panel(256, 300, seed=5, spacing_mode=1)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = rng.integers(0, pal.shape[0], (6, pan.M)).astype(np.uint8)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 5000, per_marker=True)
+    res = run_fb(idx, codes, pal, tr)
+    check_fb(idx, codes, pal, tr, res, True)
+
+
+@pytest.mark.parametrize("ne", [1000, 10000, 100000])
+def test_fb_long_chain_reaches_floor(ne):
+    # 4000 markers: the backward vector falls to the -999 floor (about 2-4 per
+    # marker) and the floor then shapes the posteriors; per-marker transitions
+    # with heavy-tailed spacing keep the model non-degenerate.
+    rng = np.random.default_rng(ne)
+    pan = synth.make_panel(400, 4000, seed=7, spacing_mode=1)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = hardcall_codes(rng, 4, pan.M, typed_every=20)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne, per_marker=True)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, True, fold=False)
+
+
+def test_fb_all_states_mismatch():
+    # hom-alt observed where every selected pair is hom-ref: the emission
+    # offset is -999 and the floor dominates the backward step.
+    ref = np.zeros((30, 16), np.uint8)
+    ref[:, 8:] = 1
+    ref[:, 3] = 1
+    idx = gpu_index(ref, 4)
+    codes = np.full((2, 30), 3, np.uint8)
+    codes[:, 10] = 2
+    codes[1, 20] = 0
+    g = np.arange(1, 31, dtype=np.uint64) * 3000
+    tr = device.Transitions.build(g, None, 10000, per_marker=True)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, True)
+
+
+@pytest.mark.parametrize("M", [1, 2, 15, 16, 17, 33])
+def test_fb_marker_count_edges(M):
+    rng = np.random.default_rng(M)
+    pan = synth.make_panel(64, M, seed=M)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = hardcall_codes(rng, 3, M, typed_every=2)
+    tr = device.Transitions.build(pan.pos, None, 10000, per_marker=False)
+    if M >= 2:  # non-degenerate transition for the compat step
+        tr.step = device.step_coefs(np.array([-0.2], np.float32), np.array([-3.0], np.float32))[0]
+        tr.Td = np.full(M - 1, -0.2, np.float32)
+        tr.To = np.full(M - 1, -3.0, np.float32)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, False)
+
+
+def test_fb_argmax_mode():
+    rng = np.random.default_rng(2)
+    pan = synth.make_panel(128, 200, seed=2, spacing_mode=1)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = hardcall_codes(rng, 5, pan.M)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 10000, per_marker=True)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, argmax=True)
+    check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, True, argmax=True)
+
+
+def test_fb_batch_split_invariance():
+    # Draws depend on (seed, global sample, marker) only: splitting a cohort into
+    # batches with the right sample_base reproduces the whole-cohort run.
+    rng = np.random.default_rng(8)
+    pan = synth.make_panel(256, 150, seed=8, spacing_mode=1)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = hardcall_codes(rng, 12, pan.M)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 10000, per_marker=True)
+    whole = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    a = run_fb(idx, codes[:5], synth.HARDCALL_PALETTE, tr, sample_base=0)
+    b = run_fb(idx, codes[5:], synth.HARDCALL_PALETTE, tr, sample_base=5)
+    for k in ("ds", "ap", "hap", "prob"):
+        assert np.array_equal(whole[k], np.concatenate([a[k], b[k]])), k
+
+
+def test_fb_config1_full():
+    """Config 1 at full size: 4432 haplotypes x 1644 markers, 444 targets,
+    L = 8, Ne = 1e4 (reference semantics: first interval's transitions)."""
+    w = synth.WORKLOADS["cfg1"]
+    pan = synth.make_panel(w["H"], w["M"], seed=12345)
+    tg = synth.make_targets(pan, w["B"], w["typed_every"], seed=12346)
+    idx = device.DeviceIndex(pan.cols, pan.H, w["L"])
+    tr = device.Transitions.build(pan.pos, None, w["ne"], per_marker=False)
+    res = run_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr)
+    check_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr, res, False, fold=False)
+
+
+def test_fb_config3_shape_subset():
+    """Config 3 shape (8000 haplotypes, heavy-tailed spacing with hotspots,
+    per-marker transitions) on 20000 markers x 16 samples, Ne 1e3/1e4/1e5."""
+    pan = synth.make_panel(8000, 20000, seed=33, spacing_mode=1)
+    tg = synth.make_targets(pan, 16, 20, seed=34, geno_error=0.02)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8, keep_prefix=False)
+    for ne in (1000, 10000, 100000):
+        tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne, per_marker=True)
+        res = run_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr)
+        check_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr, res, True, fold=False)
