@@ -52,6 +52,13 @@ __device__ __forceinline__ void lane_emission(float4 e3, uint32_t sa4, float e[4
     }
 }
 
+// Emission-table row for (palette code, class mask of marker m).  Codes at or
+// beyond the palette size are read as the last palette entry, so no input can
+// index past the table.
+__device__ __forceinline__ uint32_t emis_row(const FbParams& p, uint32_t code, uint32_t m) {
+    return min(code, p.P - 1u) * 8u + p.mmask[m];
+}
+
 __device__ __forceinline__ float2 step_tu(const FbParams& p, uint32_t m) {
     if (p.flags & SI_FB_PER_MARKER) {
         const float4 c = reinterpret_cast<const float4*>(p.steps)[m];
@@ -134,7 +141,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
                 const uint32_t m = m0 + t;
                 if (m < p.M) {
                     const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                    const uint32_t row = code * 8 + p.mmask[m];
+                    const uint32_t row = emis_row(p, code, m);
                     float e[4];
                     lane_emission(eps_l[row], sa32[size_t(m) * lp4 + r], e);
                     float S;
@@ -165,7 +172,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
                 const uint32_t n = m0 + uint32_t(t);
                 if (n < p.M && n > 0) {
                     const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                    const uint32_t row = code * 8 + p.mmask[n];
+                    const uint32_t row = emis_row(p, code, n);
                     float e[4];
                     lane_emission(eps_l[row], sa32[size_t(n) * lp4 + r], e);
                     const float2 tu = step_tu(p, n);
@@ -214,7 +221,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(FbParams p) {
             if (t < nm) {
                 const uint32_t m = m0 + t;
                 const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                const uint32_t row = code * 8 + p.mmask[m];
+                const uint32_t row = emis_row(p, code, m);
                 float e[4];
                 lane_emission(eps_l[row], sa32[size_t(m) * lp4 + r], e);
                 const float2 tu = step_tu(p, m);
@@ -236,7 +243,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(FbParams p) {
                 if (t > 0) {
                     const uint32_t n = m0 + uint32_t(t);
                     const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                    const uint32_t row = code * 8 + p.mmask[n];
+                    const uint32_t row = emis_row(p, code, n);
                     float e[4];
                     lane_emission(eps_l[row], sa32[size_t(n) * lp4 + r], e);
                     const float2 tu = step_tu(p, n);
