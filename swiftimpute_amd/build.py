@@ -101,6 +101,15 @@ def build(force: bool = True, verbose: bool = False) -> dict:
               "-pthread"])
         os.replace(tmp, api_lib)
         result["swiftimpute"] = api_lib
+        # C++ API test driver (tests/cpp/test_api.cpp), linked against the library
+        drv = os.path.join(ROOT, "tests", "cpp", "test_api.cpp")
+        if os.path.exists(drv):
+            exe = os.path.join(BUILD, "test_api")
+            _run(["g++", *HOST_FLAGS, drv, "-o", exe + ".tmp", f"-L{LIBDIR}", "-lswiftimpute",
+                  "-lsi_hip", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{LIBDIR}",
+                  f"-Wl,-rpath,{ROCM}/lib", "-Wl,-rpath,$ORIGIN/../lib", "-pthread"])
+            os.replace(exe + ".tmp", exe)
+            result["test_api"] = exe
     return result
 
 

@@ -1,0 +1,371 @@
+// Imputer: the reference's batch driver (src/api/imputer.cpp:547-1042) over
+// the C ABI of include/swiftimpute_hip.h.
+//
+// build_index() keeps the bit-packed panel, the PBWT arrays and the
+// sample-independent state table resident on the device.  impute() converts
+// nothing per sample on the host beyond the palette coding TargetData already
+// holds; per batch it uploads the codes, runs si_fb_run (one job per launch,
+// the plain form) and copies haplotypes, allele probabilities and
+// log-likelihoods back.  Without build_index(), selection falls back to the
+// reference's simple pairs (2k, 2k+1) (imputer.cpp:929-942).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/swiftimpute/imputer.hpp"
+#include "../../../include/swiftimpute_hip.h"
+
+namespace swiftimpute {
+
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e), int(e));
+}
+
+void si_check(int rc, const char* what) {
+    if (rc == SI_OK) return;
+    const std::string msg = std::string(what) + ": " + si_last_error();
+    if (rc == SI_ERR_DEVICE) throw DeviceError(msg, rc);
+    throw ImputationError(msg);
+}
+
+template <class T>
+T* dev_alloc(size_t count, size_t& total) {
+    if (count == 0) return nullptr;
+    void* p = nullptr;
+    hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    total += count * sizeof(T);
+    return static_cast<T*>(p);
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct Imputer::Device {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t bytes = 0;
+    uint64_t* cols = nullptr;
+    uint64_t* miss = nullptr;
+    uint32_t* prefix = nullptr;
+    uint32_t* div = nullptr;
+    uint32_t* states = nullptr;
+    uint8_t* sa = nullptr;
+    uint8_t* mmask = nullptr;
+    int table_mode = -1;  // 0 PBWT, 1 simple
+    // per-batch
+    uint32_t cap = 0;
+    uint32_t pitch = 0;
+    uint8_t* codes = nullptr;
+    float* ap = nullptr;
+    uint8_t* hap = nullptr;
+    double* loglik = nullptr;
+    void* ws = nullptr;
+    size_t ws_bytes = 0;
+    si_step_coef* steps = nullptr;
+    float* eps = nullptr;
+    float* emax = nullptr;
+
+    ~Device() {
+        dev_free(cols);
+        dev_free(miss);
+        dev_free(prefix);
+        dev_free(div);
+        dev_free(states);
+        dev_free(sa);
+        dev_free(mmask);
+        dev_free(codes);
+        dev_free(ap);
+        dev_free(hap);
+        dev_free(loglik);
+        if (ws) (void)hipFree(ws);
+        dev_free(steps);
+        dev_free(eps);
+        dev_free(emax);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+Imputer::Imputer(const ReferencePanel& reference, const ImputationConfig& config)
+    : reference_(reference), config_(config), dev_(std::make_unique<Device>()) {
+    const uint32_t L = config_.hmm_params.num_states;
+    if (L == 0 || L > 64) throw ImputationError("num_states must be in 1..64");
+    if (reference_.num_haplotypes() == 0) throw ImputationError("reference panel has no haplotypes");
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    dev_->device = config_.device_id < 0 ? 0 : config_.device_id;
+    if (dev_->device >= n) throw ImputationError("device_id " + std::to_string(dev_->device) + " not present");
+    hip_check(hipSetDevice(dev_->device), "hipSetDevice");
+    hip_check(hipStreamCreate(&dev_->stream), "hipStreamCreate");
+    const auto& c = reference_.packed_alleles();
+    dev_->cols = dev_alloc<uint64_t>(c.size(), dev_->bytes);
+    hip_check(hipMemcpyAsync(dev_->cols, c.data(), c.size() * 8, hipMemcpyHostToDevice, dev_->stream),
+              "panel upload");
+    const auto& ms = reference_.packed_missing();
+    if (!ms.empty()) {
+        dev_->miss = dev_alloc<uint64_t>(ms.size(), dev_->bytes);
+        hip_check(hipMemcpyAsync(dev_->miss, ms.data(), ms.size() * 8, hipMemcpyHostToDevice, dev_->stream),
+                  "panel upload");
+    }
+    hip_check(hipStreamSynchronize(dev_->stream), "panel upload");
+}
+
+Imputer::~Imputer() = default;
+
+void Imputer::build_index() {
+    hip_check(hipSetDevice(dev_->device), "hipSetDevice");
+    const uint32_t M = reference_.num_markers(), H = reference_.num_haplotypes();
+    const uint32_t L = config_.hmm_params.num_states, Lp = si_padded_states(L);
+    if (!dev_->sa) {
+        dev_->states = dev_alloc<uint32_t>(size_t(M) * L, dev_->bytes);
+        dev_->sa = dev_alloc<uint8_t>(size_t(M) * Lp, dev_->bytes);
+        dev_->mmask = dev_alloc<uint8_t>(M, dev_->bytes);
+    }
+    const int mode = config_.hmm_params.use_pbwt_selection ? 0 : 1;
+    std::vector<uint32_t> perm(std::min(L, H));
+    if (mode == 0) {
+        dev_->prefix = dev_alloc<uint32_t>(size_t(M) * H, dev_->bytes);
+        dev_->div = dev_alloc<uint32_t>(size_t(M) * H, dev_->bytes);
+        const size_t wsb = si_pbwt_workspace_bytes(H, M);
+        void* ws = nullptr;
+        if (wsb) hip_check(hipMalloc(&ws, wsb), "hipMalloc");
+        si_check(si_pbwt_sweep(dev_->cols, H, M, dev_->prefix, dev_->div, ws, wsb, dev_->stream),
+                 "PBWT sweep");
+        si_check(si_select_perm(H, L, perm.data()), "selection permutation");
+        si_check(si_state_table(dev_->prefix, dev_->cols, dev_->miss, H, M, L, 0, perm.data(), dev_->states,
+                                dev_->sa, dev_->mmask, dev_->stream),
+                 "state table");
+        hip_check(hipStreamSynchronize(dev_->stream), "build_index");
+        if (ws) (void)hipFree(ws);
+        if (!config_.keep_pbwt_arrays) {
+            dev_free(dev_->prefix);
+            dev_free(dev_->div);
+            dev_->bytes -= 2 * size_t(M) * H * 4;
+        }
+    } else {
+        si_check(si_state_table(nullptr, dev_->cols, dev_->miss, H, M, L, 1, perm.data(), dev_->states, dev_->sa,
+                                dev_->mmask, dev_->stream),
+                 "state table");
+        hip_check(hipStreamSynchronize(dev_->stream), "build_index");
+    }
+    dev_->table_mode = mode;
+    index_built_ = true;
+    Logger::instance().info("index built: " + std::to_string(M) + " markers, " + std::to_string(H) +
+                            " haplotypes, L=" + std::to_string(L));
+}
+
+void Imputer::validate_targets(const TargetData& targets) const {
+    if (targets.num_markers() != reference_.num_markers())
+        throw ImputationError("Target and reference marker counts don't match: " +
+                              std::to_string(targets.num_markers()) + " vs " +
+                              std::to_string(reference_.num_markers()));
+}
+
+std::unique_ptr<ImputationResult> Imputer::impute(const TargetData& targets) {
+    return impute_with_progress(targets, nullptr);
+}
+
+std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
+    const TargetData& targets, std::function<void(uint32_t, uint32_t)> progress_callback) {
+    validate_targets(targets);
+    hip_check(hipSetDevice(dev_->device), "hipSetDevice");
+    const uint32_t M = targets.num_markers(), B = targets.num_samples(), H = reference_.num_haplotypes();
+    const uint32_t L = config_.hmm_params.num_states;
+    if (dev_->table_mode < 0) {
+        // No index: the reference falls back to simple selection (2k, 2k+1).
+        if (2ull * L > H) throw ImputationError("simple state selection needs 2L <= haplotypes");
+        const bool pbwt = config_.hmm_params.use_pbwt_selection;
+        config_.hmm_params.use_pbwt_selection = false;
+        build_index();
+        config_.hmm_params.use_pbwt_selection = pbwt;
+        index_built_ = false;
+    }
+    auto result = std::make_unique<ImputationResult>(B, M);
+    if (B == 0 || M == 0) return result;
+
+    // transitions from the reference panel's map (imputer.cpp:596-616)
+    std::vector<uint64_t> pos(M);
+    std::vector<double> cm(M), g(M);
+    for (uint32_t m = 0; m < M; ++m) {
+        pos[m] = reference_.markers()[m].pos;
+        cm[m] = reference_.markers()[m].cM;
+    }
+    si_check(si_genetic_map(pos.data(), cm.data(), M, g.data()), "genetic map");
+    std::vector<float> Td(std::max<uint32_t>(M, 2) - 1), To(Td.size());
+    si_check(si_transition_constants(g.data(), M, config_.hmm_params.ne, config_.hmm_params.rho_rate, Td.data(),
+                                     To.data()),
+             "transitions");
+    const uint32_t nint = M >= 2 ? M - 1 : 0;
+    std::vector<si_step_coef> co(std::max<uint32_t>(nint, 1));
+    si_check(si_step_coefs(Td.data(), To.data(), nint, co.data()), "step coefficients");
+    const bool per_marker = config_.transition_mode == TransitionMode::PerMarker;
+    si_step_coef step = nint ? co[0] : si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY};
+    if (per_marker) {
+        if (!dev_->steps) dev_->steps = dev_alloc<si_step_coef>(M, dev_->bytes);
+        std::vector<si_step_coef> full(M, si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY});
+        for (uint32_t m = 1; m < M; ++m) full[m] = co[m - 1];
+        for (uint32_t m = 1; m < M; ++m)
+            if (!(full[m].t_o >= 1e-30f))
+                throw ImputationError("interval " + std::to_string(m - 1) +
+                                      ": switch probability below the linear-space kernel's range");
+        hip_check(hipMemcpyAsync(dev_->steps, full.data(), M * sizeof(si_step_coef), hipMemcpyHostToDevice,
+                                 dev_->stream),
+                  "transition upload");
+    }
+
+    // emission table for this cohort's likelihood palette
+    const auto& pal = targets.palette();
+    const uint32_t P = uint32_t(pal.size());
+    std::vector<float> palf(3 * P), eps(size_t(P) * 32), emax(size_t(P) * 8);
+    for (uint32_t p = 0; p < P; ++p) {
+        palf[3 * p] = pal[p].ll_00;
+        palf[3 * p + 1] = pal[p].ll_01;
+        palf[3 * p + 2] = pal[p].ll_11;
+    }
+    si_check(si_emission_table(palf.data(), P, eps.data(), emax.data()), "emission table");
+    dev_free(dev_->eps);
+    dev_free(dev_->emax);
+    dev_->eps = dev_alloc<float>(eps.size(), dev_->bytes);
+    dev_->emax = dev_alloc<float>(emax.size(), dev_->bytes);
+    hip_check(hipMemcpyAsync(dev_->eps, eps.data(), eps.size() * 4, hipMemcpyHostToDevice, dev_->stream), "upload");
+    hip_check(hipMemcpyAsync(dev_->emax, emax.data(), emax.size() * 4, hipMemcpyHostToDevice, dev_->stream),
+              "upload");
+
+    const uint32_t batch = std::max<uint32_t>(1, std::min(config_.batch_size, B));
+    const uint32_t pitch = targets.code_pitch();
+    if (dev_->cap < batch || dev_->pitch != pitch) {
+        dev_free(dev_->codes);
+        dev_free(dev_->ap);
+        dev_free(dev_->hap);
+        dev_free(dev_->loglik);
+        dev_->codes = dev_alloc<uint8_t>(size_t(batch) * pitch, dev_->bytes);
+        dev_->ap = dev_alloc<float>(size_t(batch) * M * 2, dev_->bytes);
+        dev_->hap = dev_alloc<uint8_t>(size_t(batch) * 2 * M, dev_->bytes);
+        dev_->loglik = dev_alloc<double>(batch, dev_->bytes);
+        dev_->cap = batch;
+        dev_->pitch = pitch;
+    }
+    si_fb_job job{};
+    job.n_markers = M;
+    job.n_states = L;
+    job.flags = (per_marker ? uint32_t(SI_FB_PER_MARKER) : 0u) |
+                (config_.sampling == SamplingMode::Argmax ? uint32_t(SI_FB_ARGMAX) : 0u);
+    job.code_pitch = pitch;
+    job.n_palette = P;
+    job.seed = config_.seed;
+    job.step = step;
+    job.steps = dev_->steps;
+    job.sa = dev_->sa;
+    job.mmask = dev_->mmask;
+    job.codes = dev_->codes;
+    job.eps = dev_->eps;
+    job.emax = dev_->emax;
+    const bool want_ap = config_.output_probabilities || config_.output_dosages;
+    job.ap = want_ap ? dev_->ap : nullptr;
+    job.hap = dev_->hap;
+    job.loglik = dev_->loglik;
+    job.n_samples = batch;
+    const size_t need = si_fb_workspace_bytes(&job, 1);
+    if (need > dev_->ws_bytes) {
+        if (dev_->ws) (void)hipFree(dev_->ws);
+        dev_->bytes -= dev_->ws_bytes;
+        hip_check(hipMalloc(&dev_->ws, need), "hipMalloc");
+        dev_->ws_bytes = need;
+        dev_->bytes += need;
+    }
+    const uint8_t* codes = targets.codes().data();
+    for (uint32_t start = 0; start < B; start += batch) {
+        const uint32_t end = std::min(start + batch, B), n = end - start;
+        hip_check(hipMemcpyAsync(dev_->codes, codes + size_t(start) * pitch, size_t(n) * pitch,
+                                 hipMemcpyHostToDevice, dev_->stream),
+                  "codes upload");
+        job.n_samples = n;
+        job.sample_base = start;
+        si_check(si_fb_run(&job, 1, dev_->ws, dev_->ws_bytes, dev_->stream), "forward-backward");
+        hip_check(hipMemcpyAsync(result->mutable_haplotypes() + size_t(start) * 2 * M, dev_->hap,
+                                 size_t(n) * 2 * M, hipMemcpyDeviceToHost, dev_->stream),
+                  "haplotype download");
+        if (want_ap)
+            hip_check(hipMemcpyAsync(result->mutable_posteriors() + size_t(start) * M * 2, dev_->ap,
+                                     size_t(n) * M * 2 * sizeof(float), hipMemcpyDeviceToHost, dev_->stream),
+                      "posterior download");
+        hip_check(hipMemcpyAsync(result->mutable_log_likelihoods().data() + start, dev_->loglik,
+                                 size_t(n) * sizeof(double), hipMemcpyDeviceToHost, dev_->stream),
+                  "log-likelihood download");
+        hip_check(hipStreamSynchronize(dev_->stream), "batch");
+        if (progress_callback) progress_callback(end, B);
+    }
+    result->compute_info_scores();
+    return result;
+}
+
+size_t Imputer::device_memory_usage() const { return dev_->bytes; }
+
+size_t Imputer::estimate_memory_requirement(const ReferencePanel& reference, uint32_t num_target_samples,
+                                            const ImputationConfig& config) {
+    const size_t M = reference.num_markers(), H = reference.num_haplotypes();
+    const uint32_t L = config.hmm_params.num_states, Lp = si_padded_states(L);
+    const size_t W = (H + 63) / 64;
+    size_t total = M * W * 8 * (reference.packed_missing().empty() ? 1 : 2);
+    if (config.hmm_params.use_pbwt_selection) total += 2 * M * H * 4;
+    total += M * (size_t(L) * 4 + Lp + 1);
+    const uint32_t batch = std::max<uint32_t>(1, std::min(config.batch_size, num_target_samples));
+    si_fb_job j{};
+    j.n_samples = batch;
+    j.n_markers = uint32_t(M);
+    j.n_states = L;
+    total += si_fb_workspace_bytes(&j, 1);
+    total += size_t(batch) * ((M + 15) / 16 * 16 + M * 2 * 4 + 2 * M + 8);
+    return total;
+}
+
+std::vector<uint32_t> Imputer::pbwt_prefix() const {
+    std::vector<uint32_t> out;
+    if (!dev_->prefix) return out;
+    out.resize(size_t(reference_.num_markers()) * reference_.num_haplotypes());
+    hip_check(hipMemcpy(out.data(), dev_->prefix, out.size() * 4, hipMemcpyDeviceToHost), "prefix download");
+    return out;
+}
+
+std::vector<uint32_t> Imputer::pbwt_divergence() const {
+    std::vector<uint32_t> out;
+    if (!dev_->div) return out;
+    out.resize(size_t(reference_.num_markers()) * reference_.num_haplotypes());
+    hip_check(hipMemcpy(out.data(), dev_->div, out.size() * 4, hipMemcpyDeviceToHost), "divergence download");
+    return out;
+}
+
+std::vector<uint32_t> Imputer::selected_states() const {
+    std::vector<uint32_t> out;
+    if (!dev_->states) return out;
+    out.resize(size_t(reference_.num_markers()) * config_.hmm_params.num_states);
+    hip_check(hipMemcpy(out.data(), dev_->states, out.size() * 4, hipMemcpyDeviceToHost), "states download");
+    return out;
+}
+
+std::unique_ptr<ImputationResult> impute(const std::string& reference_vcf, const std::string& target_vcf,
+                                         const std::string& output_vcf, const ImputationConfig& config) {
+    auto reference = ReferencePanel::load_vcf(reference_vcf);
+    auto targets = TargetData::load_vcf(target_vcf);
+    Imputer imputer(*reference, config);
+    try {
+        imputer.build_index();
+    } catch (const ImputationError& e) {
+        Logger::instance().warning(std::string("PBWT index not available, proceeding without it: ") + e.what());
+    }
+    auto result = imputer.impute(*targets);
+    result->write_vcf(output_vcf, *targets, *reference, config);
+    return result;
+}
+
+}  // namespace swiftimpute

@@ -1,0 +1,99 @@
+"""The C++ drop-in API (include/swiftimpute/imputer.hpp) through its test
+driver tests/cpp/test_api.cpp: host-only checks on CPU; on the GPU, the
+Imputer's PBWT arrays, selected states and posteriors against the oracle, and
+its sampled haplotypes against the C-ABI path (itself replay-checked)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import GOLDEN, ROOT
+from swiftimpute_amd import device, synth
+
+EXE = os.path.join(ROOT, "swiftimpute_amd", "_build", "test_api")
+
+
+def run_exe(*args, timeout=600):
+    r = subprocess.run([EXE, *map(str, args)], capture_output=True, text=True, timeout=timeout)
+    print(r.stdout[-4000:], r.stderr[-2000:])
+    return r
+
+
+def test_cpp_api_host_checks():
+    r = run_exe("cpu", GOLDEN)
+    assert r.returncode == 0 and "PASS cpu checks" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_api_fixture(tmp_path):
+    r = run_exe("fixture", GOLDEN, tmp_path)
+    assert r.returncode == 0 and "PASS fixture" in r.stdout
+    lines = [l for l in open(tmp_path / "fixture_out.vcf") if not l.startswith("#")]
+    assert len(lines) == 10
+
+
+def write_inputs(d, pan, tg, palette):
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "meta.txt"), "w") as f:
+        f.write(f"{pan.H} {pan.M} {tg.B}\n")
+    pan.cols.tofile(os.path.join(d, "cols.bin"))
+    pan.pos.astype(np.uint64).tofile(os.path.join(d, "pos.bin"))
+    cm = pan.genetic_cM()
+    (cm if cm is not None else np.zeros(pan.M)).astype(np.float64).tofile(os.path.join(d, "cm.bin"))
+    np.ascontiguousarray(tg.codes[:, : pan.M]).tofile(os.path.join(d, "codes.bin"))
+    palette.astype(np.float32).tofile(os.path.join(d, "palette.bin"))
+
+
+def read_outputs(d, pan, B, L):
+    M, H = pan.M, pan.H
+    f = lambda n, t: np.fromfile(os.path.join(d, n), t)
+    return dict(prefix=f("prefix.bin", np.uint32).reshape(M, H), div=f("div.bin", np.uint32).reshape(M, H),
+                states=f("states.bin", np.uint32).reshape(M, L), hap=f("hap.bin", np.uint8).reshape(B, 2, M),
+                ap=f("ap.bin", np.float32).reshape(B, M, 2), loglik=f("loglik.bin", np.float64))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("per_marker,spacing,L,batch", [(False, 0, 8, 100), (True, 1, 16, 7),
+                                                         (True, 1, 5, 1000)])
+def test_cpp_api_against_oracle(tmp_path, per_marker, spacing, L, batch):
+    pan = synth.make_panel(1000, 600, seed=31, spacing_mode=spacing)
+    tg = synth.make_targets(pan, 23, 4, seed=32)
+    ne = 10000
+    write_inputs(tmp_path / "in", pan, tg, synth.HARDCALL_PALETTE)
+    os.makedirs(tmp_path / "out", exist_ok=True)
+    r = run_exe("run", tmp_path / "in", tmp_path / "out", L, ne, int(per_marker), 0, batch, 99)
+    assert r.returncode == 0, r.stdout
+    o = read_outputs(tmp_path / "out", pan, tg.B, L)
+    ref = pan.alleles()
+    p, d = oracle.pbwt_build(ref)
+    assert np.array_equal(o["prefix"], p) and np.array_equal(o["div"], d)
+    h0, _, a0, a1 = oracle.state_table(ref, p, L)
+    assert np.array_equal(o["states"], h0)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne, per_marker=per_marker)
+    gl = synth.HARDCALL_PALETTE[tg.codes[:, : pan.M]]
+    fb = oracle.fb_f64(a0, a1, gl, tr.Td, tr.To, per_marker=per_marker)
+    assert np.abs(o["ap"] - fb["ap"]).max() <= 1e-5
+    assert np.abs(o["loglik"] - fb["loglik"]).max() <= 1e-5 * np.abs(fb["loglik"]).max()
+    # sampled haplotypes: identical to the C-ABI path on the same seed (whole
+    # cohort in one job; the API's batches use the matching sample_base)
+    idx = device.DeviceIndex(pan.cols, pan.H, L)
+    job = device.Job(idx, torch.from_numpy(tg.codes).cuda(), synth.HARDCALL_PALETTE, tr, seed=99)
+    job.alloc_outputs(ds=True, hap=True)
+    device.run_jobs([job.c_job()])
+    torch.cuda.synchronize()
+    assert np.array_equal(job.out.hap.cpu().numpy(), o["hap"])
+
+
+@pytest.mark.gpu
+def test_cpp_api_simple_selection(tmp_path):
+    pan = synth.make_panel(200, 300, seed=41)
+    tg = synth.make_targets(pan, 5, 4, seed=42)
+    write_inputs(tmp_path / "in", pan, tg, synth.HARDCALL_PALETTE)
+    os.makedirs(tmp_path / "out", exist_ok=True)
+    r = run_exe("run", tmp_path / "in", tmp_path / "out", 8, 10000, 0, 1, 100, 1, 0)
+    assert r.returncode == 0, r.stdout
+    states = np.fromfile(tmp_path / "out" / "states.bin", np.uint32).reshape(300, 8)
+    assert (states == np.arange(0, 16, 2)[None, :]).all()
