@@ -23,6 +23,13 @@ enum class TransitionMode : uint32_t {
     PerMarker = 1       // interval m-1 for the step into marker m (added)
 };
 
+// Numeric form of the forward-backward kernel.
+enum class Precision : uint32_t {
+    Auto = 0,       // linear FP32 unless a switch probability is below its range
+    LinearF32 = 1,  // per-marker-rescaled linear FP32, FP64 backward scale
+    LogF32 = 2      // log-space FP32 (three-term log-sum-exp per state)
+};
+
 enum class SamplingMode : uint32_t {
     Philox = 0,  // inverse-CDF draw with Philox4x32-10 (seed, sample, marker)
     Argmax = 1   // most probable state (src/kernels/sampling.cu:138-182)
@@ -49,6 +56,7 @@ struct ImputationConfig {
     // added
     TransitionMode transition_mode;
     SamplingMode sampling;
+    Precision precision;
     uint64_t seed;               // Philox key (default 12345, sampling.hpp:74)
     bool keep_pbwt_arrays;       // keep prefix/divergence resident after build_index()
 
@@ -66,6 +74,7 @@ struct ImputationConfig {
           checkpoint_interval(0),
           transition_mode(TransitionMode::FirstInterval),
           sampling(SamplingMode::Philox),
+          precision(Precision::Auto),
           seed(12345ull),
           keep_pbwt_arrays(true) {}
 };

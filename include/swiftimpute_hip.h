@@ -156,8 +156,9 @@ typedef struct si_fb_job {
     const uint8_t* sa;    /* [M][Lp] device */
     const uint8_t* mmask; /* [M] device */
     const uint8_t* codes; /* [B][code_pitch] device */
-    const float* eps;     /* [P][8][4] device */
-    const float* emax;    /* [P][8] device */
+    const float* eps;     /* [P][8][4] device (linear kernel) */
+    const float* emax;    /* [P][8] device (linear kernel) */
+    const float* palette; /* [P][3] device (log-space kernel) */
     float* ds;            /* [B][M] device or NULL */
     float* ap;            /* [B][M][2] device or NULL */
     uint8_t* hap;         /* [B][2][M] device or NULL */

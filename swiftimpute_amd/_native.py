@@ -43,7 +43,7 @@ class FbJob(ctypes.Structure):
         ("rng_stream", ctypes.c_uint32), ("sample_base", ctypes.c_uint32),
         ("seed", ctypes.c_uint64), ("step", StepCoef),
         ("steps", _vp), ("sa", _vp), ("mmask", _vp), ("codes", _vp), ("eps", _vp),
-        ("emax", _vp), ("ds", _vp), ("ap", _vp), ("hap", _vp), ("prob", _vp),
+        ("emax", _vp), ("palette", _vp), ("ds", _vp), ("ap", _vp), ("hap", _vp), ("prob", _vp),
         ("loglik", _vp),
     ]
 

@@ -75,6 +75,7 @@ struct Imputer::Device {
     si_step_coef* steps = nullptr;
     float* eps = nullptr;
     float* emax = nullptr;
+    float* palette = nullptr;
 
     ~Device() {
         dev_free(cols);
@@ -92,6 +93,7 @@ struct Imputer::Device {
         dev_free(steps);
         dev_free(eps);
         dev_free(emax);
+        dev_free(palette);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -210,14 +212,20 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     si_check(si_step_coefs(Td.data(), To.data(), nint, co.data()), "step coefficients");
     const bool per_marker = config_.transition_mode == TransitionMode::PerMarker;
     si_step_coef step = nint ? co[0] : si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY};
+    // The linear kernel needs every switch probability it uses to be a normal
+    // FP32 number; coincident markers (To = -999) need the log-space kernel.
+    float min_switch = 1.0f;
+    for (uint32_t t = 0; t < (per_marker ? nint : std::min<uint32_t>(nint, 1)); ++t)
+        min_switch = std::min(min_switch, co[t].t_o);
+    bool logspace = config_.precision == Precision::LogF32 ||
+                    (config_.precision == Precision::Auto && !(min_switch >= 1e-30f));
+    if (!logspace && !(min_switch >= 1e-30f))
+        throw ImputationError("a switch probability is below the linear FP32 kernel's range "
+                              "(coincident markers); use Precision::LogF32 or Precision::Auto");
     if (per_marker) {
         if (!dev_->steps) dev_->steps = dev_alloc<si_step_coef>(M, dev_->bytes);
         std::vector<si_step_coef> full(M, si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY});
         for (uint32_t m = 1; m < M; ++m) full[m] = co[m - 1];
-        for (uint32_t m = 1; m < M; ++m)
-            if (!(full[m].t_o >= 1e-30f))
-                throw ImputationError("interval " + std::to_string(m - 1) +
-                                      ": switch probability below the linear-space kernel's range");
         hip_check(hipMemcpyAsync(dev_->steps, full.data(), M * sizeof(si_step_coef), hipMemcpyHostToDevice,
                                  dev_->stream),
                   "transition upload");
@@ -235,8 +243,13 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     si_check(si_emission_table(palf.data(), P, eps.data(), emax.data()), "emission table");
     dev_free(dev_->eps);
     dev_free(dev_->emax);
+    dev_free(dev_->palette);
     dev_->eps = dev_alloc<float>(eps.size(), dev_->bytes);
     dev_->emax = dev_alloc<float>(emax.size(), dev_->bytes);
+    dev_->palette = dev_alloc<float>(palf.size(), dev_->bytes);
+    hip_check(hipMemcpyAsync(dev_->palette, palf.data(), palf.size() * 4, hipMemcpyHostToDevice,
+                             dev_->stream),
+              "upload");
     hip_check(hipMemcpyAsync(dev_->eps, eps.data(), eps.size() * 4, hipMemcpyHostToDevice, dev_->stream), "upload");
     hip_check(hipMemcpyAsync(dev_->emax, emax.data(), emax.size() * 4, hipMemcpyHostToDevice, dev_->stream),
               "upload");
@@ -259,7 +272,8 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     job.n_markers = M;
     job.n_states = L;
     job.flags = (per_marker ? uint32_t(SI_FB_PER_MARKER) : 0u) |
-                (config_.sampling == SamplingMode::Argmax ? uint32_t(SI_FB_ARGMAX) : 0u);
+                (config_.sampling == SamplingMode::Argmax ? uint32_t(SI_FB_ARGMAX) : 0u) |
+                (logspace ? uint32_t(SI_FB_LOGSPACE) : 0u);
     job.code_pitch = pitch;
     job.n_palette = P;
     job.seed = config_.seed;
@@ -270,6 +284,7 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     job.codes = dev_->codes;
     job.eps = dev_->eps;
     job.emax = dev_->emax;
+    job.palette = dev_->palette;
     const bool want_ap = config_.output_probabilities || config_.output_dosages;
     job.ap = want_ap ? dev_->ap : nullptr;
     job.hap = dev_->hap;

@@ -178,19 +178,21 @@ static int check_job(const si_fb_job& j, uint32_t idx) {
                     si::kMaxPalette);
     if (j.code_pitch < j.n_markers || (j.code_pitch & 15u))
         return fail(SI_ERR_ARG, "job %u: code_pitch must be >= M and a multiple of 16", idx);
-    if (j.flags & SI_FB_LOGSPACE)
-        return fail(SI_ERR_UNSUPPORTED, "job %u: log-space kernel not built", idx);
     if (j.n_samples == 0 || j.n_markers == 0) return SI_OK;
-    if (!j.sa || !j.mmask || !j.codes || !j.eps || !j.emax)
-        return fail(SI_ERR_ARG, "job %u: sa, mmask, codes, eps, emax are required", idx);
-    if (!aligned(j.codes, 16) || !aligned(j.sa, 4) || !aligned(j.eps, 16))
+    const bool logspace = (j.flags & SI_FB_LOGSPACE) != 0;
+    if (!j.sa || !j.mmask || !j.codes)
+        return fail(SI_ERR_ARG, "job %u: sa, mmask, codes are required", idx);
+    if (logspace ? !j.palette : (!j.eps || !j.emax))
+        return fail(SI_ERR_ARG, "job %u: %s required", idx,
+                    logspace ? "palette is" : "eps and emax are");
+    if (!aligned(j.codes, 16) || !aligned(j.sa, 4) || (j.eps && !aligned(j.eps, 16)))
         return fail(SI_ERR_ARG, "job %u: codes (16 B), sa (4 B), eps (16 B) must be aligned", idx);
     if ((j.flags & SI_FB_PER_MARKER) && !j.steps)
         return fail(SI_ERR_ARG, "job %u: per-marker transitions need steps[]", idx);
-    if (!(j.flags & SI_FB_PER_MARKER) && j.n_markers >= 2 && !(j.step.t_o >= 1e-30f))
+    if (!logspace && !(j.flags & SI_FB_PER_MARKER) && j.n_markers >= 2 && !(j.step.t_o >= 1e-30f))
         return fail(SI_ERR_UNSUPPORTED,
-                    "job %u: switch probability e^To = %g below the linear-space range", idx,
-                    double(j.step.t_o));
+                    "job %u: switch probability e^To = %g below the linear-space range "
+                    "(use SI_FB_LOGSPACE)", idx, double(j.step.t_o));
     if (!j.ds && !j.ap && !j.hap && !j.prob && !j.loglik)
         return fail(SI_ERR_ARG, "job %u: no output requested", idx);
     return SI_OK;
@@ -249,6 +251,7 @@ int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
         p.codes = j.codes;
         p.eps = j.eps;
         p.emax = j.emax;
+        p.palette = j.palette;
         p.ds = j.ds;
         p.ap = j.ap;
         p.hap = j.hap;

@@ -9,12 +9,17 @@
 // with t_ij = e^To + (e^Td - e^To) [i == j]: rank one plus diagonal, so a
 // step costs O(L), not O(L^2) log-sum-exps, and no matrix unit is involved.
 //
-// This file keeps both vectors in linear FP32 space:
-//   * forward: sum-normalized, a'_j = (t_o + u a_j) eps_j; the floor term is
-//     below FP32 resolution whenever To >= -80 (checked on the host);
-//   * backward: b_m = e^{Bl_m} v_m with the log scale Bl_m in FP64, so the
-//     floor enters exactly as f = exp(-999 - Bl_m) once b reaches it.
-// exp/log run once per (sample, marker), not per state.
+// Two numeric forms share the same structure (template parameter LOG):
+//   * linear FP32 (default): forward sum-normalized, a'_j = (t_o + u a_j) eps_j
+//     (the floor term is below FP32 resolution whenever To >= -80, checked on
+//     the host); backward b_m = e^{Bl_m} v_m with the log scale Bl_m in FP64,
+//     so the floor enters exactly as f = exp(-999 - Bl_m) once b reaches it.
+//     exp/log run once per (sample, marker), not per state.
+//   * log-space FP32: forward log vector normalized to log-sum 0; backward
+//     log vector relative to an FP64 offset; every step is a three-term
+//     log-sum-exp LSE(-999, To + LSE(alpha), U + alpha_j) per state,
+//     U = log(e^Td - e^To).  Valid for any To, including the -999 of
+//     coincident markers.
 //
 // Two passes per job (all samples of the job in both):
 //   chain pass: one lane group per (sample, direction) walks all M markers,
@@ -31,23 +36,39 @@
 namespace si {
 
 constexpr int kFbThreads = 128;
+constexpr float kFloor = -999.0f;
 
-__device__ __forceinline__ void load_tables(const FbParams& p, float4* eps_l, float* emax_l) {
+// Emission table staged in LDS: row[code * 8 + class mask] holds the four
+// per-class values (class 3 = padding state), off[] the row's log offset.
+//   linear: row = (exp(gl_c - off), ..., 0),  off = max over present classes
+//   log:    row = (gl_00, gl_01, gl_11, -inf), off = 0
+struct EmisLds {
+    float4 row[kMaxPalette * 8];
+    float off[kMaxPalette * 8];
+};
+
+template <bool LOG>
+__device__ __forceinline__ void load_tables(const FbParams& p, EmisLds& t) {
     const uint32_t n = p.P * 8;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        eps_l[i] = reinterpret_cast<const float4*>(p.eps)[i];
-        emax_l[i] = p.emax[i];
+        if constexpr (LOG) {
+            const float* g = p.palette + 3 * (i / 8);
+            t.row[i] = make_float4(g[0], g[1], g[2], -INFINITY);
+            t.off[i] = 0.0f;
+        } else {
+            t.row[i] = reinterpret_cast<const float4*>(p.eps)[i];
+            t.off[i] = p.emax[i];
+        }
     }
 }
 
-// Emission factors of this lane's four states: class 0/1/2 select
-// eps.x/y/z, class 3 (padding state) gives 0.
+// Emission values of this lane's four states (class bits 4-5 of each byte).
 __device__ __forceinline__ void lane_emission(float4 e3, uint32_t sa4, float e[4]) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const uint32_t cls = (sa4 >> (8 * i + 4)) & 3u;
         const float x = (cls & 1u) ? e3.y : e3.x;
-        const float y = (cls & 1u) ? 0.0f : e3.z;
+        const float y = (cls & 1u) ? e3.w : e3.z;
         e[i] = (cls & 2u) ? y : x;
     }
 }
@@ -59,65 +80,134 @@ __device__ __forceinline__ uint32_t emis_row(const FbParams& p, uint32_t code, u
     return min(code, p.P - 1u) * 8u + p.mmask[m];
 }
 
-__device__ __forceinline__ float2 step_tu(const FbParams& p, uint32_t m) {
-    if (p.flags & SI_FB_PER_MARKER) {
-        const float4 c = reinterpret_cast<const float4*>(p.steps)[m];
-        return make_float2(c.x, c.y);
+// (t_o, u, lto, lu) of the step into marker m.
+__device__ __forceinline__ float4 step_coef(const FbParams& p, uint32_t m) {
+    if (p.flags & SI_FB_PER_MARKER) return reinterpret_cast<const float4*>(p.steps)[m];
+    return make_float4(p.step.t_o, p.step.u, p.step.lto, p.step.lu);
+}
+
+template <int LPS>
+__device__ __forceinline__ float group_lse(float x0, float x1, float x2, float x3) {
+    const float mx = group_max<LPS>(fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)));
+    const float s = group_sum<LPS>((fast_exp(x0 - mx) + fast_exp(x1 - mx)) +
+                                   (fast_exp(x2 - mx) + fast_exp(x3 - mx)));
+    return mx + fast_ln(s);
+}
+
+__device__ __forceinline__ float lse3(float a, float b, float c) {
+    const float mx = fmaxf(fmaxf(a, b), c);
+    return mx + fast_ln((fast_exp(a - mx) + fast_exp(b - mx)) + fast_exp(c - mx));
+}
+
+// ---------------------------------------------------------------- forward --
+// Marker 0.  Returns the log normalizer (log-likelihood increment).
+template <int LPS, bool LOG>
+__device__ __forceinline__ double fwd_init(float4& a, const float e[4], float off) {
+    if constexpr (LOG) {
+        const float c = group_lse<LPS>(e[0], e[1], e[2], e[3]);
+        a = make_float4(e[0] - c, e[1] - c, e[2] - c, e[3] - c);
+        return double(c);
+    } else {
+        const float S = group_sum<LPS>((e[0] + e[1]) + (e[2] + e[3]));
+        const float r = __builtin_amdgcn_rcpf(S);
+        a = make_float4(e[0] * r, e[1] * r, e[2] * r, e[3] * r);
+        return double(off) + double(fast_ln(S));
     }
-    return make_float2(p.step.t_o, p.step.u);
 }
 
-// Forward: a <- normalize((t_o + u a) * e).  Returns the normalizer.
-template <int LPS>
-__device__ __forceinline__ float fwd_step(float4& a, const float e[4], float t_o, float u) {
-    const float x0 = fmaf(u, a.x, t_o) * e[0];
-    const float x1 = fmaf(u, a.y, t_o) * e[1];
-    const float x2 = fmaf(u, a.z, t_o) * e[2];
-    const float x3 = fmaf(u, a.w, t_o) * e[3];
-    const float S = group_sum<LPS>((x0 + x1) + (x2 + x3));
-    const float r = __builtin_amdgcn_rcpf(S);
-    a = make_float4(x0 * r, x1 * r, x2 * r, x3 * r);
-    return S;
+// Step into a marker with emission e / offset off and coefficients k.
+template <int LPS, bool LOG>
+__device__ __forceinline__ double fwd_step(float4& a, const float e[4], float off, float4 k) {
+    if constexpr (LOG) {
+        // a is normalized to log-sum 0, so LSE_i(a_i + To) = To.
+        const float x0 = lse3(kFloor, k.z, k.w + a.x) + e[0];
+        const float x1 = lse3(kFloor, k.z, k.w + a.y) + e[1];
+        const float x2 = lse3(kFloor, k.z, k.w + a.z) + e[2];
+        const float x3 = lse3(kFloor, k.z, k.w + a.w) + e[3];
+        const float c = group_lse<LPS>(x0, x1, x2, x3);
+        a = make_float4(x0 - c, x1 - c, x2 - c, x3 - c);
+        return double(c);
+    } else {
+        // a sums to 1, so sum_i a_i t_o = t_o.
+        const float x0 = fmaf(k.y, a.x, k.x) * e[0];
+        const float x1 = fmaf(k.y, a.y, k.x) * e[1];
+        const float x2 = fmaf(k.y, a.z, k.x) * e[2];
+        const float x3 = fmaf(k.y, a.w, k.x) * e[3];
+        const float S = group_sum<LPS>((x0 + x1) + (x2 + x3));
+        const float r = __builtin_amdgcn_rcpf(S);
+        a = make_float4(x0 * r, x1 * r, x2 * r, x3 * r);
+        return double(off) + double(fast_ln(S));
+    }
 }
 
-template <int LPS>
-__device__ __forceinline__ float fwd_init(float4& a, const float e[4]) {
-    const float S = group_sum<LPS>((e[0] + e[1]) + (e[2] + e[3]));
-    const float r = __builtin_amdgcn_rcpf(S);
-    a = make_float4(e[0] * r, e[1] * r, e[2] * r, e[3] * r);
-    return S;
+// --------------------------------------------------------------- backward --
+// From the vector at marker n (emission e, offset off) to n - 1.  The true
+// backward vector is e^{Bl} v (linear) or Bl + v (log).  With
+// nat = Bl + off + log W the new scale is max(nat, -999): when the floor
+// dominates (nat < -999) the scale pins at -999 and the non-floor part is
+// shifted down by sh = -999 - nat, so no value ever leaves a safe range.
+template <int LPS, bool LOG>
+__device__ __forceinline__ void bwd_step(float4& v, double& Bl, const float e[4], float off,
+                                         float4 k) {
+    if constexpr (LOG) {
+        const float y0 = e[0] + v.x, y1 = e[1] + v.y, y2 = e[2] + v.z, y3 = e[3] + v.w;
+        const float lw = group_lse<LPS>(y0, y1, y2, y3);
+        const double nat = Bl + double(lw);
+        const double arg = -999.0 - nat;
+        const bool floor_dominates = arg > 0.0;
+        const float fl = floor_dominates ? 0.0f : float(arg);
+        const float sh = floor_dominates ? float(arg) : 0.0f;
+        Bl = floor_dominates ? -999.0 : nat;
+        const float bo = k.z - sh, bu = k.w - lw - sh;
+        v = make_float4(lse3(fl, bo, bu + y0), lse3(fl, bo, bu + y1), lse3(fl, bo, bu + y2),
+                        lse3(fl, bo, bu + y3));
+    } else {
+        const float w0 = e[0] * v.x, w1 = e[1] * v.y, w2 = e[2] * v.z, w3 = e[3] * v.w;
+        const float W = group_sum<LPS>((w0 + w1) + (w2 + w3));
+        const double nat = Bl + double(off) + double(fast_ln(W));
+        const double arg = -999.0 - nat;
+        const float x = fast_exp(float(-fabs(arg)));
+        const bool floor_dominates = arg > 0.0;
+        const float f = floor_dominates ? 1.0f : x;
+        const float g = floor_dominates ? x : 1.0f;
+        Bl = floor_dominates ? -999.0 : nat;
+        const float gr = g * k.y * __builtin_amdgcn_rcpf(W);
+        const float base = fmaf(g, k.x, f);
+        v = make_float4(fmaf(gr, w0, base), fmaf(gr, w1, base), fmaf(gr, w2, base),
+                        fmaf(gr, w3, base));
+    }
 }
 
-// Backward: from the vector at marker n (emission e, offset emax) to n - 1.
-//   b_{n-1}[i] = phi + e^{Bl + emax} (t_o W + u w_i),  w = e * v, W = sum w
-// New scale Bl' = max(Bl + emax + log W, -999); with g = exp(Bl + emax +
-// log W - Bl') and f = exp(-999 - Bl'), v' = f + g t_o + (g u / W) w.
-template <int LPS>
-__device__ __forceinline__ void bwd_step(float4& v, double& Bl, const float e[4], float emax,
-                                         float t_o, float u) {
-    const float w0 = e[0] * v.x, w1 = e[1] * v.y, w2 = e[2] * v.z, w3 = e[3] * v.w;
-    const float W = group_sum<LPS>((w0 + w1) + (w2 + w3));
-    const double nat = Bl + double(emax) + double(__logf(W));
-    const double arg = -999.0 - nat;
-    const float x = __expf(float(-fabs(arg)));
-    const bool floor_dominates = arg > 0.0;
-    const float f = floor_dominates ? 1.0f : x;
-    const float g = floor_dominates ? x : 1.0f;
-    Bl = floor_dominates ? -999.0 : nat;
-    const float gr = g * u * __builtin_amdgcn_rcpf(W);
-    const float base = fmaf(g, t_o, f);
-    v = make_float4(fmaf(gr, w0, base), fmaf(gr, w1, base), fmaf(gr, w2, base),
-                    fmaf(gr, w3, base));
+// Initial backward vector at the last marker (b = 1).
+template <bool LOG>
+__device__ __forceinline__ float4 bwd_init() {
+    return LOG ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(1.f, 1.f, 1.f, 1.f);
+}
+
+// -------------------------------------------------------------- posterior --
+template <int LPS, bool LOG>
+__device__ __forceinline__ float4 posterior(float4 a, float4 v) {
+    if constexpr (LOG) {
+        const float q0 = a.x + v.x, q1 = a.y + v.y, q2 = a.z + v.z, q3 = a.w + v.w;
+        const float mx = group_max<LPS>(fmaxf(fmaxf(q0, q1), fmaxf(q2, q3)));
+        const float w0 = fast_exp(q0 - mx), w1 = fast_exp(q1 - mx), w2 = fast_exp(q2 - mx),
+                    w3 = fast_exp(q3 - mx);
+        const float rq = 1.0f / group_sum<LPS>((w0 + w1) + (w2 + w3));
+        return make_float4(w0 * rq, w1 * rq, w2 * rq, w3 * rq);
+    } else {
+        const float4 q = make_float4(a.x * v.x, a.y * v.y, a.z * v.z, a.w * v.w);
+        const float rq = 1.0f / group_sum<LPS>((q.x + q.y) + (q.z + q.w));
+        return make_float4(q.x * rq, q.y * rq, q.z * rq, q.w * rq);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Chain pass.
 // ---------------------------------------------------------------------------
-template <int LPS>
+template <int LPS, bool LOG>
 __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
-    __shared__ float4 eps_l[kMaxPalette * 8];
-    __shared__ float emax_l[kMaxPalette * 8];
-    load_tables(p, eps_l, emax_l);
+    __shared__ EmisLds tab;
+    load_tables<LOG>(p, tab);
     __syncthreads();
 
     const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
@@ -140,18 +230,13 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
             for (uint32_t t = 0; t < kBlock; ++t) {
                 const uint32_t m = m0 + t;
                 if (m < p.M) {
-                    const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                    const uint32_t row = emis_row(p, code, m);
+                    const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), m);
                     float e[4];
-                    lane_emission(eps_l[row], sa32[size_t(m) * lp4 + r], e);
-                    float S;
-                    if (m == 0) {
-                        S = fwd_init<LPS>(a, e);
-                    } else {
-                        const float2 tu = step_tu(p, m);
-                        S = fwd_step<LPS>(a, e, tu.x, tu.y);
-                    }
-                    ll += double(emax_l[row]) + double(__logf(S));
+                    lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
+                    if (m == 0)
+                        ll += fwd_init<LPS, LOG>(a, e, tab.off[row]);
+                    else
+                        ll += fwd_step<LPS, LOG>(a, e, tab.off[row], step_coef(p, m));
                     if (t == 0)
                         reinterpret_cast<float4*>(p.ck_alpha)[(size_t(s) * p.nblk + c) * lp4 + r] = a;
                 }
@@ -159,7 +244,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
         }
         if (r == 0 && p.loglik) p.loglik[s] = ll;
     } else {
-        float4 v = make_float4(1.f, 1.f, 1.f, 1.f);
+        float4 v = bwd_init<LOG>();
         double Bl = 0.0;
         const size_t last = size_t(s) * p.nblk + (p.nblk - 1);
         reinterpret_cast<float4*>(p.ck_beta)[last * lp4 + r] = v;
@@ -171,12 +256,10 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
             for (int t = int(kBlock) - 1; t >= 0; --t) {
                 const uint32_t n = m0 + uint32_t(t);
                 if (n < p.M && n > 0) {
-                    const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                    const uint32_t row = emis_row(p, code, n);
+                    const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), n);
                     float e[4];
-                    lane_emission(eps_l[row], sa32[size_t(n) * lp4 + r], e);
-                    const float2 tu = step_tu(p, n);
-                    bwd_step<LPS>(v, Bl, e, emax_l[row], tu.x, tu.y);
+                    lane_emission(tab.row[row], sa32[size_t(n) * lp4 + r], e);
+                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(p, n));
                     if (t == 0) {  // v is now the vector at marker m0 - 1, last of block c - 1
                         const size_t ck = size_t(s) * p.nblk + uint32_t(c - 1);
                         reinterpret_cast<float4*>(p.ck_beta)[ck * lp4 + r] = v;
@@ -191,15 +274,14 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
 // ---------------------------------------------------------------------------
 // Block pass.
 // ---------------------------------------------------------------------------
-template <int LPS>
+template <int LPS, bool LOG>
 __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(FbParams p) {
     constexpr uint32_t G = kFbThreads / LPS;  // lane groups per workgroup
     constexpr uint32_t LP = 4 * LPS;
-    __shared__ float4 eps_l[kMaxPalette * 8];
-    __shared__ float emax_l[kMaxPalette * 8];
+    __shared__ EmisLds tab;
     __shared__ float pbuf[G][kBlock][LP];
     __shared__ uint4 rng[G][kBlock / 4];
-    load_tables(p, eps_l, emax_l);
+    load_tables<LOG>(p, tab);
     __syncthreads();
 
     const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
@@ -220,13 +302,11 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(FbParams p) {
         for (uint32_t t = 1; t < kBlock; ++t) {
             if (t < nm) {
                 const uint32_t m = m0 + t;
-                const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                const uint32_t row = emis_row(p, code, m);
+                const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), m);
                 float e[4];
-                lane_emission(eps_l[row], sa32[size_t(m) * lp4 + r], e);
-                const float2 tu = step_tu(p, m);
+                lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
                 A[t] = A[t - 1];
-                fwd_step<LPS>(A[t], e, tu.x, tu.y);
+                fwd_step<LPS, LOG>(A[t], e, tab.off[row], step_coef(p, m));
             }
         }
         const size_t ck = size_t(s) * p.nblk + c;
@@ -235,29 +315,27 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(FbParams p) {
 #pragma unroll
         for (int t = int(kBlock) - 1; t >= 0; --t) {
             if (uint32_t(t) < nm) {
-                const float4 q = make_float4(A[t].x * v.x, A[t].y * v.y, A[t].z * v.z, A[t].w * v.w);
-                const float Q = group_sum<LPS>((q.x + q.y) + (q.z + q.w));
-                const float rq = 1.0f / Q;
-                *reinterpret_cast<float4*>(&pbuf[gl][t][4 * r]) =
-                    make_float4(q.x * rq, q.y * rq, q.z * rq, q.w * rq);
+                *reinterpret_cast<float4*>(&pbuf[gl][t][4 * r]) = posterior<LPS, LOG>(A[t], v);
                 if (t > 0) {
                     const uint32_t n = m0 + uint32_t(t);
-                    const uint32_t code = byte_of((&cw.x)[t >> 2], t & 3);
-                    const uint32_t row = emis_row(p, code, n);
+                    const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), n);
                     float e[4];
-                    lane_emission(eps_l[row], sa32[size_t(n) * lp4 + r], e);
-                    const float2 tu = step_tu(p, n);
-                    bwd_step<LPS>(v, Bl, e, emax_l[row], tu.x, tu.y);
+                    lane_emission(tab.row[row], sa32[size_t(n) * lp4 + r], e);
+                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(p, n));
                 }
             }
         }
         const uint2 key = make_uint2(uint32_t(p.seed), uint32_t(p.seed >> 32));
         for (uint32_t j = r; j < kBlock / 4; j += LPS)
-            rng[gl][j] = philox4x32_10(make_uint4((m0 >> 2) + j, p.sample_base + s, p.rng_stream, 0u), key);
+            rng[gl][j] =
+                philox4x32_10(make_uint4((m0 >> 2) + j, p.sample_base + s, p.rng_stream, 0u), key);
     }
     __syncthreads();
     if (!active) return;
 
+    // Sampling, dosage and allele probabilities: lane r takes markers r,
+    // r + LPS, ... of the block and walks the states in order with a
+    // sequential FP32 cumulative sum (the host replays exactly this).
     const bool argmax = (p.flags & SI_FB_ARGMAX) != 0;
     const uint32_t L = p.L;
     for (uint32_t t = r; t < nm; t += LPS) {
@@ -301,7 +379,7 @@ size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp) {
     return 2 * vec_al + ((bl + 255) & ~size_t(255));
 }
 
-template <int LPS>
+template <int LPS, bool LOG>
 static int launch_fb_lps(const FbParams& p, uint32_t passes, hipStream_t stream) {
     const uint32_t gpw = kWave / LPS;  // groups per wavefront
     FbParams q = p;
@@ -312,28 +390,34 @@ static int launch_fb_lps(const FbParams& p, uint32_t passes, hipStream_t stream)
     const size_t g2 = (block_threads + kFbThreads - 1) / kFbThreads;
     if (g1 > 0x7fffffffu || g2 > 0x7fffffffu) return SI_ERR_UNSUPPORTED;
     if (passes & SI_FB_PASS_CHAIN) {
-        hipLaunchKernelGGL(fb_chain_kernel<LPS>, dim3(uint32_t(g1)), dim3(kFbThreads), 0, stream,
-                           q);
+        hipLaunchKernelGGL((fb_chain_kernel<LPS, LOG>), dim3(uint32_t(g1)), dim3(kFbThreads), 0,
+                           stream, q);
         if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     }
     if (passes & SI_FB_PASS_BLOCK) {
-        hipLaunchKernelGGL(fb_block_kernel<LPS>, dim3(uint32_t(g2)), dim3(kFbThreads), 0, stream,
-                           q);
+        hipLaunchKernelGGL((fb_block_kernel<LPS, LOG>), dim3(uint32_t(g2)), dim3(kFbThreads), 0,
+                           stream, q);
         if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     }
     return 0;
 }
 
-int launch_fb(const FbParams& p, uint32_t passes, hipStream_t stream) {
-    if (p.B == 0 || p.M == 0) return 0;
+template <bool LOG>
+static int launch_fb_space(const FbParams& p, uint32_t passes, hipStream_t stream) {
     switch (p.Lp) {
-        case 4: return launch_fb_lps<1>(p, passes, stream);
-        case 8: return launch_fb_lps<2>(p, passes, stream);
-        case 16: return launch_fb_lps<4>(p, passes, stream);
-        case 32: return launch_fb_lps<8>(p, passes, stream);
-        case 64: return launch_fb_lps<16>(p, passes, stream);
+        case 4: return launch_fb_lps<1, LOG>(p, passes, stream);
+        case 8: return launch_fb_lps<2, LOG>(p, passes, stream);
+        case 16: return launch_fb_lps<4, LOG>(p, passes, stream);
+        case 32: return launch_fb_lps<8, LOG>(p, passes, stream);
+        case 64: return launch_fb_lps<16, LOG>(p, passes, stream);
         default: return SI_ERR_UNSUPPORTED;
     }
+}
+
+int launch_fb(const FbParams& p, uint32_t passes, hipStream_t stream) {
+    if (p.B == 0 || p.M == 0) return 0;
+    return (p.flags & SI_FB_LOGSPACE) ? launch_fb_space<true>(p, passes, stream)
+                                      : launch_fb_space<false>(p, passes, stream);
 }
 
 }  // namespace si

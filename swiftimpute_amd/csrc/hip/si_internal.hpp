@@ -28,6 +28,7 @@ struct FbParams {
     const uint8_t* codes;
     const float* eps;
     const float* emax;
+    const float* palette;
     float* ds;
     float* ap;
     uint8_t* hap;

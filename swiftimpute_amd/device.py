@@ -108,6 +108,13 @@ class Transitions:
             steps_dev = torch.from_numpy(full).to(device)
         return Transitions(per_marker, step, steps_dev, Td, To)
 
+    def min_switch(self) -> float:
+        """Smallest e^To over the steps the kernel will use."""
+        if len(self.To) == 0:
+            return 1.0
+        to = self.To if self.per_marker else self.To[:1]
+        return float(np.exp(to.astype(np.float64)).min())
+
 
 # --------------------------------------------------------------------- index
 class DeviceIndex:
@@ -168,6 +175,7 @@ class Job:
     rng_stream: int = 0
     sample_base: int = 0
     argmax: bool = False
+    logspace: bool = False
     out: JobOutputs = field(default_factory=JobOutputs)
     _keep: list = field(default_factory=list)
 
@@ -192,13 +200,18 @@ class Job:
         dev = self.codes.device
         eps_t = torch.from_numpy(eps).to(dev)
         emax_t = torch.from_numpy(emax).to(dev)
-        self._keep = [eps_t, emax_t]
+        pal_t = torch.from_numpy(np.ascontiguousarray(self.palette, np.float32)).to(dev)
+        self._keep = [eps_t, emax_t, pal_t]
+        if not self.logspace and self.trans.min_switch() < 1e-30:
+            raise ValueError("a switch probability e^To is below the linear-space kernel's range "
+                             "(coincident markers); use logspace=True")
         j = FbJob()
         j.n_samples = self.B
         j.n_markers = self.index.M
         j.n_states = self.index.L
         j.flags = ((_native.SI_FB_PER_MARKER if self.trans.per_marker else 0)
-                   | (_native.SI_FB_ARGMAX if self.argmax else 0))
+                   | (_native.SI_FB_ARGMAX if self.argmax else 0)
+                   | (_native.SI_FB_LOGSPACE if self.logspace else 0))
         j.code_pitch = self.codes.shape[1]
         j.n_palette = self.palette.shape[0]
         j.rng_stream = self.rng_stream
@@ -212,6 +225,7 @@ class Job:
         j.codes = _ptr(self.codes)
         j.eps = _ptr(eps_t)
         j.emax = _ptr(emax_t)
+        j.palette = _ptr(pal_t)
         o = self.out
         j.ds, j.ap, j.hap, j.prob, j.loglik = (_ptr(o.ds), _ptr(o.ap), _ptr(o.hap), _ptr(o.prob),
                                                _ptr(o.loglik))

@@ -39,13 +39,14 @@ def gpu_index(ref, L, mode=0):
     return idx
 
 
-def run_fb(idx, codes, palette, trans, seed=12345, argmax=False, sample_base=0, prob=True):
+def run_fb(idx, codes, palette, trans, seed=12345, argmax=False, sample_base=0, prob=True,
+           logspace=False):
     B, M = codes.shape[0], idx.M
     pitch = (M + 15) // 16 * 16
     c = np.full((B, pitch), palette.shape[0] - 1, np.uint8)
     c[:, :M] = codes[:, :M]
     job = device.Job(idx, torch.from_numpy(c).cuda(), palette, trans, seed=seed, argmax=argmax,
-                     sample_base=sample_base)
+                     sample_base=sample_base, logspace=logspace)
     job.alloc_outputs(ds=True, ap=True, hap=True, prob=prob, loglik=True)
     device.run_jobs([job.c_job()])
     torch.cuda.synchronize()
@@ -150,17 +151,19 @@ def hardcall_codes(rng, B, M, typed_every=4):
 
 @pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 8, 12, 16, 32, 64])
 @pytest.mark.parametrize("per_marker", [False, True])
-def test_fb_states_and_modes(L, per_marker):
+@pytest.mark.parametrize("logspace", [False, True])
+def test_fb_states_and_modes(L, per_marker, logspace):
     rng = np.random.default_rng(L * 7 + per_marker)
     pan = synth.make_panel(max(4 * L, 64), 97, seed=L, spacing_mode=1)
     idx = device.DeviceIndex(pan.cols, pan.H, L)
     codes = hardcall_codes(rng, 9, pan.M)
     tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 10000, per_marker=per_marker)
-    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=logspace)
     check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, per_marker)
 
 
-def test_fb_general_genotype_likelihoods():
+@pytest.mark.parametrize("logspace", [False, True])
+def test_fb_general_genotype_likelihoods(logspace):
     # a palette of soft likelihoods (not hard calls), including mixed -999s
     rng = np.random.default_rng(21)
     pal = np.concatenate([synth.HARDCALL_PALETTE[:3],
@@ -170,12 +173,13 @@ def test_fb_general_genotype_likelihoods():
     idx = device.DeviceIndex(pan.cols, pan.H, 8)
     codes = rng.integers(0, pal.shape[0], (6, pan.M)).astype(np.uint8)
     tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 5000, per_marker=True)
-    res = run_fb(idx, codes, pal, tr)
+    res = run_fb(idx, codes, pal, tr, logspace=logspace)
     check_fb(idx, codes, pal, tr, res, True)
 
 
 @pytest.mark.parametrize("ne", [1000, 10000, 100000])
-def test_fb_long_chain_reaches_floor(ne):
+@pytest.mark.parametrize("logspace", [False, True])
+def test_fb_long_chain_reaches_floor(ne, logspace):
     # 4000 markers: the backward vector falls to the -999 floor (about 2-4 per
     # marker) and the floor then shapes the posteriors; per-marker transitions
     # with heavy-tailed spacing keep the model non-degenerate.
@@ -184,11 +188,12 @@ def test_fb_long_chain_reaches_floor(ne):
     idx = device.DeviceIndex(pan.cols, pan.H, 8)
     codes = hardcall_codes(rng, 4, pan.M, typed_every=20)
     tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne, per_marker=True)
-    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=logspace)
     check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, True, fold=False)
 
 
-def test_fb_all_states_mismatch():
+@pytest.mark.parametrize("logspace", [False, True])
+def test_fb_all_states_mismatch(logspace):
     # hom-alt observed where every selected pair is hom-ref: the emission
     # offset is -999 and the floor dominates the backward step.
     ref = np.zeros((30, 16), np.uint8)
@@ -200,8 +205,30 @@ def test_fb_all_states_mismatch():
     codes[1, 20] = 0
     g = np.arange(1, 31, dtype=np.uint64) * 3000
     tr = device.Transitions.build(g, None, 10000, per_marker=True)
-    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=logspace)
     check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, True)
+
+
+def test_fb_coincident_markers_logspace():
+    # Duplicate positions give rho = 0: stay = 1, switch = -999 (no switching
+    # at all, the floor carries every path).  Only the log-space kernel
+    # represents this; the linear kernel refuses it.
+    rng = np.random.default_rng(17)
+    pan = synth.make_panel(128, 120, seed=17)
+    pos = pan.pos.copy()
+    pos[40:60] = pos[40]           # a run of coincident markers
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = hardcall_codes(rng, 6, pan.M, typed_every=3)
+    for per_marker in (True, False):
+        p2 = pos.copy()
+        if not per_marker:
+            p2[1] = p2[0]          # the first interval drives every step
+        tr = device.Transitions.build(p2, None, 10000, per_marker=per_marker)
+        assert (tr.To == -999.0).any()
+        res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=True)
+        check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, per_marker, fold=True)
+    with pytest.raises(Exception):
+        run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=False)
 
 
 @pytest.mark.parametrize("M", [1, 2, 15, 16, 17, 33])
@@ -264,5 +291,7 @@ def test_fb_config3_shape_subset():
     idx = device.DeviceIndex(pan.cols, pan.H, 8, keep_prefix=False)
     for ne in (1000, 10000, 100000):
         tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne, per_marker=True)
-        res = run_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr)
-        check_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr, res, True, fold=False)
+        for logspace in (False, True):
+            res = run_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr, logspace=logspace)
+            err = check_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr, res, True, fold=False)
+            print(f"ne={ne} {'log' if logspace else 'linear'} max |DS err| = {err:.3g}")
