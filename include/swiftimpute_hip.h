@@ -185,6 +185,28 @@ enum si_fb_pass { SI_FB_PASS_CHAIN = 1u, SI_FB_PASS_BLOCK = 2u };
 int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
                      size_t workspace_bytes, uint32_t passes, void* stream);
 
+/* ---------------------------------------------------------- batched form --
+ * Many jobs of uneven cohort size, marker count and L in one persistent
+ * work-queue launch per pass and padded-state class (jobs longest first),
+ * optionally replayed from a captured hipGraph.  No counterpart in the
+ * reference, which runs one fixed-size batch after another
+ * (src/api/imputer.cpp:765-777).
+ *
+ * All jobs share one emission table (eps/emax/palette pointers and
+ * n_palette of jobs[0]; the others must match).  si_fb_batch_create copies
+ * the job descriptors to the device and allocates the batch's queues (set-up,
+ * not a hot call); si_fb_batch_run enqueues the batch on a stream and
+ * allocates nothing.  The workspace holds every job's checkpoints at once.
+ */
+typedef struct si_fb_batch si_fb_batch;
+int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out);
+size_t si_fb_batch_workspace_bytes(const si_fb_batch* batch);
+/* use_graph: capture the launches into a hipGraph on first run for this
+ * (workspace, stream) and replay it afterwards. */
+int si_fb_batch_run(si_fb_batch* batch, void* workspace, size_t workspace_bytes, int use_graph,
+                    void* stream);
+int si_fb_batch_destroy(si_fb_batch* batch);
+
 /* Philox4x32-10 uniform the sampler uses for (sample s, marker m): word
  * (m & 3) of Philox4x32-10(counter = {m >> 2, s, rng_stream, 0} with s the
  * global sample index (sample_base + job-local index),

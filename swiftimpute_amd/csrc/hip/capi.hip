@@ -232,40 +232,200 @@ int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
     for (uint32_t i = 0; i < n_jobs; ++i) {
         const si_fb_job& j = jobs[i];
         if (j.n_samples == 0 || j.n_markers == 0) continue;
-        si::FbParams p{};
-        p.B = j.n_samples;
-        p.M = j.n_markers;
-        p.L = j.n_states;
-        p.Lp = si_padded_states(j.n_states);
-        p.flags = j.flags;
-        p.pitch = j.code_pitch;
-        p.P = j.n_palette;
-        p.rng_stream = j.rng_stream;
-        p.sample_base = j.sample_base;
-        p.nblk = (j.n_markers + si::kBlock - 1) / si::kBlock;
-        p.seed = j.seed;
-        p.step = j.step;
-        p.steps = j.steps;
-        p.sa = j.sa;
-        p.mmask = j.mmask;
-        p.codes = j.codes;
-        p.eps = j.eps;
-        p.emax = j.emax;
-        p.palette = j.palette;
-        p.ds = j.ds;
-        p.ap = j.ap;
-        p.hap = j.hap;
-        p.prob = j.prob;
-        p.loglik = j.loglik;
-        char* ws = static_cast<char*>(workspace);
-        const size_t vec = size_t(p.B) * p.nblk * p.Lp * sizeof(float);
-        const size_t vec_al = (vec + 255) & ~size_t(255);
-        p.ck_alpha = reinterpret_cast<float*>(ws);
-        p.ck_beta = reinterpret_cast<float*>(ws + vec_al);
-        p.ck_bl = reinterpret_cast<double*>(ws + 2 * vec_al);
-        const int rc = si::launch_fb(p, passes, static_cast<hipStream_t>(stream));
+        const uint32_t Lp = si_padded_states(j.n_states);
+        si::JobView v = si::fb_job_view(j);
+        si::fb_bind_workspace(v, Lp, static_cast<char*>(workspace));
+        const si::FbTables t{j.eps, j.emax, j.palette, j.n_palette};
+        const int rc = si::launch_fb(v, t, Lp, passes, static_cast<hipStream_t>(stream));
         if (rc) return device_status(rc, "si_fb_run");
     }
+    return SI_OK;
+}
+
+// ------------------------------------------------------------ batched form --
+struct si_fb_batch {
+    struct Class {
+        uint32_t Lp = 0, flags = 0;
+        std::vector<uint32_t> order;  // job indices, longest first
+        si::JobView* d_jobs = nullptr;
+        uint64_t* d_off = nullptr;    // chain offsets [n+1] then block offsets [n+1]
+        uint32_t* d_counters = nullptr;
+        uint64_t chain_items = 0, block_items = 0;
+        uint32_t grid_chain = 1, grid_block = 1;
+    };
+    std::vector<si_fb_job> jobs;
+    std::vector<size_t> ws_off;  // per job
+    size_t ws_bytes = 0;
+    si::FbTables tables{};
+    std::vector<Class> classes;
+    void* bound_ws = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* graph_ws = nullptr;
+};
+
+static void batch_free(si_fb_batch* b) {
+    if (!b) return;
+    for (auto& c : b->classes) {
+        if (c.d_jobs) (void)hipFree(c.d_jobs);
+        if (c.d_off) (void)hipFree(c.d_off);
+        if (c.d_counters) (void)hipFree(c.d_counters);
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    delete b;
+}
+
+int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out) {
+    if (!out || (n_jobs && !jobs)) return fail(SI_ERR_ARG, "si_fb_batch_create: bad arguments");
+    *out = nullptr;
+    if (n_jobs == 0) return fail(SI_ERR_ARG, "si_fb_batch_create: no jobs");
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        const int rc = check_job(jobs[i], i);
+        if (rc) return rc;
+        if (jobs[i].eps != jobs[0].eps || jobs[i].emax != jobs[0].emax ||
+            jobs[i].palette != jobs[0].palette || jobs[i].n_palette != jobs[0].n_palette)
+            return fail(SI_ERR_ARG, "si_fb_batch_create: job %u uses a different emission table", i);
+    }
+    auto* b = new si_fb_batch();
+    b->jobs.assign(jobs, jobs + n_jobs);
+    b->tables = si::FbTables{jobs[0].eps, jobs[0].emax, jobs[0].palette, jobs[0].n_palette};
+    b->ws_off.resize(n_jobs);
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        b->ws_off[i] = b->ws_bytes;
+        const size_t w = si::fb_workspace_bytes_one(jobs[i].n_samples, jobs[i].n_markers,
+                                                    si_padded_states(jobs[i].n_states));
+        b->ws_bytes += (w + 255) & ~size_t(255);
+    }
+    // classes by (padded states, numeric space); jobs longest chain first
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        if (jobs[i].n_samples == 0 || jobs[i].n_markers == 0) continue;
+        const uint32_t Lp = si_padded_states(jobs[i].n_states);
+        const uint32_t fl = jobs[i].flags & SI_FB_LOGSPACE;
+        auto it = std::find_if(b->classes.begin(), b->classes.end(),
+                               [&](const si_fb_batch::Class& c) { return c.Lp == Lp && c.flags == fl; });
+        if (it == b->classes.end()) {
+            b->classes.emplace_back();
+            it = b->classes.end() - 1;
+            it->Lp = Lp;
+            it->flags = fl;
+        }
+        it->order.push_back(i);
+    }
+    for (auto& c : b->classes) {
+        std::stable_sort(c.order.begin(), c.order.end(), [&](uint32_t x, uint32_t y) {
+            return jobs[x].n_markers > jobs[y].n_markers;
+        });
+        const size_t n = c.order.size();
+        std::vector<uint64_t> off(2 * (n + 1), 0);
+        for (size_t k = 0; k < n; ++k) {
+            const si_fb_job& j = jobs[c.order[k]];
+            const uint64_t nblk = (j.n_markers + si::kBlock - 1) / si::kBlock;
+            off[k + 1] = off[k] + 2ull * j.n_samples;
+            off[n + 1 + k + 1] = off[n + 1 + k] + uint64_t(j.n_samples) * nblk;
+        }
+        c.chain_items = off[n];
+        c.block_items = off[2 * n + 1];
+        si::BatchClass bc{};
+        bc.Lp = c.Lp;
+        bc.flags = c.flags;
+        bc.chain_items = c.chain_items;
+        bc.block_items = c.block_items;
+        si::fb_batch_grid(bc);
+        c.grid_chain = bc.grid_chain;
+        c.grid_block = bc.grid_block;
+        if (hipMalloc(&c.d_jobs, n * sizeof(si::JobView)) != hipSuccess ||
+            hipMalloc(&c.d_off, off.size() * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc(&c.d_counters, 2 * sizeof(uint32_t)) != hipSuccess ||
+            hipMemcpy(c.d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            batch_free(b);
+            return device_status(SI_ERR_DEVICE, "si_fb_batch_create");
+        }
+    }
+    *out = b;
+    return SI_OK;
+}
+
+size_t si_fb_batch_workspace_bytes(const si_fb_batch* b) { return b ? b->ws_bytes : 0; }
+
+// Writes the device job tables with checkpoint pointers into `ws`.
+static int batch_bind(si_fb_batch* b, void* ws, hipStream_t stream) {
+    for (auto& c : b->classes) {
+        std::vector<si::JobView> v(c.order.size());
+        for (size_t k = 0; k < c.order.size(); ++k) {
+            const uint32_t i = c.order[k];
+            v[k] = si::fb_job_view(b->jobs[i]);
+            si::fb_bind_workspace(v[k], c.Lp, static_cast<char*>(ws) + b->ws_off[i]);
+        }
+        if (hipMemcpyAsync(c.d_jobs, v.data(), v.size() * sizeof(si::JobView),
+                           hipMemcpyHostToDevice, stream) != hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess)
+            return device_status(SI_ERR_DEVICE, "si_fb_batch_run (bind)");
+    }
+    b->bound_ws = ws;
+    return SI_OK;
+}
+
+static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
+    for (auto& c : b->classes) {
+        si::BatchClass bc{};
+        bc.Lp = c.Lp;
+        bc.n_jobs = uint32_t(c.order.size());
+        bc.flags = c.flags;
+        bc.jobs = c.d_jobs;
+        bc.chain_off = c.d_off;
+        bc.block_off = c.d_off + c.order.size() + 1;
+        bc.chain_items = c.chain_items;
+        bc.block_items = c.block_items;
+        bc.counters = c.d_counters;
+        bc.grid_chain = c.grid_chain;
+        bc.grid_block = c.grid_block;
+        const int rc = si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, stream);
+        if (rc) return device_status(rc, "si_fb_batch_run");
+    }
+    return SI_OK;
+}
+
+int si_fb_batch_run(si_fb_batch* b, void* workspace, size_t workspace_bytes, int use_graph,
+                    void* stream_v) {
+    if (!b) return fail(SI_ERR_ARG, "si_fb_batch_run: null batch");
+    if (b->ws_bytes && (!workspace || workspace_bytes < b->ws_bytes))
+        return fail(SI_ERR_WORKSPACE, "si_fb_batch_run: workspace of %zu bytes needed, %zu given",
+                    b->ws_bytes, workspace_bytes);
+    hipStream_t stream = static_cast<hipStream_t>(stream_v);
+    if (workspace != b->bound_ws) {
+        const int rc = batch_bind(b, workspace, stream);
+        if (rc) return rc;
+    }
+    if (!use_graph) return batch_enqueue(b, stream);
+    if (!stream) return fail(SI_ERR_ARG, "si_fb_batch_run: graph capture needs a non-null stream");
+    if (!b->exec || b->graph_ws != workspace) {
+        if (b->exec) {
+            (void)hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return device_status(SI_ERR_DEVICE, "si_fb_batch_run (capture)");
+        const int rc = batch_enqueue(b, stream);
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess || hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return device_status(SI_ERR_DEVICE, "si_fb_batch_run (instantiate)");
+        }
+        (void)hipGraphDestroy(g);
+        b->graph_ws = workspace;
+    }
+    if (hipGraphLaunch(b->exec, stream) != hipSuccess)
+        return device_status(SI_ERR_DEVICE, "si_fb_batch_run (graph launch)");
+    return SI_OK;
+}
+
+int si_fb_batch_destroy(si_fb_batch* b) {
+    batch_free(b);
     return SI_OK;
 }
 

@@ -30,6 +30,8 @@
 //     through the block into registers, then walks the backward vector down
 //     the block, forming posteriors, dosages and the sampled haplotypes.
 // A lane group is LPS = Lp/4 lanes of one wavefront; each lane holds 4 states.
+#include <algorithm>
+
 #include "si_device.hpp"
 #include "si_internal.hpp"
 
@@ -48,16 +50,16 @@ struct EmisLds {
 };
 
 template <bool LOG>
-__device__ __forceinline__ void load_tables(const FbParams& p, EmisLds& t) {
-    const uint32_t n = p.P * 8;
+__device__ __forceinline__ void load_tables(const FbTables& tb, EmisLds& t) {
+    const uint32_t n = tb.P * 8;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         if constexpr (LOG) {
-            const float* g = p.palette + 3 * (i / 8);
+            const float* g = tb.palette + 3 * (i / 8);
             t.row[i] = make_float4(g[0], g[1], g[2], -INFINITY);
             t.off[i] = 0.0f;
         } else {
-            t.row[i] = reinterpret_cast<const float4*>(p.eps)[i];
-            t.off[i] = p.emax[i];
+            t.row[i] = reinterpret_cast<const float4*>(tb.eps)[i];
+            t.off[i] = tb.emax[i];
         }
     }
 }
@@ -76,14 +78,22 @@ __device__ __forceinline__ void lane_emission(float4 e3, uint32_t sa4, float e[4
 // Emission-table row for (palette code, class mask of marker m).  Codes at or
 // beyond the palette size are read as the last palette entry, so no input can
 // index past the table.
-__device__ __forceinline__ uint32_t emis_row(const FbParams& p, uint32_t code, uint32_t m) {
-    return min(code, p.P - 1u) * 8u + p.mmask[m];
+__device__ __forceinline__ uint32_t emis_row(const JobView& J, uint32_t P, uint32_t code,
+                                             uint32_t m) {
+    return min(code, P - 1u) * 8u + J.mmask[m];
 }
 
 // (t_o, u, lto, lu) of the step into marker m.
-__device__ __forceinline__ float4 step_coef(const FbParams& p, uint32_t m) {
-    if (p.flags & SI_FB_PER_MARKER) return reinterpret_cast<const float4*>(p.steps)[m];
-    return make_float4(p.step.t_o, p.step.u, p.step.lto, p.step.lu);
+__device__ __forceinline__ float4 step_coef(const JobView& J, uint32_t m) {
+    return (J.flags & SI_FB_PER_MARKER) ? J.steps[m] : J.step;
+}
+
+// Orders this wavefront's LDS accesses: lane groups exchange data through LDS
+// only within their own wavefront.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int LPS>
@@ -202,68 +212,58 @@ __device__ __forceinline__ float4 posterior(float4 a, float4 v) {
 }
 
 // ---------------------------------------------------------------------------
-// Chain pass.
+// One chain (forward or backward) of sample s of job J, by one lane group.
 // ---------------------------------------------------------------------------
 template <int LPS, bool LOG>
-__global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
-    __shared__ EmisLds tab;
-    load_tables<LOG>(p, tab);
-    __syncthreads();
-
-    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
-    const uint32_t grp = gtid / LPS, r = gtid % LPS;
-    const bool fwd = grp < p.Bpad;
-    const uint32_t s = fwd ? grp : grp - p.Bpad;
-    if (s >= p.B) return;  // whole lane groups leave together
-
-    const uint8_t* codes = p.codes + size_t(s) * p.pitch;
-    const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(p.sa);
-    const uint32_t lp4 = p.Lp / 4;
-
+__device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, uint32_t P,
+                                          uint32_t Lp, uint32_t s, bool fwd, uint32_t r) {
+    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
+    const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(J.sa);
+    const uint32_t lp4 = Lp / 4;
     if (fwd) {
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
         double ll = 0.0;
-        for (uint32_t c = 0; c < p.nblk; ++c) {
+        for (uint32_t c = 0; c < J.nblk; ++c) {
             const uint32_t m0 = c * kBlock;
             const uint4 cw = *reinterpret_cast<const uint4*>(codes + m0);
 #pragma unroll
             for (uint32_t t = 0; t < kBlock; ++t) {
                 const uint32_t m = m0 + t;
-                if (m < p.M) {
-                    const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), m);
+                if (m < J.M) {
+                    const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), m);
                     float e[4];
                     lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
                     if (m == 0)
                         ll += fwd_init<LPS, LOG>(a, e, tab.off[row]);
                     else
-                        ll += fwd_step<LPS, LOG>(a, e, tab.off[row], step_coef(p, m));
+                        ll += fwd_step<LPS, LOG>(a, e, tab.off[row], step_coef(J, m));
                     if (t == 0)
-                        reinterpret_cast<float4*>(p.ck_alpha)[(size_t(s) * p.nblk + c) * lp4 + r] = a;
+                        reinterpret_cast<float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r] = a;
                 }
             }
         }
-        if (r == 0 && p.loglik) p.loglik[s] = ll;
+        if (r == 0 && J.loglik) J.loglik[s] = ll;
     } else {
         float4 v = bwd_init<LOG>();
         double Bl = 0.0;
-        const size_t last = size_t(s) * p.nblk + (p.nblk - 1);
-        reinterpret_cast<float4*>(p.ck_beta)[last * lp4 + r] = v;
-        if (r == 0) p.ck_bl[last] = Bl;
-        for (int c = int(p.nblk) - 1; c >= 0; --c) {
+        const size_t last = size_t(s) * J.nblk + (J.nblk - 1);
+        reinterpret_cast<float4*>(J.ck_beta)[last * lp4 + r] = v;
+        if (r == 0) J.ck_bl[last] = Bl;
+        for (int c = int(J.nblk) - 1; c >= 0; --c) {
             const uint32_t m0 = uint32_t(c) * kBlock;
             const uint4 cw = *reinterpret_cast<const uint4*>(codes + m0);
 #pragma unroll
             for (int t = int(kBlock) - 1; t >= 0; --t) {
                 const uint32_t n = m0 + uint32_t(t);
-                if (n < p.M && n > 0) {
-                    const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), n);
+                if (n < J.M && n > 0) {
+                    const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), n);
                     float e[4];
                     lane_emission(tab.row[row], sa32[size_t(n) * lp4 + r], e);
-                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(p, n));
+                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(J, n));
                     if (t == 0) {  // v is now the vector at marker m0 - 1, last of block c - 1
-                        const size_t ck = size_t(s) * p.nblk + uint32_t(c - 1);
-                        reinterpret_cast<float4*>(p.ck_beta)[ck * lp4 + r] = v;
-                        if (r == 0) p.ck_bl[ck] = Bl;
+                        const size_t ck = size_t(s) * J.nblk + uint32_t(c - 1);
+                        reinterpret_cast<float4*>(J.ck_beta)[ck * lp4 + r] = v;
+                        if (r == 0) J.ck_bl[ck] = Bl;
                     }
                 }
             }
@@ -272,80 +272,71 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(FbParams p) {
 }
 
 // ---------------------------------------------------------------------------
-// Block pass.
+// Block c of sample s of job J, by one lane group: replay the forward vector
+// through the block into registers, walk the backward vector down the block
+// writing posteriors into the group's LDS slice, then sample, form dosages
+// and allele probabilities (lane r takes markers r, r + LPS, ... of the block,
+// walking the states in order with a sequential FP32 cumulative sum, which the
+// host replays exactly).  `active` groups do the work; every lane of the
+// wavefront passes the wave_sync.
 // ---------------------------------------------------------------------------
 template <int LPS, bool LOG>
-__global__ __launch_bounds__(kFbThreads) void fb_block_kernel(FbParams p) {
-    constexpr uint32_t G = kFbThreads / LPS;  // lane groups per workgroup
-    constexpr uint32_t LP = 4 * LPS;
-    __shared__ EmisLds tab;
-    __shared__ float pbuf[G][kBlock][LP];
-    __shared__ uint4 rng[G][kBlock / 4];
-    load_tables<LOG>(p, tab);
-    __syncthreads();
-
-    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
-    const uint32_t grp = gtid / LPS, r = gtid % LPS, gl = threadIdx.x / LPS;
-    const bool active = grp < p.B * p.nblk;
-    const uint32_t s = active ? grp / p.nblk : 0u;
-    const uint32_t c = active ? grp % p.nblk : 0u;
+__device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, uint32_t P,
+                                          uint32_t Lp, bool active, uint32_t s, uint32_t c,
+                                          uint32_t r, float (*pbuf)[4 * LPS], uint4* rng) {
     const uint32_t m0 = c * kBlock;
-    const uint32_t nm = active ? min(kBlock, p.M - m0) : 0u;
-    const uint32_t lp4 = p.Lp / 4;
-    const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(p.sa);
-
+    const uint32_t nm = active ? min(kBlock, J.M - m0) : 0u;
+    const uint32_t lp4 = Lp / 4;
+    const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(J.sa);
     if (active) {
-        const uint4 cw = *reinterpret_cast<const uint4*>(p.codes + size_t(s) * p.pitch + m0);
+        const uint4 cw = *reinterpret_cast<const uint4*>(J.codes + size_t(s) * J.pitch + m0);
         float4 A[kBlock];
-        A[0] = reinterpret_cast<const float4*>(p.ck_alpha)[(size_t(s) * p.nblk + c) * lp4 + r];
+        A[0] = reinterpret_cast<const float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r];
 #pragma unroll
         for (uint32_t t = 1; t < kBlock; ++t) {
             if (t < nm) {
                 const uint32_t m = m0 + t;
-                const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), m);
+                const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), m);
                 float e[4];
                 lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
                 A[t] = A[t - 1];
-                fwd_step<LPS, LOG>(A[t], e, tab.off[row], step_coef(p, m));
+                fwd_step<LPS, LOG>(A[t], e, tab.off[row], step_coef(J, m));
             }
         }
-        const size_t ck = size_t(s) * p.nblk + c;
-        float4 v = reinterpret_cast<const float4*>(p.ck_beta)[ck * lp4 + r];
-        double Bl = p.ck_bl[ck];
+        const size_t ck = size_t(s) * J.nblk + c;
+        float4 v = reinterpret_cast<const float4*>(J.ck_beta)[ck * lp4 + r];
+        double Bl = J.ck_bl[ck];
 #pragma unroll
         for (int t = int(kBlock) - 1; t >= 0; --t) {
             if (uint32_t(t) < nm) {
-                *reinterpret_cast<float4*>(&pbuf[gl][t][4 * r]) = posterior<LPS, LOG>(A[t], v);
+                *reinterpret_cast<float4*>(&pbuf[t][4 * r]) = posterior<LPS, LOG>(A[t], v);
                 if (t > 0) {
                     const uint32_t n = m0 + uint32_t(t);
-                    const uint32_t row = emis_row(p, byte_of((&cw.x)[t >> 2], t & 3), n);
+                    const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), n);
                     float e[4];
                     lane_emission(tab.row[row], sa32[size_t(n) * lp4 + r], e);
-                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(p, n));
+                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(J, n));
                 }
             }
         }
-        const uint2 key = make_uint2(uint32_t(p.seed), uint32_t(p.seed >> 32));
+        const uint2 key = make_uint2(uint32_t(J.seed), uint32_t(J.seed >> 32));
         for (uint32_t j = r; j < kBlock / 4; j += LPS)
-            rng[gl][j] =
-                philox4x32_10(make_uint4((m0 >> 2) + j, p.sample_base + s, p.rng_stream, 0u), key);
+            rng[j] = philox4x32_10(make_uint4((m0 >> 2) + j, J.sample_base + s, J.rng_stream, 0u),
+                                   key);
     }
-    __syncthreads();
+    wave_sync();
     if (!active) return;
 
-    // Sampling, dosage and allele probabilities: lane r takes markers r,
-    // r + LPS, ... of the block and walks the states in order with a
-    // sequential FP32 cumulative sum (the host replays exactly this).
-    const bool argmax = (p.flags & SI_FB_ARGMAX) != 0;
-    const uint32_t L = p.L;
+    const bool argmax = (J.flags & SI_FB_ARGMAX) != 0;
+    const uint32_t L = J.L;
     for (uint32_t t = r; t < nm; t += LPS) {
         const uint32_t m = m0 + t;
-        const float u = u24((&rng[gl][t >> 2].x)[t & 3]);
-        const uint8_t* sam = p.sa + size_t(m) * p.Lp;
+        const float u = u24((&rng[t >> 2].x)[t & 3]);
+        const uint8_t* sam = J.sa + size_t(m) * Lp;
         float cs = 0.0f, best = -1.0f, ap0 = 0.0f, ap1 = 0.0f;
         uint32_t kdraw = 0xffffffffu, kbest = 0;
         for (uint32_t k = 0; k < L; ++k) {
-            const float pk = pbuf[gl][t][k];
+            const float pk = pbuf[t][k];
             cs += pk;
             if (kdraw == 0xffffffffu && u < cs) kdraw = k;
             if (pk > best) {
@@ -358,19 +349,132 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(FbParams p) {
         }
         if (kdraw == 0xffffffffu) kdraw = 0;
         const uint32_t ks = argmax ? kbest : kdraw;
-        const size_t o = size_t(s) * p.M + m;
-        if (p.ds) p.ds[o] = ap0 + ap1;
-        if (p.ap) reinterpret_cast<float2*>(p.ap)[o] = make_float2(ap0, ap1);
-        if (p.hap) {
+        const size_t o = size_t(s) * J.M + m;
+        if (J.ds) J.ds[o] = ap0 + ap1;
+        if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0, ap1);
+        if (J.hap) {
             const uint32_t b = sam[ks];
-            p.hap[size_t(s) * 2 * p.M + m] = uint8_t((b & 4u) ? 255u : (b & 1u));
-            p.hap[size_t(s) * 2 * p.M + p.M + m] = uint8_t((b & 8u) ? 255u : ((b >> 1) & 1u));
+            J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 4u) ? 255u : (b & 1u));
+            J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 8u) ? 255u : ((b >> 1) & 1u));
         }
-        if (p.prob)
-            for (uint32_t k = 0; k < L; ++k) p.prob[o * L + k] = pbuf[gl][t][k];
+        if (J.prob)
+            for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pbuf[t][k];
+    }
+    wave_sync();  // the slice is reused by the group's next item
+}
+
+// ---------------------------------------------------------------------------
+// Plain form: one job; grid = groups.
+// ---------------------------------------------------------------------------
+template <int LPS, bool LOG>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(JobView J, FbTables tb,
+                                                              uint32_t Bpad) {
+    __shared__ EmisLds tab;
+    load_tables<LOG>(tb, tab);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / LPS, r = gtid % LPS;
+    const bool fwd = grp < Bpad;
+    const uint32_t s = fwd ? grp : grp - Bpad;
+    if (s >= J.B) return;  // whole lane groups leave together
+    run_chain<LPS, LOG>(J, tab, tb.P, 4 * LPS, s, fwd, r);
+}
+
+template <int LPS, bool LOG>
+__global__ __launch_bounds__(kFbThreads) void fb_block_kernel(JobView J, FbTables tb) {
+    constexpr uint32_t G = kFbThreads / LPS;  // lane groups per workgroup
+    __shared__ EmisLds tab;
+    __shared__ float pbuf[G][kBlock][4 * LPS];
+    __shared__ uint4 rng[G][kBlock / 4];
+    load_tables<LOG>(tb, tab);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / LPS, r = gtid % LPS, gl = threadIdx.x / LPS;
+    const bool active = grp < J.B * J.nblk;
+    const uint32_t s = active ? grp / J.nblk : 0u;
+    const uint32_t c = active ? grp % J.nblk : 0u;
+    run_block<LPS, LOG>(J, tab, tb.P, 4 * LPS, active, s, c, r, pbuf[gl], rng[gl]);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: a persistent grid; each wavefront repeatedly takes the next
+// 64/LPS items of the class's queue (one atomic per wavefront) until the
+// queue is exhausted.  Items are ordered job by job (jobs longest first for
+// the chain pass), so a wavefront's groups usually share a job.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t find_job(const uint64_t* off, uint32_t n, uint64_t item) {
+    uint32_t lo = 0, hi = n;  // off[lo] <= item < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t wave_take(uint32_t* counter, uint32_t n) {
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(counter, n);
+    return uint64_t(__builtin_amdgcn_readfirstlane(base));
+}
+
+template <int LPS, bool LOG>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass bc, FbTables tb) {
+    __shared__ EmisLds tab;
+    load_tables<LOG>(tb, tab);
+    __syncthreads();
+    constexpr uint32_t GPW = kWave / LPS;
+    const uint32_t lane = __lane_id(), gw = lane / LPS, r = lane % LPS;
+    for (;;) {
+        const uint64_t base = wave_take(&bc.counters[0], GPW);
+        if (base >= bc.chain_items) break;
+        const uint64_t item = base + gw;
+        if (item < bc.chain_items) {
+            const uint32_t j = find_job(bc.chain_off, bc.n_jobs, item);
+            const JobView J = bc.jobs[j];
+            const uint64_t k = item - bc.chain_off[j];
+            run_chain<LPS, LOG>(J, tab, tb.P, bc.Lp, uint32_t(k >> 1), (k & 1) == 0, r);
+        }
     }
 }
 
+template <int LPS, bool LOG>
+__global__ __launch_bounds__(kFbThreads) void fb_block_batch_kernel(BatchClass bc, FbTables tb) {
+    constexpr uint32_t G = kFbThreads / LPS;
+    constexpr uint32_t GPW = kWave / LPS;
+    __shared__ EmisLds tab;
+    __shared__ float pbuf[G][kBlock][4 * LPS];
+    __shared__ uint4 rng[G][kBlock / 4];
+    load_tables<LOG>(tb, tab);
+    __syncthreads();
+    const uint32_t lane = __lane_id(), gw = lane / LPS, r = lane % LPS;
+    const uint32_t gl = threadIdx.x / LPS;
+    for (;;) {
+        const uint64_t base = wave_take(&bc.counters[1], GPW);
+        if (base >= bc.block_items) break;
+        const uint64_t item = base + gw;
+        const bool active = item < bc.block_items;
+        uint32_t j = 0, s = 0, c = 0;
+        if (active) {
+            j = find_job(bc.block_off, bc.n_jobs, item);
+            const uint64_t k = item - bc.block_off[j];
+            const uint32_t nblk = bc.jobs[j].nblk;
+            s = uint32_t(k / nblk);
+            c = uint32_t(k % nblk);
+        }
+        const JobView J = bc.jobs[j];
+        run_block<LPS, LOG>(J, tab, tb.P, bc.Lp, active, s, c, r, pbuf[gl], rng[gl]);
+    }
+}
+
+__global__ void zero_words_kernel(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
 size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp) {
     const size_t nblk = (size_t(M) + kBlock - 1) / kBlock;
     const size_t vec = size_t(B) * nblk * Lp * sizeof(float);
@@ -379,45 +483,151 @@ size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp) {
     return 2 * vec_al + ((bl + 255) & ~size_t(255));
 }
 
+void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws) {
+    const size_t vec = size_t(v.B) * v.nblk * Lp * sizeof(float);
+    const size_t vec_al = (vec + 255) & ~size_t(255);
+    v.ck_alpha = reinterpret_cast<float*>(ws);
+    v.ck_beta = reinterpret_cast<float*>(ws + vec_al);
+    v.ck_bl = reinterpret_cast<double*>(ws + 2 * vec_al);
+}
+
+JobView fb_job_view(const si_fb_job& j) {
+    JobView v{};
+    v.B = j.n_samples;
+    v.M = j.n_markers;
+    v.L = j.n_states;
+    v.flags = j.flags;
+    v.pitch = j.code_pitch;
+    v.rng_stream = j.rng_stream;
+    v.sample_base = j.sample_base;
+    v.nblk = (j.n_markers + kBlock - 1) / kBlock;
+    v.seed = j.seed;
+    v.step = make_float4(j.step.t_o, j.step.u, j.step.lto, j.step.lu);
+    v.steps = reinterpret_cast<const float4*>(j.steps);
+    v.sa = j.sa;
+    v.mmask = j.mmask;
+    v.codes = j.codes;
+    v.ds = j.ds;
+    v.ap = j.ap;
+    v.hap = j.hap;
+    v.prob = j.prob;
+    v.loglik = j.loglik;
+    return v;
+}
+
 template <int LPS, bool LOG>
-static int launch_fb_lps(const FbParams& p, uint32_t passes, hipStream_t stream) {
+static int launch_plain(const JobView& v, const FbTables& t, uint32_t passes, hipStream_t stream) {
     const uint32_t gpw = kWave / LPS;  // groups per wavefront
-    FbParams q = p;
-    q.Bpad = (p.B + gpw - 1) / gpw * gpw;
-    const size_t chain_threads = size_t(q.Bpad + p.B) * LPS;
-    const size_t block_threads = size_t(p.B) * p.nblk * LPS;
+    const uint32_t Bpad = (v.B + gpw - 1) / gpw * gpw;
+    const size_t chain_threads = size_t(Bpad + v.B) * LPS;
+    const size_t block_threads = size_t(v.B) * v.nblk * LPS;
     const size_t g1 = (chain_threads + kFbThreads - 1) / kFbThreads;
     const size_t g2 = (block_threads + kFbThreads - 1) / kFbThreads;
     if (g1 > 0x7fffffffu || g2 > 0x7fffffffu) return SI_ERR_UNSUPPORTED;
     if (passes & SI_FB_PASS_CHAIN) {
         hipLaunchKernelGGL((fb_chain_kernel<LPS, LOG>), dim3(uint32_t(g1)), dim3(kFbThreads), 0,
-                           stream, q);
+                           stream, v, t, Bpad);
         if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     }
     if (passes & SI_FB_PASS_BLOCK) {
         hipLaunchKernelGGL((fb_block_kernel<LPS, LOG>), dim3(uint32_t(g2)), dim3(kFbThreads), 0,
-                           stream, q);
+                           stream, v, t);
+        if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
+    }
+    return 0;
+}
+
+template <int LPS, bool LOG>
+static int launch_batch(const BatchClass& c, const FbTables& t, uint32_t passes,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream, c.counters, 2u);
+    if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
+    if ((passes & SI_FB_PASS_CHAIN) && c.chain_items) {
+        hipLaunchKernelGGL((fb_chain_batch_kernel<LPS, LOG>), dim3(c.grid_chain), dim3(kFbThreads),
+                           0, stream, c, t);
+        if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
+    }
+    if ((passes & SI_FB_PASS_BLOCK) && c.block_items) {
+        hipLaunchKernelGGL((fb_block_batch_kernel<LPS, LOG>), dim3(c.grid_block), dim3(kFbThreads),
+                           0, stream, c, t);
         if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     }
     return 0;
 }
 
 template <bool LOG>
-static int launch_fb_space(const FbParams& p, uint32_t passes, hipStream_t stream) {
-    switch (p.Lp) {
-        case 4: return launch_fb_lps<1, LOG>(p, passes, stream);
-        case 8: return launch_fb_lps<2, LOG>(p, passes, stream);
-        case 16: return launch_fb_lps<4, LOG>(p, passes, stream);
-        case 32: return launch_fb_lps<8, LOG>(p, passes, stream);
-        case 64: return launch_fb_lps<16, LOG>(p, passes, stream);
+static int dispatch_plain(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
+                          hipStream_t stream) {
+    switch (Lp) {
+        case 4: return launch_plain<1, LOG>(v, t, passes, stream);
+        case 8: return launch_plain<2, LOG>(v, t, passes, stream);
+        case 16: return launch_plain<4, LOG>(v, t, passes, stream);
+        case 32: return launch_plain<8, LOG>(v, t, passes, stream);
+        case 64: return launch_plain<16, LOG>(v, t, passes, stream);
         default: return SI_ERR_UNSUPPORTED;
     }
 }
 
-int launch_fb(const FbParams& p, uint32_t passes, hipStream_t stream) {
-    if (p.B == 0 || p.M == 0) return 0;
-    return (p.flags & SI_FB_LOGSPACE) ? launch_fb_space<true>(p, passes, stream)
-                                      : launch_fb_space<false>(p, passes, stream);
+template <bool LOG>
+static int dispatch_batch(const BatchClass& c, const FbTables& t, uint32_t passes,
+                          hipStream_t stream) {
+    switch (c.Lp) {
+        case 4: return launch_batch<1, LOG>(c, t, passes, stream);
+        case 8: return launch_batch<2, LOG>(c, t, passes, stream);
+        case 16: return launch_batch<4, LOG>(c, t, passes, stream);
+        case 32: return launch_batch<8, LOG>(c, t, passes, stream);
+        case 64: return launch_batch<16, LOG>(c, t, passes, stream);
+        default: return SI_ERR_UNSUPPORTED;
+    }
+}
+
+int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
+              hipStream_t stream) {
+    if (v.B == 0 || v.M == 0) return 0;
+    return (v.flags & SI_FB_LOGSPACE) ? dispatch_plain<true>(v, t, Lp, passes, stream)
+                                      : dispatch_plain<false>(v, t, Lp, passes, stream);
+}
+
+// Persistent grid: as many workgroups as can be resident at once, capped by
+// the work available.
+template <int LPS, bool LOG>
+static void batch_grid_lps(BatchClass& c) {
+    int dev = 0, cus = 0, occ_c = 0, occ_b = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_c, fb_chain_batch_kernel<LPS, LOG>,
+                                                       kFbThreads, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_b, fb_block_batch_kernel<LPS, LOG>,
+                                                       kFbThreads, 0);
+    const uint64_t gpb = kFbThreads / LPS;  // lane groups per workgroup
+    const uint64_t need_c = (c.chain_items + gpb - 1) / gpb, need_b = (c.block_items + gpb - 1) / gpb;
+    const uint64_t ncu = uint64_t(cus > 0 ? cus : 1);
+    const uint64_t cap_c = ncu * uint64_t(occ_c > 0 ? occ_c : 1);
+    const uint64_t cap_b = ncu * uint64_t(occ_b > 0 ? occ_b : 1);
+    c.grid_chain = uint32_t(std::max<uint64_t>(1, std::min(need_c, cap_c)));
+    c.grid_block = uint32_t(std::max<uint64_t>(1, std::min(need_b, cap_b)));
+}
+
+template <bool LOG>
+static void batch_grid_space(BatchClass& c) {
+    switch (c.Lp) {
+        case 4: batch_grid_lps<1, LOG>(c); break;
+        case 8: batch_grid_lps<2, LOG>(c); break;
+        case 16: batch_grid_lps<4, LOG>(c); break;
+        case 32: batch_grid_lps<8, LOG>(c); break;
+        case 64: batch_grid_lps<16, LOG>(c); break;
+        default: c.grid_chain = c.grid_block = 1;
+    }
+}
+
+void fb_batch_grid(BatchClass& c) {
+    if (c.flags & SI_FB_LOGSPACE) batch_grid_space<true>(c);
+    else batch_grid_space<false>(c);
+}
+
+int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, hipStream_t stream) {
+    return (c.flags & SI_FB_LOGSPACE) ? dispatch_batch<true>(c, t, passes, stream)
+                                      : dispatch_batch<false>(c, t, passes, stream);
 }
 
 }  // namespace si

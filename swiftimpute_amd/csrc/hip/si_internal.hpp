@@ -15,20 +15,15 @@ constexpr uint32_t kBlock = 16;
 // Largest genotype-likelihood palette the kernels stage in LDS.
 constexpr uint32_t kMaxPalette = 64;
 
-// Kernel-side view of one job (passed by value).
-struct FbParams {
-    uint32_t B, M, L, Lp, flags, pitch, P, rng_stream, sample_base;
-    uint32_t nblk;  // ceil(M / kBlock)
-    uint32_t Bpad;  // forward-chain groups, rounded so no wavefront mixes directions
+// One job as a lane group sees it (plain and batched forms alike).
+struct JobView {
+    uint32_t B, M, L, flags, pitch, rng_stream, sample_base, nblk;
     uint64_t seed;
-    si_step_coef step;
-    const si_step_coef* steps;
-    const uint8_t* sa;
-    const uint8_t* mmask;
-    const uint8_t* codes;
-    const float* eps;
-    const float* emax;
-    const float* palette;
+    float4 step;             // (t_o, u, lto, lu) when not per-marker
+    const float4* steps;     // [M] per-marker coefficients
+    const uint8_t* sa;       // [M][Lp]
+    const uint8_t* mmask;    // [M]
+    const uint8_t* codes;    // [B][pitch]
     float* ds;
     float* ap;
     uint8_t* hap;
@@ -39,8 +34,40 @@ struct FbParams {
     double* ck_bl;    // [B][nblk]      its log scale
 };
 
+// Emission tables shared by every job of a launch.
+struct FbTables {
+    const float* eps;      // [P][8][4] linear kernel
+    const float* emax;     // [P][8]
+    const float* palette;  // [P][3]   log-space kernel
+    uint32_t P;
+};
+
 size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp);
-int launch_fb(const FbParams& p, uint32_t passes, hipStream_t stream);
+void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws);
+JobView fb_job_view(const si_fb_job& j);
+
+// Plain form: one job, two launches.
+int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
+              hipStream_t stream);
+
+// Batched form: every job of one padded-state class in one persistent launch
+// per pass.  jobs: device array in launch order; item_off: device prefix of
+// per-job item counts (n_jobs + 1) for the chain pass (2 chains per sample)
+// and for the block pass (one item per (sample, block)); counters: 2 device
+// words, zeroed by the launcher.
+struct BatchClass {
+    uint32_t Lp;
+    uint32_t n_jobs;
+    uint32_t flags;  // SI_FB_LOGSPACE
+    const JobView* jobs;
+    const uint64_t* chain_off;
+    const uint64_t* block_off;
+    uint64_t chain_items, block_items;
+    uint32_t* counters;
+    uint32_t grid_chain, grid_block;  // persistent grid sizes (workgroups)
+};
+void fb_batch_grid(BatchClass& c);
+int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, hipStream_t stream);
 
 int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
                       uint32_t* div, hipStream_t stream);

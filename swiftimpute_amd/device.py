@@ -246,3 +246,53 @@ def run_jobs(cjobs, workspace: torch.Tensor | None = None, stream=None):
     check(hip().si_fb_run(arr, len(cjobs), _ptr(workspace), workspace.numel(),
                           _stream_handle(stream)), "si_fb_run")
     return workspace
+
+
+class Batch:
+    """Batched form: many jobs in persistent work-queue launches (one per pass
+    and padded-state class), optionally replayed from a captured hipGraph.
+    All jobs must share one emission table (build them with the same
+    ``tables``)."""
+
+    def __init__(self, cjobs):
+        self._arr = (FbJob * len(cjobs))(*cjobs)
+        h = ctypes.c_void_p()
+        check(hip().si_fb_batch_create(self._arr, len(cjobs), ctypes.byref(h)), "si_fb_batch_create")
+        self.handle = h
+        self.ws_bytes = int(hip().si_fb_batch_workspace_bytes(h))
+        self.workspace = None
+
+    def run(self, use_graph=False, stream=None, workspace=None):
+        if workspace is not None:
+            self.workspace = workspace
+        if self.workspace is None or self.workspace.numel() < self.ws_bytes:
+            self.workspace = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=DEV)
+        check(hip().si_fb_batch_run(self.handle, _ptr(self.workspace), self.workspace.numel(),
+                                    int(bool(use_graph)), _stream_handle(stream)), "si_fb_batch_run")
+
+    def close(self):
+        if self.handle:
+            hip().si_fb_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SharedTables:
+    """One emission table on the device, shared by the jobs of a batch."""
+
+    def __init__(self, palette, device=DEV):
+        self.palette = np.ascontiguousarray(palette, np.float32)
+        eps, emax = emission_table(self.palette)
+        self.eps = torch.from_numpy(eps).to(device)
+        self.emax = torch.from_numpy(emax).to(device)
+        self.pal = torch.from_numpy(self.palette).to(device)
+
+    def apply(self, j: FbJob) -> FbJob:
+        j.eps, j.emax, j.palette = _ptr(self.eps), _ptr(self.emax), _ptr(self.pal)
+        j.n_palette = self.palette.shape[0]
+        return j

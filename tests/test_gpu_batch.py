@@ -1,0 +1,140 @@
+"""Batched form: uneven jobs (regions of one shared panel, cohorts of 1 to a
+few hundred samples, L in {4, 5, 8, 16}, linear and log-space) in persistent
+work-queue launches, eager and replayed from a hipGraph.  Every output must
+equal the plain one-job-per-launch path bit for bit (same arithmetic), and the
+plain path is itself checked against the oracle in test_gpu_parity.py."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from swiftimpute_amd import _native, device, synth
+
+pytestmark = pytest.mark.gpu
+
+
+class RegionJob:
+    """One job = (marker region of the shared panel, cohort, L)."""
+
+    def __init__(self, idx, pan, m0, m1, codes_full, ne, tables, seed, logspace=False,
+                 per_marker=True):
+        self.M = m1 - m0
+        B = codes_full.shape[0]
+        pitch = (self.M + 15) // 16 * 16
+        c = np.full((B, pitch), 3, np.uint8)
+        c[:, : self.M] = codes_full[:, m0:m1]
+        self.codes_np = c
+        self.codes = torch.from_numpy(c).cuda()
+        cm = pan.genetic_cM()
+        self.trans = device.Transitions.build(pan.pos[m0:m1], None if cm is None else cm[m0:m1], ne,
+                                              per_marker=per_marker)
+        self.idx, self.m0, self.B, self.seed = idx, m0, B, seed
+        self.tables, self.logspace = tables, logspace
+
+    def cjob(self):
+        L, Lp, M, B = self.idx.L, self.idx.Lp, self.M, self.B
+        self.out = {
+            "ds": torch.full((B, M), -1.0, device="cuda"),
+            "ap": torch.full((B, M, 2), -1.0, device="cuda"),
+            "hap": torch.full((B, 2, M), 7, dtype=torch.uint8, device="cuda"),
+            "prob": torch.full((B, M, L), -1.0, device="cuda"),
+            "loglik": torch.zeros(B, dtype=torch.float64, device="cuda"),
+        }
+        j = _native.FbJob()
+        j.n_samples, j.n_markers, j.n_states = B, M, L
+        j.flags = ((_native.SI_FB_PER_MARKER if self.trans.per_marker else 0)
+                   | (_native.SI_FB_LOGSPACE if self.logspace else 0))
+        j.code_pitch = self.codes_np.shape[1]
+        j.seed = self.seed
+        s = self.trans.step
+        j.step = _native.StepCoef(float(s[0]), float(s[1]), float(s[2]), float(s[3]))
+        j.steps = device._ptr(self.trans.steps_dev)
+        j.sa = ctypes.c_void_p(self.idx.sa.data_ptr() + self.m0 * Lp)
+        j.mmask = ctypes.c_void_p(self.idx.mmask.data_ptr() + self.m0)
+        j.codes = device._ptr(self.codes)
+        self.tables.apply(j)
+        for k in ("ds", "ap", "hap", "prob", "loglik"):
+            setattr(j, k, device._ptr(self.out[k]))
+        return j
+
+    def result(self):
+        torch.cuda.synchronize()
+        return {k: v.cpu().numpy() for k, v in self.out.items()}
+
+
+def make_jobs(seed=5, n_jobs=14):
+    rng = np.random.default_rng(seed)
+    pan = synth.make_panel(1500, 2600, seed=seed, spacing_mode=1)
+    tg = synth.make_targets(pan, 300, 10, seed=seed + 1)
+    tables = device.SharedTables(synth.HARDCALL_PALETTE)
+    idx = {L: device.DeviceIndex(pan.cols, pan.H, L, keep_prefix=False) for L in (4, 5, 8, 16)}
+    jobs = []
+    for n in range(n_jobs):
+        L = int(rng.choice([4, 5, 8, 16]))
+        M = int(np.exp(rng.uniform(np.log(1), np.log(2500))))
+        m0 = int(rng.integers(0, pan.M - M + 1))
+        B = int(np.exp(rng.uniform(0, np.log(300))))
+        s0 = int(rng.integers(0, tg.B - B + 1))
+        jobs.append(RegionJob(idx[L], pan, m0, m0 + M, tg.codes[s0:s0 + B], 10000, tables,
+                              seed=1000 + n, logspace=bool(n % 5 == 4)))
+    return pan, jobs
+
+
+def plain_results(jobs):
+    res = []
+    for rj in jobs:
+        cj = rj.cjob()
+        device.run_jobs([cj])
+        res.append(rj.result())
+    return res
+
+
+def test_batch_equals_plain_eager():
+    pan, jobs = make_jobs()
+    ref = plain_results(jobs)
+    b = device.Batch([rj.cjob() for rj in jobs])
+    b.run(use_graph=False)
+    for rj, r in zip(jobs, ref):
+        got = rj.result()
+        for k in r:
+            assert np.array_equal(got[k], r[k]), (k, rj.B, rj.M, rj.idx.L)
+    b.close()
+
+
+def test_batch_graph_replays():
+    pan, jobs = make_jobs(seed=9, n_jobs=10)
+    ref = plain_results(jobs)
+    b = device.Batch([rj.cjob() for rj in jobs])
+    stream = torch.cuda.Stream()
+    for rep in range(3):
+        for rj in jobs:  # clobber outputs between replays
+            for v in rj.out.values():
+                v.fill_(0)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            b.run(use_graph=True, stream=stream)
+        stream.synchronize()
+        _ = torch.randn(1000, device="cuda").sum().item()  # unrelated eager work between replays
+        for rj, r in zip(jobs, ref):
+            got = rj.result()
+            for k in r:
+                assert np.array_equal(got[k], r[k]), (rep, k)
+    b.close()
+
+
+def test_batch_against_oracle_subset():
+    pan, jobs = make_jobs(seed=13, n_jobs=6)
+    b = device.Batch([rj.cjob() for rj in jobs])
+    b.run()
+    for rj in jobs:
+        got = rj.result()
+        a0, a1 = rj.idx.state_alleles()
+        a0, a1 = a0[rj.m0:rj.m0 + rj.M], a1[rj.m0:rj.m0 + rj.M]
+        gl = synth.HARDCALL_PALETTE[rj.codes_np[:, : rj.M]]
+        o = oracle.fb_f64_fast(a0, a1, gl, rj.trans.Td, rj.trans.To, per_marker=True)
+        assert np.abs(got["ds"] - o["ds"]).max() <= 1e-5
+        hap = oracle.sample_replay(got["prob"], a0, a1, rj.seed)
+        assert np.array_equal(got["hap"], hap)
+    b.close()
