@@ -1,0 +1,79 @@
+/*
+ * server.h -- the serving form (C ABI of libswiftimpute.so).
+ *
+ * A server keeps one reference panel resident in HBM (bit-packed alleles, the
+ * PBWT-selected state table and the per-marker transition coefficients) and
+ * imputes requests -- a target cohort over a contiguous marker region of the
+ * panel -- as they arrive.  A scheduler thread admits queued requests in
+ * arrival order while their device footprint (codes, outputs, checkpoints)
+ * fits the HBM budget, forms a batch of every admitted request, and launches
+ * it (si_fb_batch_*) on one of several streams, so requests that arrive while
+ * others run are admitted as memory frees up and their work overlaps on the
+ * device.  Each request's result depends only on its own inputs and seed:
+ * alone or in any mix it is bit-for-bit the same.
+ *
+ * Region semantics: a request over [m0, m1) uses the resident full-panel
+ * PBWT order at each marker of the region (the reference's --region path
+ * re-sweeps the PBWT from the region start instead).
+ *
+ * No counterpart in the reference (its Imputer processes one cohort file at
+ * a time, src/api/imputer.cpp:747-785).
+ */
+#ifndef SWIFTIMPUTE_SERVER_H
+#define SWIFTIMPUTE_SERVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct si_server si_server;
+
+typedef struct si_server_cfg {
+    uint32_t n_states;          /* L */
+    uint32_t ne;                /* effective population size */
+    float rho_rate;             /* recombination multiplier (4.0) */
+    uint32_t per_marker;        /* 0: each request uses its region's first interval */
+    uint32_t max_streams;       /* batches in flight at once (>= 1) */
+    uint32_t max_batch;         /* requests per batch (>= 1) */
+    uint64_t hbm_budget_bytes;  /* device memory for admitted requests */
+    int device_id;
+    uint32_t max_queue_delay_us; /* a partial batch waits up to this long for company (0) */
+} si_server_cfg;
+
+void si_server_defaults(si_server_cfg* cfg);
+
+/* cols [M][ceil(H/64)] host bit-packed panel; pos [M] bp; cM [M] or NULL.
+ * Builds the PBWT index and state table on the device, then starts the
+ * scheduler thread.  Returns 0 or a negative status (si_server_last_error). */
+int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H, uint32_t M,
+                     const uint64_t* pos, const double* cM, si_server** out);
+
+/* Queue a request: cohort of B samples over markers [m0, m1) with hard-call
+ * codes [B][m1-m0] (0 = 0/0, 1 = 0/1, 2 = 1/1, 3 = missing), Philox seed.
+ * codes is copied; the call returns at once with the request id. */
+int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const uint8_t* codes,
+                     uint64_t seed, uint64_t* request_id);
+
+/* Wait for a request and copy its results: hap [B][2][M'] and ds [B][M']
+ * (either may be NULL); latency_s receives arrival-to-result seconds. */
+int si_server_wait(si_server* s, uint64_t request_id, uint8_t* hap, float* ds, double* latency_s);
+
+/* Counters since creation: completed requests, batches, imputed genotypes
+ * (untyped (sample, marker) pairs), largest batch, peak admitted bytes. */
+typedef struct si_server_stats {
+    uint64_t requests, batches, genotypes, untyped_genotypes, max_batch_requests;
+    uint64_t peak_bytes, deferred_admissions;
+} si_server_stats;
+int si_server_get_stats(si_server* s, si_server_stats* out);
+
+int si_server_destroy(si_server* s);
+const char* si_server_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWIFTIMPUTE_SERVER_H */

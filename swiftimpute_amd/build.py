@@ -38,7 +38,7 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
 
 HIP_SOURCES = ["hip/capi.hip", "hip/index.hip", "hip/hmm.hip"]
 HIP_HEADERS = ["hip/si_device.hpp", "hip/si_internal.hpp"]
-API_SOURCES = ["api/panel.cpp", "api/imputer.cpp", "api/synth.cpp"]
+API_SOURCES = ["api/panel.cpp", "api/imputer.cpp", "api/synth.cpp", "api/server.cpp"]
 
 
 def _run(cmd):
@@ -72,7 +72,8 @@ def build(force: bool = True, verbose: bool = False) -> dict:
         if force or _newer(o, [s] + hdrs):
             jobs.append([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
     api_objs = []
-    api_hdrs = hdrs + [os.path.join(INCLUDE, "swiftimpute", f) for f in ("types.hpp", "imputer.hpp")]
+    api_hdrs = hdrs + [os.path.join(INCLUDE, "swiftimpute", f)
+                       for f in ("types.hpp", "imputer.hpp", "synth.h", "server.h")]
     for src in API_SOURCES:
         s = os.path.join(CSRC, src)
         if not os.path.exists(s):

@@ -1,0 +1,514 @@
+// Serving form (include/swiftimpute/server.h): resident panel + continuous
+// batching of imputation requests over several streams.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../../include/swiftimpute/server.h"
+#include "../../../include/swiftimpute/synth.h"
+#include "../../../include/swiftimpute_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+using Clock = std::chrono::steady_clock;
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// First-fit allocator over one device buffer (host-side bookkeeping only).
+class Arena {
+  public:
+    void init(char* base, size_t bytes) {
+        base_ = base;
+        size_ = bytes;
+        free_.clear();
+        free_[0] = bytes;
+        used_ = 0;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = align256(std::max<size_t>(bytes, 1));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes, start = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[start] = rest;
+                used_ += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = align256(std::max<size_t>(bytes, 1));
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    char* ptr(size_t off) const { return base_ + off; }
+    size_t used() const { return used_; }
+    size_t size() const { return size_; }
+
+  private:
+    char* base_ = nullptr;
+    size_t size_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+struct Request {
+    uint64_t id = 0;
+    uint32_t m0 = 0, M = 0, B = 0, pitch = 0;
+    uint64_t seed = 0;
+    std::vector<uint8_t> codes;  // [B][pitch]
+    uint64_t untyped = 0;
+    Clock::time_point arrival, done_at;
+    size_t off_codes = 0, off_hap = 0, off_ds = 0, own_bytes = 0;
+    std::vector<uint8_t> hap;
+    std::vector<float> ds;
+    bool done = false;
+    int status = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    std::vector<std::shared_ptr<Request>> reqs;
+    si_fb_batch* batch = nullptr;
+    size_t ws_off = 0, ws_bytes = 0;
+    char* pinned = nullptr;  // staging for codes in and results out
+    size_t pinned_bytes = 0;
+};
+
+}  // namespace
+
+struct si_server {
+    si_server_cfg cfg{};
+    uint32_t H = 0, M = 0, Lp = 0;
+    int device = 0;
+    // resident
+    uint64_t* d_cols = nullptr;
+    uint8_t* d_sa = nullptr;
+    uint8_t* d_mmask = nullptr;
+    si_step_coef* d_steps = nullptr;  // [M], entry m = step into m
+    std::vector<si_step_coef> steps;  // host copy
+    std::vector<uint32_t> tiny_switch_prefix;  // intervals below the linear range, prefix count
+    float* d_eps = nullptr;
+    float* d_emax = nullptr;
+    float* d_palette = nullptr;
+    char* d_arena = nullptr;
+    Arena arena;
+    // scheduling
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> queue;
+    std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
+    std::vector<Slot> slots;
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_id = 1;
+    // finished batch descriptors; released when no batch is in flight, since
+    // freeing device memory can wait on every stream of the device
+    std::vector<si_fb_batch*> finished;
+    si_server_stats stats{};
+    std::string worker_error;
+
+    ~si_server() {
+        for (auto* b : finished) si_fb_batch_destroy(b);
+        for (auto& sl : slots) {
+            if (sl.batch) si_fb_batch_destroy(sl.batch);
+            if (sl.pinned) (void)hipHostFree(sl.pinned);
+            if (sl.done) (void)hipEventDestroy(sl.done);
+            if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        }
+        for (void* p : {static_cast<void*>(d_cols), static_cast<void*>(d_sa), static_cast<void*>(d_mmask),
+                        static_cast<void*>(d_steps), static_cast<void*>(d_eps), static_cast<void*>(d_emax),
+                        static_cast<void*>(d_palette), static_cast<void*>(d_arena)})
+            if (p) (void)hipFree(p);
+    }
+
+    bool uses_logspace(uint32_t m0, uint32_t m1) const {
+        // steps used: compat -> interval m0 only; per-marker -> intervals m0 .. m1-2
+        if (m1 - m0 < 2) return false;
+        const uint32_t a = m0, b = cfg.per_marker ? m1 - 1 : m0 + 1;
+        return tiny_switch_prefix[b] - tiny_switch_prefix[a] > 0;
+    }
+
+    // Device footprint of a request apart from its checkpoint workspace.
+    static size_t own_bytes(uint32_t B, uint32_t M, uint32_t pitch) {
+        return align256(size_t(B) * pitch) + align256(size_t(B) * 2 * M) +
+               align256(size_t(B) * M * sizeof(float));
+    }
+    size_t ws_bytes_of(const Request& r) const {
+        si_fb_job j{};
+        j.n_samples = r.B;
+        j.n_markers = r.M;
+        j.n_states = cfg.n_states;
+        return align256(si_fb_workspace_bytes(&j, 1));
+    }
+
+    si_fb_job job_of(const Request& r) const {
+        si_fb_job j{};
+        j.n_samples = r.B;
+        j.n_markers = r.M;
+        j.n_states = cfg.n_states;
+        j.flags = (cfg.per_marker ? uint32_t(SI_FB_PER_MARKER) : 0u) |
+                  (uses_logspace(r.m0, r.m0 + r.M) ? uint32_t(SI_FB_LOGSPACE) : 0u);
+        j.code_pitch = r.pitch;
+        j.n_palette = 4;
+        j.seed = r.seed;
+        j.step = (r.M >= 2) ? steps[r.m0 + 1] : si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY};
+        j.steps = d_steps + r.m0;
+        j.sa = d_sa + size_t(r.m0) * Lp;
+        j.mmask = d_mmask + r.m0;
+        j.codes = reinterpret_cast<const uint8_t*>(arena.ptr(r.off_codes));
+        j.eps = d_eps;
+        j.emax = d_emax;
+        j.palette = d_palette;
+        j.hap = reinterpret_cast<uint8_t*>(arena.ptr(r.off_hap));
+        j.ds = reinterpret_cast<float*>(arena.ptr(r.off_ds));
+        return j;
+    }
+
+    void retire(Slot& sl) {
+        // results are in the slot's pinned buffer after the codes region
+        size_t in_bytes = 0;
+        for (auto& r : sl.reqs) in_bytes += size_t(r->B) * r->pitch;
+        size_t o = in_bytes;
+        const auto now = Clock::now();
+        for (auto& r : sl.reqs) {
+            const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
+            r->hap.assign(sl.pinned + o, sl.pinned + o + hb);
+            o += hb;
+            r->ds.resize(size_t(r->B) * r->M);
+            std::memcpy(r->ds.data(), sl.pinned + o, db);
+            o += db;
+            arena.release(r->off_codes, r->own_bytes);
+            r->done = true;
+            r->done_at = now;
+            stats.requests++;
+            stats.genotypes += uint64_t(r->B) * r->M;
+            stats.untyped_genotypes += r->untyped;
+        }
+        arena.release(sl.ws_off, sl.ws_bytes);
+        finished.push_back(sl.batch);
+        sl.batch = nullptr;
+        sl.reqs.clear();
+        sl.busy = false;
+        cv_done.notify_all();
+    }
+
+    // Admits queued requests (arrival order) into a free slot; false if none.
+    bool launch(Slot& sl) {
+        std::vector<std::shared_ptr<Request>> take;
+        size_t ws_total = 0;
+        while (!queue.empty() && take.size() < cfg.max_batch) {
+            auto& r = queue.front();
+            const size_t own = own_bytes(r->B, r->M, r->pitch), ws = ws_bytes_of(*r);
+            size_t off = 0;
+            // the batch workspace is allocated after selection; keep room for it
+            if (arena.size() - arena.used() < own + ws_total + ws + 4096 || !arena.alloc(own, &off)) {
+                if (take.empty()) stats.deferred_admissions++;
+                break;
+            }
+            r->own_bytes = own;
+            r->off_codes = off;
+            r->off_hap = off + align256(size_t(r->B) * r->pitch);
+            r->off_ds = r->off_hap + align256(size_t(r->B) * 2 * r->M);
+            ws_total += ws;
+            take.push_back(r);
+            queue.pop_front();
+        }
+        if (take.empty()) return false;
+        size_t ws_off = 0;
+        if (!arena.alloc(ws_total, &ws_off)) {  // fragmentation: put them back
+            for (auto it = take.rbegin(); it != take.rend(); ++it) {
+                arena.release((*it)->off_codes, (*it)->own_bytes);
+                queue.push_front(*it);
+            }
+            stats.deferred_admissions++;
+            return false;
+        }
+        std::vector<si_fb_job> jobs;
+        size_t in_bytes = 0, out_bytes = 0;
+        for (auto& r : take) {
+            jobs.push_back(job_of(*r));
+            in_bytes += size_t(r->B) * r->pitch;
+            out_bytes += size_t(r->B) * r->M * (2 + sizeof(float));
+        }
+        if (sl.pinned_bytes < in_bytes + out_bytes) {
+            if (sl.pinned) (void)hipHostFree(sl.pinned);
+            sl.pinned_bytes = std::max<size_t>(in_bytes + out_bytes, sl.pinned_bytes * 2);
+            if (hipHostMalloc(reinterpret_cast<void**>(&sl.pinned), sl.pinned_bytes, 0) != hipSuccess)
+                throw std::runtime_error("hipHostMalloc failed");
+        }
+        size_t o = 0;
+        for (auto& r : take) {
+            std::memcpy(sl.pinned + o, r->codes.data(), r->codes.size());
+            if (hipMemcpyAsync(arena.ptr(r->off_codes), sl.pinned + o, r->codes.size(),
+                               hipMemcpyHostToDevice, sl.stream) != hipSuccess)
+                throw std::runtime_error("codes upload failed");
+            o += r->codes.size();
+        }
+        if (si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch) != SI_OK ||
+            si_fb_batch_run(sl.batch, arena.ptr(ws_off), ws_total, 0, sl.stream) != SI_OK)
+            throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
+        for (auto& r : take) {
+            const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
+            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_hap), hb, hipMemcpyDeviceToHost, sl.stream);
+            o += hb;
+            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_ds), db, hipMemcpyDeviceToHost, sl.stream);
+            o += db;
+        }
+        if (hipEventRecord(sl.done, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
+        sl.reqs = std::move(take);
+        sl.ws_off = ws_off;
+        sl.ws_bytes = ws_total;
+        sl.busy = true;
+        stats.batches++;
+        stats.max_batch_requests = std::max<uint64_t>(stats.max_batch_requests, sl.reqs.size());
+        stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena.used());
+        return true;
+    }
+
+    void run() {
+        (void)hipSetDevice(device);
+        std::unique_lock<std::mutex> lk(mu);
+        try {
+            for (;;) {
+                bool inflight = false;
+                for (auto& sl : slots)
+                    if (sl.busy) {
+                        if (hipEventQuery(sl.done) == hipSuccess) retire(sl);
+                        else inflight = true;
+                    }
+                bool launched = false;
+                // a partial batch launches once its oldest request has waited
+                // max_queue_delay_us (or at shutdown)
+                const bool ready =
+                    !queue.empty() &&
+                    (stop || queue.size() >= cfg.max_batch ||
+                     Clock::now() - queue.front()->arrival >=
+                         std::chrono::microseconds(cfg.max_queue_delay_us));
+                for (auto& sl : slots)
+                    if (ready && !sl.busy && !queue.empty() && launch(sl)) {
+                        launched = inflight = true;
+                    }
+                if (!inflight && !finished.empty()) {
+                    for (auto* b : finished) si_fb_batch_destroy(b);
+                    finished.clear();
+                }
+                if (stop && queue.empty() && !inflight) break;
+                if (launched) continue;
+                if (inflight) {
+                    lk.unlock();
+                    std::this_thread::sleep_for(std::chrono::microseconds(50));
+                    lk.lock();
+                } else {
+                    cv_work.wait_for(lk, std::chrono::milliseconds(5));
+                }
+            }
+        } catch (const std::exception& e) {
+            worker_error = e.what();
+            for (auto& kv : reqs)
+                if (!kv.second->done) {
+                    kv.second->done = true;
+                    kv.second->status = SI_ERR_DEVICE;
+                }
+            cv_done.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+const char* si_server_last_error(void) { return g_err.c_str(); }
+
+void si_server_defaults(si_server_cfg* c) {
+    c->n_states = 8;
+    c->ne = 10000;
+    c->rho_rate = 4.0f;
+    c->per_marker = 0;
+    c->max_streams = 4;
+    c->max_batch = 256;
+    c->hbm_budget_bytes = 32ull << 30;
+    c->device_id = 0;
+    c->max_queue_delay_us = 0;
+}
+
+int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H, uint32_t M,
+                     const uint64_t* pos, const double* cM, si_server** out) {
+    if (!cfg || !cols || !pos || !out || H == 0 || M == 0)
+        return fail(SI_ERR_ARG, "si_server_create: bad arguments");
+    if (cfg->n_states == 0 || cfg->n_states > 64) return fail(SI_ERR_ARG, "n_states must be in 1..64");
+    auto s = std::make_unique<si_server>();
+    s->cfg = *cfg;
+    s->cfg.max_streams = std::max(1u, cfg->max_streams);
+    s->cfg.max_batch = std::max(1u, cfg->max_batch);
+    s->H = H;
+    s->M = M;
+    s->Lp = si_padded_states(cfg->n_states);
+    s->device = cfg->device_id < 0 ? 0 : cfg->device_id;
+    if (hipSetDevice(s->device) != hipSuccess) return fail(SI_ERR_DEVICE, "hipSetDevice failed");
+    const size_t W = (H + 63) / 64;
+    const uint32_t L = cfg->n_states;
+    // index build: PBWT sweep into a temporary prefix array, then the state table
+    uint32_t* prefix = nullptr;
+    hipStream_t st = nullptr;
+    if (hipStreamCreate(&st) != hipSuccess ||
+        hipMalloc(&s->d_cols, M * W * 8) != hipSuccess ||
+        hipMalloc(&s->d_sa, size_t(M) * s->Lp) != hipSuccess ||
+        hipMalloc(&s->d_mmask, M) != hipSuccess ||
+        hipMalloc(&prefix, size_t(M) * H * 4) != hipSuccess ||
+        hipMemcpy(s->d_cols, cols, M * W * 8, hipMemcpyHostToDevice) != hipSuccess) {
+        if (prefix) (void)hipFree(prefix);
+        if (st) (void)hipStreamDestroy(st);
+        return fail(SI_ERR_DEVICE, "si_server_create: device allocation failed");
+    }
+    std::vector<uint32_t> perm(std::min(L, H));
+    int rc = si_pbwt_sweep(s->d_cols, H, M, prefix, nullptr, nullptr, 0, st);
+    if (rc == SI_OK) rc = si_select_perm(H, L, perm.data());
+    if (rc == SI_OK)
+        rc = si_state_table(prefix, s->d_cols, nullptr, H, M, L, 0, perm.data(), nullptr, s->d_sa,
+                            s->d_mmask, st);
+    if (rc == SI_OK && hipStreamSynchronize(st) != hipSuccess) rc = SI_ERR_DEVICE;
+    (void)hipFree(prefix);
+    (void)hipStreamDestroy(st);
+    if (rc != SI_OK) return fail(rc, std::string("index build failed: ") + si_last_error());
+    // transitions over the whole panel
+    std::vector<double> g(M);
+    si_genetic_map(pos, cM, M, g.data());
+    std::vector<float> Td(M > 1 ? M - 1 : 1), To(Td.size());
+    si_transition_constants(g.data(), M, cfg->ne, cfg->rho_rate, Td.data(), To.data());
+    std::vector<si_step_coef> co(Td.size());
+    si_step_coefs(Td.data(), To.data(), M > 1 ? M - 1 : 0, co.data());
+    s->steps.assign(M, si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY});
+    s->tiny_switch_prefix.assign(M + 1, 0);
+    for (uint32_t m = 1; m < M; ++m) s->steps[m] = co[m - 1];
+    // tiny_switch_prefix[t+1] counts intervals 0..t whose switch is below the linear range
+    for (uint32_t t = 0; t < M; ++t)
+        s->tiny_switch_prefix[t + 1] =
+            s->tiny_switch_prefix[t] + ((t + 1 < M && !(s->steps[t + 1].t_o >= 1e-30f)) ? 1u : 0u);
+    float pal[12], eps[4 * 32], emax[32];
+    si_hardcall_palette(pal);
+    si_emission_table(pal, 4, eps, emax);
+    if (hipMalloc(&s->d_steps, M * sizeof(si_step_coef)) != hipSuccess ||
+        hipMalloc(&s->d_eps, sizeof(eps)) != hipSuccess || hipMalloc(&s->d_emax, sizeof(emax)) != hipSuccess ||
+        hipMalloc(&s->d_palette, sizeof(pal)) != hipSuccess ||
+        hipMemcpy(s->d_steps, s->steps.data(), M * sizeof(si_step_coef), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_eps, eps, sizeof(eps), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_emax, emax, sizeof(emax), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_palette, pal, sizeof(pal), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc(&s->d_arena, cfg->hbm_budget_bytes) != hipSuccess)
+        return fail(SI_ERR_DEVICE, "si_server_create: device allocation failed");
+    s->arena.init(s->d_arena, cfg->hbm_budget_bytes);
+    s->slots.resize(s->cfg.max_streams);
+    for (auto& sl : s->slots)
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+            return fail(SI_ERR_DEVICE, "si_server_create: stream creation failed");
+    si_server* raw = s.release();
+    raw->worker = std::thread([raw] { raw->run(); });
+    *out = raw;
+    return SI_OK;
+}
+
+int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const uint8_t* codes,
+                     uint64_t seed, uint64_t* id) {
+    if (!s || !id || m1 <= m0 || m1 > s->M || B == 0 || !codes)
+        return fail(SI_ERR_ARG, "si_server_submit: bad arguments");
+    auto r = std::make_shared<Request>();
+    r->arrival = Clock::now();
+    r->m0 = m0;
+    r->M = m1 - m0;
+    r->B = B;
+    r->pitch = (r->M + 15) / 16 * 16;
+    r->seed = seed;
+    r->codes.assign(size_t(B) * r->pitch, 3);
+    for (uint32_t b = 0; b < B; ++b) {
+        const uint8_t* src = codes + size_t(b) * r->M;
+        std::memcpy(&r->codes[size_t(b) * r->pitch], src, r->M);
+        for (uint32_t m = 0; m < r->M; ++m) {
+            if (src[m] > 3) return fail(SI_ERR_ARG, "si_server_submit: code outside 0..3");
+            r->untyped += src[m] == 3;
+        }
+    }
+    const size_t need = si_server::own_bytes(B, r->M, r->pitch) + s->ws_bytes_of(*r) + 8192;
+    if (need > s->cfg.hbm_budget_bytes)
+        return fail(SI_ERR_ARG, "si_server_submit: request needs " + std::to_string(need) +
+                                    " bytes, above the server's HBM budget");
+    std::lock_guard<std::mutex> g(s->mu);
+    if (!s->worker_error.empty()) return fail(SI_ERR_DEVICE, s->worker_error);
+    r->id = s->next_id++;
+    s->reqs[r->id] = r;
+    s->queue.push_back(r);
+    *id = r->id;
+    s->cv_work.notify_one();
+    return SI_OK;
+}
+
+int si_server_wait(si_server* s, uint64_t id, uint8_t* hap, float* ds, double* latency_s) {
+    if (!s) return fail(SI_ERR_ARG, "si_server_wait: null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(SI_ERR_ARG, "si_server_wait: unknown request");
+    auto r = it->second;
+    s->cv_done.wait(lk, [&] { return r->done; });
+    s->reqs.erase(it);
+    if (r->status != 0) return fail(r->status, "request failed: " + s->worker_error);
+    if (hap) std::memcpy(hap, r->hap.data(), r->hap.size());
+    if (ds) std::memcpy(ds, r->ds.data(), r->ds.size() * sizeof(float));
+    if (latency_s) *latency_s = std::chrono::duration<double>(r->done_at - r->arrival).count();
+    return SI_OK;
+}
+
+int si_server_get_stats(si_server* s, si_server_stats* out) {
+    if (!s || !out) return fail(SI_ERR_ARG, "si_server_get_stats: bad arguments");
+    std::lock_guard<std::mutex> g(s->mu);
+    *out = s->stats;
+    return SI_OK;
+}
+
+int si_server_destroy(si_server* s) {
+    if (!s) return SI_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+        s->cv_work.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    delete s;
+    return SI_OK;
+}
+
+}  // extern "C"

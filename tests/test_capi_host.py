@@ -34,6 +34,24 @@ def test_api_library_exports_synth_symbols():
         assert hasattr(lib, n), n
 
 
+def test_api_library_exports_server_symbols():
+    from swiftimpute_amd import serving
+    lib = serving._lib()
+    names = declared(os.path.join("swiftimpute", "server.h"))
+    assert {n for n, _, _ in serving._SYMBOLS} == set(names)
+    cfg = serving.ServerCfg()
+    lib.si_server_defaults(ctypes.byref(cfg))
+    assert (cfg.n_states, cfg.ne, cfg.max_streams, cfg.max_queue_delay_us) == (8, 10000, 4, 0)
+    assert abs(cfg.rho_rate - 4.0) < 1e-7
+    h = ctypes.c_void_p()
+    one = np.zeros(1, np.uint64)
+    # H = 0 is refused before any device call
+    assert lib.si_server_create(ctypes.byref(cfg), one.ctypes.data_as(_native._u64p), 0, 1,
+                                one.ctypes.data_as(_native._u64p), None, ctypes.byref(h)) != 0
+    assert b"bad arguments" in lib.si_server_last_error()
+    assert lib.si_server_submit(None, 0, 1, 1, None, 0, None) != 0
+
+
 def test_abi_version_and_padding():
     lib = _native.hip()
     assert lib.si_abi_version() == 1

@@ -1,0 +1,124 @@
+"""Serving form: a resident panel answering (cohort, region) requests.
+
+A request's result must not depend on what it was batched with: each request
+served alone (one request per batch, one stream) equals the same request
+served in a mix (many per batch, several streams), and both equal the plain
+one-job-per-launch path over the same region (itself checked against the
+oracle in test_gpu_parity.py)."""
+import numpy as np
+import pytest
+import torch
+
+from swiftimpute_amd import _native, device, serving, synth
+
+pytestmark = pytest.mark.gpu
+
+L = 8
+NE = 10000
+
+
+def _requests(pan, tg, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        M = int(np.exp(rng.uniform(0, np.log(pan.M))))
+        m0 = int(rng.integers(0, pan.M - M + 1))
+        B = int(rng.integers(1, tg.B + 1))
+        s0 = int(rng.integers(0, tg.B - B + 1))
+        out.append((m0, m0 + M, np.ascontiguousarray(tg.codes[s0:s0 + B, m0:m0 + M]), 700 + i))
+    return out
+
+
+@pytest.fixture(scope="module")
+def world():
+    pan = synth.make_panel(1200, 1800, seed=31)
+    tg = synth.make_targets(pan, 48, 10, seed=32)
+    return pan, tg
+
+
+def _plain(pan, idx, tables, m0, m1, codes, seed):
+    M = m1 - m0
+    B = codes.shape[0]
+    pitch = (M + 15) // 16 * 16
+    c = np.full((B, pitch), 3, np.uint8)
+    c[:, :M] = codes
+    cm = pan.genetic_cM()
+    tr = device.Transitions.build(pan.pos[m0:m1], None if cm is None else cm[m0:m1], NE)
+    cd = torch.from_numpy(c).cuda()
+    ds = torch.full((B, M), -1.0, device="cuda")
+    hap = torch.full((B, 2, M), 7, dtype=torch.uint8, device="cuda")
+    j = _native.FbJob()
+    j.n_samples, j.n_markers, j.n_states = B, M, L
+    j.flags = _native.SI_FB_LOGSPACE if tr.min_switch() < 1e-30 else 0
+    j.code_pitch = pitch
+    j.seed = seed
+    s = tr.step
+    j.step = _native.StepCoef(float(s[0]), float(s[1]), float(s[2]), float(s[3]))
+    j.steps = device._ptr(tr.steps_dev)
+    import ctypes
+    j.sa = ctypes.c_void_p(idx.sa.data_ptr() + m0 * idx.Lp)
+    j.mmask = ctypes.c_void_p(idx.mmask.data_ptr() + m0)
+    j.codes = device._ptr(cd)
+    tables.apply(j)
+    j.ds, j.hap = device._ptr(ds), device._ptr(hap)
+    device.run_jobs([j])
+    torch.cuda.synchronize()
+    return hap.cpu().numpy(), ds.cpu().numpy()
+
+
+def test_alone_equals_mix_equals_plain(world):
+    pan, tg = world
+    reqs = _requests(pan, tg, 14, seed=4)
+    cm = pan.genetic_cM()
+    with serving.Server(pan.cols, pan.H, pan.pos, cm, L=L, ne=NE, max_streams=1,
+                        max_batch=1) as srv:
+        alone = [srv.wait(srv.submit(*r)) for r in reqs]
+        st1 = srv.stats()
+    assert st1["batches"] == len(reqs) and st1["max_batch_requests"] == 1
+    with serving.Server(pan.cols, pan.H, pan.pos, cm, L=L, ne=NE, max_streams=3, max_batch=6,
+                        max_queue_delay_us=200000) as srv:
+        ids = [srv.submit(*r) for r in reqs]
+        mix = [srv.wait(i) for i in ids]
+        st2 = srv.stats()
+    assert st2["requests"] == len(reqs)
+    assert st2["max_batch_requests"] > 1, st2
+    assert st2["batches"] < len(reqs), st2
+    for r, a, b in zip(reqs, alone, mix):
+        assert np.array_equal(a.hap, b.hap), r[:2]
+        assert np.array_equal(a.ds, b.ds), r[:2]
+        assert a.latency_s > 0 and b.latency_s > 0
+
+    idx = device.DeviceIndex(pan.cols, pan.H, L, keep_prefix=False)
+    tables = device.SharedTables(synth.HARDCALL_PALETTE)
+    for r, a in zip(reqs, alone):
+        hap, ds = _plain(pan, idx, tables, *r)
+        assert np.array_equal(a.hap, hap), r[:2]
+        assert np.array_equal(a.ds, ds), r[:2]
+
+
+def test_admission_under_tight_budget(world):
+    pan, tg = world
+    reqs = _requests(pan, tg, 10, seed=8)
+    big = max(r[2].shape[0] * (r[1] - r[0]) for r in reqs)
+    # room for about two of the largest requests at a time
+    budget = 3 * (big * (2 + 4 + 1) + big * L * 4) / (1 << 30) + 1e-4
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE, max_streams=4,
+                        max_batch=8) as ref_srv:
+        ref = [ref_srv.wait(ref_srv.submit(*r)) for r in reqs]
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE, max_streams=4,
+                        max_batch=8, hbm_budget_gb=budget) as srv:
+        ids = [srv.submit(*r) for r in reqs]
+        got = [srv.wait(i) for i in ids]
+        st = srv.stats()
+    assert st["requests"] == len(reqs)
+    assert st["peak_bytes"] <= budget * (1 << 30)
+    for a, b in zip(ref, got):
+        assert np.array_equal(a.hap, b.hap) and np.array_equal(a.ds, b.ds)
+
+
+def test_request_above_budget_is_refused(world):
+    pan, tg = world
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE,
+                        hbm_budget_gb=1e-4) as srv:
+        with pytest.raises(_native.NativeError, match="budget"):
+            srv.submit(0, pan.M, np.ascontiguousarray(tg.codes[:, : pan.M]))
