@@ -147,7 +147,7 @@ typedef struct si_fb_job {
     uint32_t n_states;    /* L (1..64) */
     uint32_t flags;       /* si_fb_flags */
     uint32_t code_pitch;  /* row pitch of codes, >= M, multiple of 16 */
-    uint32_t n_palette;   /* P (1..256) */
+    uint32_t n_palette;   /* P (1..64) */
     uint32_t rng_stream;  /* third Philox counter word */
     uint32_t sample_base; /* global index of sample 0 (Philox counter word 1) */
     uint64_t seed;        /* Philox key */
@@ -157,7 +157,7 @@ typedef struct si_fb_job {
     const uint8_t* mmask; /* [M] device */
     const uint8_t* codes; /* [B][code_pitch] device */
     const float* eps;     /* [P][8][4] device (linear kernel) */
-    const float* emax;    /* [P][8] device (linear kernel) */
+    const float* emax;    /* [P][8] device (both kernels: the FP64-carried offset) */
     const float* palette; /* [P][3] device (log-space kernel) */
     float* ds;            /* [B][M] device or NULL */
     float* ap;            /* [B][M][2] device or NULL */

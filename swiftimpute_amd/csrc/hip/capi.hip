@@ -182,9 +182,9 @@ static int check_job(const si_fb_job& j, uint32_t idx) {
     const bool logspace = (j.flags & SI_FB_LOGSPACE) != 0;
     if (!j.sa || !j.mmask || !j.codes)
         return fail(SI_ERR_ARG, "job %u: sa, mmask, codes are required", idx);
-    if (logspace ? !j.palette : (!j.eps || !j.emax))
+    if (!j.emax || (logspace ? !j.palette : !j.eps))
         return fail(SI_ERR_ARG, "job %u: %s required", idx,
-                    logspace ? "palette is" : "eps and emax are");
+                    logspace ? "palette and emax are" : "eps and emax are");
     if (!aligned(j.codes, 16) || !aligned(j.sa, 4) || (j.eps && !aligned(j.eps, 16)))
         return fail(SI_ERR_ARG, "job %u: codes (16 B), sa (4 B), eps (16 B) must be aligned", idx);
     if ((j.flags & SI_FB_PER_MARKER) && !j.steps)

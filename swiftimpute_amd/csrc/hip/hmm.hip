@@ -15,11 +15,16 @@
 //     the host); backward b_m = e^{Bl_m} v_m with the log scale Bl_m in FP64,
 //     so the floor enters exactly as f = exp(-999 - Bl_m) once b reaches it.
 //     exp/log run once per (sample, marker), not per state.
-//   * log-space FP32: forward log vector normalized to log-sum 0; backward
-//     log vector relative to an FP64 offset; every step is a three-term
-//     log-sum-exp LSE(-999, To + LSE(alpha), U + alpha_j) per state,
-//     U = log(e^Td - e^To).  Valid for any To, including the -999 of
-//     coincident markers.
+//   * log-space FP32: forward log vector normalized to maximum 0 (as the
+//     reference normalizes, so its -999 floor weighs the same), its log mass
+//     ls = LSE(alpha) carried beside it; backward log vector relative to an
+//     FP64 offset; every step is a three-term log-sum-exp
+//     LSE(-999, To + ls, U + alpha_j) per state, U = log(e^Td - e^To).
+//     Valid for any To, including the -999 of coincident markers.
+// Both forms take emissions relative to the best class present at the marker
+// (so a marker where every state mismatches does not push the FP32 values
+// towards -1000, where their resolution is 6e-5) and carry that offset in
+// FP64, in the log-likelihood and the backward scale.
 //
 // Two passes per job (all samples of the job in both):
 //   chain pass: one lane group per (sample, direction) walks all M markers,
@@ -41,9 +46,10 @@ constexpr int kFbThreads = 128;
 constexpr float kFloor = -999.0f;
 
 // Emission table staged in LDS: row[code * 8 + class mask] holds the four
-// per-class values (class 3 = padding state), off[] the row's log offset.
-//   linear: row = (exp(gl_c - off), ..., 0),  off = max over present classes
-//   log:    row = (gl_00, gl_01, gl_11, -inf), off = 0
+// per-class values (class 3 = padding state), off[] the row's log offset,
+// off = max of gl over the classes present (0 if none):
+//   linear: row = (exp(gl_c - off), ..., 0), absent classes 0
+//   log:    row = (gl_c - off, ..., -inf),   absent classes -inf
 struct EmisLds {
     float4 row[kMaxPalette * 8];
     float off[kMaxPalette * 8];
@@ -55,8 +61,11 @@ __device__ __forceinline__ void load_tables(const FbTables& tb, EmisLds& t) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         if constexpr (LOG) {
             const float* g = tb.palette + 3 * (i / 8);
-            t.row[i] = make_float4(g[0], g[1], g[2], -INFINITY);
-            t.off[i] = 0.0f;
+            const uint32_t c = i % 8;
+            const float o = tb.emax[i];
+            t.row[i] = make_float4((c & 1u) ? g[0] - o : -INFINITY, (c & 2u) ? g[1] - o : -INFINITY,
+                                   (c & 4u) ? g[2] - o : -INFINITY, -INFINITY);
+            t.off[i] = o;
         } else {
             t.row[i] = reinterpret_cast<const float4*>(tb.eps)[i];
             t.off[i] = tb.emax[i];
@@ -109,14 +118,23 @@ __device__ __forceinline__ float lse3(float a, float b, float c) {
     return mx + fast_ln((fast_exp(a - mx) + fast_exp(b - mx)) + fast_exp(c - mx));
 }
 
+// Log mass LSE_i(a_i) of a max-normalized log vector (a value in [0, ln Lp]).
+template <int LPS>
+__device__ __forceinline__ float log_mass(float4 a) {
+    return fast_ln(group_sum<LPS>((fast_exp(a.x) + fast_exp(a.y)) + (fast_exp(a.z) + fast_exp(a.w))));
+}
+
 // ---------------------------------------------------------------- forward --
-// Marker 0.  Returns the log normalizer (log-likelihood increment).
+// Marker 0.  Returns the log normalizer (log-likelihood increment); ls is the
+// log mass of the normalized log-space vector (unused by the linear form,
+// whose vector sums to 1).
 template <int LPS, bool LOG>
-__device__ __forceinline__ double fwd_init(float4& a, const float e[4], float off) {
+__device__ __forceinline__ double fwd_init(float4& a, float& ls, const float e[4], float off) {
     if constexpr (LOG) {
-        const float c = group_lse<LPS>(e[0], e[1], e[2], e[3]);
+        const float c = group_max<LPS>(fmaxf(fmaxf(e[0], e[1]), fmaxf(e[2], e[3])));
         a = make_float4(e[0] - c, e[1] - c, e[2] - c, e[3] - c);
-        return double(c);
+        ls = log_mass<LPS>(a);
+        return double(off) + double(c);
     } else {
         const float S = group_sum<LPS>((e[0] + e[1]) + (e[2] + e[3]));
         const float r = __builtin_amdgcn_rcpf(S);
@@ -127,16 +145,19 @@ __device__ __forceinline__ double fwd_init(float4& a, const float e[4], float of
 
 // Step into a marker with emission e / offset off and coefficients k.
 template <int LPS, bool LOG>
-__device__ __forceinline__ double fwd_step(float4& a, const float e[4], float off, float4 k) {
+__device__ __forceinline__ double fwd_step(float4& a, float& ls, const float e[4], float off,
+                                           float4 k) {
     if constexpr (LOG) {
-        // a is normalized to log-sum 0, so LSE_i(a_i + To) = To.
-        const float x0 = lse3(kFloor, k.z, k.w + a.x) + e[0];
-        const float x1 = lse3(kFloor, k.z, k.w + a.y) + e[1];
-        const float x2 = lse3(kFloor, k.z, k.w + a.z) + e[2];
-        const float x3 = lse3(kFloor, k.z, k.w + a.w) + e[3];
-        const float c = group_lse<LPS>(x0, x1, x2, x3);
+        // LSE_i(a_i + To) = To + ls
+        const float lo = k.z + ls;
+        const float x0 = lse3(kFloor, lo, k.w + a.x) + e[0];
+        const float x1 = lse3(kFloor, lo, k.w + a.y) + e[1];
+        const float x2 = lse3(kFloor, lo, k.w + a.z) + e[2];
+        const float x3 = lse3(kFloor, lo, k.w + a.w) + e[3];
+        const float c = group_max<LPS>(fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)));
         a = make_float4(x0 - c, x1 - c, x2 - c, x3 - c);
-        return double(c);
+        ls = log_mass<LPS>(a);
+        return double(off) + double(c);
     } else {
         // a sums to 1, so sum_i a_i t_o = t_o.
         const float x0 = fmaf(k.y, a.x, k.x) * e[0];
@@ -162,7 +183,7 @@ __device__ __forceinline__ void bwd_step(float4& v, double& Bl, const float e[4]
     if constexpr (LOG) {
         const float y0 = e[0] + v.x, y1 = e[1] + v.y, y2 = e[2] + v.z, y3 = e[3] + v.w;
         const float lw = group_lse<LPS>(y0, y1, y2, y3);
-        const double nat = Bl + double(lw);
+        const double nat = Bl + double(off) + double(lw);
         const double arg = -999.0 - nat;
         const bool floor_dominates = arg > 0.0;
         const float fl = floor_dominates ? 0.0f : float(arg);
@@ -222,6 +243,7 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
     const uint32_t lp4 = Lp / 4;
     if (fwd) {
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        float ls = 0.0f;
         double ll = 0.0;
         for (uint32_t c = 0; c < J.nblk; ++c) {
             const uint32_t m0 = c * kBlock;
@@ -234,14 +256,15 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
                     float e[4];
                     lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
                     if (m == 0)
-                        ll += fwd_init<LPS, LOG>(a, e, tab.off[row]);
+                        ll += fwd_init<LPS, LOG>(a, ls, e, tab.off[row]);
                     else
-                        ll += fwd_step<LPS, LOG>(a, e, tab.off[row], step_coef(J, m));
+                        ll += fwd_step<LPS, LOG>(a, ls, e, tab.off[row], step_coef(J, m));
                     if (t == 0)
                         reinterpret_cast<float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r] = a;
                 }
             }
         }
+        if (LOG) ll += double(ls);  // log mass of the last normalized vector
         if (r == 0 && J.loglik) J.loglik[s] = ll;
     } else {
         float4 v = bwd_init<LOG>();
@@ -292,6 +315,8 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
         const uint4 cw = *reinterpret_cast<const uint4*>(J.codes + size_t(s) * J.pitch + m0);
         float4 A[kBlock];
         A[0] = reinterpret_cast<const float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r];
+        // the chain pass derived ls from the same vector by the same function
+        float ls = LOG ? log_mass<LPS>(A[0]) : 0.0f;
 #pragma unroll
         for (uint32_t t = 1; t < kBlock; ++t) {
             if (t < nm) {
@@ -300,7 +325,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
                 float e[4];
                 lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
                 A[t] = A[t - 1];
-                fwd_step<LPS, LOG>(A[t], e, tab.off[row], step_coef(J, m));
+                fwd_step<LPS, LOG>(A[t], ls, e, tab.off[row], step_coef(J, m));
             }
         }
         const size_t ck = size_t(s) * J.nblk + c;
