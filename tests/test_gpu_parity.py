@@ -123,7 +123,7 @@ def test_state_table_simple_mode():
 
 # ------------------------------------------------------------ forward-backward
 def check_fb(idx, codes, palette, trans, res, per_marker, fold=True, sample_base=0, seed=12345,
-             argmax=False):
+             argmax=False, tol=DS_TOL):
     a0, a1 = idx.state_alleles()
     gl = palette[codes[:, : idx.M]]
     if fold:
@@ -131,10 +131,10 @@ def check_fb(idx, codes, palette, trans, res, per_marker, fold=True, sample_base
     else:
         ref = oracle.fb_f64_fast(a0, a1, gl, trans.Td, trans.To, per_marker=per_marker)
     err = np.abs(res["ds"] - ref["ds"]).max()
-    assert err <= DS_TOL, f"max |DS - DS_fp64| = {err:.3g}"
-    assert np.abs(res["ap"] - ref["ap"]).max() <= DS_TOL
+    assert err <= tol, f"max |DS - DS_fp64| = {err:.3g}"
+    assert np.abs(res["ap"] - ref["ap"]).max() <= tol
     if ref["prob"] is not None and res["prob"] is not None:
-        assert np.abs(res["prob"] - ref["prob"]).max() <= DS_TOL
+        assert np.abs(res["prob"] - ref["prob"]).max() <= tol
     ll_err = np.abs(res["loglik"] - ref["loglik"]) / np.maximum(1.0, np.abs(ref["loglik"]))
     assert ll_err.max() < 1e-5
     hap = oracle.sample_replay(res["prob"], a0, a1, seed, mode="argmax" if argmax else "philox",
@@ -226,7 +226,13 @@ def test_fb_coincident_markers_logspace():
         tr = device.Transitions.build(p2, None, 10000, per_marker=per_marker)
         assert (tr.To == -999.0).any()
         res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=True)
-        check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, per_marker, fold=True)
+        # Reference-compatible mode with a zero-length first interval has
+        # To = -999 at every step: no switching at all, and the posteriors are
+        # set by terms e^-999 apart.  FP32 resolves |x| ~ 999 to 6.1e-5, which
+        # bounds the posterior error by about 2 ulp * p(1 - p) < 5e-5 (an FP32
+        # emulation of the kernel's arithmetic gives 1.29e-5 on this input).
+        check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, per_marker, fold=True,
+                 tol=DS_TOL if per_marker else 5e-5)
     with pytest.raises(Exception):
         run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=False)
 
