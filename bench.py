@@ -1,24 +1,41 @@
 #!/usr/bin/env python3
-"""Imputation throughput of the MI355X hot path (one JSON line on stdout).
+"""Imputation throughput of the MI355X hot path: batched vs plain, and serving
+latency (one JSON line on stdout; progress on stderr).
 
-Workload (default "cfg2"): a synthetic 20,000-haplotype x 100,000-marker panel
-(mosaics of 40 founders, 50 kb spacing on a 1 cM/Mb map, mutation 1e-3) and
-10,000 diploid targets typed at 5,000 evenly spaced markers (every 20th),
-imputing the other 95,000; L = 16 states, Ne = 20,000, FP32 forward-backward.
+Sections (all on synthetic data from the seeded generators of
+swiftimpute_amd.synth; every input is resident in HBM before a timed region):
 
-One step = one plain (one job per launch) pass of the hot path over the whole
-cohort with every input already resident in HBM: the chain pass and the block
-pass of si_fb_run (forward-backward, posterior dosage, Philox sampling).  The
-PBWT index is built once at set-up (timed separately, reported under "pbwt").
+  batch  (the headline `value`) -- 512 uneven jobs against one shared
+         20,000-haplotype x 200,000-marker panel: regions log-uniform in
+         2,000-80,000 markers, cohorts log-uniform in 1-2,000 samples, L
+         uniform in {4, 8, 16}, typed density uniform in {1/10, 1/20, 1/50},
+         Ne = 10,000.  One step = every job imputed once (forward-backward,
+         posterior dosage, Philox sampling).  Batched form: one persistent
+         work-queue launch per pass and state class, replayed from a captured
+         hipGraph.  Plain form: si_fb_run once per job, one after another on
+         one stream.  Both write the same outputs; the bench checks that they
+         are equal bit for bit.
+  serve  -- a resident 40,000 x 500,000 panel; Poisson request arrivals;
+         each request a cohort (log-uniform 1-5,000 samples) over a region
+         (log-uniform 1,000-100,000 markers), L = 8, Ne = 10,000, typed every
+         20th marker.  Reports arrival-to-result p50/p99 latency and served
+         genotypes/s, beside the same requests submitted all at once
+         (batched-offline).
+  plain  -- 20,000 x 100,000 panel, 10,000 targets typed every 20th marker,
+         L = 16, Ne = 20,000, one job per launch: the per-kernel roofline
+         (chain and block passes timed with HIP events on their stream), the
+         PBWT index build, and the CPU baseline.
 
-Progress goes to stderr; the JSON result is the only stdout line.
+Imputed genotype = one (target sample, untyped marker) pair.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -33,16 +50,217 @@ def log(msg):
 
 
 def load_traffic(kernel_key):
-    """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary
-    (profiles/traffic.json), or None."""
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (profiles/traffic.json, plain 20,000 x 100,000 workload), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel_key, {}).get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
+            return json.load(f)["kernels"].get(kernel_key, {}).get("hbm_bytes_per_launch")
+    except (OSError, ValueError, KeyError):
         return None
 
 
+def sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+# --------------------------------------------------------------------------
+# batch: config-4 job set, batched vs plain
+# --------------------------------------------------------------------------
+def bench_batch(args):
+    import numpy as np
+    import torch
+
+    from swiftimpute_amd import _native, device, synth
+
+    H, M, pool_B, ne = 20000, 200000, 2000, 10000
+    n_jobs = args.jobs
+    log(f"[batch] generating panel H={H} M={M}")
+    pan = synth.make_panel(H, M, seed=12345)
+    log(f"[batch] generating a pool of {pool_B} targets typed at every marker")
+    pool = synth.make_targets(pan, pool_B, 1, seed=12346, with_truth=False)
+    specs = synth.uneven_jobs(n_jobs, M, pool_B)
+    log("[batch] building the PBWT index and state tables for L in {4, 8, 16}")
+    idx16 = device.DeviceIndex(pan.cols, H, 16, keep_prefix=True, with_div=False)
+    idx = {16: idx16, 8: idx16.derive(8), 4: idx16.derive(4)}
+    idx16.drop_prefix()
+    torch.cuda.empty_cache()
+    tables = device.SharedTables(synth.HARDCALL_PALETTE)
+    cm = pan.genetic_cM()
+
+    log(f"[batch] uploading {n_jobs} jobs")
+    jobs = []
+    units = untyped = 0
+    for sp in specs:
+        pitch = (sp.M + 15) // 16 * 16
+        c = synth.region_codes(pool.codes, sp, pitch)
+        untyped += int((c[:, : sp.M] == synth.CODE_MISSING).sum())
+        units += sp.B * sp.M
+        tr = device.Transitions.build(pan.pos[sp.m0:sp.m0 + sp.M],
+                                      None if cm is None else cm[sp.m0:sp.m0 + sp.M], ne)
+        jobs.append(device.RegionJob(idx[sp.L], sp.m0, sp.M, torch.from_numpy(c).cuda(), tr,
+                                     sp.seed, tables))
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[batch] CPU baseline: reference CPU path on a sample of the jobs, {threads} threads")
+        v, dt, n, parts = cpu_baseline_jobs(pan, pool.codes, specs, ne, args.cpu_jobs,
+                                            args.cpu_samples, args.cpu_markers, threads)
+        cpu = {"value": v, "unit": "genotypes/s", "cores": threads, "kind": "port",
+               "sample": f"the first {args.cpu_jobs} jobs of the same job set, each cut to at most "
+                         f"{args.cpu_samples} samples x {args.cpu_markers} markers "
+                         f"({', '.join(parts)}; {n} imputed genotypes in {dt:.1f} s): PBWT sweep "
+                         "+ per-(sample, marker) std::sort selection + L^2 FP32 forward-backward "
+                         "with sqrt(M) replay + sampling, threaded over samples"}
+        log(f"[batch] CPU baseline {v:.4g} genotypes/s")
+    del pool
+    cjobs = [j.c_job() for j in jobs]
+    n_log = sum(j.logspace for j in jobs)
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    lib = _native.hip()
+
+    # plain: one job per launch
+    arr1 = [(_native.FbJob * 1)(cj) for cj in cjobs]
+    ws_plain = max(int(lib.si_fb_workspace_bytes(a, 1)) for a in arr1)
+    ws = torch.empty(ws_plain, dtype=torch.uint8, device="cuda")
+
+    def plain_step():
+        for a in arr1:
+            _native.check(lib.si_fb_run(a, 1, ctypes.c_void_p(ws.data_ptr()), ws_plain, sh),
+                          "si_fb_run")
+
+    plain_steps = max(1, min(args.steps, args.plain_steps))
+    log(f"[batch] plain: {min(args.warmup, 1)} warmup + {plain_steps} timed steps")
+    for _ in range(min(args.warmup, 1)):
+        plain_step()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(plain_steps):
+        plain_step()
+    sync()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / plain_steps
+    ref = [{k: v.clone() for k, v in j.out.items()} for j in jobs]
+    del ws
+    torch.cuda.empty_cache()
+    log(f"[batch] plain {plain_ms:.1f} ms/step")
+
+    # batched: persistent work-queue launches replayed from a hipGraph
+    batch = device.Batch(cjobs)
+    batch.run(use_graph=False, stream=stream)   # eager once (also a correctness reference)
+    sync()
+    log(f"[batch] batched: {args.warmup} warmup + {args.steps} timed graph replays "
+        f"(workspace {batch.ws_bytes / 2**30:.2f} GiB)")
+    for _ in range(args.warmup):
+        batch.run(use_graph=True, stream=stream)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(use_graph=True, stream=stream)
+    sync()
+    batch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"[batch] batched {batch_ms:.1f} ms/step")
+    equal = all(torch.equal(j.out[k], r[k]) for j, r in zip(jobs, ref) for k in r)
+    if not equal:
+        raise SystemExit("batched outputs differ from the plain path")
+    batch.close()
+    return {
+        "cpu_baseline": cpu,
+        "units": units, "untyped": untyped, "batch_ms": batch_ms, "plain_ms": plain_ms,
+        "plain_steps": plain_steps, "n_jobs": n_jobs, "n_logspace_jobs": n_log,
+        "L_counts": {str(L): sum(sp.L == L for sp in specs) for L in (4, 8, 16)},
+    }
+
+
+# --------------------------------------------------------------------------
+# serve: config-5 request stream against a resident panel
+# --------------------------------------------------------------------------
+def bench_serve(args):
+    import numpy as np
+
+    from swiftimpute_amd import serving, synth
+
+    H, M, pool_B, L, ne, te = args.serve_H, args.serve_M, 5000, 8, 10000, 20
+    log(f"[serve] generating panel H={H} M={M}")
+    pan = synth.make_panel(H, M, seed=12345)
+    log(f"[serve] generating a pool of {pool_B} targets typed at every marker")
+    pool = synth.make_targets(pan, pool_B, 1, seed=12346, with_truth=False)
+    reqs = synth.poisson_requests(args.rate, args.serve_seconds, M, pool_B, L=L, typed_every=te)
+    log(f"[serve] {len(reqs)} requests at {args.rate} req/s over {args.serve_seconds} s")
+    codes = [synth.region_codes(pool.codes, sp) for _, sp in reqs]
+    untyped = sum(int((c == synth.CODE_MISSING).sum()) for c in codes)
+    del pool
+    log("[serve] starting the server (PBWT index build on the device)")
+    t0 = time.perf_counter()
+    srv = serving.Server(pan.cols, H, pan.pos, pan.genetic_cM(), L=L, ne=ne, max_streams=4,
+                         max_batch=64, hbm_budget_gb=args.serve_budget_gb,
+                         max_queue_delay_us=args.queue_delay_us)
+    startup_s = time.perf_counter() - t0
+    log(f"[serve] server ready in {startup_s:.1f} s")
+
+    # warm up the kernels once
+    srv.wait(srv.submit(0, 2000, np.full((4, 2000), 3, np.uint8)))
+
+    lat = [None] * len(reqs)
+    ids = [None] * len(reqs)
+    done = threading.Event()
+
+    def waiter():
+        for i in range(len(reqs)):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            lat[i] = srv.wait(ids[i]).latency_s
+        done.set()
+
+    th = threading.Thread(target=waiter, daemon=True)
+    th.start()
+    t_start = time.perf_counter()
+    for i, ((t_arr, sp), c) in enumerate(zip(reqs, codes)):
+        now = time.perf_counter() - t_start
+        if t_arr > now:
+            time.sleep(t_arr - now)
+        ids[i] = srv.submit(sp.m0, sp.m0 + sp.M, c, seed=sp.seed)
+    done.wait()
+    t_online = time.perf_counter() - t_start
+    st_online = srv.stats()
+    lat_ms = np.array(lat) * 1e3
+    log(f"[serve] online: p50 {np.percentile(lat_ms, 50):.1f} ms, p99 {np.percentile(lat_ms, 99):.1f} ms")
+
+    # the same requests, all queued at once (batched-offline)
+    t1 = time.perf_counter()
+    ids2 = [srv.submit(sp.m0, sp.m0 + sp.M, c, seed=sp.seed) for (_, sp), c in zip(reqs, codes)]
+    for i in ids2:
+        srv.wait(i)
+    t_offline = time.perf_counter() - t1
+    st_all = srv.stats()
+    srv.close()
+    log(f"[serve] offline: {t_offline:.2f} s for {len(reqs)} requests")
+    busy_s = t_online - reqs[0][0] if reqs else t_online
+    return {
+        "workload": f"{H} x {M} resident panel; Poisson arrivals at {args.rate} req/s over "
+                    f"{args.serve_seconds} s; cohorts log-uniform 1-5000, regions log-uniform "
+                    f"1000-100000 markers; L={L}, Ne={ne}, typed every {te}th marker",
+        "requests": len(reqs),
+        "rate_req_per_s": args.rate,
+        "p50_ms": float(np.percentile(lat_ms, 50)),
+        "p99_ms": float(np.percentile(lat_ms, 99)),
+        "mean_ms": float(lat_ms.mean()),
+        "genotypes_per_s": untyped / busy_s,
+        "offline_genotypes_per_s": untyped / t_offline,
+        "untyped_genotypes": untyped,
+        "batches_online": st_online["batches"] - 1,
+        "max_batch_requests": st_online["max_batch_requests"],
+        "peak_admitted_bytes": st_all["peak_bytes"],
+        "server_startup_s": startup_s,
+        "note": "online genotypes/s is bounded by the arrival rate; offline = every request "
+                "queued at once on the same server",
+    }
+
+
+# --------------------------------------------------------------------------
+# plain: config-2 cohort, per-kernel roofline, PBWT build, CPU baseline
+# --------------------------------------------------------------------------
 def cpu_baseline(pan, tg, L, ne, n_samples, n_markers, threads):
     """The reference CPU path (oracle.cpu_reference_impute: PBWT sweep,
     per-(sample, marker) std::sort selection, L^2 FP32 forward-backward with
@@ -52,11 +270,9 @@ def cpu_baseline(pan, tg, L, ne, n_samples, n_markers, threads):
     import oracle
     from swiftimpute_amd import synth
 
-    ref = pan.alleles()[:n_markers] if pan.M * pan.H <= 4e8 else None
-    if ref is None:  # unpack only the rows we need
-        W = pan.W
-        b = pan.cols[:n_markers].view(np.uint8).reshape(n_markers, W * 8)
-        ref = np.unpackbits(b, axis=1, bitorder="little")[:, : pan.H].copy()
+    W = pan.W
+    b = pan.cols[:n_markers].view(np.uint8).reshape(n_markers, W * 8)
+    ref = np.unpackbits(b, axis=1, bitorder="little")[:, : pan.H].copy()
     gl = synth.HARDCALL_PALETTE[tg.codes[:n_samples, :n_markers]]
     g = oracle.genetic_map(pan.pos[:n_markers])
     Td, To = oracle.transitions(g, ne)
@@ -67,63 +283,64 @@ def cpu_baseline(pan, tg, L, ne, n_samples, n_markers, threads):
     return untyped / dt, dt, untyped
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg3"])
-    ap.add_argument("--cpu-samples", type=int, default=64)
-    ap.add_argument("--cpu-markers", type=int, default=5000)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    args = ap.parse_args()
-
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "path is not part of this build (see DESIGN.md, sharding).", file=sys.stderr)
-        sys.exit(2)
-
+def cpu_baseline_jobs(pan, pool_codes, specs, ne, n_jobs, max_samples, max_markers, threads):
+    """The reference CPU path on a bounded sample of a job set: the first
+    n_jobs jobs, each cut to its first max_samples samples and first
+    max_markers markers of its region."""
     import numpy as np
+
+    import oracle
+    from swiftimpute_amd import synth
+
+    cm = pan.genetic_cM()
+    total_dt, untyped, parts = 0.0, 0, []
+    for sp in specs[:n_jobs]:
+        Mc, Bc = min(sp.M, max_markers), min(sp.B, max_samples)
+        b = pan.cols[sp.m0:sp.m0 + Mc].view(np.uint8).reshape(Mc, pan.W * 8)
+        ref = np.unpackbits(b, axis=1, bitorder="little")[:, : pan.H].copy()
+        cut = synth.JobSpec(sp.m0, Mc, sp.s0, Bc, sp.L, sp.typed_every, sp.seed)
+        c = synth.region_codes(pool_codes, cut)
+        gl = synth.HARDCALL_PALETTE[c]
+        g = oracle.genetic_map(pan.pos[sp.m0:sp.m0 + Mc], None if cm is None else cm[sp.m0:sp.m0 + Mc])
+        Td, To = oracle.transitions(g, ne)
+        t0 = time.perf_counter()
+        oracle.cpu_reference_impute(ref, gl, Td, To, sp.L, sp.seed, nthreads=threads)
+        total_dt += time.perf_counter() - t0
+        untyped += int((c == synth.CODE_MISSING).sum())
+        parts.append(f"{Bc}x{Mc} L={sp.L}")
+    return untyped / total_dt, total_dt, untyped, parts
+
+
+def bench_plain(args):
     import torch
 
     from swiftimpute_amd import _native, device, synth
 
-    w = synth.WORKLOADS[args.workload]
+    w = synth.WORKLOADS["cfg2"]
     H, M, B, L, ne, te = w["H"], w["M"], w["B"], w["L"], w["ne"], w["typed_every"]
-    threads = min(16, os.cpu_count() or 1)
-    torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
-
-    log(f"generating panel H={H} M={M}")
-    pan = synth.make_panel(H, M, seed=12345, spacing_mode=w["spacing_mode"])
-    log(f"generating {B} targets typed every {te}")
+    log(f"[plain] generating panel H={H} M={M} and {B} targets typed every {te}")
+    pan = synth.make_panel(H, M, seed=12345)
     tg = synth.make_targets(pan, B, te, seed=12346, with_truth=False)
     untyped = int((tg.codes[:, :M] == synth.CODE_MISSING).sum())
 
-    # ---- index build (PBWT sweep + state table), timed once
-    log("building the PBWT index on the GPU")
+    log("[plain] building the PBWT index on the GPU")
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-    torch.cuda.synchronize()
+    sync()
     ev[0].record(stream)
     idx = device.DeviceIndex(pan.cols, H, L, stream=stream)
     ev[1].record(stream)
-    torch.cuda.synchronize()
+    sync()
     index_ms = ev[0].elapsed_time(ev[1])
-    W = (H + 63) // 64
-    pbwt_bytes = M * (W * 8 + 4 * H + 4 * H)
-    log(f"index built in {index_ms:.1f} ms")
-    idx.prefix = None
-    idx.div = None
+    pbwt_bytes = M * (pan.W * 8 + 4 * H + 4 * H)
+    log(f"[plain] index built in {index_ms:.1f} ms")
+    idx.drop_prefix()
     torch.cuda.empty_cache()
 
-    per_marker = w["spacing_mode"] != 0
-    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne, per_marker=per_marker)
-    codes = torch.from_numpy(tg.codes).cuda()
-    job = device.Job(idx, codes, synth.HARDCALL_PALETTE, tr, seed=12345)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne)
+    job = device.Job(idx, torch.from_numpy(tg.codes).cuda(), synth.HARDCALL_PALETTE, tr, seed=12345)
     job.alloc_outputs(ds=True, ap=False, hap=True, prob=False, loglik=False)
-    cj = job.c_job()
-    import ctypes
-    arr = (_native.FbJob * 1)(cj)
+    arr = (_native.FbJob * 1)(job.c_job())
     lib = _native.hip()
     ws_bytes = int(lib.si_fb_workspace_bytes(arr, 1))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
@@ -141,28 +358,23 @@ def main():
         if evs:
             evs[2].record(stream)
 
-    log(f"warmup {args.warmup} steps")
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    sync()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    log(f"timing {args.steps} steps")
-    torch.cuda.synchronize()
+    log(f"[plain] timing {args.steps} steps")
     t0 = time.perf_counter()
     for k in range(args.steps):
         step(evs[k])
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1000.0 / args.steps
+    sync()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
     chain_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
     block_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
-    log(f"{ms:.3f} ms/step (chain {chain_ms:.3f} ms, block {block_ms:.3f} ms)")
-
-    # sanity of the outputs of the last step (size-independent properties)
+    log(f"[plain] {ms:.3f} ms/step (chain {chain_ms:.3f} ms, block {block_ms:.3f} ms)")
     ds = job.out.ds
     assert torch.isfinite(ds).all().item() and float(ds.min()) >= -1e-6 and float(ds.max()) <= 2 + 1e-5
-    hap = job.out.hap
-    assert int(hap.max()) <= 1
+    assert int(job.out.hap.max()) <= 1
+    del ws
 
     units = B * M
     block_flops = 12.0 * L * units   # forward replay 4L + backward 4L + posterior/dosage 4L
@@ -170,31 +382,16 @@ def main():
     dom, dom_ms, dom_flops = (("fb_block_kernel", block_ms, block_flops) if block_ms >= chain_ms
                               else ("fb_chain_kernel", chain_ms, chain_flops))
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
-    result = {
-        "metric": "imputed genotypes/s",
-        "value": untyped / (ms * 1e-3),
-        "unit": "genotypes/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{args.workload}: {H} haplotypes x {M} markers panel, {B} targets typed "
-                        f"every {te}th marker, L={L}, Ne={ne}, plain one-job-per-launch",
-            "haplotypes": H, "markers": M, "samples": B, "states": L, "ne": ne,
-            "typed_every": te, "transitions": "per-marker" if per_marker else "first-interval",
-            "form": "plain",
-        },
+    out = {
+        "workload": f"{H} haplotypes x {M} markers panel, {B} targets typed every {te}th marker, "
+                    f"L={L}, Ne={ne}, plain one-job-per-launch",
+        "genotypes_per_s": untyped / (ms * 1e-3),
         "all_marker_genotypes_per_s": units / (ms * 1e-3),
+        "ms_per_step": ms,
         "passes_ms": {"fb_chain_kernel": chain_ms, "fb_block_kernel": block_ms},
         "roofline": {
             "bound": "valu",
-            "kernel": dom,
+            "kernel": dom + " (plain 20,000 x 100,000 workload)",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -208,23 +405,127 @@ def main():
             "bytes": pbwt_bytes,
             "achieved_GBps_incl_state_table": pbwt_bytes / (index_ms * 1e-3) / 1e9,
             "peak_GBps": PEAK_HBM_GBS,
+            "frac": pbwt_bytes / (index_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
         },
     }
-    del ws
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and "batch" not in args.sections.split(","):
+        threads = min(16, os.cpu_count() or 1)
         ns, nm = min(args.cpu_samples, B), min(args.cpu_markers, M)
-        log(f"CPU baseline: reference CPU path on {ns} samples x {nm} markers, {threads} threads")
+        log(f"[plain] CPU baseline: reference CPU path on {ns} samples x {nm} markers, "
+            f"{threads} threads")
         v, dt, n = cpu_baseline(pan, tg, L, ne, ns, nm, threads)
-        result["cpu_baseline"] = {
+        out["cpu_baseline"] = {
             "value": v, "unit": "genotypes/s", "cores": threads, "kind": "port",
-            "sample": f"first {ns} targets x first {nm} markers of the same panel/cohort "
-                      f"({n} imputed genotypes in {dt:.1f} s); PBWT sweep + per-(sample, marker) "
-                      "std::sort selection + L^2 FP32 forward-backward with sqrt(M) replay + "
-                      "sampling; the replay cost per marker grows with sqrt(M), so the full "
-                      "100k-marker run would be slower per genotype",
+            "sample": f"first {ns} targets x first {nm} markers of the 20,000 x 100,000 panel "
+                      f"and cohort ({n} imputed genotypes in {dt:.1f} s); PBWT sweep + "
+                      "per-(sample, marker) std::sort selection + L^2 FP32 forward-backward "
+                      "with sqrt(M) replay + sampling",
         }
-        log(f"CPU baseline {v:.4g} genotypes/s")
-    print(json.dumps(result), flush=True)
+        log(f"[plain] CPU baseline {v:.4g} genotypes/s")
+    return out
+
+
+def copy_calibration():
+    """A plain device copy of 1 GiB (three launches), the reference for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    sync()
+    print(json.dumps({"copy_bytes_per_launch": n * 4}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sections", default="batch,serve,plain")
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="timed steps of the one-job-per-launch comparison")
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-H", type=int, default=40000)
+    ap.add_argument("--serve-M", type=int, default=500000)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--queue-delay-us", type=int, default=2000)
+    ap.add_argument("--cpu-jobs", type=int, default=8)
+    ap.add_argument("--cpu-samples", type=int, default=16)
+    ap.add_argument("--cpu-markers", type=int, default=4000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the multi-GPU "
+              "path is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.set_device(0)
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    sections = [s for s in args.sections.split(",") if s]
+    res = {}
+    for s in sections:
+        res[s] = {"batch": bench_batch, "serve": bench_serve, "plain": bench_plain}[s](args)
+        torch.cuda.empty_cache()
+
+    out = {
+        "metric": "imputed genotypes/s, batched vs plain kernel on 1 GPU; p50/p99 request latency",
+        "unit": "genotypes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+    }
+    if "batch" in res:
+        b = res["batch"]
+        out.update({
+            "value": b["untyped"] / (b["batch_ms"] * 1e-3),
+            "ms_per_step": b["batch_ms"],
+            "config": {
+                "workload": f"{b['n_jobs']} uneven jobs on one 20000 x 200000 panel: regions "
+                            "log-uniform 2000-80000 markers, cohorts log-uniform 1-2000 samples, "
+                            "L in {4,8,16}, typed 1/10, 1/20 or 1/50, Ne=10000; batched "
+                            "persistent work-queue launches replayed from a hipGraph",
+                "jobs": b["n_jobs"], "haplotypes": 20000, "markers": 200000,
+                "sample_marker_units": b["units"], "untyped_genotypes": b["untyped"],
+                "jobs_per_L": b["L_counts"], "logspace_jobs": b["n_logspace_jobs"],
+                "form": "batched",
+            },
+            "batched_vs_plain": {
+                "plain_genotypes_per_s": b["untyped"] / (b["plain_ms"] * 1e-3),
+                "plain_ms_per_step": b["plain_ms"],
+                "plain_timed_steps": b["plain_steps"],
+                "speedup": b["plain_ms"] / b["batch_ms"],
+                "outputs_bit_equal": True,
+            },
+        })
+    elif "plain" in res:
+        p = res["plain"]
+        out.update({"value": p["genotypes_per_s"], "ms_per_step": p["ms_per_step"],
+                    "config": {"workload": p["workload"], "form": "plain"}})
+    if "batch" in res and res["batch"]["cpu_baseline"]:
+        out["cpu_baseline"] = res["batch"]["cpu_baseline"]
+    if "serve" in res:
+        out["serving"] = res["serve"]
+    if "plain" in res:
+        p = dict(res["plain"])
+        out["roofline"] = p.pop("roofline")
+        out["pbwt"] = p.pop("pbwt")
+        if "cpu_baseline" in p:
+            out["cpu_baseline"] = p.pop("cpu_baseline")
+        out["plain_20000x100000"] = p
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -145,6 +145,29 @@ class DeviceIndex:
             self.prefix = None
             self.div = None
 
+    def derive(self, L: int, mode: int = 0, stream=None) -> "DeviceIndex":
+        """Another state table (L states) over this index's PBWT arrays (which
+        must have been kept), sharing the resident panel and prefix rows."""
+        if self.prefix is None and mode == 0:
+            raise ValueError("derive() needs the prefix rows (keep_prefix=True)")
+        d = object.__new__(DeviceIndex)
+        d.H, d.M, d.L, d.Lp = self.H, self.M, L, padded_states(L)
+        d.cols, d.miss, d.prefix, d.div = self.cols, self.miss, self.prefix, self.div
+        dev = self.cols.device
+        d.perm = select_perm(self.H, L)
+        d.states = torch.empty((self.M, L), dtype=torch.int32, device=dev)
+        d.sa = torch.empty((self.M, d.Lp), dtype=torch.uint8, device=dev)
+        d.mmask = torch.empty((self.M,), dtype=torch.uint8, device=dev)
+        check(hip().si_state_table(_ptr(self.prefix), _ptr(self.cols), _ptr(self.miss), self.H,
+                                   self.M, L, mode, d.perm.ctypes.data_as(_native._u32p),
+                                   _ptr(d.states), _ptr(d.sa), _ptr(d.mmask),
+                                   _stream_handle(stream)), "si_state_table")
+        return d
+
+    def drop_prefix(self):
+        self.prefix = None
+        self.div = None
+
     def state_alleles(self):
         """(a0, a1) [M][L] uint8 raw alleles of the selected state pairs, from
         the device state table (for the oracle)."""
@@ -295,4 +318,53 @@ class SharedTables:
     def apply(self, j: FbJob) -> FbJob:
         j.eps, j.emax, j.palette = _ptr(self.eps), _ptr(self.emax), _ptr(self.pal)
         j.n_palette = self.palette.shape[0]
+        return j
+
+
+class RegionJob:
+    """A job over the marker region [m0, m0 + M) of a resident index: its
+    state table and transition steps are slices of the panel-wide ones, its
+    codes a [B][pitch] device tensor.  Outputs (device tensors) are allocated
+    for the names in `outputs`.  logspace=None picks the log-space kernel when
+    a switch probability the job uses is below the linear kernel's range."""
+
+    def __init__(self, index: DeviceIndex, m0: int, M: int, codes: torch.Tensor,
+                 trans: Transitions, seed: int, tables: "SharedTables", logspace=None,
+                 outputs=("ds", "hap"), rng_stream: int = 0, sample_base: int = 0):
+        if m0 < 0 or M < 1 or m0 + M > index.M:
+            raise ValueError("region outside the panel")
+        if codes.dim() != 2 or codes.shape[1] < M or codes.shape[1] % 16:
+            raise ValueError("codes must be [B][pitch] with pitch >= M, a multiple of 16")
+        self.index, self.m0, self.M, self.codes = index, m0, M, codes
+        self.B = codes.shape[0]
+        self.trans, self.seed, self.tables = trans, seed, tables
+        self.logspace = (trans.min_switch() < 1e-30) if logspace is None else bool(logspace)
+        self.rng_stream, self.sample_base = rng_stream, sample_base
+        dev = codes.device
+        B, L = self.B, index.L
+        shapes = {"ds": ((B, M), torch.float32), "ap": ((B, M, 2), torch.float32),
+                  "hap": ((B, 2, M), torch.uint8), "prob": ((B, M, L), torch.float32),
+                  "loglik": ((B,), torch.float64)}
+        self.out = {k: torch.empty(shapes[k][0], dtype=shapes[k][1], device=dev) for k in outputs}
+
+    def c_job(self) -> FbJob:
+        if not self.logspace and self.trans.min_switch() < 1e-30:
+            raise ValueError("a switch probability e^To is below the linear-space kernel's range; "
+                             "use logspace=True")
+        ix = self.index
+        j = FbJob()
+        j.n_samples, j.n_markers, j.n_states = self.B, self.M, ix.L
+        j.flags = ((_native.SI_FB_PER_MARKER if self.trans.per_marker else 0)
+                   | (_native.SI_FB_LOGSPACE if self.logspace else 0))
+        j.code_pitch = self.codes.shape[1]
+        j.rng_stream, j.sample_base, j.seed = self.rng_stream, self.sample_base, self.seed
+        s = self.trans.step
+        j.step = StepCoef(float(s[0]), float(s[1]), float(s[2]), float(s[3]))
+        j.steps = _ptr(self.trans.steps_dev)
+        j.sa = ctypes.c_void_p(ix.sa.data_ptr() + self.m0 * ix.Lp)
+        j.mmask = ctypes.c_void_p(ix.mmask.data_ptr() + self.m0)
+        j.codes = _ptr(self.codes)
+        self.tables.apply(j)
+        for k in ("ds", "ap", "hap", "prob", "loglik"):
+            setattr(j, k, _ptr(self.out.get(k)))
         return j

@@ -111,3 +111,75 @@ WORKLOADS = {
     "cfg2": dict(H=20000, M=100000, B=10000, L=16, ne=20000, typed_every=20, spacing_mode=0),
     "cfg3": dict(H=8000, M=100000, B=2000, L=8, ne=10000, typed_every=20, spacing_mode=1),
 }
+
+
+# --------------------------------------------------------------------------
+# Uneven job sets and request streams over one shared panel.  A job is a
+# contiguous marker region of the panel and a contiguous slice of a target
+# pool typed at every marker; the job's own typed density keeps every
+# typed_every-th marker of its region (counted from the region start) and
+# masks the rest.
+# --------------------------------------------------------------------------
+@dataclass
+class JobSpec:
+    m0: int           # first marker of the region
+    M: int            # region length
+    s0: int           # first sample of the cohort slice of the pool
+    B: int            # cohort size
+    L: int            # states
+    typed_every: int  # typed-marker density 1/typed_every
+    seed: int         # Philox seed of the job's sampling
+
+
+def _log_uniform(rng, lo, hi):
+    return int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+
+
+def uneven_jobs(n_jobs: int, panel_M: int, pool_B: int, m_lo: int = 2000, m_hi: int = 80000,
+                b_lo: int = 1, b_hi: int = 2000, Ls=(4, 8, 16), densities=(10, 20, 50),
+                seed: int = 12345) -> list:
+    """Config-4 style job set: region length and cohort size log-uniform,
+    L and typed density uniform over the given choices, region start uniform
+    in the panel; job j draws from seed + j."""
+    out = []
+    for j in range(n_jobs):
+        rng = np.random.default_rng(seed + j)
+        M = min(_log_uniform(rng, m_lo, m_hi), panel_M)
+        B = min(_log_uniform(rng, b_lo, b_hi), pool_B)
+        m0 = int(rng.integers(0, panel_M - M + 1))
+        s0 = int(rng.integers(0, pool_B - B + 1))
+        L = int(rng.choice(Ls))
+        te = int(rng.choice(densities))
+        out.append(JobSpec(m0, M, s0, B, L, te, seed + j))
+    return out
+
+
+def poisson_requests(rate: float, duration_s: float, panel_M: int, pool_B: int,
+                     m_lo: int = 1000, m_hi: int = 100000, b_lo: int = 1, b_hi: int = 5000,
+                     L: int = 8, typed_every: int = 20, seed: int = 12345) -> list:
+    """Config-5 style request stream: Poisson arrivals at `rate` per second
+    over `duration_s`; cohort and region log-uniform.  Returns
+    [(arrival_s, JobSpec)]."""
+    rng = np.random.default_rng(seed)
+    t, out, j = 0.0, [], 0
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t >= duration_s:
+            return out
+        M = min(_log_uniform(rng, m_lo, m_hi), panel_M)
+        B = min(_log_uniform(rng, b_lo, b_hi), pool_B)
+        m0 = int(rng.integers(0, panel_M - M + 1))
+        s0 = int(rng.integers(0, pool_B - B + 1))
+        out.append((t, JobSpec(m0, M, s0, B, L, typed_every, seed + 1 + j)))
+        j += 1
+
+
+def region_codes(pool: np.ndarray, spec: JobSpec, pitch: int | None = None) -> np.ndarray:
+    """[B][pitch] codes of a job from a pool typed at every marker: markers of
+    the region with (m - m0) % typed_every != 0 become missing (code 3)."""
+    M = spec.M
+    if pitch is None:
+        pitch = M
+    c = np.full((spec.B, pitch), CODE_MISSING, np.uint8)
+    c[:, :M:spec.typed_every] = pool[spec.s0:spec.s0 + spec.B, spec.m0:spec.m0 + M:spec.typed_every]
+    return c
