@@ -117,7 +117,8 @@ def bench_batch(args):
     del pool
     cjobs = [j.c_job() for j in jobs]
     n_log = sum(j.logspace for j in jobs)
-    stream = torch.cuda.current_stream()
+    sync()
+    stream = torch.cuda.Stream()   # graph capture needs a created (non-null) stream
     sh = ctypes.c_void_p(stream.cuda_stream)
     lib = _native.hip()
 
