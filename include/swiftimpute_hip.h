@@ -63,11 +63,21 @@ uint32_t si_padded_states(uint32_t L);
 /* ------------------------------------------------------------------ PBWT --
  * Replaces PBWTBuilder::build / build_parallel / build_marker
  * (src/pbwt/pbwt_index.cpp:95-247, declared src/pbwt/pbwt_index.hpp:105-135),
- * which the reference runs on the host.  div may be NULL (not written).
+ * which the reference runs on the host.
+ *   cols   [M][ceil(H/64)] device, bit h of word h/64 set iff allele != 0
+ *   prefix [M][H] uint32 device; div [M][H] uint32 device or NULL (not written)
+ * H <= 65536.  The sweep works in chunks of markers (default 512); workspace
+ * (si_pbwt_workspace_bytes, device) may be NULL, in which case the call takes
+ * stream-ordered scratch of its own.  Asynchronous on `stream`.
  */
 size_t si_pbwt_workspace_bytes(uint32_t H, uint32_t M);
 int si_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
                   uint32_t* div, void* workspace, size_t workspace_bytes, void* stream);
+/* The same with an explicit chunk length (markers per chunk, >= 1). */
+size_t si_pbwt_chunked_workspace_bytes(uint32_t H, uint32_t M, uint32_t chunk);
+int si_pbwt_sweep_chunked(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
+                          uint32_t* div, uint32_t chunk, void* workspace, size_t workspace_bytes,
+                          void* stream);
 
 /* ------------------------------------------------------------- selection --
  * Host: the permutation libstdc++ std::sort applies to H equal keys under the

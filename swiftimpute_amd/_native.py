@@ -22,6 +22,11 @@ _f32p = ctypes.POINTER(ctypes.c_float)
 _f64p = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 
+SI_OK = 0
+SI_ERR_ARG = -1
+SI_ERR_DEVICE = -2
+SI_ERR_UNSUPPORTED = -3
+SI_ERR_WORKSPACE = -4
 SI_FB_PER_MARKER = 1
 SI_FB_ARGMAX = 2
 SI_FB_LOGSPACE = 4
@@ -72,6 +77,10 @@ HIP_SYMBOLS = [
     ("si_pbwt_workspace_bytes", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32]),
     ("si_pbwt_sweep", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
                                      ctypes.c_size_t, _vp]),
+    ("si_pbwt_chunked_workspace_bytes", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32,
+                                                          ctypes.c_uint32]),
+    ("si_pbwt_sweep_chunked", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp,
+                                             ctypes.c_uint32, _vp, ctypes.c_size_t, _vp]),
     ("si_select_perm", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p]),
     ("si_state_table", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.c_uint32, _u32p, _vp, _vp, _vp,

@@ -36,7 +36,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
               f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["hip/capi.hip", "hip/index.hip", "hip/hmm.hip"]
+HIP_SOURCES = ["hip/capi.hip", "hip/index.hip", "hip/pbwt.hip", "hip/hmm.hip"]
 HIP_HEADERS = ["hip/si_device.hpp", "hip/si_internal.hpp"]
 API_SOURCES = ["api/panel.cpp", "api/imputer.cpp", "api/synth.cpp", "api/server.cpp"]
 

@@ -55,21 +55,42 @@ uint32_t si_padded_states(uint32_t L) {
 
 // ------------------------------------------------------------------- PBWT --
 size_t si_pbwt_workspace_bytes(uint32_t H, uint32_t M) {
-    (void)H;
-    (void)M;
-    return 0;
+    return si::pbwt_workspace_bytes(H, M, si::pbwt_default_chunk(H, M));
+}
+
+size_t si_pbwt_chunked_workspace_bytes(uint32_t H, uint32_t M, uint32_t chunk) {
+    return si::pbwt_workspace_bytes(H, M, chunk);
+}
+
+int si_pbwt_sweep_chunked(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
+                          uint32_t* div, uint32_t chunk, void* workspace, size_t workspace_bytes,
+                          void* stream) {
+    if (H == 0) return fail(SI_ERR_ARG, "si_pbwt_sweep: H must be positive");
+    if (H > si::kPbwtMaxH)
+        return fail(SI_ERR_UNSUPPORTED, "si_pbwt_sweep: H = %u above the supported %u haplotypes", H,
+                    si::kPbwtMaxH);
+    if (chunk == 0) return fail(SI_ERR_ARG, "si_pbwt_sweep: chunk must be positive");
+    if (M == 0) return SI_OK;
+    if (!cols || !prefix) return fail(SI_ERR_ARG, "si_pbwt_sweep: cols and prefix are required");
+    auto st = static_cast<hipStream_t>(stream);
+    const size_t need = si::pbwt_workspace_bytes(H, M, chunk);
+    void* ws = workspace;
+    if (!ws) {  // stream-ordered scratch of the call's own
+        if (hipMallocAsync(&ws, need, st) != hipSuccess)
+            return fail(SI_ERR_DEVICE, "si_pbwt_sweep: workspace allocation of %zu bytes failed", need);
+    } else if (workspace_bytes < need) {
+        return fail(SI_ERR_WORKSPACE, "si_pbwt_sweep: workspace of %zu bytes, %zu needed",
+                    workspace_bytes, need);
+    }
+    const int rc = si::launch_pbwt_sweep(cols, H, M, prefix, div, chunk, ws, need, st);
+    if (!workspace) (void)hipFreeAsync(ws, st);
+    return device_status(rc, "si_pbwt_sweep");
 }
 
 int si_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix, uint32_t* div,
                   void* workspace, size_t workspace_bytes, void* stream) {
-    (void)workspace;
-    (void)workspace_bytes;
-    if (H == 0) return fail(SI_ERR_ARG, "si_pbwt_sweep: H must be positive");
-    if (M == 0) return SI_OK;
-    if (!cols || !prefix) return fail(SI_ERR_ARG, "si_pbwt_sweep: cols and prefix are required");
-    return device_status(
-        si::launch_pbwt_sweep(cols, H, M, prefix, div, static_cast<hipStream_t>(stream)),
-        "si_pbwt_sweep");
+    return si_pbwt_sweep_chunked(cols, H, M, prefix, div, si::pbwt_default_chunk(H, M), workspace,
+                                 workspace_bytes, stream);
 }
 
 // -------------------------------------------------------------- selection --

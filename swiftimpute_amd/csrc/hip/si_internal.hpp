@@ -69,8 +69,12 @@ struct BatchClass {
 void fb_batch_grid(BatchClass& c);
 int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, hipStream_t stream);
 
+// PBWT prefix/divergence construction (pbwt.hip): chunks of C markers.
+constexpr uint32_t kPbwtMaxH = 65536;
+uint32_t pbwt_default_chunk(uint32_t H, uint32_t M);
+size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C);
 int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
-                      uint32_t* div, hipStream_t stream);
+                      uint32_t* div, uint32_t C, void* ws, size_t ws_bytes, hipStream_t stream);
 int launch_state_table(const uint32_t* prefix, const uint64_t* cols, const uint64_t* miss,
                        uint32_t H, uint32_t M, uint32_t L, uint32_t Lp, uint32_t mode,
                        const uint32_t* perm_host, uint32_t* states, uint8_t* sa, uint8_t* mmask,

@@ -117,6 +117,24 @@ class Transitions:
 
 
 # --------------------------------------------------------------------- index
+def pbwt_sweep(cols: np.ndarray, H: int, chunk: int | None = None, with_div: bool = True,
+               stream=None, device=DEV):
+    """PBWT prefix (and divergence) rows [M][H] int32 on the device from a
+    bit-packed panel; chunk=None uses the library's default chunk length."""
+    M = cols.shape[0]
+    c = torch.from_numpy(np.ascontiguousarray(cols).view(np.int64)).to(device)
+    prefix = torch.empty((M, H), dtype=torch.int32, device=device)
+    div = torch.empty((M, H), dtype=torch.int32, device=device) if with_div else None
+    sh = _stream_handle(stream)
+    if chunk is None:
+        check(hip().si_pbwt_sweep(_ptr(c), H, M, _ptr(prefix), _ptr(div), None, 0, sh),
+              "si_pbwt_sweep")
+    else:
+        check(hip().si_pbwt_sweep_chunked(_ptr(c), H, M, _ptr(prefix), _ptr(div), chunk, None, 0,
+                                          sh), "si_pbwt_sweep_chunked")
+    return prefix, div
+
+
 class DeviceIndex:
     """Resident panel, PBWT index and state table (Imputer::build_index)."""
 

@@ -52,6 +52,22 @@ def test_api_library_exports_server_symbols():
     assert lib.si_server_submit(None, 0, 1, 1, None, 0, None) != 0
 
 
+def test_pbwt_argument_errors():
+    lib = _native.hip()
+    one = ctypes.c_void_p(8)  # never dereferenced: the checks come first
+    assert lib.si_pbwt_sweep(one, 0, 5, one, None, None, 0, None) == _native.SI_ERR_ARG
+    assert lib.si_pbwt_sweep(one, 70000, 5, one, None, None, 0, None) == _native.SI_ERR_UNSUPPORTED
+    assert b"65536" in lib.si_last_error()
+    assert lib.si_pbwt_sweep_chunked(one, 10, 5, one, None, 0, None, 0, None) == _native.SI_ERR_ARG
+    assert lib.si_pbwt_sweep(None, 10, 5, one, None, None, 0, None) == _native.SI_ERR_ARG
+    assert lib.si_pbwt_sweep(one, 10, 0, one, None, None, 0, None) == 0  # nothing to do
+    # workspace: three phases' buffers, growing with the number of chunks
+    w512 = lib.si_pbwt_chunked_workspace_bytes(20000, 100000, 512)
+    w64 = lib.si_pbwt_chunked_workspace_bytes(20000, 100000, 64)
+    assert lib.si_pbwt_workspace_bytes(20000, 100000) == w512
+    assert w64 > w512 >= 196 * 3 * 20000 * 4
+
+
 def test_abi_version_and_padding():
     lib = _native.hip()
     assert lib.si_abi_version() == 1

@@ -77,6 +77,43 @@ def test_pbwt_random(M, H):
     assert np.array_equal(idx.div.cpu().numpy().view(np.uint32), d)
 
 
+def _dup_panel(rng, M, H, n_distinct=6, flip=0.01):
+    """Panels of near-identical haplotypes: large groups in every chunk."""
+    base = (rng.random((M, n_distinct)) < 0.5).astype(np.uint8)
+    ref = base[:, rng.integers(0, n_distinct, H)]
+    ref[rng.random((M, H)) < flip] ^= 1
+    return ref
+
+
+@pytest.mark.parametrize("M,H,chunk", [(1, 1, 1), (5, 2, 1), (40, 63, 7), (33, 64, 4),
+                                       (50, 65, 5), (64, 300, 3), (100, 1100, 16), (37, 1025, 2),
+                                       (300, 4432, 64), (30, 40000, 7), (9, 65536, 4),
+                                       (700, 2000, 512), (1200, 500, 100)])
+@pytest.mark.parametrize("dup", [False, True])
+def test_pbwt_chunked(M, H, chunk, dup):
+    rng = np.random.default_rng(M * 7 + H + chunk + dup)
+    if dup:
+        ref = _dup_panel(rng, M, H)
+    else:
+        ref = (rng.random((M, H)) < rng.uniform(0.05, 0.9)).astype(np.uint8)
+    ref[rng.random((M, H)) < 0.01] = 255      # missing alleles sort with the ones
+    cols, _ = pack_cols(ref)
+    prefix, div = device.pbwt_sweep(cols, H, chunk=chunk)
+    torch.cuda.synchronize()
+    p, d = oracle.pbwt_build(ref)
+    assert np.array_equal(prefix.cpu().numpy().view(np.uint32), p)
+    assert np.array_equal(div.cpu().numpy().view(np.uint32), d)
+
+
+def test_pbwt_default_chunk_matches_all_chunk_lengths():
+    # size-independent property: every chunk length gives the same rows
+    pan = synth.make_panel(3000, 2100, seed=99)
+    ref_p, _ = device.pbwt_sweep(pan.cols, pan.H, with_div=False)
+    for chunk in (1, 3, 33, 256, 1024, 4096):
+        p, _ = device.pbwt_sweep(pan.cols, pan.H, chunk=chunk, with_div=False)
+        assert torch.equal(p, ref_p), chunk
+
+
 def test_pbwt_config1_panel():
     pan = synth.make_panel(4432, 1644, seed=12345)
     ref = pan.alleles()
