@@ -57,8 +57,17 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
 int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const uint8_t* codes,
                      uint64_t seed, uint64_t* request_id);
 
+/* The same, with the results written straight into caller buffers hap
+ * [B][2][m1-m0] and ds [B][m1-m0] (either may be NULL), which must stay valid
+ * until si_server_wait returns for this request; saves one host copy. */
+int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
+                          const uint8_t* codes, uint64_t seed, uint8_t* hap, float* ds,
+                          uint64_t* request_id);
+
 /* Wait for a request and copy its results: hap [B][2][M'] and ds [B][M']
- * (either may be NULL); latency_s receives arrival-to-result seconds. */
+ * (either may be NULL; ignored for si_server_submit_into requests, whose
+ * results are already in place); latency_s receives arrival-to-result
+ * seconds. */
 int si_server_wait(si_server* s, uint64_t request_id, uint8_t* hap, float* ds, double* latency_s);
 
 /* Counters since creation: completed requests, batches, imputed genotypes

@@ -95,14 +95,19 @@ struct Request {
     size_t off_codes = 0, off_hap = 0, off_ds = 0, own_bytes = 0;
     std::vector<uint8_t> hap;
     std::vector<float> ds;
+    uint8_t* dst_hap = nullptr;  // caller buffers (si_server_submit_into), else hap/ds above
+    float* dst_ds = nullptr;
+    bool into = false;
     bool done = false;
     int status = 0;
 };
 
+enum class SlotState { Free, Running, Copying };
+
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    bool busy = false;
+    SlotState state = SlotState::Free;
     std::vector<std::shared_ptr<Request>> reqs;
     si_fb_batch* batch = nullptr;
     size_t ws_off = 0, ws_bytes = 0;
@@ -135,7 +140,13 @@ struct si_server {
     std::unordered_map<uint64_t, std::shared_ptr<Request>> reqs;
     std::vector<Slot> slots;
     std::thread worker;
+    // host-side result copies run on their own threads, so the scheduler keeps
+    // launching while finished batches are copied out of pinned staging
+    std::vector<std::thread> copiers;
+    std::deque<Slot*> copy_queue;
+    std::condition_variable cv_copy;
     bool stop = false;
+    bool copiers_stop = false;
     uint64_t next_id = 1;
     // finished batch descriptors; released when no batch is in flight, since
     // freeing device memory can wait on every stream of the device
@@ -200,36 +211,61 @@ struct si_server {
         return j;
     }
 
+    // The batch of a slot finished on the device: its device memory is free
+    // again, and its results wait in the slot's pinned staging for a copier.
     void retire(Slot& sl) {
-        // results are in the slot's pinned buffer after the codes region
-        size_t in_bytes = 0;
-        for (auto& r : sl.reqs) in_bytes += size_t(r->B) * r->pitch;
-        size_t o = in_bytes;
-        const auto now = Clock::now();
-        for (auto& r : sl.reqs) {
-            const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
-            r->hap.assign(sl.pinned + o, sl.pinned + o + hb);
-            o += hb;
-            r->ds.resize(size_t(r->B) * r->M);
-            std::memcpy(r->ds.data(), sl.pinned + o, db);
-            o += db;
-            arena.release(r->off_codes, r->own_bytes);
-            r->done = true;
-            r->done_at = now;
-            stats.requests++;
-            stats.genotypes += uint64_t(r->B) * r->M;
-            stats.untyped_genotypes += r->untyped;
-        }
+        for (auto& r : sl.reqs) arena.release(r->off_codes, r->own_bytes);
         arena.release(sl.ws_off, sl.ws_bytes);
         finished.push_back(sl.batch);
         sl.batch = nullptr;
-        sl.reqs.clear();
-        sl.busy = false;
-        cv_done.notify_all();
+        sl.state = SlotState::Copying;
+        copy_queue.push_back(&sl);
+        cv_copy.notify_one();
+    }
+
+    // Copier thread: results from pinned staging to their destinations
+    // (outside the lock), then the requests complete and the slot frees.
+    void copy_loop() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv_copy.wait(lk, [&] { return !copy_queue.empty() || copiers_stop; });
+            if (copy_queue.empty()) return;
+            Slot* sl = copy_queue.front();
+            copy_queue.pop_front();
+            lk.unlock();
+            size_t o = 0;
+            for (auto& r : sl->reqs) o += size_t(r->B) * r->pitch;
+            for (auto& r : sl->reqs) {
+                const size_t hb = size_t(r->B) * 2 * r->M, n = size_t(r->B) * r->M;
+                if (r->into) {
+                    if (r->dst_hap) std::memcpy(r->dst_hap, sl->pinned + o, hb);
+                    if (r->dst_ds) std::memcpy(r->dst_ds, sl->pinned + o + hb, n * sizeof(float));
+                } else {
+                    r->hap.assign(sl->pinned + o, sl->pinned + o + hb);
+                    r->ds.resize(n);
+                    std::memcpy(r->ds.data(), sl->pinned + o + hb, n * sizeof(float));
+                }
+                o += hb + n * sizeof(float);
+            }
+            const auto now = Clock::now();
+            lk.lock();
+            for (auto& r : sl->reqs) {
+                r->done = true;
+                r->done_at = now;
+                stats.requests++;
+                stats.genotypes += uint64_t(r->B) * r->M;
+                stats.untyped_genotypes += r->untyped;
+            }
+            sl->reqs.clear();
+            sl->state = SlotState::Free;
+            cv_done.notify_all();
+            cv_work.notify_one();
+        }
     }
 
     // Admits queued requests (arrival order) into a free slot; false if none.
-    bool launch(Slot& sl) {
+    // Called with the lock held; drops it while staging and enqueueing.
+    bool launch(Slot& sl, std::unique_lock<std::mutex>& lk) {
         std::vector<std::shared_ptr<Request>> take;
         size_t ws_total = 0;
         while (!queue.empty() && take.size() < cfg.max_batch) {
@@ -259,10 +295,23 @@ struct si_server {
             stats.deferred_admissions++;
             return false;
         }
+        sl.state = SlotState::Running;
+        sl.reqs = std::move(take);
+        sl.ws_off = ws_off;
+        sl.ws_bytes = ws_total;
+        stats.batches++;
+        stats.max_batch_requests = std::max<uint64_t>(stats.max_batch_requests, sl.reqs.size());
+        stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena.used());
+        const auto& take_ref = sl.reqs;
         std::vector<si_fb_job> jobs;
+        for (auto& r : take_ref) jobs.push_back(job_of(*r));
+        lk.unlock();  // the slot is ours; staging and launch need no shared state
+        struct Relock {
+            std::unique_lock<std::mutex>& l;
+            ~Relock() { l.lock(); }
+        } relock{lk};
         size_t in_bytes = 0, out_bytes = 0;
-        for (auto& r : take) {
-            jobs.push_back(job_of(*r));
+        for (auto& r : take_ref) {
             in_bytes += size_t(r->B) * r->pitch;
             out_bytes += size_t(r->B) * r->M * (2 + sizeof(float));
         }
@@ -273,7 +322,7 @@ struct si_server {
                 throw std::runtime_error("hipHostMalloc failed");
         }
         size_t o = 0;
-        for (auto& r : take) {
+        for (auto& r : take_ref) {
             std::memcpy(sl.pinned + o, r->codes.data(), r->codes.size());
             if (hipMemcpyAsync(arena.ptr(r->off_codes), sl.pinned + o, r->codes.size(),
                                hipMemcpyHostToDevice, sl.stream) != hipSuccess)
@@ -283,7 +332,7 @@ struct si_server {
         if (si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch) != SI_OK ||
             si_fb_batch_run(sl.batch, arena.ptr(ws_off), ws_total, 0, sl.stream) != SI_OK)
             throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
-        for (auto& r : take) {
+        for (auto& r : take_ref) {
             const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
             (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_hap), hb, hipMemcpyDeviceToHost, sl.stream);
             o += hb;
@@ -291,13 +340,6 @@ struct si_server {
             o += db;
         }
         if (hipEventRecord(sl.done, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
-        sl.reqs = std::move(take);
-        sl.ws_off = ws_off;
-        sl.ws_bytes = ws_total;
-        sl.busy = true;
-        stats.batches++;
-        stats.max_batch_requests = std::max<uint64_t>(stats.max_batch_requests, sl.reqs.size());
-        stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena.used());
         return true;
     }
 
@@ -306,12 +348,13 @@ struct si_server {
         std::unique_lock<std::mutex> lk(mu);
         try {
             for (;;) {
-                bool inflight = false;
+                bool inflight = false, copying = false;
                 for (auto& sl : slots)
-                    if (sl.busy) {
+                    if (sl.state == SlotState::Running) {
                         if (hipEventQuery(sl.done) == hipSuccess) retire(sl);
                         else inflight = true;
                     }
+                for (auto& sl : slots) copying |= sl.state == SlotState::Copying;
                 bool launched = false;
                 // a partial batch launches once its oldest request has waited
                 // max_queue_delay_us (or at shutdown)
@@ -321,14 +364,14 @@ struct si_server {
                      Clock::now() - queue.front()->arrival >=
                          std::chrono::microseconds(cfg.max_queue_delay_us));
                 for (auto& sl : slots)
-                    if (ready && !sl.busy && !queue.empty() && launch(sl)) {
+                    if (ready && sl.state == SlotState::Free && !queue.empty() && launch(sl, lk)) {
                         launched = inflight = true;
                     }
                 if (!inflight && !finished.empty()) {
                     for (auto* b : finished) si_fb_batch_destroy(b);
                     finished.clear();
                 }
-                if (stop && queue.empty() && !inflight) break;
+                if (stop && queue.empty() && !inflight && !copying) break;
                 if (launched) continue;
                 if (inflight) {
                     lk.unlock();
@@ -439,12 +482,15 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
             return fail(SI_ERR_DEVICE, "si_server_create: stream creation failed");
     si_server* raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
+    for (uint32_t i = 0; i < raw->cfg.max_streams; ++i)
+        raw->copiers.emplace_back([raw] { raw->copy_loop(); });
     *out = raw;
     return SI_OK;
 }
 
-int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const uint8_t* codes,
-                     uint64_t seed, uint64_t* id) {
+int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
+                          const uint8_t* codes, uint64_t seed, uint8_t* hap, float* ds,
+                          uint64_t* id) {
     if (!s || !id || m1 <= m0 || m1 > s->M || B == 0 || !codes)
         return fail(SI_ERR_ARG, "si_server_submit: bad arguments");
     auto r = std::make_shared<Request>();
@@ -454,6 +500,9 @@ int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const u
     r->B = B;
     r->pitch = (r->M + 15) / 16 * 16;
     r->seed = seed;
+    r->into = hap != nullptr || ds != nullptr;
+    r->dst_hap = hap;
+    r->dst_ds = ds;
     r->codes.assign(size_t(B) * r->pitch, 3);
     for (uint32_t b = 0; b < B; ++b) {
         const uint8_t* src = codes + size_t(b) * r->M;
@@ -477,6 +526,11 @@ int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const u
     return SI_OK;
 }
 
+int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const uint8_t* codes,
+                     uint64_t seed, uint64_t* id) {
+    return si_server_submit_into(s, m0, m1, B, codes, seed, nullptr, nullptr, id);
+}
+
 int si_server_wait(si_server* s, uint64_t id, uint8_t* hap, float* ds, double* latency_s) {
     if (!s) return fail(SI_ERR_ARG, "si_server_wait: null server");
     std::unique_lock<std::mutex> lk(s->mu);
@@ -486,8 +540,10 @@ int si_server_wait(si_server* s, uint64_t id, uint8_t* hap, float* ds, double* l
     s->cv_done.wait(lk, [&] { return r->done; });
     s->reqs.erase(it);
     if (r->status != 0) return fail(r->status, "request failed: " + s->worker_error);
-    if (hap) std::memcpy(hap, r->hap.data(), r->hap.size());
-    if (ds) std::memcpy(ds, r->ds.data(), r->ds.size() * sizeof(float));
+    if (!r->into) {
+        if (hap) std::memcpy(hap, r->hap.data(), r->hap.size());
+        if (ds) std::memcpy(ds, r->ds.data(), r->ds.size() * sizeof(float));
+    }
     if (latency_s) *latency_s = std::chrono::duration<double>(r->done_at - r->arrival).count();
     return SI_OK;
 }
@@ -507,6 +563,12 @@ int si_server_destroy(si_server* s) {
         s->cv_work.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->copiers_stop = true;
+    }
+    s->cv_copy.notify_all();
+    for (auto& th : s->copiers) th.join();
     delete s;
     return SI_OK;
 }

@@ -84,17 +84,56 @@ __device__ __forceinline__ void lane_emission(float4 e3, uint32_t sa4, float e[4
     }
 }
 
-// Emission-table row for (palette code, class mask of marker m).  Codes at or
-// beyond the palette size are read as the last palette entry, so no input can
-// index past the table.
-__device__ __forceinline__ uint32_t emis_row(const JobView& J, uint32_t P, uint32_t code,
-                                             uint32_t m) {
-    return min(code, P - 1u) * 8u + J.mmask[m];
+// The class-present mask of a marker rides in bit 7 of the first three state
+// bytes of every lane's word (see the state table), so the emission row needs
+// no load of its own.
+__device__ __forceinline__ uint32_t class_mask(uint32_t w) {
+    return ((w >> 7) & 1u) | ((w >> 14) & 2u) | ((w >> 21) & 4u);
 }
 
-// (t_o, u, lto, lu) of the step into marker m.
-__device__ __forceinline__ float4 step_coef(const JobView& J, uint32_t m) {
-    return (J.flags & SI_FB_PER_MARKER) ? J.steps[m] : J.step;
+// Emission values of this lane's four states at a marker with palette code
+// `code` and state word w; returns the row's log offset.  Codes at or beyond
+// the palette size are read as the last palette entry, so no input can index
+// past the table.
+__device__ __forceinline__ float emission(const EmisLds& tab, uint32_t P, uint32_t code, uint32_t w,
+                                          float e[4]) {
+    const uint32_t row = min(code, P - 1u) * 8u + class_mask(w);
+    lane_emission(tab.row[row], w, e);
+    return tab.off[row];
+}
+
+// Component-wise select (keeps a select of two register vectors from being
+// turned into a select of their addresses).
+__device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b) {
+    return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
+}
+
+// One 16-marker block of a lane's inputs: the sample's 16 genotype codes and,
+// per marker, the lane's four state bytes.  Loaded whole (indices clamped in
+// bounds past the last marker), so every load of the block is in flight at
+// once and the steps that follow read registers only.
+struct BlockIn {
+    uint4 cw;
+    uint32_t sa[kBlock];
+    __device__ __forceinline__ void load(const JobView& J, const uint8_t* codes, uint32_t lp4,
+                                         uint32_t r, uint32_t c) {
+        const uint32_t m0 = c * kBlock, ml = J.M - 1u;
+        cw = *reinterpret_cast<const uint4*>(codes + m0);
+        const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(J.sa);
+#pragma unroll
+        for (uint32_t t = 0; t < kBlock; ++t) sa[t] = sa32[size_t(min(m0 + t, ml)) * lp4 + r];
+    }
+    __device__ __forceinline__ uint32_t code(uint32_t t) const {
+        return byte_of((&cw.x)[t >> 2], t & 3);
+    }
+};
+
+// Coefficients of the step into marker m: (t_o, u) for the linear form,
+// (lto, lu) for the log form; per marker (PM) or one for every step.
+template <bool LOG, bool PM>
+__device__ __forceinline__ float2 coef(const JobView& J, uint32_t m) {
+    const float4 k = PM ? J.steps[m] : J.step;
+    return LOG ? make_float2(k.z, k.w) : make_float2(k.x, k.y);
 }
 
 // Orders this wavefront's LDS accesses: lane groups exchange data through LDS
@@ -146,20 +185,20 @@ __device__ __forceinline__ double fwd_init(float4& a, float& ls, const float e[4
 // Step into a marker with emission e / offset off and coefficients k.
 template <int LPS, bool LOG>
 __device__ __forceinline__ double fwd_step(float4& a, float& ls, const float e[4], float off,
-                                           float4 k) {
+                                           float2 k) {
     if constexpr (LOG) {
-        // LSE_i(a_i + To) = To + ls
-        const float lo = k.z + ls;
-        const float x0 = lse3(kFloor, lo, k.w + a.x) + e[0];
-        const float x1 = lse3(kFloor, lo, k.w + a.y) + e[1];
-        const float x2 = lse3(kFloor, lo, k.w + a.z) + e[2];
-        const float x3 = lse3(kFloor, lo, k.w + a.w) + e[3];
+        // k = (lto, lu); LSE_i(a_i + To) = To + ls
+        const float lo = k.x + ls;
+        const float x0 = lse3(kFloor, lo, k.y + a.x) + e[0];
+        const float x1 = lse3(kFloor, lo, k.y + a.y) + e[1];
+        const float x2 = lse3(kFloor, lo, k.y + a.z) + e[2];
+        const float x3 = lse3(kFloor, lo, k.y + a.w) + e[3];
         const float c = group_max<LPS>(fmaxf(fmaxf(x0, x1), fmaxf(x2, x3)));
         a = make_float4(x0 - c, x1 - c, x2 - c, x3 - c);
         ls = log_mass<LPS>(a);
         return double(off) + double(c);
     } else {
-        // a sums to 1, so sum_i a_i t_o = t_o.
+        // k = (t_o, u); a sums to 1, so sum_i a_i t_o = t_o.
         const float x0 = fmaf(k.y, a.x, k.x) * e[0];
         const float x1 = fmaf(k.y, a.y, k.x) * e[1];
         const float x2 = fmaf(k.y, a.z, k.x) * e[2];
@@ -179,7 +218,7 @@ __device__ __forceinline__ double fwd_step(float4& a, float& ls, const float e[4
 // shifted down by sh = -999 - nat, so no value ever leaves a safe range.
 template <int LPS, bool LOG>
 __device__ __forceinline__ void bwd_step(float4& v, double& Bl, const float e[4], float off,
-                                         float4 k) {
+                                         float2 k) {
     if constexpr (LOG) {
         const float y0 = e[0] + v.x, y1 = e[1] + v.y, y2 = e[2] + v.z, y3 = e[3] + v.w;
         const float lw = group_lse<LPS>(y0, y1, y2, y3);
@@ -189,7 +228,7 @@ __device__ __forceinline__ void bwd_step(float4& v, double& Bl, const float e[4]
         const float fl = floor_dominates ? 0.0f : float(arg);
         const float sh = floor_dominates ? float(arg) : 0.0f;
         Bl = floor_dominates ? -999.0 : nat;
-        const float bo = k.z - sh, bu = k.w - lw - sh;
+        const float bo = k.x - sh, bu = k.y - lw - sh;
         v = make_float4(lse3(fl, bo, bu + y0), lse3(fl, bo, bu + y1), lse3(fl, bo, bu + y2),
                         lse3(fl, bo, bu + y3));
     } else {
@@ -234,35 +273,46 @@ __device__ __forceinline__ float4 posterior(float4 a, float4 v) {
 
 // ---------------------------------------------------------------------------
 // One chain (forward or backward) of sample s of job J, by one lane group.
+// The next block's inputs are fetched while the current block is stepped;
+// every step is computed and committed by a select where it lies inside the
+// chain, so the block's 16 steps are straight-line code.
 // ---------------------------------------------------------------------------
-template <int LPS, bool LOG>
+template <int LPS, bool LOG, bool PM>
 __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, uint32_t P,
                                           uint32_t Lp, uint32_t s, bool fwd, uint32_t r) {
     const uint8_t* codes = J.codes + size_t(s) * J.pitch;
-    const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(J.sa);
-    const uint32_t lp4 = Lp / 4;
+    const uint32_t lp4 = Lp / 4, ml = J.M - 1u;
+    BlockIn cur, nxt;
     if (fwd) {
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        cur.load(J, codes, lp4, r, 0);
+        float4 a;
         float ls = 0.0f;
-        double ll = 0.0;
+        double ll;
+        {
+            float e[4];
+            const float off = emission(tab, P, cur.code(0), cur.sa[0], e);
+            ll = fwd_init<LPS, LOG>(a, ls, e, off);
+        }
         for (uint32_t c = 0; c < J.nblk; ++c) {
+            if (c + 1 < J.nblk) nxt.load(J, codes, lp4, r, c + 1);
             const uint32_t m0 = c * kBlock;
-            const uint4 cw = *reinterpret_cast<const uint4*>(codes + m0);
 #pragma unroll
             for (uint32_t t = 0; t < kBlock; ++t) {
                 const uint32_t m = m0 + t;
-                if (m < J.M) {
-                    const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), m);
-                    float e[4];
-                    lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
-                    if (m == 0)
-                        ll += fwd_init<LPS, LOG>(a, ls, e, tab.off[row]);
-                    else
-                        ll += fwd_step<LPS, LOG>(a, ls, e, tab.off[row], step_coef(J, m));
-                    if (t == 0)
-                        reinterpret_cast<float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r] = a;
+                float e[4];
+                const float off = emission(tab, P, cur.code(t), cur.sa[t], e);
+                float4 an = a;
+                float lsn = ls;
+                const double inc = fwd_step<LPS, LOG>(an, lsn, e, off, coef<LOG, PM>(J, min(m, ml)));
+                if (m > 0 && m < J.M) {
+                    a = an;
+                    ls = lsn;
+                    ll += inc;
                 }
+                if (t == 0)
+                    reinterpret_cast<float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r] = a;
             }
+            cur = nxt;
         }
         if (LOG) ll += double(ls);  // log mass of the last normalized vector
         if (r == 0 && J.loglik) J.loglik[s] = ll;
@@ -272,75 +322,101 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
         const size_t last = size_t(s) * J.nblk + (J.nblk - 1);
         reinterpret_cast<float4*>(J.ck_beta)[last * lp4 + r] = v;
         if (r == 0) J.ck_bl[last] = Bl;
+        cur.load(J, codes, lp4, r, J.nblk - 1);
         for (int c = int(J.nblk) - 1; c >= 0; --c) {
+            if (c > 0) nxt.load(J, codes, lp4, r, uint32_t(c - 1));
             const uint32_t m0 = uint32_t(c) * kBlock;
-            const uint4 cw = *reinterpret_cast<const uint4*>(codes + m0);
 #pragma unroll
             for (int t = int(kBlock) - 1; t >= 0; --t) {
                 const uint32_t n = m0 + uint32_t(t);
-                if (n < J.M && n > 0) {
-                    const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), n);
-                    float e[4];
-                    lane_emission(tab.row[row], sa32[size_t(n) * lp4 + r], e);
-                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(J, n));
-                    if (t == 0) {  // v is now the vector at marker m0 - 1, last of block c - 1
-                        const size_t ck = size_t(s) * J.nblk + uint32_t(c - 1);
-                        reinterpret_cast<float4*>(J.ck_beta)[ck * lp4 + r] = v;
-                        if (r == 0) J.ck_bl[ck] = Bl;
-                    }
+                float e[4];
+                const float off = emission(tab, P, cur.code(uint32_t(t)), cur.sa[t], e);
+                float4 vn = v;
+                double Bln = Bl;
+                bwd_step<LPS, LOG>(vn, Bln, e, off, coef<LOG, PM>(J, min(n, ml)));
+                if (n > 0 && n < J.M) {
+                    v = vn;
+                    Bl = Bln;
+                }
+                if (t == 0 && c > 0) {  // v is the vector at marker m0 - 1, last of block c - 1
+                    const size_t ck = size_t(s) * J.nblk + uint32_t(c - 1);
+                    reinterpret_cast<float4*>(J.ck_beta)[ck * lp4 + r] = v;
+                    if (r == 0) J.ck_bl[ck] = Bl;
                 }
             }
+            cur = nxt;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Block c of sample s of job J, by one lane group: replay the forward vector
-// through the block into registers, walk the backward vector down the block
-// writing posteriors into the group's LDS slice, then sample, form dosages
-// and allele probabilities (lane r takes markers r, r + LPS, ... of the block,
-// walking the states in order with a sequential FP32 cumulative sum, which the
-// host replays exactly).  `active` groups do the work; every lane of the
-// wavefront passes the wave_sync.
+// through the block into registers, then walk the backward vector down the
+// block forming posteriors (into the group's LDS slice) and the allele
+// probabilities (group sums).  Then sample: lane r takes markers r, r + LPS,
+// ... of the block and walks the states in order with a sequential FP32
+// cumulative sum, which the host replays exactly.  `active` groups do the
+// work; every lane of the wavefront passes the wave_sync.
 // ---------------------------------------------------------------------------
-template <int LPS, bool LOG>
+template <int LPS, bool LOG, bool PM>
 __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, uint32_t P,
                                           uint32_t Lp, bool active, uint32_t s, uint32_t c,
-                                          uint32_t r, float (*pbuf)[4 * LPS], uint4* rng) {
+                                          uint32_t r, float (*pbuf)[4 * LPS],
+                                          uint32_t (*sbuf)[LPS], uint4* rng) {
+    constexpr uint32_t kPer = kBlock / LPS;  // markers per lane in the sampling phase
     const uint32_t m0 = c * kBlock;
     const uint32_t nm = active ? min(kBlock, J.M - m0) : 0u;
     const uint32_t lp4 = Lp / 4;
-    const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(J.sa);
+    float ap0r[kPer], ap1r[kPer];
     if (active) {
-        const uint4 cw = *reinterpret_cast<const uint4*>(J.codes + size_t(s) * J.pitch + m0);
-        float4 A[kBlock];
-        A[0] = reinterpret_cast<const float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r];
+        BlockIn in;
+        in.load(J, J.codes + size_t(s) * J.pitch, lp4, r, c);
+#pragma unroll
+        for (uint32_t t = 0; t < kBlock; ++t) sbuf[t][r] = in.sa[t];
+        // forward vectors of the block, into this lane's part of the LDS slice
+        float4 a = reinterpret_cast<const float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r];
+        *reinterpret_cast<float4*>(&pbuf[0][4 * r]) = a;
         // the chain pass derived ls from the same vector by the same function
-        float ls = LOG ? log_mass<LPS>(A[0]) : 0.0f;
+        float ls = LOG ? log_mass<LPS>(a) : 0.0f;
 #pragma unroll
         for (uint32_t t = 1; t < kBlock; ++t) {
-            if (t < nm) {
-                const uint32_t m = m0 + t;
-                const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), m);
-                float e[4];
-                lane_emission(tab.row[row], sa32[size_t(m) * lp4 + r], e);
-                A[t] = A[t - 1];
-                fwd_step<LPS, LOG>(A[t], ls, e, tab.off[row], step_coef(J, m));
-            }
+            float e[4];
+            const float off = emission(tab, P, in.code(t), in.sa[t], e);
+            float4 an = a;
+            float lsn = ls;
+            fwd_step<LPS, LOG>(an, lsn, e, off, coef<LOG, PM>(J, min(m0 + t, J.M - 1u)));
+            const bool live = t < nm;
+            a = sel4(live, an, a);
+            ls = live ? lsn : ls;
+            *reinterpret_cast<float4*>(&pbuf[t][4 * r]) = a;
         }
         const size_t ck = size_t(s) * J.nblk + c;
         float4 v = reinterpret_cast<const float4*>(J.ck_beta)[ck * lp4 + r];
         double Bl = J.ck_bl[ck];
 #pragma unroll
         for (int t = int(kBlock) - 1; t >= 0; --t) {
-            if (uint32_t(t) < nm) {
-                *reinterpret_cast<float4*>(&pbuf[t][4 * r]) = posterior<LPS, LOG>(A[t], v);
-                if (t > 0) {
-                    const uint32_t n = m0 + uint32_t(t);
-                    const uint32_t row = emis_row(J, P, byte_of((&cw.x)[t >> 2], t & 3), n);
-                    float e[4];
-                    lane_emission(tab.row[row], sa32[size_t(n) * lp4 + r], e);
-                    bwd_step<LPS, LOG>(v, Bl, e, tab.off[row], step_coef(J, n));
+            const float4 p =
+                posterior<LPS, LOG>(*reinterpret_cast<const float4*>(&pbuf[t][4 * r]), v);
+            const uint32_t w = in.sa[t];
+            const float x0 = ((w & 1u) ? p.x : 0.0f) + ((w & 0x100u) ? p.y : 0.0f) +
+                             ((w & 0x10000u) ? p.z : 0.0f) + ((w & 0x1000000u) ? p.w : 0.0f);
+            const float x1 = ((w & 2u) ? p.x : 0.0f) + ((w & 0x200u) ? p.y : 0.0f) +
+                             ((w & 0x20000u) ? p.z : 0.0f) + ((w & 0x2000000u) ? p.w : 0.0f);
+            const float ap0 = group_sum<LPS>(x0), ap1 = group_sum<LPS>(x1);
+            if (uint32_t(t) < nm) *reinterpret_cast<float4*>(&pbuf[t][4 * r]) = p;
+            if (uint32_t(t) % LPS == r) {
+                ap0r[uint32_t(t) / LPS] = ap0;
+                ap1r[uint32_t(t) / LPS] = ap1;
+            }
+            if (t > 0) {
+                float e[4];
+                const float off = emission(tab, P, in.code(uint32_t(t)), w, e);
+                float4 vn = v;
+                double Bln = Bl;
+                bwd_step<LPS, LOG>(vn, Bln, e, off, coef<LOG, PM>(J, min(m0 + uint32_t(t), J.M - 1u)));
+                if (uint32_t(t) < nm) {
+                    v = vn;
+                    Bl = Bln;
                 }
             }
         }
@@ -354,38 +430,42 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
 
     const bool argmax = (J.flags & SI_FB_ARGMAX) != 0;
     const uint32_t L = J.L;
-    for (uint32_t t = r; t < nm; t += LPS) {
+#pragma unroll
+    for (uint32_t q = 0; q < kPer; ++q) {
+        const uint32_t t = r + q * LPS;
+        if (t >= nm) continue;
         const uint32_t m = m0 + t;
-        const float u = u24((&rng[t >> 2].x)[t & 3]);
-        const uint8_t* sam = J.sa + size_t(m) * Lp;
-        float cs = 0.0f, best = -1.0f, ap0 = 0.0f, ap1 = 0.0f;
-        uint32_t kdraw = 0xffffffffu, kbest = 0;
-        for (uint32_t k = 0; k < L; ++k) {
-            const float pk = pbuf[t][k];
-            cs += pk;
-            if (kdraw == 0xffffffffu && u < cs) kdraw = k;
-            if (pk > best) {
-                best = pk;
-                kbest = k;
+        const float* pt = pbuf[t];
+        uint32_t ks = 0;
+        if (argmax) {  // first maximum
+            float best = -1.0f;
+            for (uint32_t k = 0; k < L; ++k)
+                if (pt[k] > best) {
+                    best = pt[k];
+                    ks = k;
+                }
+        } else {  // first k with u < cumsum, else 0
+            const float u = u24((&rng[t >> 2].x)[t & 3]);
+            float cs = 0.0f;
+            uint32_t kd = 0xffffffffu;
+            for (uint32_t k = 0; k < L; ++k) {
+                cs += pt[k];
+                if (kd == 0xffffffffu && u < cs) kd = k;
             }
-            const uint32_t b = sam[k];
-            ap0 += (b & 1u) ? pk : 0.0f;
-            ap1 += (b & 2u) ? pk : 0.0f;
+            ks = kd == 0xffffffffu ? 0u : kd;
         }
-        if (kdraw == 0xffffffffu) kdraw = 0;
-        const uint32_t ks = argmax ? kbest : kdraw;
         const size_t o = size_t(s) * J.M + m;
-        if (J.ds) J.ds[o] = ap0 + ap1;
-        if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0, ap1);
+        if (J.ds) J.ds[o] = ap0r[q] + ap1r[q];
+        if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0r[q], ap1r[q]);
         if (J.hap) {
-            const uint32_t b = sam[ks];
+            const uint32_t b = reinterpret_cast<const uint8_t*>(sbuf[t])[ks];
             J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 4u) ? 255u : (b & 1u));
             J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 8u) ? 255u : ((b >> 1) & 1u));
         }
         if (J.prob)
-            for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pbuf[t][k];
+            for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pt[k];
     }
-    wave_sync();  // the slice is reused by the group's next item
+    wave_sync();  // the slices are reused by the group's next item
 }
 
 // ---------------------------------------------------------------------------
@@ -402,7 +482,8 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(JobView J, FbTable
     const bool fwd = grp < Bpad;
     const uint32_t s = fwd ? grp : grp - Bpad;
     if (s >= J.B) return;  // whole lane groups leave together
-    run_chain<LPS, LOG>(J, tab, tb.P, 4 * LPS, s, fwd, r);
+    if (J.flags & SI_FB_PER_MARKER) run_chain<LPS, LOG, true>(J, tab, tb.P, 4 * LPS, s, fwd, r);
+    else run_chain<LPS, LOG, false>(J, tab, tb.P, 4 * LPS, s, fwd, r);
 }
 
 template <int LPS, bool LOG>
@@ -410,6 +491,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(JobView J, FbTable
     constexpr uint32_t G = kFbThreads / LPS;  // lane groups per workgroup
     __shared__ EmisLds tab;
     __shared__ float pbuf[G][kBlock][4 * LPS];
+    __shared__ uint32_t sbuf[G][kBlock][LPS];
     __shared__ uint4 rng[G][kBlock / 4];
     load_tables<LOG>(tb, tab);
     __syncthreads();
@@ -418,7 +500,10 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_kernel(JobView J, FbTable
     const bool active = grp < J.B * J.nblk;
     const uint32_t s = active ? grp / J.nblk : 0u;
     const uint32_t c = active ? grp % J.nblk : 0u;
-    run_block<LPS, LOG>(J, tab, tb.P, 4 * LPS, active, s, c, r, pbuf[gl], rng[gl]);
+    if (J.flags & SI_FB_PER_MARKER)
+        run_block<LPS, LOG, true>(J, tab, tb.P, 4 * LPS, active, s, c, r, pbuf[gl], sbuf[gl], rng[gl]);
+    else
+        run_block<LPS, LOG, false>(J, tab, tb.P, 4 * LPS, active, s, c, r, pbuf[gl], sbuf[gl], rng[gl]);
 }
 
 // ---------------------------------------------------------------------------
@@ -458,7 +543,10 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass b
             const uint32_t j = find_job(bc.chain_off, bc.n_jobs, item);
             const JobView J = bc.jobs[j];
             const uint64_t k = item - bc.chain_off[j];
-            run_chain<LPS, LOG>(J, tab, tb.P, bc.Lp, uint32_t(k >> 1), (k & 1) == 0, r);
+            if (J.flags & SI_FB_PER_MARKER)
+                run_chain<LPS, LOG, true>(J, tab, tb.P, bc.Lp, uint32_t(k >> 1), (k & 1) == 0, r);
+            else
+                run_chain<LPS, LOG, false>(J, tab, tb.P, bc.Lp, uint32_t(k >> 1), (k & 1) == 0, r);
         }
     }
 }
@@ -469,6 +557,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_batch_kernel(BatchClass b
     constexpr uint32_t GPW = kWave / LPS;
     __shared__ EmisLds tab;
     __shared__ float pbuf[G][kBlock][4 * LPS];
+    __shared__ uint32_t sbuf[G][kBlock][LPS];
     __shared__ uint4 rng[G][kBlock / 4];
     load_tables<LOG>(tb, tab);
     __syncthreads();
@@ -488,7 +577,10 @@ __global__ __launch_bounds__(kFbThreads) void fb_block_batch_kernel(BatchClass b
             c = uint32_t(k % nblk);
         }
         const JobView J = bc.jobs[j];
-        run_block<LPS, LOG>(J, tab, tb.P, bc.Lp, active, s, c, r, pbuf[gl], rng[gl]);
+        if (J.flags & SI_FB_PER_MARKER)
+            run_block<LPS, LOG, true>(J, tab, tb.P, bc.Lp, active, s, c, r, pbuf[gl], sbuf[gl], rng[gl]);
+        else
+            run_block<LPS, LOG, false>(J, tab, tb.P, bc.Lp, active, s, c, r, pbuf[gl], sbuf[gl], rng[gl]);
     }
 }
 

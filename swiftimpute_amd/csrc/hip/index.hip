@@ -31,29 +31,36 @@ __global__ __launch_bounds__(256) void state_table_kernel(
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
     const uint32_t W = (H + 63) / 64;
-    uint32_t present = 0;
-    for (uint32_t k = 0; k < Lp; ++k) {
-        uint8_t b;
-        if (k < L) {
-            uint32_t h0, h1;
-            if (mode == 1) {
-                h0 = 2 * k;
-                h1 = 2 * k + 1;
-            } else {
-                h0 = (k < H) ? prefix[size_t(m) * H + perm.v[k]] : (k % H);
-                h1 = (h0 + 1 == H) ? 0u : h0 + 1;
-            }
-            if (states) states[size_t(m) * L + k] = h0;
-            const uint32_t a0 = allele_raw(cols, miss, W, m, h0);
-            const uint32_t a1 = allele_raw(cols, miss, W, m, h1);
-            const uint32_t sum = a0 + a1;
-            const uint32_t cls = sum >= 2 ? 2u : sum;
-            b = uint8_t((a0 == 1) | ((a1 == 1) << 1) | ((a0 == 255) << 2) | ((a1 == 255) << 3) |
-                        (cls << 4));
-            present |= 1u << cls;
+    // state pair of slot k and its allele byte (without the mask bit)
+    auto slot = [&](uint32_t k, uint32_t& cls) -> uint8_t {
+        uint32_t h0, h1;
+        if (mode == 1) {
+            h0 = 2 * k;
+            h1 = 2 * k + 1;
         } else {
-            b = uint8_t(0x40 | (3u << 4));  // padding state
+            h0 = (k < H) ? prefix[size_t(m) * H + perm.v[k]] : (k % H);
+            h1 = (h0 + 1 == H) ? 0u : h0 + 1;
         }
+        if (states) states[size_t(m) * L + k] = h0;
+        const uint32_t a0 = allele_raw(cols, miss, W, m, h0);
+        const uint32_t a1 = allele_raw(cols, miss, W, m, h1);
+        const uint32_t sum = a0 + a1;
+        cls = sum >= 2 ? 2u : sum;
+        return uint8_t((a0 == 1) | ((a1 == 1) << 1) | ((a0 == 255) << 2) | ((a1 == 255) << 3) |
+                       (cls << 4));
+    };
+    uint32_t present = 0;
+    for (uint32_t k = 0; k < L; ++k) {
+        uint32_t cls;
+        (void)slot(k, cls);
+        present |= 1u << cls;
+    }
+    for (uint32_t k = 0; k < Lp; ++k) {
+        uint32_t cls = 3;
+        uint8_t b = (k < L) ? slot(k, cls) : uint8_t(0x40 | (3u << 4));  // padding state
+        // every lane's 4-byte word carries the marker's class-present mask in
+        // bit 7 of its bytes 0-2
+        if ((k & 3u) < 3u) b |= uint8_t(((present >> (k & 3u)) & 1u) << 7);
         sa[size_t(m) * Lp + k] = b;
     }
     mmask[m] = uint8_t(present);

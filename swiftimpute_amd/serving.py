@@ -36,6 +36,9 @@ _SYMBOLS = [
     ("si_server_submit", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                         _native._u8p, ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.c_uint64)]),
+    ("si_server_submit_into", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                             _native._u8p, ctypes.c_uint64, _native._u8p,
+                                             _native._f32p, ctypes.POINTER(ctypes.c_uint64)]),
     ("si_server_wait", ctypes.c_int, [_vp, ctypes.c_uint64, _native._u8p, _native._f32p,
                                       ctypes.POINTER(ctypes.c_double)]),
     ("si_server_get_stats", ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
@@ -107,7 +110,32 @@ class Server:
         self._shapes[rid.value] = (B, Mr)
         return rid.value
 
+    def submit_into(self, m0: int, m1: int, codes: np.ndarray, hap: np.ndarray, ds: np.ndarray,
+                    seed: int = 12345) -> int:
+        """Submit with results written into the given arrays (hap [B][2][M]
+        uint8, ds [B][M] float32, C-contiguous), which must stay alive until
+        wait() returns for this request."""
+        codes = np.ascontiguousarray(codes, np.uint8)
+        B, Mr = codes.shape
+        assert Mr == m1 - m0
+        assert hap.shape == (B, 2, Mr) and hap.dtype == np.uint8 and hap.flags.c_contiguous
+        assert ds.shape == (B, Mr) and ds.dtype == np.float32 and ds.flags.c_contiguous
+        rid = ctypes.c_uint64()
+        _check(_lib().si_server_submit_into(self.h, m0, m1, B, codes.ctypes.data_as(_native._u8p),
+                                            seed, hap.ctypes.data_as(_native._u8p),
+                                            ds.ctypes.data_as(_native._f32p), ctypes.byref(rid)),
+               "si_server_submit_into")
+        self._shapes[rid.value] = (hap, ds)
+        return rid.value
+
     def wait(self, rid: int) -> Result:
+        got = self._shapes.get(rid)
+        if got is not None and isinstance(got[0], np.ndarray):  # submitted into caller arrays
+            self._shapes.pop(rid)
+            lat = ctypes.c_double()
+            _check(_lib().si_server_wait(self.h, rid, None, None, ctypes.byref(lat)),
+                   "si_server_wait")
+            return Result(got[0], got[1], lat.value)
         B, Mr = self._shapes.pop(rid)
         hap = np.empty((B, 2, Mr), np.uint8)
         ds = np.empty((B, Mr), np.float32)

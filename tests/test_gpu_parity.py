@@ -142,10 +142,14 @@ def test_state_table(H, L):
     sa = idx.sa.cpu().numpy()
     cls = np.minimum(a0.astype(int) + a1.astype(int), 2)
     assert np.array_equal((sa[:, :L] >> 4) & 3, cls)
-    assert (sa[:, L:] == 0x70).all()
+    assert ((sa[:, L:] & 0x7f) == 0x70).all()
     mm = idx.mmask.cpu().numpy()
     for m in range(M):
         assert mm[m] == sum(1 << c for c in set(cls[m].tolist()))
+    # the class mask rides in bit 7 of bytes 0-2 of every 4-byte word
+    k = np.arange(idx.Lp)
+    want = np.where((k % 4)[None, :] < 3, (mm[:, None] >> (k % 4)[None, :]) & 1, 0)
+    assert np.array_equal(sa >> 7, want)
 
 
 def test_state_table_simple_mode():

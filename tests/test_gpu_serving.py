@@ -87,6 +87,15 @@ def test_alone_equals_mix_equals_plain(world):
         assert np.array_equal(a.hap, b.hap), r[:2]
         assert np.array_equal(a.ds, b.ds), r[:2]
         assert a.latency_s > 0 and b.latency_s > 0
+    # results written straight into caller arrays: the same values
+    with serving.Server(pan.cols, pan.H, pan.pos, cm, L=L, ne=NE, max_streams=2, max_batch=4) as srv:
+        outs = [(np.full((r[2].shape[0], 2, r[1] - r[0]), 9, np.uint8),
+                 np.full((r[2].shape[0], r[1] - r[0]), -1, np.float32)) for r in reqs]
+        ids = [srv.submit_into(r[0], r[1], r[2], h, d, seed=r[3]) for r, (h, d) in zip(reqs, outs)]
+        into = [srv.wait(i) for i in ids]
+    for r, a, b, (h, d) in zip(reqs, alone, into, outs):
+        assert b.hap is h and b.ds is d
+        assert np.array_equal(a.hap, h) and np.array_equal(a.ds, d), r[:2]
 
     idx = device.DeviceIndex(pan.cols, pan.H, L, keep_prefix=False)
     tables = device.SharedTables(synth.HARDCALL_PALETTE)
