@@ -343,7 +343,24 @@ def bench_plain(args):
     sync()
     index_ms = ev[0].elapsed_time(ev[1])
     pbwt_bytes = M * (pan.W * 8 + 4 * H + 4 * H)
-    log(f"[plain] index built in {index_ms:.1f} ms")
+    log(f"[plain] index built in {index_ms:.1f} ms (panel upload, PBWT, state table)")
+    # the PBWT sweep alone (three launches), on resident inputs and outputs
+    lib = _native.hip()
+    wsb = int(lib.si_pbwt_workspace_bytes(H, M))
+    pws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    reps = 3
+    for k in range(reps + 1):
+        if k == 1:
+            ev[0].record(stream)
+        _native.check(lib.si_pbwt_sweep(device._ptr(idx.cols), H, M, device._ptr(idx.prefix),
+                                        device._ptr(idx.div), device._ptr(pws), wsb, sh),
+                      "si_pbwt_sweep")
+    ev[1].record(stream)
+    sync()
+    sweep_ms = ev[0].elapsed_time(ev[1]) / reps
+    log(f"[plain] PBWT sweep {sweep_ms:.2f} ms ({pbwt_bytes / (sweep_ms * 1e-3) / 1e9:.0f} GB/s)")
+    del pws
     idx.drop_prefix()
     torch.cuda.empty_cache()
 
@@ -351,10 +368,8 @@ def bench_plain(args):
     job = device.Job(idx, torch.from_numpy(tg.codes).cuda(), synth.HARDCALL_PALETTE, tr, seed=12345)
     job.alloc_outputs(ds=True, ap=False, hap=True, prob=False, loglik=False)
     arr = (_native.FbJob * 1)(job.c_job())
-    lib = _native.hip()
     ws_bytes = int(lib.si_fb_workspace_bytes(arr, 1))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
-    sh = ctypes.c_void_p(stream.cuda_stream)
 
     def step(evs=None):
         if evs:
@@ -411,11 +426,15 @@ def bench_plain(args):
                            f" x {units} units per launch",
         },
         "pbwt": {
-            "index_build_ms": index_ms,
+            "workload": f"{H} haplotypes x {M} markers, prefix and divergence rows",
+            "sweep_ms": sweep_ms,
             "bytes": pbwt_bytes,
-            "achieved_GBps_incl_state_table": pbwt_bytes / (index_ms * 1e-3) / 1e9,
+            "algorithmic": "M * (8*ceil(H/64) + 4H + 4H) bytes: column read, prefix and "
+                           "divergence rows written",
+            "achieved_GBps": pbwt_bytes / (sweep_ms * 1e-3) / 1e9,
             "peak_GBps": PEAK_HBM_GBS,
-            "frac": pbwt_bytes / (index_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "frac": pbwt_bytes / (sweep_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "index_build_ms_incl_upload_and_state_table": index_ms,
         },
     }
     if not args.no_cpu_baseline and "batch" not in args.sections.split(","):
@@ -432,6 +451,83 @@ def bench_plain(args):
                       "with sqrt(M) replay + sampling",
         }
         log(f"[plain] CPU baseline {v:.4g} genotypes/s")
+    return out
+
+
+def bench_precision(args):
+    """Config-3 precision stress: 8,000 x 100,000 panel with heavy-tailed
+    spacing (log-normal, median 20 kb, sigma 1.5) and 50x hotspots on 1% of
+    intervals, 2,000 targets at 2% genotype error typed every 20th marker,
+    L = 8, per-marker transitions, Ne in {1e3, 1e4, 1e5}.  Both FP32 kernels
+    run on the whole cohort; the FP64 oracle on the first `precision_samples`
+    samples gives the max dosage error and the genotype (argmax GP)
+    discordance; discordance against the generator's truth is over the whole
+    cohort."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from swiftimpute_amd import device, synth
+
+    H, M, B, L, te = 8000, 100000, 2000, 8, 20
+    log(f"[precision] generating panel H={H} M={M} (heavy-tailed spacing, hotspots)")
+    pan = synth.make_panel(H, M, seed=12345, spacing_mode=1)
+    tg = synth.make_targets(pan, B, te, seed=12346, geno_error=0.02, with_truth=True)
+    idx = device.DeviceIndex(pan.cols, H, L, keep_prefix=False, with_div=False)
+    a0, a1 = idx.state_alleles()
+    codes = torch.from_numpy(tg.codes).cuda()
+    untyped = tg.codes[:, :M] == synth.CODE_MISSING
+    truth_g = tg.truth[:, 0, :].astype(np.int32) + tg.truth[:, 1, :]
+    ns = min(args.precision_samples, B)
+    gl_sub = synth.HARDCALL_PALETTE[tg.codes[:ns, :M]]
+
+    def gp(ap0, ap1):
+        ap0 = ap0.astype(np.float64)
+        ap1 = ap1.astype(np.float64)
+        return np.stack([(1 - ap0) * (1 - ap1), ap0 * (1 - ap1) + (1 - ap0) * ap1, ap0 * ap1])
+
+    def gp_call(ap0, ap1):
+        return np.argmax(gp(ap0, ap1), axis=0)
+
+    out = {"workload": f"{H} x {M} panel, heavy-tailed spacing + hotspots, {B} targets at 2% "
+                       f"genotype error typed every {te}th marker, L={L}, per-marker "
+                       f"transitions; FP64 oracle on the first {ns} samples",
+           "per_ne": {}}
+    for ne in (1000, 10000, 100000):
+        tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne, per_marker=True)
+        o = oracle.fb_f64_fast(a0, a1, gl_sub, tr.Td, tr.To, per_marker=True, want_prob=False)
+        ref_gp = np.sort(gp(o["ap"][..., 0], o["ap"][..., 1]), axis=0)
+        ref_call = gp_call(o["ap"][..., 0], o["ap"][..., 1])
+        # calls the FP64 posterior decides by more than 1e-5 (the model often
+        # gives exactly tied genotype probabilities, where any rounding flips
+        # the argmax)
+        decisive = (ref_gp[2] - ref_gp[1]) > 1e-5
+        row = {"min_switch_probability": tr.min_switch()}
+        for name, logspace in (("linear_f32", False), ("log_f32", True)):
+            if not logspace and tr.min_switch() < 1e-30:
+                row[name] = None
+                continue
+            job = device.Job(idx, codes, synth.HARDCALL_PALETTE, tr, seed=12345, logspace=logspace)
+            job.alloc_outputs(ds=True, ap=True, hap=False, prob=False, loglik=False)
+            device.run_jobs([job.c_job()])
+            sync()
+            ds = job.out.ds.cpu().numpy()
+            ap = job.out.ap.cpu().numpy()
+            call = gp_call(ap[..., 0], ap[..., 1])
+            u = untyped[:ns]
+            row[name] = {
+                "max_abs_ds_err_vs_f64": float(np.abs(ds[:ns] - o["ds"]).max()),
+                "discordance_vs_f64": float((call[:ns][u] != ref_call[u]).mean()),
+                "discordance_vs_f64_decisive_calls": float(
+                    (call[:ns][u & decisive] != ref_call[u & decisive]).mean()),
+                "discordance_vs_truth": float((call[untyped] != truth_g[untyped]).mean()),
+            }
+            del job
+        row["f64_tied_call_fraction"] = float(1.0 - decisive[untyped[:ns]].mean())
+        row["f64_discordance_vs_truth_subset"] = float(
+            (ref_call[untyped[:ns]] != truth_g[:ns][untyped[:ns]]).mean())
+        out["per_ne"][str(ne)] = row
+        log(f"[precision] Ne={ne}: {row}")
     return out
 
 
@@ -453,7 +549,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--sections", default="batch,serve,plain")
+    ap.add_argument("--sections", default="batch,serve,plain,precision")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="timed steps of the one-job-per-launch comparison")
@@ -467,6 +563,7 @@ def main():
     ap.add_argument("--cpu-samples", type=int, default=16)
     ap.add_argument("--cpu-markers", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision-samples", type=int, default=32)
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
 
@@ -482,7 +579,8 @@ def main():
     sections = [s for s in args.sections.split(",") if s]
     res = {}
     for s in sections:
-        res[s] = {"batch": bench_batch, "serve": bench_serve, "plain": bench_plain}[s](args)
+        res[s] = {"batch": bench_batch, "serve": bench_serve, "plain": bench_plain,
+                  "precision": bench_precision}[s](args)
         torch.cuda.empty_cache()
 
     out = {
@@ -524,6 +622,8 @@ def main():
         p = res["plain"]
         out.update({"value": p["genotypes_per_s"], "ms_per_step": p["ms_per_step"],
                     "config": {"workload": p["workload"], "form": "plain"}})
+    if "precision" in res:
+        out["precision"] = res["precision"]
     if "batch" in res and res["batch"]["cpu_baseline"]:
         out["cpu_baseline"] = res["batch"]["cpu_baseline"]
     if "serve" in res:

@@ -108,13 +108,17 @@ __device__ __forceinline__ float4 sel4(bool c, float4 a, float4 b) {
     return make_float4(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z, c ? a.w : b.w);
 }
 
-// One 16-marker block of a lane's inputs: the sample's 16 genotype codes and,
-// per marker, the lane's four state bytes.  Loaded whole (indices clamped in
-// bounds past the last marker), so every load of the block is in flight at
-// once and the steps that follow read registers only.
+// One 16-marker block of a lane's inputs: the sample's 16 genotype codes, per
+// marker the lane's four state bytes and (per-marker transitions) the step
+// coefficients: (t_o, u) for the linear form, (lto, lu) for the log form.
+// Loaded whole (indices clamped in bounds past the last marker), so every
+// load of the block is in flight at once and the steps that follow read
+// registers only.
+template <bool LOG, bool PM>
 struct BlockIn {
     uint4 cw;
     uint32_t sa[kBlock];
+    float2 k[PM ? kBlock : 1];
     __device__ __forceinline__ void load(const JobView& J, const uint8_t* codes, uint32_t lp4,
                                          uint32_t r, uint32_t c) {
         const uint32_t m0 = c * kBlock, ml = J.M - 1u;
@@ -122,19 +126,33 @@ struct BlockIn {
         const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(J.sa);
 #pragma unroll
         for (uint32_t t = 0; t < kBlock; ++t) sa[t] = sa32[size_t(min(m0 + t, ml)) * lp4 + r];
+        if constexpr (PM) {
+#pragma unroll
+            for (uint32_t t = 0; t < kBlock; ++t) {
+                const float4 q = J.steps[min(m0 + t, ml)];
+                k[t] = LOG ? make_float2(q.z, q.w) : make_float2(q.x, q.y);
+            }
+        } else {
+            k[0] = LOG ? make_float2(J.step.z, J.step.w) : make_float2(J.step.x, J.step.y);
+        }
     }
     __device__ __forceinline__ uint32_t code(uint32_t t) const {
         return byte_of((&cw.x)[t >> 2], t & 3);
     }
+    __device__ __forceinline__ float2 coef(uint32_t t) const { return k[PM ? t : 0]; }
 };
 
-// Coefficients of the step into marker m: (t_o, u) for the linear form,
-// (lto, lu) for the log form; per marker (PM) or one for every step.
-template <bool LOG, bool PM>
-__device__ __forceinline__ float2 coef(const JobView& J, uint32_t m) {
-    const float4 k = PM ? J.steps[m] : J.step;
-    return LOG ? make_float2(k.z, k.w) : make_float2(k.x, k.y);
-}
+// Emissions of a block's 16 markers for this lane, read from the LDS table
+// together before any step needs them.
+struct BlockEm {
+    float e[kBlock][4];
+    float off[kBlock];
+    template <bool LOG, bool PM>
+    __device__ __forceinline__ void fill(const EmisLds& tab, uint32_t P, const BlockIn<LOG, PM>& in) {
+#pragma unroll
+        for (uint32_t t = 0; t < kBlock; ++t) off[t] = emission(tab, P, in.code(t), in.sa[t], e[t]);
+    }
+};
 
 // Orders this wavefront's LDS accesses: lane groups exchange data through LDS
 // only within their own wavefront.
@@ -281,8 +299,9 @@ template <int LPS, bool LOG, bool PM>
 __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, uint32_t P,
                                           uint32_t Lp, uint32_t s, bool fwd, uint32_t r) {
     const uint8_t* codes = J.codes + size_t(s) * J.pitch;
-    const uint32_t lp4 = Lp / 4, ml = J.M - 1u;
-    BlockIn cur, nxt;
+    const uint32_t lp4 = Lp / 4;
+    BlockIn<LOG, PM> cur, nxt;
+    BlockEm em;
     if (fwd) {
         cur.load(J, codes, lp4, r, 0);
         float4 a;
@@ -295,20 +314,18 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
         }
         for (uint32_t c = 0; c < J.nblk; ++c) {
             if (c + 1 < J.nblk) nxt.load(J, codes, lp4, r, c + 1);
+            em.fill(tab, P, cur);
             const uint32_t m0 = c * kBlock;
 #pragma unroll
             for (uint32_t t = 0; t < kBlock; ++t) {
                 const uint32_t m = m0 + t;
-                float e[4];
-                const float off = emission(tab, P, cur.code(t), cur.sa[t], e);
                 float4 an = a;
                 float lsn = ls;
-                const double inc = fwd_step<LPS, LOG>(an, lsn, e, off, coef<LOG, PM>(J, min(m, ml)));
-                if (m > 0 && m < J.M) {
-                    a = an;
-                    ls = lsn;
-                    ll += inc;
-                }
+                const double inc = fwd_step<LPS, LOG>(an, lsn, em.e[t], em.off[t], cur.coef(t));
+                const bool live = m > 0 && m < J.M;
+                a = sel4(live, an, a);
+                ls = live ? lsn : ls;
+                ll += live ? inc : 0.0;
                 if (t == 0)
                     reinterpret_cast<float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r] = a;
             }
@@ -325,19 +342,17 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
         cur.load(J, codes, lp4, r, J.nblk - 1);
         for (int c = int(J.nblk) - 1; c >= 0; --c) {
             if (c > 0) nxt.load(J, codes, lp4, r, uint32_t(c - 1));
+            em.fill(tab, P, cur);
             const uint32_t m0 = uint32_t(c) * kBlock;
 #pragma unroll
             for (int t = int(kBlock) - 1; t >= 0; --t) {
                 const uint32_t n = m0 + uint32_t(t);
-                float e[4];
-                const float off = emission(tab, P, cur.code(uint32_t(t)), cur.sa[t], e);
                 float4 vn = v;
                 double Bln = Bl;
-                bwd_step<LPS, LOG>(vn, Bln, e, off, coef<LOG, PM>(J, min(n, ml)));
-                if (n > 0 && n < J.M) {
-                    v = vn;
-                    Bl = Bln;
-                }
+                bwd_step<LPS, LOG>(vn, Bln, em.e[t], em.off[t], cur.coef(uint32_t(t)));
+                const bool live = n > 0 && n < J.M;
+                v = sel4(live, vn, v);
+                Bl = live ? Bln : Bl;
                 if (t == 0 && c > 0) {  // v is the vector at marker m0 - 1, last of block c - 1
                     const size_t ck = size_t(s) * J.nblk + uint32_t(c - 1);
                     reinterpret_cast<float4*>(J.ck_beta)[ck * lp4 + r] = v;
@@ -369,8 +384,10 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
     const uint32_t lp4 = Lp / 4;
     float ap0r[kPer], ap1r[kPer];
     if (active) {
-        BlockIn in;
+        BlockIn<LOG, PM> in;
         in.load(J, J.codes + size_t(s) * J.pitch, lp4, r, c);
+        BlockEm em;
+        em.fill(tab, P, in);
 #pragma unroll
         for (uint32_t t = 0; t < kBlock; ++t) sbuf[t][r] = in.sa[t];
         // forward vectors of the block, into this lane's part of the LDS slice
@@ -380,11 +397,9 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
         float ls = LOG ? log_mass<LPS>(a) : 0.0f;
 #pragma unroll
         for (uint32_t t = 1; t < kBlock; ++t) {
-            float e[4];
-            const float off = emission(tab, P, in.code(t), in.sa[t], e);
             float4 an = a;
             float lsn = ls;
-            fwd_step<LPS, LOG>(an, lsn, e, off, coef<LOG, PM>(J, min(m0 + t, J.M - 1u)));
+            fwd_step<LPS, LOG>(an, lsn, em.e[t], em.off[t], in.coef(t));
             const bool live = t < nm;
             a = sel4(live, an, a);
             ls = live ? lsn : ls;
@@ -409,15 +424,12 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
                 ap1r[uint32_t(t) / LPS] = ap1;
             }
             if (t > 0) {
-                float e[4];
-                const float off = emission(tab, P, in.code(uint32_t(t)), w, e);
                 float4 vn = v;
                 double Bln = Bl;
-                bwd_step<LPS, LOG>(vn, Bln, e, off, coef<LOG, PM>(J, min(m0 + uint32_t(t), J.M - 1u)));
-                if (uint32_t(t) < nm) {
-                    v = vn;
-                    Bl = Bln;
-                }
+                bwd_step<LPS, LOG>(vn, Bln, em.e[t], em.off[t], in.coef(uint32_t(t)));
+                const bool live = uint32_t(t) < nm;
+                v = sel4(live, vn, v);
+                Bl = live ? Bln : Bl;
             }
         }
         const uint2 key = make_uint2(uint32_t(J.seed), uint32_t(J.seed >> 32));
