@@ -358,17 +358,23 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
             __syncthreads();
             continue;
         }
+        // element words, gathered once: haplotype | group id << 16, into the
+        // buffer pass 0 reads
+        {
+            uint32_t* X0 = a.xb + a.H;
+#pragma unroll 8
+            for (uint32_t j = 0; j < K; ++j) {
+                const uint32_t h = prev[q.pos(j)];
+                const uint32_t v = h | (gid[h] << 16);
+                if (q.valid(j)) X0[q.base + j * 64u + q.lane] = v;
+            }
+            __syncthreads();
+        }
         for (uint32_t pass = 0; pass < npass; ++pass) {
             const uint32_t sh = 16u + 8u * pass;
-            const uint32_t* X = a.xb + ((pass + 1u) & 1u) * a.H;  // previous pass's output
+            const uint32_t* X = a.xb + ((pass + 1u) & 1u) * a.H;  // this pass's input
             uint32_t* Y = (pass + 1 == npass) ? nullptr : a.xb + (pass & 1u) * a.H;
-            auto elem = [&](uint32_t j) -> uint32_t {
-                if (pass == 0) {
-                    const uint32_t h = prev[q.pos(j)];
-                    return h | (gid[h] << 16);
-                }
-                return X[q.pos(j)];
-            };
+            auto elem = [&](uint32_t j) -> uint32_t { return X[q.pos(j)]; };
             auto match_of = [&](bool valid, uint32_t d) -> uint64_t {
                 uint64_t match = __ballot(valid);
 #pragma unroll
@@ -381,7 +387,7 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
             };
             for (uint32_t i = q.t; i < uint32_t(kPW) * 256u; i += kPT) (&cnt[0][0])[i] = 0u;
             __syncthreads();
-#pragma unroll 2
+#pragma unroll 4
             for (uint32_t j = 0; j < K; ++j) {
                 const bool valid = q.valid(j);
                 const uint32_t d = (elem(j) >> sh) & 255u;
@@ -411,7 +417,7 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
                 dbase[q.t] = off + incl - tot;
             }
             __syncthreads();
-#pragma unroll 2
+#pragma unroll 4
             for (uint32_t j = 0; j < K; ++j) {
                 const bool valid = q.valid(j);
                 const uint32_t v = elem(j);
@@ -431,14 +437,232 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// LDS-resident forms (H <= 49152).  The current order lives in LDS as 16-bit
+// haplotype ids (and, for the group sweep, a bit per position marking group
+// starts); each lane holds its K elements in registers across the step's
+// barrier, so a step touches global memory only for the next column and the
+// rows it writes.  NPOS = 16 * 64 * K positions are allocated; those at or
+// beyond H hold 0 and are never written.
+// ---------------------------------------------------------------------------
+// Keeps the scheduler from hoisting later rounds' ballots (each round holds
+// several 64-bit masks; hoisting every round's would overflow the scalar
+// registers).
+__device__ __forceinline__ void round_end() { __builtin_amdgcn_sched_barrier(0); }
+
+// The same value, opaque to the optimizer: per-round addresses derived from it
+// inside a loop are recomputed there (an immediate offset per round) instead
+// of being hoisted out of the loop as K live 64-bit registers.
+__device__ __forceinline__ uint32_t opaque(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+template <int K>
+struct LdsGeom {
+    static constexpr uint32_t NPOS = uint32_t(kPW) * 64u * uint32_t(K);
+    static constexpr uint32_t NW32 = NPOS / 32u;
+};
+
+template <int K>
+__global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
+    using G = LdsGeom<K>;
+    __shared__ uint16_t ord[G::NPOS];
+    __shared__ uint32_t col[2][G::NW32];
+    __shared__ uint32_t wz[kPW];
+    const uint32_t c = blockIdx.x;
+    const uint32_t m_first = c * a.C, m_end = min(a.M, m_first + a.C);
+    if (m_end - m_first < 2) return;
+    const WaveCtx q(a);
+
+    const uint32_t* start = (c == 0) ? nullptr : a.prefix + size_t(m_first - 1) * a.H;
+    for (uint32_t i = q.t; i < G::NPOS; i += kPT)
+        ord[i] = uint16_t(i < a.H ? (start ? start[i] : i) : 0u);
+    load_column(a, m_first, col[0]);
+    __syncthreads();
+    for (uint32_t m = m_first; m + 1 < m_end; ++m) {
+        const uint32_t* cb = col[(m - m_first) & 1u];
+        const bool pre = m + 2 < m_end;
+        ColPrefetch nx;
+        if (pre) nx.fetch(a, m + 1);
+
+        uint32_t pk[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) pk[j] = ord[q.base + uint32_t(j) * 64u + q.lane];
+        uint32_t xb[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) xb[j] = allele_bit(cb, pk[j]);
+        uint32_t zsum = 0;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t x = xb[j] | uint32_t(!q.valid(uint32_t(j)));
+            const uint64_t Z = __ballot(x == 0u);
+            pk[j] |= ((zsum + mbcnt64(Z)) << 16) | (x << 31);
+            zsum += uint32_t(__popcll(Z));
+        }
+        if (q.lane == 0) wz[q.w] = zsum;
+        __syncthreads();
+        uint32_t zbefore, ztot;
+        zero_offsets(wz, q.w, zbefore, ztot);
+        uint32_t* row = a.prefix + size_t(m) * a.H;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t e = uint32_t(j) * 64u + q.lane;
+            const uint32_t v = pk[j], zb = (v >> 16) & 0x1fffu, h = v & 0xffffu;
+            const uint32_t dst = (v >> 31) ? ztot + (q.base - zbefore) + (e - zb) : zbefore + zb;
+            if (e < q.nvalid) {
+                ord[dst] = uint16_t(h);
+                row[dst] = h;
+            }
+        }
+        if (pre) nx.store(a, col[((m - m_first) & 1u) ^ 1u]);
+        __syncthreads();
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(kPT) void pbwt_groups_lds_kernel(PbwtArgs a) {
+    using G = LdsGeom<K>;
+    __shared__ uint16_t ord[G::NPOS];
+    __shared__ uint32_t fl[2][G::NW32];  // bit p: position p starts a group
+    __shared__ uint32_t col[2][G::NW32];
+    __shared__ uint32_t s_z[kPW], s_hs[kPW], s_zs[kPW], s_os[kPW];
+    const uint32_t c = blockIdx.x;
+    const uint32_t m_first = c * a.C, m_end = min(a.M, m_first + a.C);
+    const WaveCtx q(a);
+
+    for (uint32_t i = q.t; i < G::NPOS; i += kPT) ord[i] = uint16_t(i < a.H ? i : 0u);
+    for (uint32_t i = q.t; i < G::NW32; i += kPT) fl[0][i] = (i == 0) ? 1u : 0u;  // one group
+    load_column(a, m_first, col[0]);
+    __syncthreads();
+    for (uint32_t m = m_first; m < m_end; ++m) {
+        const uint32_t k = m - m_first, cur = k & 1u, nxt = cur ^ 1u;
+        const uint32_t* cb = col[cur];
+        const bool pre = m + 1 < m_end;
+        ColPrefetch nx;
+        if (pre) nx.fetch(a, m + 1);
+        for (uint32_t i = q.t; i < G::NW32; i += kPT) fl[nxt][i] = 0u;
+
+        uint32_t pk[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t p = q.base + uint32_t(j) * 64u + q.lane;
+            pk[j] = ord[p] | (((fl[cur][p >> 5] >> (p & 31u)) & 1u) << 31);
+        }
+        uint32_t xb[K];
+#pragma unroll
+        for (int j = 0; j < K; ++j) xb[j] = allele_bit(cb, pk[j] & 0xffffu);
+        uint32_t zsum = 0;
+        GroupRun g;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const bool valid = q.valid(uint32_t(j));
+            const uint32_t v = pk[j];
+            const uint32_t x = xb[j] | uint32_t(!valid);
+            const uint64_t Z = __ballot(x == 0u);
+            const uint64_t O = __ballot(valid && x != 0u);
+            const uint64_t Bm = __ballot(valid && (v >> 31) != 0u);
+            bool fl_, op;
+            g.round(q, Z, O, Bm, x, fl_, op);
+            pk[j] = (v & 0xffffu) | ((zsum + mbcnt64(Z)) << 16) | (uint32_t(fl_) << 29) |
+                    (uint32_t(op) << 30) | (x << 31);
+            zsum += uint32_t(__popcll(Z));
+            if (K > 12) round_end();
+        }
+        if (q.lane == 0) {
+            s_z[q.w] = zsum;
+            s_hs[q.w] = g.hs ? 1u : 0u;
+            s_zs[q.w] = g.zs;
+            s_os[q.w] = g.os;
+        }
+        __syncthreads();
+        uint32_t zbefore = 0, ztot = 0, cz = 0, co = 0;
+#pragma unroll
+        for (int v = 0; v < kPW; ++v) {
+            const uint32_t z = s_z[v];
+            ztot += z;
+            if (uint32_t(v) < q.w) {
+                zbefore += z;
+                if (s_hs[v]) {
+                    cz = s_zs[v];
+                    co = s_os[v];
+                } else {
+                    cz += z;
+                    co += wave_valid(a, uint32_t(v)) - z;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            const uint32_t e = uint32_t(j) * 64u + q.lane;
+            const uint32_t in = pk[j];
+            const uint32_t x = in >> 31, open = (in >> 30) & 1u, f = (in >> 29) & 1u;
+            const uint32_t zb = (in >> 16) & 0x1fffu;
+            const bool first = f && (!open || (x ? co : cz) == 0u);
+            const uint32_t dst = x ? ztot + (q.base - zbefore) + (e - zb) : zbefore + zb;
+            if (e < q.nvalid) {
+                ord[dst] = uint16_t(in & 0xffffu);
+                if (first) atomicOr(&fl[nxt][dst >> 5], 1u << (dst & 31u));
+            }
+        }
+        if (a.div) {
+            uint32_t* drow = a.div + size_t(m) * a.H;
+            for (uint32_t i = q.t; i < a.H; i += kPT) __builtin_nontemporal_store(m, drow + i);
+        }
+        if (pre) nx.store(a, col[nxt]);
+        __syncthreads();
+    }
+
+    // group ids: dense rank of the group in the final order
+    const uint32_t* ff = fl[(m_end - m_first) & 1u];
+    uint32_t pk[K];  // haplotype | inclusive count of group starts in the wave << 16
+    uint32_t bsum = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t p = q.base + uint32_t(j) * 64u + q.lane;
+        const uint64_t Bm = __ballot(q.valid(uint32_t(j)) && ((ff[p >> 5] >> (p & 31u)) & 1u));
+        pk[j] = uint32_t(ord[p]) | ((bsum + uint32_t(__popcll(Bm & q.le))) << 16);
+        bsum += uint32_t(__popcll(Bm));
+    }
+    if (q.lane == 0) s_z[q.w] = bsum;
+    __syncthreads();
+    uint32_t bbefore, Gn;
+    zero_offsets(s_z, q.w, bbefore, Gn);
+    uint32_t* gid = a.gid + size_t(c) * a.H;
+    uint32_t* lastrow = a.prefix + size_t(m_end - 1) * a.H;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t e = uint32_t(j) * 64u + q.lane;
+        if (e < q.nvalid) {
+            gid[pk[j] & 0xffffu] = bbefore + (pk[j] >> 16) - 1u;
+            if (c == 0) lastrow[q.base + e] = pk[j] & 0xffffu;
+        }
+    }
+    if (q.t == 0) a.ngroups[c] = Gn;
+}
+
+// ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
 namespace {
 
-// Rounds per wave: 16 * 64 * K >= H.
-uint32_t pbwt_rounds(uint32_t H) { return (H + kPT - 1) / kPT; }
+// Rounds per wave K (16 * 64 * K >= H): the smallest instantiated value.
+constexpr uint32_t kRounds[] = {1, 2, 4, 6, 8, 12, 16, 20, 24, 32, 40, 48, 64};
+
+uint32_t pbwt_rounds(uint32_t H) {
+    const uint32_t need = (H + kPT - 1) / kPT;
+    for (uint32_t k : kRounds)
+        if (k >= need) return k;
+    return 0;
+}
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+template <int K>
+void launch_lds(const PbwtArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(pbwt_groups_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
+    if (a.nchunk > 1) hipLaunchKernelGGL(pbwt_compose_kernel, dim3(1), dim3(kPT), 0, st, a);
+    hipLaunchKernelGGL(pbwt_rows_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
+}
 
 }  // namespace
 
@@ -479,9 +703,25 @@ int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* pr
     a.ngroups = reinterpret_cast<uint32_t*>(p);
     p += align256(n * 4);
     a.xb = reinterpret_cast<uint32_t*>(p);
-    hipLaunchKernelGGL(pbwt_groups_kernel, dim3(a.nchunk), dim3(kPT), 0, stream, a);
-    if (a.nchunk > 1) hipLaunchKernelGGL(pbwt_compose_kernel, dim3(1), dim3(kPT), 0, stream, a);
-    hipLaunchKernelGGL(pbwt_rows_kernel, dim3(a.nchunk), dim3(kPT), 0, stream, a);
+    switch (K) {
+        case 1: launch_lds<1>(a, stream); break;
+        case 2: launch_lds<2>(a, stream); break;
+        case 4: launch_lds<4>(a, stream); break;
+        case 6: launch_lds<6>(a, stream); break;
+        case 8: launch_lds<8>(a, stream); break;
+        case 12: launch_lds<12>(a, stream); break;
+        case 16: launch_lds<16>(a, stream); break;
+        case 20: launch_lds<20>(a, stream); break;
+        case 24: launch_lds<24>(a, stream); break;
+        case 32: launch_lds<32>(a, stream); break;
+        case 40: launch_lds<40>(a, stream); break;
+        case 48: launch_lds<48>(a, stream); break;
+        default:  // H > 49152: the order exchanges through global memory
+            hipLaunchKernelGGL(pbwt_groups_kernel, dim3(a.nchunk), dim3(kPT), 0, stream, a);
+            if (a.nchunk > 1)
+                hipLaunchKernelGGL(pbwt_compose_kernel, dim3(1), dim3(kPT), 0, stream, a);
+            hipLaunchKernelGGL(pbwt_rows_kernel, dim3(a.nchunk), dim3(kPT), 0, stream, a);
+    }
     return hipGetLastError() == hipSuccess ? 0 : SI_ERR_DEVICE;
 }
 
