@@ -142,16 +142,22 @@ struct BlockIn {
     __device__ __forceinline__ float2 coef(uint32_t t) const { return k[PM ? t : 0]; }
 };
 
-// Emissions of a block's 16 markers for this lane, read from the LDS table
-// together before any step needs them.
+// What a block's 16 steps consume: this lane's emissions (read from the LDS
+// table together, before any step needs them) and the step coefficients.
+// Filling it frees the BlockIn for the next block's loads, which are then in
+// flight while the steps run.
+template <bool LOG, bool PM>
 struct BlockEm {
     float e[kBlock][4];
     float off[kBlock];
-    template <bool LOG, bool PM>
+    float2 k[PM ? kBlock : 1];
     __device__ __forceinline__ void fill(const EmisLds& tab, uint32_t P, const BlockIn<LOG, PM>& in) {
 #pragma unroll
         for (uint32_t t = 0; t < kBlock; ++t) off[t] = emission(tab, P, in.code(t), in.sa[t], e[t]);
+#pragma unroll
+        for (uint32_t t = 0; t < (PM ? kBlock : 1); ++t) k[t] = in.k[t];
     }
+    __device__ __forceinline__ float2 coef(uint32_t t) const { return k[PM ? t : 0]; }
 };
 
 // Orders this wavefront's LDS accesses: lane groups exchange data through LDS
@@ -300,8 +306,8 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
                                           uint32_t Lp, uint32_t s, bool fwd, uint32_t r) {
     const uint8_t* codes = J.codes + size_t(s) * J.pitch;
     const uint32_t lp4 = Lp / 4;
-    BlockIn<LOG, PM> cur, nxt;
-    BlockEm em;
+    BlockIn<LOG, PM> cur;
+    BlockEm<LOG, PM> em;
     if (fwd) {
         cur.load(J, codes, lp4, r, 0);
         float4 a;
@@ -313,15 +319,15 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
             ll = fwd_init<LPS, LOG>(a, ls, e, off);
         }
         for (uint32_t c = 0; c < J.nblk; ++c) {
-            if (c + 1 < J.nblk) nxt.load(J, codes, lp4, r, c + 1);
             em.fill(tab, P, cur);
+            if (c + 1 < J.nblk) cur.load(J, codes, lp4, r, c + 1);
             const uint32_t m0 = c * kBlock;
 #pragma unroll
             for (uint32_t t = 0; t < kBlock; ++t) {
                 const uint32_t m = m0 + t;
                 float4 an = a;
                 float lsn = ls;
-                const double inc = fwd_step<LPS, LOG>(an, lsn, em.e[t], em.off[t], cur.coef(t));
+                const double inc = fwd_step<LPS, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
                 const bool live = m > 0 && m < J.M;
                 a = sel4(live, an, a);
                 ls = live ? lsn : ls;
@@ -329,7 +335,6 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
                 if (t == 0)
                     reinterpret_cast<float4*>(J.ck_alpha)[(size_t(s) * J.nblk + c) * lp4 + r] = a;
             }
-            cur = nxt;
         }
         if (LOG) ll += double(ls);  // log mass of the last normalized vector
         if (r == 0 && J.loglik) J.loglik[s] = ll;
@@ -341,15 +346,15 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
         if (r == 0) J.ck_bl[last] = Bl;
         cur.load(J, codes, lp4, r, J.nblk - 1);
         for (int c = int(J.nblk) - 1; c >= 0; --c) {
-            if (c > 0) nxt.load(J, codes, lp4, r, uint32_t(c - 1));
             em.fill(tab, P, cur);
+            if (c > 0) cur.load(J, codes, lp4, r, uint32_t(c - 1));
             const uint32_t m0 = uint32_t(c) * kBlock;
 #pragma unroll
             for (int t = int(kBlock) - 1; t >= 0; --t) {
                 const uint32_t n = m0 + uint32_t(t);
                 float4 vn = v;
                 double Bln = Bl;
-                bwd_step<LPS, LOG>(vn, Bln, em.e[t], em.off[t], cur.coef(uint32_t(t)));
+                bwd_step<LPS, LOG>(vn, Bln, em.e[t], em.off[t], em.coef(uint32_t(t)));
                 const bool live = n > 0 && n < J.M;
                 v = sel4(live, vn, v);
                 Bl = live ? Bln : Bl;
@@ -359,7 +364,6 @@ __device__ __forceinline__ void run_chain(const JobView& J, const EmisLds& tab, 
                     if (r == 0) J.ck_bl[ck] = Bl;
                 }
             }
-            cur = nxt;
         }
     }
 }
@@ -386,7 +390,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
     if (active) {
         BlockIn<LOG, PM> in;
         in.load(J, J.codes + size_t(s) * J.pitch, lp4, r, c);
-        BlockEm em;
+        BlockEm<LOG, PM> em;
         em.fill(tab, P, in);
 #pragma unroll
         for (uint32_t t = 0; t < kBlock; ++t) sbuf[t][r] = in.sa[t];
@@ -399,7 +403,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
         for (uint32_t t = 1; t < kBlock; ++t) {
             float4 an = a;
             float lsn = ls;
-            fwd_step<LPS, LOG>(an, lsn, em.e[t], em.off[t], in.coef(t));
+            fwd_step<LPS, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
             const bool live = t < nm;
             a = sel4(live, an, a);
             ls = live ? lsn : ls;
@@ -426,7 +430,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const EmisLds& tab, 
             if (t > 0) {
                 float4 vn = v;
                 double Bln = Bl;
-                bwd_step<LPS, LOG>(vn, Bln, em.e[t], em.off[t], in.coef(uint32_t(t)));
+                bwd_step<LPS, LOG>(vn, Bln, em.e[t], em.off[t], em.coef(uint32_t(t)));
                 const bool live = uint32_t(t) < nm;
                 v = sel4(live, vn, v);
                 Bl = live ? Bln : Bl;

@@ -337,14 +337,31 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_kernel(PbwtArgs a) {
 // Element word: haplotype | group id << 16; LSD radix over 8-bit digits of
 // the id.  Lanes of a round with the same digit agree on all 8 digit-bit
 // ballots; the wave's running count per digit lives in LDS and is advanced by
-// the highest lane of each digit.
+// the highest lane of each digit.  The exchange buffers hold every position a
+// wave owns (16 * S >= H), so loads and stores need no validity branch; the
+// rounds run in batches of kBatch with the batch's loads issued together.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kBatch = 8;
+
+__device__ __forceinline__ uint64_t digit_match(uint32_t d, bool valid) {
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = ((d >> b) & 1u) != 0u;
+        const uint64_t Bb = __ballot(valid && bit);
+        match &= bit ? Bb : ~Bb;
+    }
+    return match;
+}
+
 __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
     __shared__ uint32_t cnt[kPW][256];  // per-wave digit counts, then running offsets
     __shared__ uint32_t dbase[256];     // exclusive prefix of the digit totals
     __shared__ uint32_t dsum[4];
     const WaveCtx q(a);
     const uint32_t K = a.S / 64u;
+    const uint32_t npos = kPW * a.S;    // padded buffer length
+    const uint32_t own = q.base + q.lane;
 
     for (uint32_t c = 1; c < a.nchunk; ++c) {
         const uint32_t* prev = a.prefix + size_t(c * a.C - 1u) * a.H;
@@ -358,48 +375,49 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
             __syncthreads();
             continue;
         }
-        // element words, gathered once: haplotype | group id << 16, into the
-        // buffer pass 0 reads
+        // element words, gathered once into the buffer pass 0 reads
         {
-            uint32_t* X0 = a.xb + a.H;
-#pragma unroll 8
-            for (uint32_t j = 0; j < K; ++j) {
-                const uint32_t h = prev[q.pos(j)];
-                const uint32_t v = h | (gid[h] << 16);
-                if (q.valid(j)) X0[q.base + j * 64u + q.lane] = v;
+            uint32_t* X0 = a.xb + npos;
+            for (uint32_t j0 = 0; j0 < K; j0 += kBatch) {
+                uint32_t h[kBatch];
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) h[u] = prev[min(own + (j0 + u) * 64u, q.last)];
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) h[u] |= gid[h[u] & 0xffffu] << 16;
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u)
+                    if (j0 + u < K) X0[own + (j0 + u) * 64u] = h[u];
             }
             __syncthreads();
         }
         for (uint32_t pass = 0; pass < npass; ++pass) {
             const uint32_t sh = 16u + 8u * pass;
-            const uint32_t* X = a.xb + ((pass + 1u) & 1u) * a.H;  // this pass's input
-            uint32_t* Y = (pass + 1 == npass) ? nullptr : a.xb + (pass & 1u) * a.H;
-            auto elem = [&](uint32_t j) -> uint32_t { return X[q.pos(j)]; };
-            auto match_of = [&](bool valid, uint32_t d) -> uint64_t {
-                uint64_t match = __ballot(valid);
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const bool bit = ((d >> b) & 1u) != 0u;
-                    const uint64_t Bb = __ballot(valid && bit);
-                    match &= bit ? Bb : ~Bb;
-                }
-                return match;
-            };
+            const uint32_t* X = a.xb + ((pass + 1u) & 1u) * npos;  // this pass's input
+            const bool last = pass + 1 == npass;
+            uint32_t* Y = a.xb + (pass & 1u) * npos;
             for (uint32_t i = q.t; i < uint32_t(kPW) * 256u; i += kPT) (&cnt[0][0])[i] = 0u;
             __syncthreads();
-#pragma unroll 4
-            for (uint32_t j = 0; j < K; ++j) {
-                const bool valid = q.valid(j);
-                const uint32_t d = (elem(j) >> sh) & 255u;
-                const uint64_t match = match_of(valid, d);
-                if (valid && (match >> q.lane) == 1ull) cnt[q.w][d] += uint32_t(__popcll(match));
+            for (uint32_t j0 = 0; j0 < K; j0 += kBatch) {
+                uint32_t v[kBatch];
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) v[u] = X[own + min(j0 + u, K - 1u) * 64u];
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    if (j0 + u < K) {
+                        const bool valid = q.valid(j0 + u);
+                        const uint32_t d = (v[u] >> sh) & 255u;
+                        const uint64_t match = digit_match(d, valid);
+                        if (valid && (match >> q.lane) == 1ull)
+                            cnt[q.w][d] += uint32_t(__popcll(match));
+                    }
+                }
             }
             __syncthreads();
             uint32_t tot = 0, incl = 0;
             if (q.t < 256u) {
-                for (int v = 0; v < kPW; ++v) {
-                    const uint32_t x = cnt[v][q.t];
-                    cnt[v][q.t] = tot;
+                for (int w = 0; w < kPW; ++w) {
+                    const uint32_t x = cnt[w][q.t];
+                    cnt[w][q.t] = tot;
                     tot += x;
                 }
                 incl = tot;
@@ -413,22 +431,28 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
             __syncthreads();
             if (q.t < 256u) {
                 uint32_t off = 0;
-                for (uint32_t v = 0; v < q.w; ++v) off += dsum[v];
+                for (uint32_t w = 0; w < q.w; ++w) off += dsum[w];
                 dbase[q.t] = off + incl - tot;
             }
             __syncthreads();
-#pragma unroll 4
-            for (uint32_t j = 0; j < K; ++j) {
-                const bool valid = q.valid(j);
-                const uint32_t v = elem(j);
-                const uint32_t d = (v >> sh) & 255u;
-                const uint64_t match = match_of(valid, d);
-                const uint32_t c0 = cnt[q.w][d];
-                const uint32_t dst = dbase[d] + c0 + mbcnt64(match);
-                if (valid) {
-                    if (Y) Y[dst] = v;
-                    else out[dst] = v & 0xffffu;
-                    if ((match >> q.lane) == 1ull) cnt[q.w][d] = c0 + uint32_t(__popcll(match));
+            for (uint32_t j0 = 0; j0 < K; j0 += kBatch) {
+                uint32_t v[kBatch];
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) v[u] = X[own + min(j0 + u, K - 1u) * 64u];
+#pragma unroll
+                for (uint32_t u = 0; u < kBatch; ++u) {
+                    if (j0 + u < K) {
+                        const bool valid = q.valid(j0 + u);
+                        const uint32_t d = (v[u] >> sh) & 255u;
+                        const uint64_t match = digit_match(d, valid);
+                        const uint32_t c0 = cnt[q.w][d];
+                        const uint32_t dst = dbase[d] + c0 + mbcnt64(match);
+                        if (valid) {
+                            if (last) out[dst] = v[u] & 0xffffu;
+                            else Y[dst] = v[u];
+                            if ((match >> q.lane) == 1ull) cnt[q.w][d] = c0 + uint32_t(__popcll(match));
+                        }
+                    }
                 }
             }
             __syncthreads();
@@ -548,16 +572,13 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_lds_kernel(PbwtArgs a) {
             const uint32_t p = q.base + uint32_t(j) * 64u + q.lane;
             pk[j] = ord[p] | (((fl[cur][p >> 5] >> (p & 31u)) & 1u) << 31);
         }
-        uint32_t xb[K];
-#pragma unroll
-        for (int j = 0; j < K; ++j) xb[j] = allele_bit(cb, pk[j] & 0xffffu);
         uint32_t zsum = 0;
         GroupRun g;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
             const bool valid = q.valid(uint32_t(j));
             const uint32_t v = pk[j];
-            const uint32_t x = xb[j] | uint32_t(!valid);
+            const uint32_t x = allele_bit(cb, v & 0xffffu) | uint32_t(!valid);
             const uint64_t Z = __ballot(x == 0u);
             const uint64_t O = __ballot(valid && x != 0u);
             const uint64_t Bm = __ballot(valid && (v >> 31) != 0u);
@@ -675,7 +696,8 @@ uint32_t pbwt_default_chunk(uint32_t H, uint32_t M) {
 size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C) {
     if (H == 0 || M == 0 || C == 0) return 0;
     const size_t n = (size_t(M) + C - 1) / C, h = H;
-    return align256(n * 2 * h * 4) + align256(n * h * 4) + align256(n * 4) + align256(2 * h * 4);
+    const size_t npos = size_t(kPW) * 64u * pbwt_rounds(H);  // padded exchange length
+    return align256(n * 2 * h * 4) + align256(n * h * 4) + align256(n * 4) + align256(2 * npos * 4);
 }
 
 int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
