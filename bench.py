@@ -147,29 +147,64 @@ def bench_batch(args):
     torch.cuda.empty_cache()
     log(f"[batch] plain {plain_ms:.1f} ms/step")
 
-    # batched: persistent work-queue launches replayed from a hipGraph
+    # batched: persistent work-queue launches replayed from a hipGraph, lane
+    # shapes and checkpoint block picked per class by the autotuner
     batch = device.Batch(cjobs)
+    sync()
+    t0 = time.perf_counter()
+    batch.tune(stream=stream)
+    sync()
+    tune_s = time.perf_counter() - t0
+    classes = batch.classes()
+    for lp, lg, var, _ in classes:
+        log(f"[batch] class Lp={lp}{' log' if lg else ''}: chain {var[0]} lanes/sample, "
+            f"block {var[1]} lanes/sample, checkpoint block {var[2]}")
+    log(f"[batch] autotuning took {tune_s:.2f} s")
+
+    def time_graph(steps):
+        for _ in range(args.warmup):
+            batch.run(use_graph=True, stream=stream)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.run(use_graph=True, stream=stream)
+        sync()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
     batch.run(use_graph=False, stream=stream)   # eager once (also a correctness reference)
     sync()
     log(f"[batch] batched: {args.warmup} warmup + {args.steps} timed graph replays "
         f"(workspace {batch.ws_bytes / 2**30:.2f} GiB)")
-    for _ in range(args.warmup):
-        batch.run(use_graph=True, stream=stream)
-    sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.run(use_graph=True, stream=stream)
-    sync()
-    batch_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    batch_ms = time_graph(args.steps)
     log(f"[batch] batched {batch_ms:.1f} ms/step")
     equal = all(torch.equal(j.out[k], r[k]) for j, r in zip(jobs, ref) for k in r)
     if not equal:
         raise SystemExit("batched outputs differ from the plain path")
+    # the same batch with every class at the default variant
+    for k, (lp, _, _, _) in enumerate(classes):
+        batch.set_variant(k, device.default_variant(lp))
+    default_ms = time_graph(max(1, min(args.steps, 2)))
+    log(f"[batch] batched with default variants {default_ms:.1f} ms/step")
+    equal = all(torch.equal(j.out[k], r[k]) for j, r in zip(jobs, ref) for k in r)
+    if not equal:
+        raise SystemExit("batched outputs at the default variants differ from the plain path")
     batch.close()
+    autotune = {
+        "tune_s": tune_s,
+        "classes": [{"padded_states": lp, "logspace": lg, "chain_lanes_per_sample": v[0],
+                     "block_lanes_per_sample": v[1], "checkpoint_block": v[2]}
+                    for lp, lg, v, _ in classes],
+        "tuned_ms_per_step": batch_ms,
+        "default_variant_ms_per_step": default_ms,
+        "candidates": "lanes per sample Lp/8, Lp/4, Lp/2 (chain), Lp/4, Lp/2 (block); "
+                      "checkpoint block 16 or 32; each candidate warmed on 16-marker cuts, "
+                      "then timed once on each class's whole jobs",
+    }
     return {
         "cpu_baseline": cpu,
         "units": units, "untyped": untyped, "batch_ms": batch_ms, "plain_ms": plain_ms,
         "plain_steps": plain_steps, "n_jobs": n_jobs, "n_logspace_jobs": n_log,
+        "autotune": autotune,
         "L_counts": {str(L): sum(sp.L == L for sp in specs) for L in (4, 8, 16)},
     }
 
@@ -617,6 +652,7 @@ def main():
                 "speedup": b["plain_ms"] / b["batch_ms"],
                 "outputs_bit_equal": True,
             },
+            "autotune": b["autotune"],
         })
     elif "plain" in res:
         p = res["plain"]

@@ -178,8 +178,30 @@ typedef struct si_fb_job {
     double* loglik;       /* [B] device or NULL */
 } si_fb_job;
 
-/* Workspace (device bytes) si_fb_run needs for these jobs. */
+/* Workspace (device bytes) si_fb_run needs for these jobs (enough for every
+ * kernel variant). */
 size_t si_fb_workspace_bytes(const si_fb_job* jobs, uint32_t n_jobs);
+
+/* Kernel variant: lanes of a wavefront per sample in the chain pass and in
+ * the block pass (states per lane = Lp / lanes, Lp = si_padded_states(L)),
+ * and the checkpoint block length in markers (16 or 32; 32 halves checkpoint
+ * memory for 16 extra forward steps per block).  Every sum over states is
+ * the same binary tree whatever the variant, so outputs are bit-identical
+ * across variants: the choice is performance only.  No counterpart in the
+ * reference (one thread per (sample, state), forward_backward.cu:41-317). */
+typedef struct si_fb_variant {
+    uint32_t chain_lanes;
+    uint32_t block_lanes;
+    uint32_t ckpt_block;
+} si_fb_variant;
+
+/* The variant si_fb_run uses for L states (four states per lane, 16-marker
+ * blocks). */
+int si_fb_default_variant(uint32_t n_states, si_fb_variant* out);
+
+/* Lane counts built for a pass (pass = SI_FB_PASS_CHAIN or SI_FB_PASS_BLOCK)
+ * at L states, written to lanes[0..max); returns how many exist. */
+uint32_t si_fb_lane_candidates(uint32_t n_states, uint32_t pass, uint32_t* lanes, uint32_t max);
 
 /* Plain form: one launch sequence per job, jobs processed in order on the
  * stream.  jobs is a host array. */
@@ -196,6 +218,18 @@ int si_fb_run(const si_fb_job* jobs, uint32_t n_jobs, void* workspace, size_t wo
 enum si_fb_pass { SI_FB_PASS_CHAIN = 1u, SI_FB_PASS_BLOCK = 2u };
 int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
                      size_t workspace_bytes, uint32_t passes, void* stream);
+
+/* si_fb_run_passes with an explicit variant (NULL: the default). */
+int si_fb_run_variant(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_variant* variant,
+                      void* workspace, size_t workspace_bytes, uint32_t passes, void* stream);
+
+/* Autotuner for the plain form: times every candidate variant on these jobs
+ * (one padded-state class, one numeric space), each job cut to its first
+ * max_markers markers (0: 2048) with every output disabled, so the jobs'
+ * output buffers are not touched; writes the fastest to *out.  The workspace
+ * is the one si_fb_run takes for these jobs.  Synchronizes the stream. */
+int si_fb_tune(const si_fb_job* jobs, uint32_t n_jobs, void* workspace, size_t workspace_bytes,
+               uint32_t max_markers, void* stream, si_fb_variant* out);
 
 /* ---------------------------------------------------------- batched form --
  * Many jobs of uneven cohort size, marker count and L in one persistent
@@ -214,10 +248,29 @@ typedef struct si_fb_batch si_fb_batch;
 int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out);
 size_t si_fb_batch_workspace_bytes(const si_fb_batch* batch);
 /* use_graph: capture the launches into a hipGraph on first run for this
- * (workspace, stream) and replay it afterwards. */
+ * (workspace, stream) and replay it afterwards.  A class without a variant
+ * set or tuned is tuned first (si_fb_batch_tune with the defaults), before
+ * any capture. */
 int si_fb_batch_run(si_fb_batch* batch, void* workspace, size_t workspace_bytes, int use_graph,
                     void* stream);
 int si_fb_batch_destroy(si_fb_batch* batch);
+
+/* Batch classes (one per padded-state count and numeric space) and their
+ * variants.  A batch tunes each class with its own persistent kernels on its
+ * own jobs (cut to max_markers markers, 0: whole jobs), outputs disabled:
+ * every candidate is warmed on 16-marker cuts, then timed once per pass,
+ * about five runs of the class in all.  Results are kept per process, keyed
+ * by (device, Lp, numeric space, per-marker transitions, size bucket of the
+ * class's chain count), so later batches of the same kind reuse them without
+ * timing.  Synchronizes the stream. */
+uint32_t si_fb_batch_n_classes(const si_fb_batch* batch);
+int si_fb_batch_class_info(const si_fb_batch* batch, uint32_t k, uint32_t* padded_states,
+                           uint32_t* flags, si_fb_variant* variant, int* tuned);
+int si_fb_batch_set_variant(si_fb_batch* batch, uint32_t k, const si_fb_variant* variant);
+int si_fb_batch_tune(si_fb_batch* batch, void* workspace, size_t workspace_bytes,
+                     uint32_t max_markers, void* stream);
+/* Forget every tuning kept by this process (later batches time again). */
+void si_fb_tune_cache_clear(void);
 
 /* Philox4x32-10 uniform the sampler uses for (sample s, marker m): word
  * (m & 3) of Philox4x32-10(counter = {m >> 2, s, rng_stream, 0} with s the

@@ -39,6 +39,16 @@ class StepCoef(ctypes.Structure):
                 ("lu", ctypes.c_float)]
 
 
+class FbVariant(ctypes.Structure):
+    """Mirror of ``si_fb_variant``: lanes per sample in the chain and block
+    passes and the checkpoint block length (markers)."""
+    _fields_ = [("chain_lanes", ctypes.c_uint32), ("block_lanes", ctypes.c_uint32),
+                ("ckpt_block", ctypes.c_uint32)]
+
+    def as_tuple(self):
+        return (self.chain_lanes, self.block_lanes, self.ckpt_block)
+
+
 class FbJob(ctypes.Structure):
     """Mirror of ``si_fb_job`` (include/swiftimpute_hip.h)."""
     _fields_ = [
@@ -100,6 +110,21 @@ HIP_SYMBOLS = [
     ("si_fb_batch_workspace_bytes", ctypes.c_size_t, [_vp]),
     ("si_fb_batch_run", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]),
     ("si_fb_batch_destroy", ctypes.c_int, [_vp]),
+    ("si_fb_default_variant", ctypes.c_int, [ctypes.c_uint32, ctypes.POINTER(FbVariant)]),
+    ("si_fb_lane_candidates", ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                                ctypes.c_uint32]),
+    ("si_fb_run_variant", ctypes.c_int, [ctypes.POINTER(FbJob), ctypes.c_uint32,
+                                         ctypes.POINTER(FbVariant), _vp, ctypes.c_size_t,
+                                         ctypes.c_uint32, _vp]),
+    ("si_fb_tune", ctypes.c_int, [ctypes.POINTER(FbJob), ctypes.c_uint32, _vp, ctypes.c_size_t,
+                                  ctypes.c_uint32, _vp, ctypes.POINTER(FbVariant)]),
+    ("si_fb_batch_n_classes", ctypes.c_uint32, [_vp]),
+    ("si_fb_batch_class_info", ctypes.c_int, [_vp, ctypes.c_uint32, _u32p, _u32p,
+                                              ctypes.POINTER(FbVariant),
+                                              ctypes.POINTER(ctypes.c_int)]),
+    ("si_fb_batch_set_variant", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.POINTER(FbVariant)]),
+    ("si_fb_batch_tune", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_uint32, _vp]),
+    ("si_fb_tune_cache_clear", None, []),
     ("si_sample_uniform", ctypes.c_float, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
                                            ctypes.c_uint32]),
 ]

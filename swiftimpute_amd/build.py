@@ -26,9 +26,11 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = "gfx950"
 
 # Packed-FP32 instructions are disabled for every HIP object: the kernels use
-# ordinary scalar FP32 arithmetic only.
+# ordinary scalar FP32 arithmetic only.  No floating-point contraction: the
+# forward-backward kernels fuse multiply-adds only where written (fmaf), so
+# their arithmetic is the same for every lane shape.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",
@@ -37,7 +39,11 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
               f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
 HIP_SOURCES = ["hip/capi.hip", "hip/index.hip", "hip/pbwt.hip", "hip/hmm.hip"]
-HIP_HEADERS = ["hip/si_device.hpp", "hip/si_internal.hpp"]
+HIP_HEADERS = ["hip/si_device.hpp", "hip/si_internal.hpp", "hip/fb_kernels.hpp"]
+# Forward-backward lane shapes (padded states Lp, lanes per sample G), one
+# object each from hip/fb_inst.hip: 8, 4 and 2 states per lane, at most 32 lanes.
+FB_SHAPES = [(4, 1), (4, 2), (8, 1), (8, 2), (8, 4), (16, 2), (16, 4), (16, 8),
+             (32, 4), (32, 8), (32, 16), (64, 8), (64, 16), (64, 32)]
 API_SOURCES = ["api/panel.cpp", "api/imputer.cpp", "api/synth.cpp", "api/server.cpp"]
 
 
@@ -71,6 +77,14 @@ def build(force: bool = True, verbose: bool = False) -> dict:
         hip_objs.append(o)
         if force or _newer(o, [s] + hdrs):
             jobs.append([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
+    inst = os.path.join(CSRC, "hip", "fb_inst.hip")
+    for lp, g in FB_SHAPES:
+        o = os.path.join(BUILD, f"fb_lp{lp}_g{g}.o")
+        hip_objs.append(o)
+        if force or _newer(o, [inst] + hdrs):
+            jobs.append([HIPCC, *HIP_FLAGS, f"-DSI_FB_LP={lp}", f"-DSI_FB_G={g}", "-c", inst,
+                         "-o", o])
+    jobs.sort(key=lambda c: 0 if any("fb_inst" in x for x in c) else 1)
     api_objs = []
     api_hdrs = hdrs + [os.path.join(INCLUDE, "swiftimpute", f)
                        for f in ("types.hpp", "imputer.hpp", "synth.h", "server.h")]

@@ -8,7 +8,9 @@
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -227,14 +229,50 @@ size_t si_fb_workspace_bytes(const si_fb_job* jobs, uint32_t n_jobs) {
     return mx;
 }
 
+int si_fb_default_variant(uint32_t n_states, si_fb_variant* out) {
+    if (!out || n_states == 0 || n_states > 64)
+        return fail(SI_ERR_ARG, "si_fb_default_variant: n_states must be in 1..64");
+    const si::FbVariant v = si::fb_default_variant(si_padded_states(n_states));
+    *out = si_fb_variant{v.gc, v.gb, v.cb};
+    return SI_OK;
+}
+
+uint32_t si_fb_lane_candidates(uint32_t n_states, uint32_t pass, uint32_t* lanes, uint32_t max) {
+    if (n_states == 0 || n_states > 64 || (pass != SI_FB_PASS_CHAIN && pass != SI_FB_PASS_BLOCK))
+        return 0;
+    return si::fb_candidates(si_padded_states(n_states), pass == SI_FB_PASS_CHAIN ? 0 : 1, lanes,
+                             lanes ? max : 0);
+}
+
 int si_fb_run(const si_fb_job* jobs, uint32_t n_jobs, void* workspace, size_t workspace_bytes,
               void* stream) {
-    return si_fb_run_passes(jobs, n_jobs, workspace, workspace_bytes,
-                            SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, stream);
+    return si_fb_run_variant(jobs, n_jobs, nullptr, workspace, workspace_bytes,
+                             SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, stream);
 }
 
 int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
                      size_t workspace_bytes, uint32_t passes, void* stream) {
+    return si_fb_run_variant(jobs, n_jobs, nullptr, workspace, workspace_bytes, passes, stream);
+}
+
+extern "C++" {
+static int check_variant(uint32_t Lp, const si_fb_variant* v, const char* what) {
+    if (!v) return SI_OK;
+    if (!si::fb_variant_valid(Lp, si::FbVariant{v->chain_lanes, v->block_lanes, v->ckpt_block}))
+        return fail(SI_ERR_UNSUPPORTED,
+                    "%s: variant (chain %u lanes, block %u lanes, checkpoint block %u) not built "
+                    "for %u padded states", what, v->chain_lanes, v->block_lanes, v->ckpt_block, Lp);
+    return SI_OK;
+}
+
+static si::FbVariant variant_or_default(uint32_t Lp, const si_fb_variant* v) {
+    return v ? si::FbVariant{v->chain_lanes, v->block_lanes, v->ckpt_block}
+             : si::fb_default_variant(Lp);
+}
+}  // extern "C++"
+
+int si_fb_run_variant(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_variant* variant,
+                      void* workspace, size_t workspace_bytes, uint32_t passes, void* stream) {
     if (n_jobs && !jobs) return fail(SI_ERR_ARG, "si_fb_run: jobs is null");
     if (passes == 0 || (passes & ~3u))
         return fail(SI_ERR_ARG, "si_fb_run_passes: passes must be a nonzero CHAIN|BLOCK mask");
@@ -243,7 +281,8 @@ int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
     if (passes != 3u && n_jobs > 1)
         return fail(SI_ERR_ARG, "si_fb_run_passes: a single pass runs one job at a time");
     for (uint32_t i = 0; i < n_jobs; ++i) {
-        const int rc = check_job(jobs[i], i);
+        int rc = check_job(jobs[i], i);
+        if (!rc) rc = check_variant(si_padded_states(jobs[i].n_states), variant, "si_fb_run_variant");
         if (rc) return rc;
     }
     const size_t need = si_fb_workspace_bytes(jobs, n_jobs);
@@ -254,24 +293,209 @@ int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
         const si_fb_job& j = jobs[i];
         if (j.n_samples == 0 || j.n_markers == 0) continue;
         const uint32_t Lp = si_padded_states(j.n_states);
-        si::JobView v = si::fb_job_view(j);
+        const si::FbVariant var = variant_or_default(Lp, variant);
+        si::JobView v = si::fb_job_view(j, var.cb);
         si::fb_bind_workspace(v, Lp, static_cast<char*>(workspace));
         const si::FbTables t{j.eps, j.emax, j.palette, j.n_palette};
-        const int rc = si::launch_fb(v, t, Lp, passes, static_cast<hipStream_t>(stream));
+        const int rc = si::launch_fb(v, t, Lp, passes, var, static_cast<hipStream_t>(stream));
         if (rc) return device_status(rc, "si_fb_run");
     }
+    return SI_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- autotuner --
+namespace {
+
+constexpr uint32_t kCkptBlocks[2] = {16, 32};
+
+// Event pair timing one launch sequence on a stream (milliseconds).
+struct Timer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = false;
+    Timer() { ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess; }
+    ~Timer() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    template <class F>
+    int time_once(hipStream_t st, F&& f, float* ms) {
+        if (hipEventRecord(e0, st) != hipSuccess) return SI_ERR_DEVICE;
+        const int rc = f();
+        if (rc) return rc;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+            return SI_ERR_DEVICE;
+        return 0;
+    }
+    // Runs f twice (the first warms caches and code objects), times the second.
+    template <class F>
+    int time(hipStream_t st, F&& f, float* ms) {
+        const int rc = f();
+        return rc ? rc : time_once(st, f, ms);
+    }
+};
+
+// A job cut to its first `cap` markers with every output disabled.
+si::JobView tuning_view(const si_fb_job& j, uint32_t cap, uint32_t cb) {
+    si_fb_job c = j;
+    c.n_markers = std::min(j.n_markers, cap);
+    c.ds = c.ap = c.prob = nullptr;
+    c.hap = nullptr;
+    c.loglik = nullptr;
+    return si::fb_job_view(c, cb);
+}
+
+struct CacheKey {
+    int device;
+    uint32_t Lp, flags, pm, bucket;
+    bool operator==(const CacheKey& o) const {
+        return device == o.device && Lp == o.Lp && flags == o.flags && pm == o.pm &&
+               bucket == o.bucket;
+    }
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<CacheKey, si::FbVariant>> g_tune_cache;
+
+uint32_t size_bucket(uint64_t chains) {
+    uint32_t b = 0;
+    while (chains >= 4 && b < 31) {
+        chains >>= 2;
+        ++b;
+    }
+    return b;
+}
+
+bool cache_get(const CacheKey& k, si::FbVariant* v) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& e : g_tune_cache)
+        if (e.first == k) {
+            *v = e.second;
+            return true;
+        }
+    return false;
+}
+
+void cache_put(const CacheKey& k, const si::FbVariant& v) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& e : g_tune_cache)
+        if (e.first == k) {
+            e.second = v;
+            return;
+        }
+    g_tune_cache.emplace_back(k, v);
+}
+
+// SI_FB_TUNE_LOG=1 prints every timing the tuners take to stderr.
+bool tune_log() {
+    static const bool on = [] {
+        const char* e = std::getenv("SI_FB_TUNE_LOG");
+        return e && *e && *e != '0';
+    }();
+    return on;
+}
+
+// Best variant from per-(checkpoint block, lanes) times of each pass.
+si::FbVariant pick(const float chain_ms[2][8], const uint32_t* gc, uint32_t nc,
+                   const float block_ms[2][8], const uint32_t* gb, uint32_t nb, uint32_t Lp) {
+    if (tune_log())
+        for (int k = 0; k < 2; ++k) {
+            for (uint32_t i = 0; i < nc; ++i)
+                std::fprintf(stderr, "[si_fb_tune] Lp=%u ckpt=%u chain lanes=%u: %.3f ms\n", Lp,
+                             kCkptBlocks[k], gc[i], double(chain_ms[k][i]));
+            for (uint32_t i = 0; i < nb; ++i)
+                std::fprintf(stderr, "[si_fb_tune] Lp=%u ckpt=%u block lanes=%u: %.3f ms\n", Lp,
+                             kCkptBlocks[k], gb[i], double(block_ms[k][i]));
+        }
+    si::FbVariant best{gc[0], gb[0], kCkptBlocks[0]};
+    float best_ms = 3.4e38f;
+    for (int k = 0; k < 2; ++k) {
+        uint32_t ic = 0, ib = 0;
+        for (uint32_t i = 1; i < nc; ++i)
+            if (chain_ms[k][i] < chain_ms[k][ic]) ic = i;
+        for (uint32_t i = 1; i < nb; ++i)
+            if (block_ms[k][i] < block_ms[k][ib]) ib = i;
+        const float t = chain_ms[k][ic] + block_ms[k][ib];
+        if (t < best_ms) {
+            best_ms = t;
+            best = si::FbVariant{gc[ic], gb[ib], kCkptBlocks[k]};
+        }
+    }
+    return best;
+}
+
+}  // namespace
+
+extern "C" {
+
+int si_fb_tune(const si_fb_job* jobs, uint32_t n_jobs, void* workspace, size_t workspace_bytes,
+               uint32_t max_markers, void* stream_v, si_fb_variant* out) {
+    if (!jobs || n_jobs == 0 || !out) return fail(SI_ERR_ARG, "si_fb_tune: bad arguments");
+    for (uint32_t i = 0; i < n_jobs; ++i) {
+        const int rc = check_job(jobs[i], i);
+        if (rc) return rc;
+        if (si_padded_states(jobs[i].n_states) != si_padded_states(jobs[0].n_states) ||
+            (jobs[i].flags & SI_FB_LOGSPACE) != (jobs[0].flags & SI_FB_LOGSPACE))
+            return fail(SI_ERR_ARG, "si_fb_tune: jobs must share padded states and numeric space");
+    }
+    const size_t need = si_fb_workspace_bytes(jobs, n_jobs);
+    if (need && (!workspace || workspace_bytes < need))
+        return fail(SI_ERR_WORKSPACE, "si_fb_tune: workspace of %zu bytes needed, %zu given", need,
+                    workspace_bytes);
+    hipStream_t st = static_cast<hipStream_t>(stream_v);
+    const uint32_t cap = max_markers ? max_markers : 2048u;
+    const uint32_t Lp = si_padded_states(jobs[0].n_states);
+    uint32_t gc[8], gb[8];
+    const uint32_t nc = std::min(8u, si::fb_candidates(Lp, 0, gc, 8));
+    const uint32_t nb = std::min(8u, si::fb_candidates(Lp, 1, gb, 8));
+    if (!nc || !nb) return fail(SI_ERR_UNSUPPORTED, "si_fb_tune: no kernels for %u states", Lp);
+    float chain_ms[2][8] = {}, block_ms[2][8] = {};
+    Timer tm;
+    if (!tm.ok) return device_status(SI_ERR_DEVICE, "si_fb_tune (events)");
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t cb = kCkptBlocks[k];
+        for (uint32_t i = 0; i < n_jobs; ++i) {
+            const si_fb_job& j = jobs[i];
+            if (j.n_samples == 0 || j.n_markers == 0) continue;
+            si::JobView v = tuning_view(j, cap, cb);
+            si::fb_bind_workspace(v, Lp, static_cast<char*>(workspace));
+            const si::FbTables t{j.eps, j.emax, j.palette, j.n_palette};
+            for (uint32_t a = 0; a < nc; ++a) {
+                const si::FbVariant var{gc[a], gb[0], cb};
+                float ms = 0.0f;
+                const int rc = tm.time(st, [&] { return si::launch_fb(v, t, Lp, SI_FB_PASS_CHAIN, var, st); }, &ms);
+                if (rc) return device_status(rc, "si_fb_tune");
+                chain_ms[k][a] += ms;
+            }
+            for (uint32_t a = 0; a < nb; ++a) {
+                const si::FbVariant var{gc[0], gb[a], cb};
+                float ms = 0.0f;
+                const int rc = tm.time(st, [&] { return si::launch_fb(v, t, Lp, SI_FB_PASS_BLOCK, var, st); }, &ms);
+                if (rc) return device_status(rc, "si_fb_tune");
+                block_ms[k][a] += ms;
+            }
+        }
+    }
+    const si::FbVariant best = pick(chain_ms, gc, nc, block_ms, gb, nb, Lp);
+    *out = si_fb_variant{best.gc, best.gb, best.cb};
     return SI_OK;
 }
 
 // ------------------------------------------------------------ batched form --
 struct si_fb_batch {
     struct Class {
-        uint32_t Lp = 0, flags = 0;
+        uint32_t Lp = 0, flags = 0, pm = 0;
         std::vector<uint32_t> order;  // job indices, longest first
         si::JobView* d_jobs = nullptr;
         uint64_t* d_off = nullptr;    // chain offsets [n+1] then block offsets [n+1]
-        uint32_t* d_counters = nullptr;
+        si::JobView* d_tjobs = nullptr;  // tuning copies
+        uint64_t* d_toff = nullptr;
+        unsigned long long* d_counters = nullptr;
         uint64_t chain_items = 0, block_items = 0;
+        uint32_t off_cb = 0;          // checkpoint block the offsets were made for
+        si::FbVariant var{};
+        bool tuned = false, fixed = false;
         uint32_t grid_chain = 1, grid_block = 1;
     };
     std::vector<si_fb_job> jobs;
@@ -289,11 +513,59 @@ static void batch_free(si_fb_batch* b) {
     for (auto& c : b->classes) {
         if (c.d_jobs) (void)hipFree(c.d_jobs);
         if (c.d_off) (void)hipFree(c.d_off);
+        if (c.d_tjobs) (void)hipFree(c.d_tjobs);
+        if (c.d_toff) (void)hipFree(c.d_toff);
         if (c.d_counters) (void)hipFree(c.d_counters);
     }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     delete b;
 }
+
+extern "C++" {
+// Item offsets of a class for checkpoint block cb (markers capped at cap).
+static void class_offsets(const si_fb_batch& b, const si_fb_batch::Class& c, uint32_t cb,
+                          uint32_t cap, std::vector<uint64_t>& off) {
+    const size_t n = c.order.size();
+    off.assign(2 * (n + 1), 0);
+    for (size_t k = 0; k < n; ++k) {
+        const si_fb_job& j = b.jobs[c.order[k]];
+        const uint64_t M = std::min(j.n_markers, cap);
+        const uint64_t nblk = (M + cb - 1) / cb;
+        off[k + 1] = off[k] + 2ull * j.n_samples;
+        // block items padded to whole wavefront rounds (see fb_block_batch_kernel)
+        off[n + 1 + k + 1] = off[n + 1 + k] + ((uint64_t(j.n_samples) * nblk + 63) & ~uint64_t(63));
+    }
+}
+
+// Offsets and persistent grids of a class for its current variant.
+static int class_prepare(si_fb_batch* b, si_fb_batch::Class& c) {
+    if (c.off_cb != c.var.cb) {
+        std::vector<uint64_t> off;
+        class_offsets(*b, c, c.var.cb, 0xffffffffu, off);
+        if (hipMemcpy(c.d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            return SI_ERR_DEVICE;
+        const size_t n = c.order.size();
+        c.chain_items = off[n];
+        c.block_items = off[2 * n + 1];
+        c.off_cb = c.var.cb;
+        b->bound_ws = nullptr;  // job views carry the block count: rebind
+        if (b->exec) {
+            (void)hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+        }
+    }
+    si::BatchClass bc{};
+    bc.Lp = c.Lp;
+    bc.flags = c.flags;
+    bc.chain_items = c.chain_items;
+    bc.block_items = c.block_items;
+    si::fb_batch_grid(bc, c.var, b->tables.P);
+    c.grid_chain = bc.grid_chain;
+    c.grid_block = bc.grid_block;
+    return SI_OK;
+}
+}  // extern "C++"
 
 int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out) {
     if (!out || (n_jobs && !jobs)) return fail(SI_ERR_ARG, "si_fb_batch_create: bad arguments");
@@ -328,36 +600,22 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
             it = b->classes.end() - 1;
             it->Lp = Lp;
             it->flags = fl;
+            it->var = si::fb_default_variant(Lp);
         }
         it->order.push_back(i);
+        if (jobs[i].flags & SI_FB_PER_MARKER) it->pm = 1;
     }
     for (auto& c : b->classes) {
         std::stable_sort(c.order.begin(), c.order.end(), [&](uint32_t x, uint32_t y) {
             return jobs[x].n_markers > jobs[y].n_markers;
         });
         const size_t n = c.order.size();
-        std::vector<uint64_t> off(2 * (n + 1), 0);
-        for (size_t k = 0; k < n; ++k) {
-            const si_fb_job& j = jobs[c.order[k]];
-            const uint64_t nblk = (j.n_markers + si::kBlock - 1) / si::kBlock;
-            off[k + 1] = off[k] + 2ull * j.n_samples;
-            off[n + 1 + k + 1] = off[n + 1 + k] + uint64_t(j.n_samples) * nblk;
-        }
-        c.chain_items = off[n];
-        c.block_items = off[2 * n + 1];
-        si::BatchClass bc{};
-        bc.Lp = c.Lp;
-        bc.flags = c.flags;
-        bc.chain_items = c.chain_items;
-        bc.block_items = c.block_items;
-        si::fb_batch_grid(bc);
-        c.grid_chain = bc.grid_chain;
-        c.grid_block = bc.grid_block;
         if (hipMalloc(&c.d_jobs, n * sizeof(si::JobView)) != hipSuccess ||
-            hipMalloc(&c.d_off, off.size() * sizeof(uint64_t)) != hipSuccess ||
-            hipMalloc(&c.d_counters, 2 * sizeof(uint32_t)) != hipSuccess ||
-            hipMemcpy(c.d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) !=
-                hipSuccess) {
+            hipMalloc(&c.d_off, 2 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc(&c.d_tjobs, n * sizeof(si::JobView)) != hipSuccess ||
+            hipMalloc(&c.d_toff, 2 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc(&c.d_counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
+            class_prepare(b, c) != SI_OK) {
             batch_free(b);
             return device_status(SI_ERR_DEVICE, "si_fb_batch_create");
         }
@@ -368,13 +626,156 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
 
 size_t si_fb_batch_workspace_bytes(const si_fb_batch* b) { return b ? b->ws_bytes : 0; }
 
+uint32_t si_fb_batch_n_classes(const si_fb_batch* b) { return b ? uint32_t(b->classes.size()) : 0u; }
+
+int si_fb_batch_class_info(const si_fb_batch* b, uint32_t k, uint32_t* padded_states,
+                           uint32_t* flags, si_fb_variant* variant, int* tuned) {
+    if (!b || k >= b->classes.size()) return fail(SI_ERR_ARG, "si_fb_batch_class_info: no class %u", k);
+    const auto& c = b->classes[k];
+    if (padded_states) *padded_states = c.Lp;
+    if (flags) *flags = c.flags;
+    if (variant) *variant = si_fb_variant{c.var.gc, c.var.gb, c.var.cb};
+    if (tuned) *tuned = c.tuned ? 1 : 0;
+    return SI_OK;
+}
+
+int si_fb_batch_set_variant(si_fb_batch* b, uint32_t k, const si_fb_variant* v) {
+    if (!b || !v || k >= b->classes.size())
+        return fail(SI_ERR_ARG, "si_fb_batch_set_variant: bad arguments");
+    auto& c = b->classes[k];
+    const int rc = check_variant(c.Lp, v, "si_fb_batch_set_variant");
+    if (rc) return rc;
+    const si::FbVariant nv{v->chain_lanes, v->block_lanes, v->ckpt_block};
+    if (nv.gc != c.var.gc || nv.gb != c.var.gb || nv.cb != c.var.cb) {
+        c.var = nv;
+        if (b->exec) {
+            (void)hipGraphExecDestroy(b->exec);
+            b->exec = nullptr;
+        }
+    }
+    c.fixed = true;
+    c.tuned = false;
+    if (class_prepare(b, c) != SI_OK) return device_status(SI_ERR_DEVICE, "si_fb_batch_set_variant");
+    return SI_OK;
+}
+
+extern "C++" {
+// Times every candidate of a class with the batched kernels on its jobs cut
+// to `cap` markers, outputs disabled, checkpoints in the batch workspace.
+static int class_tune(si_fb_batch* b, si_fb_batch::Class& c, char* ws, uint32_t cap,
+                      hipStream_t st, si::FbVariant* best) {
+    uint32_t gc[8], gb[8];
+    const uint32_t nc = std::min(8u, si::fb_candidates(c.Lp, 0, gc, 8));
+    const uint32_t nb = std::min(8u, si::fb_candidates(c.Lp, 1, gb, 8));
+    if (!nc || !nb) return SI_ERR_UNSUPPORTED;
+    float chain_ms[2][8] = {}, block_ms[2][8] = {};
+    Timer tm;
+    if (!tm.ok) return SI_ERR_DEVICE;
+    const size_t n = c.order.size();
+    // Job views cut to `mk` markers at checkpoint block cb, on the device.
+    auto upload = [&](uint32_t cb, uint32_t mk, si::BatchClass& bc) -> int {
+        std::vector<si::JobView> v(n);
+        for (size_t i = 0; i < n; ++i) {
+            const uint32_t ji = c.order[i];
+            v[i] = tuning_view(b->jobs[ji], mk, cb);
+            si::fb_bind_workspace(v[i], c.Lp, ws + b->ws_off[ji]);
+        }
+        std::vector<uint64_t> off;
+        class_offsets(*b, c, cb, mk, off);
+        if (hipMemcpy(c.d_tjobs, v.data(), n * sizeof(si::JobView), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c.d_toff, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) != hipSuccess)
+            return SI_ERR_DEVICE;
+        bc = si::BatchClass{};
+        bc.Lp = c.Lp;
+        bc.n_jobs = uint32_t(n);
+        bc.flags = c.flags;
+        bc.jobs = c.d_tjobs;
+        bc.chain_off = c.d_toff;
+        bc.block_off = c.d_toff + n + 1;
+        bc.chain_items = off[n];
+        bc.block_items = off[2 * n + 1];
+        bc.counters = c.d_counters;
+        return 0;
+    };
+    for (int k = 0; k < 2; ++k) {
+        const uint32_t cb = kCkptBlocks[k];
+        si::BatchClass bc{};
+        // warm every candidate (code objects, caches) on 16-marker cuts
+        int rc = upload(cb, 16, bc);
+        if (rc) return rc;
+        for (uint32_t a = 0; a < std::max(nc, nb) && !rc; ++a) {
+            const si::FbVariant var{gc[std::min(a, nc - 1)], gb[std::min(a, nb - 1)], cb};
+            si::fb_batch_grid(bc, var, b->tables.P);
+            rc = si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, var, st);
+        }
+        if (rc || hipStreamSynchronize(st) != hipSuccess) return rc ? rc : SI_ERR_DEVICE;
+        // then one timed run per candidate on the jobs cut to `cap`
+        rc = upload(cb, cap, bc);
+        if (rc) return rc;
+        for (uint32_t a = 0; a < nc; ++a) {
+            const si::FbVariant var{gc[a], gb[0], cb};
+            si::fb_batch_grid(bc, var, b->tables.P);
+            rc = tm.time_once(st, [&] { return si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN, var, st); },
+                              &chain_ms[k][a]);
+            if (rc) return rc;
+        }
+        for (uint32_t a = 0; a < nb; ++a) {
+            const si::FbVariant var{gc[0], gb[a], cb};
+            si::fb_batch_grid(bc, var, b->tables.P);
+            rc = tm.time_once(st, [&] { return si::launch_fb_batch(bc, b->tables, SI_FB_PASS_BLOCK, var, st); },
+                              &block_ms[k][a]);
+            if (rc) return rc;
+        }
+    }
+    *best = pick(chain_ms, gc, nc, block_ms, gb, nb, c.Lp);
+    return SI_OK;
+}
+}  // extern "C++"
+
+void si_fb_tune_cache_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache.clear();
+}
+
+int si_fb_batch_tune(si_fb_batch* b, void* workspace, size_t workspace_bytes, uint32_t max_markers,
+                     void* stream_v) {
+    if (!b) return fail(SI_ERR_ARG, "si_fb_batch_tune: null batch");
+    if (b->ws_bytes && (!workspace || workspace_bytes < b->ws_bytes))
+        return fail(SI_ERR_WORKSPACE, "si_fb_batch_tune: workspace of %zu bytes needed, %zu given",
+                    b->ws_bytes, workspace_bytes);
+    hipStream_t st = static_cast<hipStream_t>(stream_v);
+    const uint32_t cap = max_markers ? max_markers : 0xffffffffu;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    for (auto& c : b->classes) {
+        if (c.fixed) continue;
+        const CacheKey key{dev, c.Lp, c.flags, c.pm, size_bucket(c.chain_items)};
+        si::FbVariant v{};
+        if (!cache_get(key, &v)) {
+            const int rc = class_tune(b, c, static_cast<char*>(workspace), cap, st, &v);
+            if (rc) return device_status(rc, "si_fb_batch_tune");
+            cache_put(key, v);
+        }
+        if (v.gc != c.var.gc || v.gb != c.var.gb || v.cb != c.var.cb) {
+            c.var = v;
+            if (b->exec) {
+                (void)hipGraphExecDestroy(b->exec);
+                b->exec = nullptr;
+            }
+        }
+        c.tuned = true;
+        if (class_prepare(b, c) != SI_OK) return device_status(SI_ERR_DEVICE, "si_fb_batch_tune");
+    }
+    return SI_OK;
+}
+
 // Writes the device job tables with checkpoint pointers into `ws`.
 static int batch_bind(si_fb_batch* b, void* ws, hipStream_t stream) {
     for (auto& c : b->classes) {
         std::vector<si::JobView> v(c.order.size());
         for (size_t k = 0; k < c.order.size(); ++k) {
             const uint32_t i = c.order[k];
-            v[k] = si::fb_job_view(b->jobs[i]);
+            v[k] = si::fb_job_view(b->jobs[i], c.var.cb);
             si::fb_bind_workspace(v[k], c.Lp, static_cast<char*>(ws) + b->ws_off[i]);
         }
         if (hipMemcpyAsync(c.d_jobs, v.data(), v.size() * sizeof(si::JobView),
@@ -400,7 +801,8 @@ static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
         bc.counters = c.d_counters;
         bc.grid_chain = c.grid_chain;
         bc.grid_block = c.grid_block;
-        const int rc = si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, stream);
+        const int rc = si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, c.var,
+                                           stream);
         if (rc) return device_status(rc, "si_fb_batch_run");
     }
     return SI_OK;
@@ -413,6 +815,12 @@ int si_fb_batch_run(si_fb_batch* b, void* workspace, size_t workspace_bytes, int
         return fail(SI_ERR_WORKSPACE, "si_fb_batch_run: workspace of %zu bytes needed, %zu given",
                     b->ws_bytes, workspace_bytes);
     hipStream_t stream = static_cast<hipStream_t>(stream_v);
+    for (const auto& c : b->classes)
+        if (!c.fixed && !c.tuned) {
+            const int rc = si_fb_batch_tune(b, workspace, workspace_bytes, 0, stream_v);
+            if (rc) return rc;
+            break;
+        }
     if (workspace != b->bound_ws) {
         const int rc = batch_bind(b, workspace, stream);
         if (rc) return rc;

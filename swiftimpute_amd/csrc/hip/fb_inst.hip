@@ -1,0 +1,96 @@
+// One lane shape of the forward-backward kernels: compiled once per
+// (SI_FB_LP padded states, SI_FB_G lanes per sample), each object registering
+// its launchers with the host side (hmm.hip) when the library loads.  Block
+// kernels exist for R = Lp / G <= 4; R = 8 is a chain-pass shape only.
+#include "fb_kernels.hpp"
+
+#if !defined(SI_FB_LP) || !defined(SI_FB_G)
+#error "compile with -DSI_FB_LP=<padded states> -DSI_FB_G=<lanes per sample>"
+#endif
+
+namespace si {
+namespace {
+
+constexpr int kLp = SI_FB_LP, kG = SI_FB_G, kR = kLp / kG;
+constexpr bool kHasBlock = kR <= 4;
+
+template <bool LOG, int CB>
+int launch_t(const FbLaunch& L) {
+    const size_t dyn = fb_table_lds_bytes(L.tb.P);
+    const dim3 grid(L.grid), block(kFbThreads);
+    switch (L.form) {
+        case kFbPlainChain:
+            hipLaunchKernelGGL((fb_chain_kernel<kG, kR, LOG, CB>), grid, block, dyn, L.stream, *L.job,
+                               L.tb, L.Bpad);
+            break;
+        case kFbBatchChain:
+            hipLaunchKernelGGL((fb_chain_batch_kernel<kG, kR, LOG, CB>), grid, block, dyn, L.stream,
+                               *L.bc, L.tb);
+            break;
+        case kFbPlainBlock:
+            if constexpr (kHasBlock) {
+                hipLaunchKernelGGL((fb_block_kernel<kG, kR, LOG, CB>), grid, block, dyn, L.stream,
+                                   *L.job, L.tb);
+                break;
+            } else {
+                return SI_ERR_UNSUPPORTED;
+            }
+        case kFbBatchBlock:
+            if constexpr (kHasBlock) {
+                hipLaunchKernelGGL((fb_block_batch_kernel<kG, kR, LOG, CB>), grid, block, dyn, L.stream,
+                                   *L.bc, L.tb);
+                break;
+            } else {
+                return SI_ERR_UNSUPPORTED;
+            }
+        default:
+            return SI_ERR_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : SI_ERR_DEVICE;
+}
+
+int launch(const FbLaunch& L) {
+    if (L.cb == 16) return L.log ? launch_t<true, 16>(L) : launch_t<false, 16>(L);
+    if (L.cb == 32) return L.log ? launch_t<true, 32>(L) : launch_t<false, 32>(L);
+    return SI_ERR_UNSUPPORTED;
+}
+
+template <bool LOG, int CB>
+int occupancy_t(int form, uint32_t P) {
+    int n = 0;
+    const size_t dyn = fb_table_lds_bytes(P);
+    hipError_t e = hipErrorInvalidValue;
+    switch (form) {
+        case kFbPlainChain:
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_chain_kernel<kG, kR, LOG, CB>,
+                                                             kFbThreads, dyn);
+            break;
+        case kFbBatchChain:
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, fb_chain_batch_kernel<kG, kR, LOG, CB>, kFbThreads, dyn);
+            break;
+        case kFbPlainBlock:
+            if constexpr (kHasBlock)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_block_kernel<kG, kR, LOG, CB>,
+                                                                 kFbThreads, dyn);
+            break;
+        case kFbBatchBlock:
+            if constexpr (kHasBlock)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &n, fb_block_batch_kernel<kG, kR, LOG, CB>, kFbThreads, dyn);
+            break;
+        default:
+            break;
+    }
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int occupancy(int form, bool log, uint32_t cb, uint32_t P) {
+    if (cb == 32) return log ? occupancy_t<true, 32>(form, P) : occupancy_t<false, 32>(form, P);
+    return log ? occupancy_t<true, 16>(form, P) : occupancy_t<false, 16>(form, P);
+}
+
+const bool registered = (fb_register(FbImpl{kLp, kG, kHasBlock, launch, occupancy}), true);
+
+}  // namespace
+}  // namespace si

@@ -1,0 +1,954 @@
+// Li-Stephens forward-backward, posterior dosage and Philox sampling for gfx950:
+// the device code, templated on the lane shape.  Each (padded states Lp,
+// lanes per sample G) pair is instantiated in a translation unit of its own
+// (fb_inst.hip); hmm.hip holds the host side.
+//
+// Reference semantics (src/kernels/forward_backward.cu:41-317,
+// src/kernels/sampling.cu:52-131): with the -999 floor of every log-sum-exp
+// fold written out, the quantities are
+//   forward   a_m[j] ~ (e^-999 + sum_i a_{m-1}[i] t_ij) e^{E_m[j]}   (rescaled per marker)
+//   backward  b_m[i] =  e^-999 + sum_j t_ij e^{E_{m+1}[j]} b_{m+1}[j] (never rescaled)
+//   posterior p_m[k] = a_m[k] b_m[k] / sum_l a_m[l] b_m[l]
+// with t_ij = e^To + (e^Td - e^To) [i == j]: rank one plus diagonal, so a
+// step costs O(L), not O(L^2) log-sum-exps, and no matrix unit is involved.
+//
+// Two numeric forms share the structure (template parameter LOG):
+//   * linear FP32: forward sum-normalized, a'_j = (t_o + u a_j) eps_j (the
+//     floor term is below FP32 resolution whenever To >= -80, checked on the
+//     host); backward b_m = e^{Bl_m} v_m with the log scale Bl_m in FP64, so
+//     the floor enters exactly as f = exp(-999 - Bl_m) once b reaches it.
+//     exp/log run once per (sample, marker), not per state.
+//   * log-space FP32: forward log vector normalized to maximum 0 (as the
+//     reference normalizes, so its -999 floor weighs the same), its log mass
+//     ls = LSE(alpha) carried beside it; backward log vector relative to an
+//     FP64 offset; every step is a three-term log-sum-exp
+//     LSE(-999, To + ls, U + alpha_j) per state, U = log(e^Td - e^To), whose
+//     first two terms are shared by all states (one exp and one log per state).
+//     Valid for any To, including the -999 of coincident markers.
+// Both take emissions relative to the best class present at the marker (so a
+// marker where every state mismatches does not push FP32 values towards
+// -1000, where their resolution is 6e-5) and carry that offset in FP64.
+//
+// Work split.  A lane group of G adjacent lanes of one wavefront owns one
+// sample's vector, R = Lp / G states per lane (lane r holds states
+// [r R, r R + R)).  Every sum over states is the same binary tree over the Lp
+// states in index order (pairwise within the lane, then lane pairs by DPP),
+// and every maximum is exact, so results are bit-identical for every (G, R):
+// the lane shape is a pure performance choice, made by the autotuner.
+//
+// Two passes per job (all samples of the job in both):
+//   chain pass: one lane group per (sample, direction) walks all M markers;
+//     forward groups store the forward vector at the first marker of every
+//     checkpoint block (CB markers), backward groups the backward vector and
+//     its FP64 scale at every block's last marker;
+//   block pass: one lane group per (sample, checkpoint block).  The block is
+//     handled as CB/16 sub-blocks of 16 markers, last first: the forward
+//     vector is replayed from the block checkpoint to the sub-block (kept in
+//     LDS for its 16 markers), then the backward vector walks down the
+//     sub-block forming posteriors, dosages and allele probabilities; lanes
+//     then sample the sub-block's markers from the LDS posteriors.  CB = 32
+//     halves checkpoint memory for 16 extra forward steps per block.
+#pragma once
+
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "si_device.hpp"
+#include "si_internal.hpp"
+
+namespace si {
+
+constexpr int kFbThreads = 128;
+constexpr float kFloor = -999.0f;
+constexpr uint32_t kSub = 16;  // markers per load block and per block-pass sub-block
+
+// Lane shape: G lanes per sample, R states per lane.
+template <int G_, int R_>
+struct Shape {
+    static constexpr int G = G_, R = R_, Lp = G_ * R_;
+    static constexpr int NW = R_ >= 4 ? R_ / 4 : 1;  // state-table words a lane loads per marker
+    static constexpr int GPW = kWave / G_;           // lane groups per wavefront
+    static constexpr int GPB = kFbThreads / G_;      // lane groups per workgroup
+    static_assert(G_ >= 1 && G_ <= 32 && R_ >= 2 && R_ <= 8, "lane shape");
+};
+
+// Block-pass LDS of one lane group: posteriors [16][T] floats (T = Lp + pad,
+// group stride S floats) and the state bytes [16][Lp] (group stride SW
+// words).  The paddings make the vector writes of the backward walk and the
+// 16-byte reads of the sampling phase free of bank conflicts (searched over
+// 32- and 64-bank models for every lane shape).
+__host__ __device__ constexpr int pbuf_T(int Lp) { return Lp == 4 ? 4 : Lp == 8 ? 12 : Lp + 4; }
+__host__ __device__ constexpr int pbuf_E(int Lp, int G) {
+    return Lp == 4 ? 4 : Lp == 8 ? 8 : Lp == 16 ? (G == 2 ? 8 : 16) : Lp == 32 ? (G == 4 ? 4 : 32)
+                                                                              : (G == 8 ? 4 : 0);
+}
+template <class S>
+struct BlockLds {
+    static constexpr int T = pbuf_T(S::Lp);
+    static constexpr int PS = int(kSub) * T + pbuf_E(S::Lp, S::G);  // floats per group
+    static constexpr int SW = int(kSub) * S::Lp / 4 + 1;            // words per group
+    static_assert(T % 4 == 0 && PS % 4 == 0, "16-byte aligned rows");
+};
+
+// ------------------------------------------------------------ emissions --
+// Emission table staged in LDS (dynamic): row = code * 8 + class mask, five
+// floats per row: the four per-class values (class 3 = padding state) and the
+// row's log offset off = max of gl over the classes present (0 if none):
+//   linear: (exp(gl_c - off), ..., 0, off), absent classes 0
+//   log:    (gl_c - off, ..., -inf, off),   absent classes -inf
+// A state's value is one LDS read at row * 5 + class.
+struct Tab {
+    const float* f;
+};
+constexpr uint32_t kTabRow = 5;
+
+template <bool LOG>
+__device__ __forceinline__ Tab load_tables(const FbTables& tb) {
+    extern __shared__ float4 fb_dyn_lds[];
+    float* f = reinterpret_cast<float*>(fb_dyn_lds);
+    const uint32_t n = tb.P * 8;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        float* row = f + i * kTabRow;
+        const float o = tb.emax[i];
+        if constexpr (LOG) {
+            const float* g = tb.palette + 3 * (i / 8);
+            const uint32_t c = i % 8;
+            row[0] = (c & 1u) ? g[0] - o : -INFINITY;
+            row[1] = (c & 2u) ? g[1] - o : -INFINITY;
+            row[2] = (c & 4u) ? g[2] - o : -INFINITY;
+            row[3] = -INFINITY;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) row[k] = tb.eps[4 * i + k];
+        }
+        row[4] = o;
+    }
+    return Tab{f};
+}
+
+__host__ __device__ constexpr size_t fb_table_lds_bytes(uint32_t P) {
+    return size_t(P) * 8 * kTabRow * sizeof(float);
+}
+
+// The class-present mask of a marker rides in bit 7 of the first three state
+// bytes of every aligned state-table word, so the emission row needs no load
+// of its own.  The 24-bit product moves bits 7, 15, 23 to 21, 22, 23 (the
+// other partial products land on distinct bits, so nothing carries).
+__device__ __forceinline__ uint32_t class_mask(uint32_t w) {
+    return (__umul24(w & 0x808080u, 0x4081u) >> 21) & 7u;
+}
+
+// The lane's state bytes of one marker: NW aligned words; for R < 4 the
+// lane's bytes start at byte `sub` of its word.
+template <class S>
+struct LaneWords {
+    uint32_t w[S::NW];
+    __device__ __forceinline__ uint32_t byte(int i, uint32_t sub) const {
+        if constexpr (S::R >= 4) return (w[i >> 2] >> (8 * (i & 3))) & 0xffu;
+        else return (w[0] >> (8 * (sub + uint32_t(i)))) & 0xffu;
+    }
+    // Bit b of state byte i as an all-ones / all-zeros mask.
+    __device__ __forceinline__ uint32_t bitmask(int i, uint32_t sub, int b) const {
+        const uint32_t x = S::R >= 4 ? w[i >> 2] : w[0];
+        const uint32_t pos = S::R >= 4 ? uint32_t(8 * (i & 3) + b) : 8u * (sub + uint32_t(i)) + b;
+        return uint32_t(int32_t(x << (31u - pos)) >> 31);
+    }
+};
+
+template <class S>
+__device__ __forceinline__ uint32_t lane_sub(uint32_t r) {
+    return S::R >= 4 ? 0u : (r * S::R) & 3u;
+}
+
+// Emission values of the lane's R states at a marker with palette code `code`;
+// codes at or beyond the palette size read as the last entry.  Returns the
+// row's log offset.
+template <class S>
+__device__ __forceinline__ float lane_emission(const Tab& tab, uint32_t P, uint32_t code,
+                                               const LaneWords<S>& sw, uint32_t sub,
+                                               float (&e)[S::R]) {
+    const float* rp = tab.f + (min(code, P - 1u) * 8u + class_mask(sw.w[0])) * kTabRow;
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) e[i] = rp[(sw.byte(i, sub) >> 4) & 3u];
+    return rp[4];
+}
+
+// ------------------------------------------------------------ reductions --
+template <int N>
+__device__ __forceinline__ float tree_sum(const float* x) {
+    if constexpr (N == 1) {
+        return x[0];
+    } else {
+        float h[N / 2];
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) h[i] = x[2 * i] + x[2 * i + 1];
+        return tree_sum<N / 2>(h);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ float tree_max(const float* x) {
+    float m = x[0];
+#pragma unroll
+    for (int i = 1; i < N; ++i) m = fmaxf(m, x[i]);
+    return m;
+}
+
+// Sum / max over the lane group (G lanes), every lane receiving the result.
+template <int G>
+__device__ __forceinline__ float gsum(float v) {
+    if constexpr (G <= 16) {
+        return group_sum<G>(v);
+    } else {
+        v = group_sum<16>(v);
+        return v + __shfl_xor(v, 16, kWave);
+    }
+}
+template <int G>
+__device__ __forceinline__ float gmax(float v) {
+    if constexpr (G <= 16) {
+        return group_max<G>(v);
+    } else {
+        v = group_max<16>(v);
+        return fmaxf(v, __shfl_xor(v, 16, kWave));
+    }
+}
+
+template <class S>
+__device__ __forceinline__ float vsum(const float (&x)[S::R]) {
+    return gsum<S::G>(tree_sum<S::R>(x));
+}
+template <class S>
+__device__ __forceinline__ float vmax(const float (&x)[S::R]) {
+    return gmax<S::G>(tree_max<S::R>(x));
+}
+
+// LSE(a, b, c_i) for a pair (a, b) shared by every state and a per-state c_i:
+// the pair is folded once per step into (M0, S0) with M0 = max(a, b),
+// S0 = e^{a-M0} + e^{b-M0} in [1, 2]; each state then costs one exp and one
+// log: with d = c - M0,
+//   d <= 0:  M0 + ln(S0 + e^d)
+//   d >  0:  c  + ln(1 + S0 e^-d)
+struct LsePair {
+    float M0, S0;
+};
+__device__ __forceinline__ LsePair lse_pair(float a, float b) {
+    const float m = fmaxf(a, b);
+    return LsePair{m, fast_exp(a - m) + fast_exp(b - m)};
+}
+__device__ __forceinline__ float lse_with(const LsePair& q, float c) {
+    const float d = c - q.M0;
+    const float e = fast_exp(-fabsf(d));
+    return d <= 0.0f ? q.M0 + fast_ln(q.S0 + e) : c + fast_ln(fmaf(q.S0, e, 1.0f));
+}
+
+// Log mass LSE_i(a_i) of a max-normalized log vector (a value in [0, ln Lp]).
+template <class S>
+__device__ __forceinline__ float log_mass(const float (&a)[S::R]) {
+    float w[S::R];
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) w[i] = fast_exp(a[i]);
+    return fast_ln(vsum<S>(w));
+}
+
+template <int R>
+__device__ __forceinline__ void vsel(bool c, float (&dst)[R], const float (&src)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) dst[i] = c ? src[i] : dst[i];
+}
+
+// ---------------------------------------------------------------- forward --
+// Marker 0.  Returns the log normalizer (log-likelihood increment); ls is the
+// log mass of the normalized log-space vector (unused by the linear form,
+// whose vector sums to 1).
+template <class S, bool LOG>
+__device__ __forceinline__ double fwd_init(float (&a)[S::R], float& ls, const float (&e)[S::R],
+                                           float off) {
+    if constexpr (LOG) {
+        const float c = vmax<S>(e);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) a[i] = e[i] - c;
+        ls = log_mass<S>(a);
+        return double(off) + double(c);
+    } else {
+        const float Sm = vsum<S>(e);
+        const float r = __builtin_amdgcn_rcpf(Sm);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) a[i] = e[i] * r;
+        return double(off) + double(fast_ln(Sm));
+    }
+}
+
+// Step into a marker with emission e / offset off and coefficients k:
+// k = (t_o, u) for the linear form, (lto, lu) for the log form.
+template <class S, bool LOG>
+__device__ __forceinline__ double fwd_step(float (&a)[S::R], float& ls, const float (&e)[S::R],
+                                           float off, float2 k) {
+    float x[S::R];
+    if constexpr (LOG) {
+        const LsePair q = lse_pair(kFloor, k.x + ls);  // LSE_i(a_i + To) = To + ls
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) x[i] = lse_with(q, k.y + a[i]) + e[i];
+        const float c = vmax<S>(x);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) a[i] = x[i] - c;
+        ls = log_mass<S>(a);
+        return double(off) + double(c);
+    } else {
+        // a sums to 1, so sum_i a_i t_o = t_o
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) x[i] = fmaf(k.y, a[i], k.x) * e[i];
+        const float Sm = vsum<S>(x);
+        const float r = __builtin_amdgcn_rcpf(Sm);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) a[i] = x[i] * r;
+        return double(off) + double(fast_ln(Sm));
+    }
+}
+
+// --------------------------------------------------------------- backward --
+// From the vector at marker n (emission e, offset off) to n - 1.  The true
+// backward vector is e^{Bl} v (linear) or Bl + v (log).  With
+// nat = Bl + off + log W the new scale is max(nat, -999): when the floor
+// dominates (nat < -999) the scale pins at -999 and the non-floor part is
+// shifted down by sh = -999 - nat, so no value ever leaves a safe range.
+template <class S, bool LOG>
+__device__ __forceinline__ void bwd_step(float (&v)[S::R], double& Bl, const float (&e)[S::R],
+                                         float off, float2 k) {
+    float y[S::R];
+    if constexpr (LOG) {
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) y[i] = e[i] + v[i];
+        const float mx = vmax<S>(y);
+        float w[S::R];
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) w[i] = fast_exp(y[i] - mx);
+        const float lw = mx + fast_ln(vsum<S>(w));
+        const double nat = Bl + double(off) + double(lw);
+        const double arg = -999.0 - nat;
+        const bool floor_dominates = arg > 0.0;
+        const float fl = floor_dominates ? 0.0f : float(arg);
+        const float sh = floor_dominates ? float(arg) : 0.0f;
+        Bl = floor_dominates ? -999.0 : nat;
+        const LsePair q = lse_pair(fl, k.x - sh);
+        const float bu = k.y - lw - sh;
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) v[i] = lse_with(q, bu + y[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) y[i] = e[i] * v[i];
+        const float W = vsum<S>(y);
+        const double nat = Bl + double(off) + double(fast_ln(W));
+        const double arg = -999.0 - nat;
+        const float x = fast_exp(float(-fabs(arg)));
+        const bool floor_dominates = arg > 0.0;
+        const float f = floor_dominates ? 1.0f : x;
+        const float g = floor_dominates ? x : 1.0f;
+        Bl = floor_dominates ? -999.0 : nat;
+        const float gr = g * k.y * __builtin_amdgcn_rcpf(W);
+        const float base = fmaf(g, k.x, f);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) v[i] = fmaf(gr, y[i], base);
+    }
+}
+
+template <class S, bool LOG>
+__device__ __forceinline__ void bwd_init(float (&v)[S::R]) {
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) v[i] = LOG ? 0.0f : 1.0f;
+}
+
+// -------------------------------------------------------------- posterior --
+template <class S, bool LOG>
+__device__ __forceinline__ void posterior(const float (&a)[S::R], const float (&v)[S::R],
+                                          float (&p)[S::R]) {
+    if constexpr (LOG) {
+        float q[S::R];
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) q[i] = a[i] + v[i];
+        const float mx = vmax<S>(q);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) p[i] = fast_exp(q[i] - mx);
+    } else {
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) p[i] = a[i] * v[i];
+    }
+    const float rq = __builtin_amdgcn_rcpf(vsum<S>(p));
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) p[i] *= rq;
+}
+
+// ------------------------------------------------------------ block inputs --
+// One 16-marker load block of a lane's inputs: the sample's 16 genotype
+// codes, per marker the lane's state-table words and (per-marker
+// transitions) the step coefficients.  Loaded whole, with indices clamped
+// inside the job (markers past the end repeat the last one, a load block
+// past the end reads the last whole one), so every load of the block is in
+// flight at once and the steps that follow read registers only.
+template <class S, bool LOG, bool PM>
+struct BlockIn {
+    uint4 cw;
+    LaneWords<S> sa[kSub];
+    float2 k[PM ? kSub : 1];
+    __device__ __forceinline__ void load(const JobView& J, const uint8_t* codes, uint32_t r,
+                                         uint32_t m0) {
+        const uint32_t ml = J.M - 1u;
+        m0 = min(m0, ml & ~(kSub - 1u));
+        cw = *reinterpret_cast<const uint4*>(codes + m0);
+        const uint32_t wpr = uint32_t(S::Lp) / 4u;       // words per state-table row
+        const uint32_t w0 = (r * uint32_t(S::R)) / 4u;   // the lane's first word
+        const uint32_t* sa32 = reinterpret_cast<const uint32_t*>(J.sa);
+#pragma unroll
+        for (uint32_t t = 0; t < kSub; ++t) {
+            const uint32_t* p = sa32 + size_t(min(m0 + t, ml)) * wpr + w0;
+#pragma unroll
+            for (int w = 0; w < S::NW; ++w) sa[t].w[w] = p[w];  // the table is 4-byte aligned
+        }
+        if constexpr (PM) {
+#pragma unroll
+            for (uint32_t t = 0; t < kSub; ++t) {
+                const float4 q = J.steps[min(m0 + t, ml)];
+                k[t] = LOG ? make_float2(q.z, q.w) : make_float2(q.x, q.y);
+            }
+        } else {
+            k[0] = LOG ? make_float2(J.step.z, J.step.w) : make_float2(J.step.x, J.step.y);
+        }
+    }
+    __device__ __forceinline__ uint32_t code(uint32_t t) const {
+        return byte_of((&cw.x)[t >> 2], t & 3);
+    }
+};
+
+// What a load block's 16 steps consume: the lane's emissions (read from the
+// LDS table together, before any step needs them) and the coefficients.
+// Filling it frees the BlockIn for the next block's loads.
+template <class S, bool LOG, bool PM>
+struct BlockEm {
+    float e[kSub][S::R];
+    float off[kSub];
+    float2 k[PM ? kSub : 1];
+    __device__ __forceinline__ void fill(const Tab& tab, uint32_t P, const BlockIn<S, LOG, PM>& in,
+                                         uint32_t sub) {
+#pragma unroll
+        for (uint32_t t = 0; t < kSub; ++t)
+            off[t] = lane_emission<S>(tab, P, in.code(t), in.sa[t], sub, e[t]);
+#pragma unroll
+        for (uint32_t t = 0; t < (PM ? kSub : 1); ++t) k[t] = in.k[t];
+    }
+    __device__ __forceinline__ float2 coef(uint32_t t) const { return k[PM ? t : 0]; }
+};
+
+// R consecutive floats of a checkpoint vector.
+template <int R>
+__device__ __forceinline__ void store_vec(float* p, const float (&x)[R]) {
+    if constexpr (R == 2) {
+        *reinterpret_cast<float2*>(p) = make_float2(x[0], x[1]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; i += 4)
+            *reinterpret_cast<float4*>(p + i) = make_float4(x[i], x[i + 1], x[i + 2], x[i + 3]);
+    }
+}
+template <int R>
+__device__ __forceinline__ void load_vec(const float* p, float (&x)[R]) {
+    if constexpr (R == 2) {
+        const float2 q = *reinterpret_cast<const float2*>(p);
+        x[0] = q.x;
+        x[1] = q.y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < R; i += 4) {
+            const float4 q = *reinterpret_cast<const float4*>(p + i);
+            x[i] = q.x;
+            x[i + 1] = q.y;
+            x[i + 2] = q.z;
+            x[i + 3] = q.w;
+        }
+    }
+}
+
+// Orders this wavefront's LDS accesses: lane groups exchange data through LDS
+// only within their own wavefront.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------
+// One chain (forward or backward) of sample s of job J, by one lane group.
+// The next load block's inputs are fetched while the current one is stepped;
+// every step is computed and committed by a select where it lies inside the
+// chain, so a load block's 16 steps are straight-line code.  J.nblk counts
+// checkpoint blocks of CB markers.
+// ---------------------------------------------------------------------------
+template <class S, bool LOG, bool PM, int CB>
+__device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint32_t P, uint32_t s,
+                                          bool fwd, uint32_t r) {
+    constexpr uint32_t NS = uint32_t(CB) / kSub;  // load blocks per checkpoint block
+    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
+    const uint32_t nlb = (J.M + kSub - 1) / kSub;
+    const uint32_t sub = lane_sub<S>(r);
+    const size_t vo = size_t(r) * S::R;  // the lane's offset in a checkpoint vector
+    BlockIn<S, LOG, PM> cur;
+    BlockEm<S, LOG, PM> em;
+    if (fwd) {
+        cur.load(J, codes, r, 0);
+        float a[S::R];
+        float ls = 0.0f;
+        double ll;
+        {
+            float e[S::R];
+            const float off = lane_emission<S>(tab, P, cur.code(0), cur.sa[0], sub, e);
+            ll = fwd_init<S, LOG>(a, ls, e, off);
+        }
+        for (uint32_t c = 0; c < nlb; ++c) {
+            em.fill(tab, P, cur, sub);
+            if (c + 1 < nlb) cur.load(J, codes, r, (c + 1) * kSub);
+            const uint32_t m0 = c * kSub;
+            const bool ck = (c % NS) == 0;
+            float* ckp = J.ck_alpha + (size_t(s) * J.nblk + m0 / CB) * S::Lp + vo;
+            if (m0 > 0 && m0 + kSub <= J.M) {  // every step inside the chain: no selects
+#pragma unroll
+                for (uint32_t t = 0; t < kSub; ++t) {
+                    ll += fwd_step<S, LOG>(a, ls, em.e[t], em.off[t], em.coef(t));
+                    if (t == 0 && ck) store_vec<S::R>(ckp, a);
+                }
+            } else {
+#pragma unroll
+                for (uint32_t t = 0; t < kSub; ++t) {
+                    const uint32_t m = m0 + t;
+                    float an[S::R];
+#pragma unroll
+                    for (int i = 0; i < S::R; ++i) an[i] = a[i];
+                    float lsn = ls;
+                    const double inc = fwd_step<S, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
+                    const bool live = m > 0 && m < J.M;
+                    vsel(live, a, an);
+                    ls = live ? lsn : ls;
+                    ll += live ? inc : 0.0;
+                    if (t == 0 && ck) store_vec<S::R>(ckp, a);
+                }
+            }
+        }
+        if (LOG) ll += double(ls);  // log mass of the last normalized vector
+        if (r == 0 && J.loglik) J.loglik[s] = ll;
+    } else {
+        float v[S::R];
+        bwd_init<S, LOG>(v);
+        double Bl = 0.0;
+        const size_t last = size_t(s) * J.nblk + (J.nblk - 1);
+        store_vec<S::R>(J.ck_beta + last * S::Lp + vo, v);
+        if (r == 0) J.ck_bl[last] = Bl;
+        cur.load(J, codes, r, (nlb - 1) * kSub);
+        for (int c = int(nlb) - 1; c >= 0; --c) {
+            em.fill(tab, P, cur, sub);
+            if (c > 0) cur.load(J, codes, r, uint32_t(c - 1) * kSub);
+            const uint32_t m0 = uint32_t(c) * kSub;
+            const bool ck = c > 0 && (uint32_t(c) % NS) == 0;
+            // v at marker m0 - 1 is the last of checkpoint block m0 / CB - 1
+            const size_t b = size_t(s) * J.nblk + (m0 / CB) - 1u;
+            if (m0 > 0 && m0 + kSub <= J.M) {  // every step inside the chain: no selects
+#pragma unroll
+                for (int t = int(kSub) - 1; t >= 0; --t) {
+                    bwd_step<S, LOG>(v, Bl, em.e[t], em.off[t], em.coef(uint32_t(t)));
+                    if (t == 0 && ck) {
+                        store_vec<S::R>(J.ck_beta + b * S::Lp + vo, v);
+                        if (r == 0) J.ck_bl[b] = Bl;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int t = int(kSub) - 1; t >= 0; --t) {
+                    const uint32_t n = m0 + uint32_t(t);
+                    float vn[S::R];
+#pragma unroll
+                    for (int i = 0; i < S::R; ++i) vn[i] = v[i];
+                    double Bln = Bl;
+                    bwd_step<S, LOG>(vn, Bln, em.e[t], em.off[t], em.coef(uint32_t(t)));
+                    const bool live = n > 0 && n < J.M;
+                    vsel(live, v, vn);
+                    Bl = live ? Bln : Bl;
+                    if (t == 0 && ck) {
+                        store_vec<S::R>(J.ck_beta + b * S::Lp + vo, v);
+                        if (r == 0) J.ck_bl[b] = Bl;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoint block c of sample s of job J, by one lane group.  For each
+// 16-marker sub-block j (last first): replay the forward vector from the
+// block checkpoint through sub-blocks 0..j-1 (registers only) and through j
+// (into the group's LDS rows), walk the backward vector down j forming the
+// posteriors (in place in LDS) and the allele probabilities (group sums),
+// then sample: lane r takes markers r, r + G, ... of the sub-block and walks
+// the states in order with a sequential FP32 cumulative sum, which the host
+// replays exactly.  Every lane of the wavefront passes the wave_syncs.
+// ---------------------------------------------------------------------------
+template <class S, bool LOG, bool PM, int CB>
+__device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint32_t P, bool active,
+                                          uint32_t s, uint32_t c, uint32_t r, float* pb,
+                                          uint32_t* sb, uint4* rng) {
+    using Lds = BlockLds<S>;
+    constexpr uint32_t NS = uint32_t(CB) / kSub;
+    // markers per lane in a sampling phase (with 32 lanes, half the lanes sample)
+    constexpr uint32_t kPer = (kSub + uint32_t(S::G) - 1) / uint32_t(S::G);
+    constexpr int T = Lds::T;
+    const uint32_t m0 = c * CB;
+    const uint32_t nm = active ? min(uint32_t(CB), J.M - m0) : 0u;
+    const uint32_t sub = lane_sub<S>(r);
+    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
+    const size_t ck = size_t(s) * J.nblk + c;
+    const bool argmax = (J.flags & SI_FB_ARGMAX) != 0;
+    const uint2 key = make_uint2(uint32_t(J.seed), uint32_t(J.seed >> 32));
+    float v[S::R];
+    double Bl = 0.0;
+    if (active) {
+        load_vec<S::R>(J.ck_beta + ck * S::Lp + size_t(r) * S::R, v);
+        Bl = J.ck_bl[ck];
+    }
+    BlockIn<S, LOG, PM> in;
+    BlockEm<S, LOG, PM> em;
+    for (int j = int(NS) - 1; j >= 0; --j) {
+        const uint32_t base = uint32_t(j) * kSub;  // sub-block start within the block
+        float ap0r[kPer], ap1r[kPer];
+        if (active) {
+            float a[S::R];
+            load_vec<S::R>(J.ck_alpha + ck * S::Lp + size_t(r) * S::R, a);
+            float ls = LOG ? log_mass<S>(a) : 0.0f;  // as the chain pass derived it
+            in.load(J, codes, r, m0);
+            // replay through the sub-blocks before j
+            for (uint32_t u = 0; u < uint32_t(j); ++u) {
+                em.fill(tab, P, in, sub);
+                in.load(J, codes, r, m0 + (u + 1) * kSub);
+#pragma unroll
+                for (uint32_t t = 0; t < kSub; ++t) {
+                    const uint32_t tt = u * kSub + t;
+                    float an[S::R];
+#pragma unroll
+                    for (int i = 0; i < S::R; ++i) an[i] = a[i];
+                    float lsn = ls;
+                    fwd_step<S, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
+                    const bool live = tt > 0 && tt < nm;
+                    vsel(live, a, an);
+                    ls = live ? lsn : ls;
+                }
+            }
+            em.fill(tab, P, in, sub);
+            // Sub-block j: forward vectors into LDS (state bytes beside them),
+            // then the backward walk.  A block inside the job (nm == CB) needs
+            // no per-step selects: only its very first step is not taken.
+            auto walk = [&](auto full_c) {
+                constexpr bool FULL = decltype(full_c)::value;
+#pragma unroll
+                for (uint32_t t = 0; t < kSub; ++t) {
+                    if constexpr (S::R >= 4) {
+#pragma unroll
+                        for (int w = 0; w < S::NW; ++w)
+                            sb[t * (S::Lp / 4) + r * S::NW + w] = in.sa[t].w[w];
+                    } else {
+                        reinterpret_cast<uint16_t*>(sb)[t * (S::Lp / 2) + r] =
+                            uint16_t(in.sa[t].w[0] >> (8 * sub));
+                    }
+                    const uint32_t tt = base + t;
+                    if (FULL && t > 0) {
+                        fwd_step<S, LOG>(a, ls, em.e[t], em.off[t], em.coef(t));
+                    } else if (!(FULL && NS == 1)) {  // with one sub-block, step 0 is never taken
+                        float an[S::R];
+#pragma unroll
+                        for (int i = 0; i < S::R; ++i) an[i] = a[i];
+                        float lsn = ls;
+                        fwd_step<S, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
+                        const bool live = tt > 0 && tt < nm;
+                        vsel(live, a, an);
+                        ls = live ? lsn : ls;
+                    }
+                    store_vec<S::R>(pb + t * T + r * S::R, a);
+                }
+#pragma unroll
+                for (int t = int(kSub) - 1; t >= 0; --t) {
+                    const uint32_t tt = base + uint32_t(t);
+                    float af[S::R], p[S::R];
+                    load_vec<S::R>(pb + t * T + r * S::R, af);
+                    posterior<S, LOG>(af, v, p);
+                    float x0[S::R], x1[S::R];
+#pragma unroll
+                    for (int i = 0; i < S::R; ++i) {
+                        const uint32_t pi = __float_as_uint(p[i]);
+                        x0[i] = __uint_as_float(pi & in.sa[t].bitmask(i, sub, 0));
+                        x1[i] = __uint_as_float(pi & in.sa[t].bitmask(i, sub, 1));
+                    }
+                    const float ap0 = vsum<S>(x0), ap1 = vsum<S>(x1);
+                    if (FULL || tt < nm) store_vec<S::R>(pb + t * T + r * S::R, p);
+                    if (uint32_t(t) % S::G == r) {
+                        ap0r[uint32_t(t) / S::G] = ap0;
+                        ap1r[uint32_t(t) / S::G] = ap1;
+                    }
+                    if (FULL) {
+                        if (t > 0 || base > 0) bwd_step<S, LOG>(v, Bl, em.e[t], em.off[t], em.coef(uint32_t(t)));
+                    } else if (tt > 0) {
+                        float vn[S::R];
+#pragma unroll
+                        for (int i = 0; i < S::R; ++i) vn[i] = v[i];
+                        double Bln = Bl;
+                        bwd_step<S, LOG>(vn, Bln, em.e[t], em.off[t], em.coef(uint32_t(t)));
+                        const bool live = tt < nm;
+                        vsel(live, v, vn);
+                        Bl = live ? Bln : Bl;
+                    }
+                }
+            };
+            if (nm == uint32_t(CB)) walk(std::true_type{});
+            else walk(std::false_type{});
+            for (uint32_t q = r; q < kSub / 4; q += S::G)
+                rng[q] = philox4x32_10(
+                    make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
+        }
+        wave_sync();
+        if (active) {
+            const uint32_t L = J.L;
+#pragma unroll
+            for (uint32_t q = 0; q < kPer; ++q) {
+                const uint32_t t = r + q * S::G;
+                const uint32_t tt = base + t;
+                if (t >= kSub || tt >= nm) continue;
+                const uint32_t m = m0 + tt;
+                const float* pt = pb + t * T;
+                uint32_t ks = 0;
+                if (argmax) {  // first maximum
+                    float best = -1.0f;
+#pragma unroll
+                    for (int k4 = 0; k4 < S::Lp; k4 += 4) {
+                        const float4 x = *reinterpret_cast<const float4*>(pt + k4);
+                        const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const bool gt = xs[i] > best;
+                            best = gt ? xs[i] : best;
+                            ks = gt ? uint32_t(k4 + i) : ks;
+                        }
+                    }
+                } else {  // first k with u < cumsum, else 0 (padding states add 0)
+                    const float uu = u24((&rng[t >> 2].x)[t & 3]);
+                    float cs = 0.0f;
+                    uint32_t kd = 0xffffffffu;
+#pragma unroll
+                    for (int k4 = 0; k4 < S::Lp; k4 += 4) {
+                        const float4 x = *reinterpret_cast<const float4*>(pt + k4);
+                        const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            cs += xs[i];
+                            kd = (kd == 0xffffffffu && uu < cs) ? uint32_t(k4 + i) : kd;
+                        }
+                    }
+                    ks = kd == 0xffffffffu ? 0u : kd;
+                }
+                const size_t o = size_t(s) * J.M + m;
+                if (J.ds) J.ds[o] = ap0r[q] + ap1r[q];
+                if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0r[q], ap1r[q]);
+                if (J.hap) {
+                    const uint32_t b = reinterpret_cast<const uint8_t*>(sb)[t * S::Lp + ks];
+                    J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 4u) ? 255u : (b & 1u));
+                    J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 8u) ? 255u : ((b >> 1) & 1u));
+                }
+                if (J.prob)
+                    for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pt[k];
+            }
+        }
+        wave_sync();  // the group's LDS rows are reused by the next sub-block
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain form: one job; grid = lane groups.  Chain: forward groups first,
+// padded to whole wavefronts (Bpad), then backward groups.
+// ---------------------------------------------------------------------------
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_kernel(JobView J, FbTables tb, uint32_t Bpad) {
+    using S = Shape<G, R>;
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / G, r = gtid % G;
+    const bool fwd = grp < Bpad;
+    const uint32_t s = fwd ? grp : grp - Bpad;
+    if (s >= J.B) return;  // whole lane groups leave together
+    if (J.flags & SI_FB_PER_MARKER) run_chain<S, LOG, true, CB>(J, tab, tb.P, s, fwd, r);
+    else run_chain<S, LOG, false, CB>(J, tab, tb.P, s, fwd, r);
+}
+
+// Wavefronts per SIMD the block kernels are compiled for: with two states per
+// lane the live state fits three wavefronts' registers.
+template <int R>
+constexpr int kBlockWaves = R <= 2 ? 3 : 2;
+
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_kernel(JobView J, FbTables tb) {
+    using S = Shape<G, R>;
+    using Lds = BlockLds<S>;
+    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
+    __shared__ uint32_t sbuf[S::GPB * Lds::SW];
+    __shared__ uint4 rng[S::GPB * (kSub / 4)];
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / G, r = gtid % G, gl = threadIdx.x / G;
+    const bool active = grp < J.B * J.nblk;
+    const uint32_t s = active ? grp / J.nblk : 0u;
+    const uint32_t c = active ? grp % J.nblk : 0u;
+    float* pb = pbuf + gl * Lds::PS;
+    uint32_t* sb = sbuf + gl * Lds::SW;
+    uint4* rg = rng + gl * (kSub / 4);
+    if (J.flags & SI_FB_PER_MARKER) run_block<S, LOG, true, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+    else run_block<S, LOG, false, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: a persistent grid; each wavefront repeatedly takes the next
+// GPW items of the class's queue (one atomic per wavefront) until the queue
+// is exhausted.  Items are ordered job by job (jobs longest first), so a
+// wavefront's groups usually share a job.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t find_job(const uint64_t* off, uint32_t n, uint64_t item) {
+    uint32_t lo = 0, hi = n;  // off[lo] <= item < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Next chunk of a queue for this wavefront (guided self-scheduling): about a
+// quarter of the remaining items spread over the grid's wavefronts, at least
+// one round (GPW items) and at most max_rounds, in whole rounds, so that the
+// queue counter sees few atomics while the tail still balances.  Chains are
+// long items, taken one round at a time; block items are short.
+struct Chunk {
+    uint64_t base;
+    uint32_t n;
+};
+__device__ __forceinline__ Chunk wave_take(unsigned long long* counter, uint64_t total, uint32_t gpw,
+                                           uint32_t max_rounds) {
+    uint32_t lo = 0, hi = 0, n = 0;
+    if (__lane_id() == 0) {
+        const uint64_t seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t waves = uint64_t(gridDim.x) * (kFbThreads / kWave);
+        const uint64_t rem = seen < total ? total - seen : 0;
+        uint64_t want = rem / (4 * waves) / gpw * gpw;
+        const uint64_t cap = uint64_t(max_rounds) * gpw;
+        want = want < gpw ? gpw : (want > cap ? cap : want);
+        n = uint32_t(want);
+        const uint64_t b = atomicAdd(counter, static_cast<unsigned long long>(want));
+        lo = uint32_t(b);
+        hi = uint32_t(b >> 32);
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    hi = __builtin_amdgcn_readfirstlane(hi);
+    n = __builtin_amdgcn_readfirstlane(n);
+    return Chunk{(uint64_t(hi) << 32) | lo, n};
+}
+
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass bc, FbTables tb) {
+    using S = Shape<G, R>;
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
+    for (;;) {
+        const Chunk ch = wave_take(&bc.counters[0], bc.chain_items, S::GPW, 1);
+        if (ch.base >= bc.chain_items) break;
+        for (uint32_t k = 0; k < ch.n; k += S::GPW) {
+            const uint64_t item = ch.base + k + gw;
+            if (item < bc.chain_items) {
+                const uint32_t j = find_job(bc.chain_off, bc.n_jobs, item);
+                const JobView J = bc.jobs[j];
+                const uint64_t q = item - bc.chain_off[j];
+                if (J.flags & SI_FB_PER_MARKER)
+                    run_chain<S, LOG, true, CB>(J, tab, tb.P, uint32_t(q >> 1), (q & 1) == 0, r);
+                else
+                    run_chain<S, LOG, false, CB>(J, tab, tb.P, uint32_t(q >> 1), (q & 1) == 0, r);
+            }
+        }
+    }
+}
+
+// The job a whole wavefront works on, in scalar registers.
+__device__ __forceinline__ JobView uniform_job(const JobView* p) {
+    static_assert(sizeof(JobView) % 4 == 0, "JobView words");
+    JobView J;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&J);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(JobView) / 4; ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return J;
+}
+
+__device__ __forceinline__ uint32_t find_job_uniform(const uint64_t* off, uint32_t n, uint64_t item) {
+    uint32_t lo = 0, hi = n;  // off[lo] <= item < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint64_t o = off[mid];
+        const uint64_t ou = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(o >> 32))) << 32) |
+                            __builtin_amdgcn_readfirstlane(uint32_t(o));
+        if (ou <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Block items of each job are padded to whole wavefront rounds (the block
+// offsets are multiples of 64), so the groups of one round share a job and
+// its descriptor is wavefront-uniform; padding items are inactive.
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_batch_kernel(BatchClass bc,
+                                                                                    FbTables tb) {
+    using S = Shape<G, R>;
+    using Lds = BlockLds<S>;
+    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
+    __shared__ uint32_t sbuf[S::GPB * Lds::SW];
+    __shared__ uint4 rng[S::GPB * (kSub / 4)];
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
+    const uint32_t gl = threadIdx.x / G;
+    float* pb = pbuf + gl * Lds::PS;
+    uint32_t* sb = sbuf + gl * Lds::SW;
+    uint4* rg = rng + gl * (kSub / 4);
+    uint64_t lo = 0, hi = 0;  // item range of the job in J (wavefront-uniform)
+    JobView J{};
+    for (;;) {
+        const Chunk ch = wave_take(&bc.counters[1], bc.block_items, S::GPW, 32);
+        if (ch.base >= bc.block_items) break;
+        for (uint32_t k = 0; k < ch.n; k += S::GPW) {
+            const uint64_t round = ch.base + k;  // wavefront-uniform
+            if (round >= bc.block_items) break;
+            if (round < lo || round >= hi) {  // another job: look it up once
+                const uint32_t j = find_job_uniform(bc.block_off, bc.n_jobs, round);
+                J = uniform_job(bc.jobs + j);
+                lo = bc.block_off[j];
+                hi = bc.block_off[j + 1];
+                lo = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(lo >> 32))) << 32) |
+                     __builtin_amdgcn_readfirstlane(uint32_t(lo));
+                hi = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(hi >> 32))) << 32) |
+                     __builtin_amdgcn_readfirstlane(uint32_t(hi));
+            }
+            const uint64_t q = round - lo + gw;
+            const bool active = q < uint64_t(J.B) * J.nblk;
+            const uint32_t s = active ? uint32_t(q / J.nblk) : 0u;
+            const uint32_t c = active ? uint32_t(q % J.nblk) : 0u;
+            if (J.flags & SI_FB_PER_MARKER)
+                run_block<S, LOG, true, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+            else
+                run_block<S, LOG, false, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+        }
+    }
+}
+
+}  // namespace si

@@ -9,15 +9,14 @@
 
 namespace si {
 
-// Markers per forward-backward block: the checkpoint interval of the chain
-// pass and the replay length of the block pass.
-constexpr uint32_t kBlock = 16;
+// Smallest checkpoint block of the forward-backward passes (markers).
+constexpr uint32_t kMinCkptBlock = 16;
 // Largest genotype-likelihood palette the kernels stage in LDS.
 constexpr uint32_t kMaxPalette = 64;
 
 // One job as a lane group sees it (plain and batched forms alike).
 struct JobView {
-    uint32_t B, M, L, flags, pitch, rng_stream, sample_base, nblk;
+    uint32_t B, M, L, flags, pitch, rng_stream, sample_base, nblk;  // nblk: checkpoint blocks
     uint64_t seed;
     float4 step;             // (t_o, u, lto, lu) when not per-marker
     const float4* steps;     // [M] per-marker coefficients
@@ -42,19 +41,31 @@ struct FbTables {
     uint32_t P;
 };
 
+// Forward-backward kernel variant: lanes per sample in the chain and block
+// passes (states per lane = Lp / lanes) and the checkpoint block length.
+struct FbVariant {
+    uint32_t gc, gb, cb;
+};
+constexpr uint32_t kMaxCkptBlock = 32;
+FbVariant fb_default_variant(uint32_t Lp);
+// Lane counts instantiated for a pass (0: chain, 1: block); returns how many.
+uint32_t fb_candidates(uint32_t Lp, int pass, uint32_t* lanes, uint32_t max);
+bool fb_variant_valid(uint32_t Lp, const FbVariant& v);
+
+// Workspace of one job at the smallest checkpoint block (enough for all).
 size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp);
 void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws);
-JobView fb_job_view(const si_fb_job& j);
+JobView fb_job_view(const si_fb_job& j, uint32_t cb);
 
-// Plain form: one job, two launches.
+// Plain form: one job, one launch per pass.
 int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
-              hipStream_t stream);
+              const FbVariant& var, hipStream_t stream);
 
 // Batched form: every job of one padded-state class in one persistent launch
 // per pass.  jobs: device array in launch order; item_off: device prefix of
 // per-job item counts (n_jobs + 1) for the chain pass (2 chains per sample)
-// and for the block pass (one item per (sample, block)); counters: 2 device
-// words, zeroed by the launcher.
+// and for the block pass (one item per (sample, checkpoint block)); counters:
+// 2 device 64-bit words, zeroed by the launcher.
 struct BatchClass {
     uint32_t Lp;
     uint32_t n_jobs;
@@ -63,11 +74,36 @@ struct BatchClass {
     const uint64_t* chain_off;
     const uint64_t* block_off;
     uint64_t chain_items, block_items;
-    uint32_t* counters;
+    unsigned long long* counters;  // [2] queue positions (chain, block)
     uint32_t grid_chain, grid_block;  // persistent grid sizes (workgroups)
 };
-void fb_batch_grid(BatchClass& c);
-int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, hipStream_t stream);
+// Persistent grid sizes for a variant (as many workgroups as fit at once,
+// capped by the work).
+void fb_batch_grid(BatchClass& c, const FbVariant& var, uint32_t P);
+int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, const FbVariant& var,
+                    hipStream_t stream);
+
+// Per-(Lp, lanes) kernel sets, registered by the instantiation units.
+enum FbForm : int { kFbPlainChain = 0, kFbPlainBlock = 1, kFbBatchChain = 2, kFbBatchBlock = 3 };
+struct FbLaunch {
+    int form;
+    bool log;
+    uint32_t cb;
+    const JobView* job;    // plain forms
+    uint32_t Bpad;         // plain chain: forward groups padded to whole wavefronts
+    const BatchClass* bc;  // batched forms
+    FbTables tb;
+    uint32_t grid;
+    hipStream_t stream;
+};
+struct FbImpl {
+    uint32_t Lp, G;
+    bool has_block;
+    int (*launch)(const FbLaunch&);
+    int (*occupancy)(int form, bool log, uint32_t cb, uint32_t P);  // workgroups per CU
+};
+void fb_register(const FbImpl& impl);
+const FbImpl* fb_impl(uint32_t Lp, uint32_t G);
 
 // PBWT prefix/divergence construction (pbwt.hip): chunks of C markers.
 constexpr uint32_t kPbwtMaxH = 65536;

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _native
-from ._native import FbJob, StepCoef, check, hip
+from ._native import SI_FB_LOGSPACE, FbJob, FbVariant, StepCoef, check, hip
 
 DEV = "cuda"
 
@@ -278,15 +278,48 @@ def workspace_bytes(jobs) -> int:
     return int(hip().si_fb_workspace_bytes(arr, len(jobs)))
 
 
-def run_jobs(cjobs, workspace: torch.Tensor | None = None, stream=None):
-    """si_fb_run over prepared FbJob structs (plain form: one job after another)."""
+def run_jobs(cjobs, workspace: torch.Tensor | None = None, stream=None, variant=None):
+    """si_fb_run over prepared FbJob structs (plain form: one job after another).
+    variant: (chain lanes, block lanes, checkpoint block) or None for the default."""
     arr = (FbJob * len(cjobs))(*cjobs)
     need = int(hip().si_fb_workspace_bytes(arr, len(cjobs)))
     if workspace is None or workspace.numel() < need:
         workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=DEV)
-    check(hip().si_fb_run(arr, len(cjobs), _ptr(workspace), workspace.numel(),
-                          _stream_handle(stream)), "si_fb_run")
+    var = None if variant is None else ctypes.byref(FbVariant(*variant))
+    check(hip().si_fb_run_variant(arr, len(cjobs), var, _ptr(workspace), workspace.numel(),
+                                  3, _stream_handle(stream)), "si_fb_run")
     return workspace
+
+
+def default_variant(L: int):
+    v = FbVariant()
+    check(hip().si_fb_default_variant(L, ctypes.byref(v)), "si_fb_default_variant")
+    return v.as_tuple()
+
+
+def lane_candidates(L: int, pass_: int):
+    """Lane counts built for a pass (SI_FB_PASS_CHAIN or SI_FB_PASS_BLOCK) at L states."""
+    buf = (ctypes.c_uint32 * 8)()
+    n = int(hip().si_fb_lane_candidates(L, pass_, buf, 8))
+    return [int(buf[i]) for i in range(min(n, 8))]
+
+
+def variants(L: int):
+    """Every (chain lanes, block lanes, checkpoint block) built for L states."""
+    return [(gc, gb, cb) for cb in (16, 32) for gc in lane_candidates(L, 1)
+            for gb in lane_candidates(L, 2)]
+
+
+def tune_jobs(cjobs, max_markers=0, stream=None, workspace=None):
+    """si_fb_tune: the fastest variant for these jobs (outputs untouched)."""
+    arr = (FbJob * len(cjobs))(*cjobs)
+    need = int(hip().si_fb_workspace_bytes(arr, len(cjobs)))
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=DEV)
+    v = FbVariant()
+    check(hip().si_fb_tune(arr, len(cjobs), _ptr(workspace), workspace.numel(), max_markers,
+                           _stream_handle(stream), ctypes.byref(v)), "si_fb_tune")
+    return v.as_tuple()
 
 
 class Batch:
@@ -310,6 +343,32 @@ class Batch:
             self.workspace = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=DEV)
         check(hip().si_fb_batch_run(self.handle, _ptr(self.workspace), self.workspace.numel(),
                                     int(bool(use_graph)), _stream_handle(stream)), "si_fb_batch_run")
+
+    def _ensure_ws(self):
+        if self.workspace is None or self.workspace.numel() < self.ws_bytes:
+            self.workspace = torch.empty(max(self.ws_bytes, 1), dtype=torch.uint8, device=DEV)
+
+    def tune(self, max_markers=0, stream=None):
+        """Tune every class without a fixed variant (si_fb_batch_tune)."""
+        self._ensure_ws()
+        check(hip().si_fb_batch_tune(self.handle, _ptr(self.workspace), self.workspace.numel(),
+                                     max_markers, _stream_handle(stream)), "si_fb_batch_tune")
+
+    def classes(self):
+        """[(padded states, logspace, (chain lanes, block lanes, ckpt block), tuned)]"""
+        out = []
+        for k in range(int(hip().si_fb_batch_n_classes(self.handle))):
+            lp, fl, t = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int()
+            v = FbVariant()
+            check(hip().si_fb_batch_class_info(self.handle, k, ctypes.byref(lp), ctypes.byref(fl),
+                                               ctypes.byref(v), ctypes.byref(t)),
+                  "si_fb_batch_class_info")
+            out.append((lp.value, bool(fl.value & SI_FB_LOGSPACE), v.as_tuple(), bool(t.value)))
+        return out
+
+    def set_variant(self, k, variant):
+        check(hip().si_fb_batch_set_variant(self.handle, k, ctypes.byref(FbVariant(*variant))),
+              "si_fb_batch_set_variant")
 
     def close(self):
         if self.handle:

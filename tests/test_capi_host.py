@@ -174,3 +174,33 @@ def test_state_table_argument_errors():
     assert lib.si_state_table(None, None, None, 0, 10, 8, 0, p32, None, None, None, None) == -1
     assert lib.si_state_table(None, None, None, 10, 10, 8, 1, p32, None, None, None, None) == -1
     assert lib.si_state_table(None, None, None, 10, 10, 8, 0, None, None, None, None, None) == -1
+
+
+def test_variant_queries():
+    lib = _native.hip()
+    v = _native.FbVariant()
+    assert lib.si_fb_default_variant(16, ctypes.byref(v)) == 0
+    assert v.as_tuple() == (4, 4, 16)
+    assert lib.si_fb_default_variant(0, ctypes.byref(v)) == _native.SI_ERR_ARG
+    assert lib.si_fb_default_variant(65, ctypes.byref(v)) == _native.SI_ERR_ARG
+    assert device.lane_candidates(16, _native.SI_FB_PASS_CHAIN) == [2, 4, 8]
+    assert device.lane_candidates(16, _native.SI_FB_PASS_BLOCK) == [4, 8]
+    assert device.lane_candidates(64, _native.SI_FB_PASS_CHAIN) == [8, 16, 32]
+    assert device.lane_candidates(3, _native.SI_FB_PASS_BLOCK) == [1, 2]
+    assert lib.si_fb_lane_candidates(16, 3, None, 0) == 0  # not a single pass
+    for L in (1, 4, 5, 8, 9, 16, 17, 32, 33, 64):
+        vs = device.variants(L)
+        assert device.default_variant(L) in vs
+        assert {cb for *_, cb in vs} == {16, 32}
+
+
+def test_fb_variant_argument_errors():
+    lib = _native.hip()
+    j = _job()
+    bad = _native.FbVariant(3, 2, 16)
+    assert lib.si_fb_run_variant(ctypes.byref(j), 1, ctypes.byref(bad), None, 0, 3, None) == -1
+    out = _native.FbVariant()
+    assert lib.si_fb_tune(None, 0, None, 0, 0, None, ctypes.byref(out)) == _native.SI_ERR_ARG
+    assert lib.si_fb_batch_n_classes(None) == 0
+    assert lib.si_fb_batch_set_variant(None, 0, ctypes.byref(out)) == _native.SI_ERR_ARG
+    assert lib.si_fb_batch_tune(None, None, 0, 0, None) == _native.SI_ERR_ARG
