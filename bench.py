@@ -405,16 +405,20 @@ def bench_plain(args):
     arr = (_native.FbJob * 1)(job.c_job())
     ws_bytes = int(lib.si_fb_workspace_bytes(arr, 1))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+    variant = device.tune_jobs([arr[0]], stream=stream, workspace=ws)
+    var = ctypes.byref(_native.FbVariant(*variant))
+    log(f"[plain] tuned variant: chain {variant[0]} lanes/sample, block {variant[1]} "
+        f"lanes/sample, checkpoint block {variant[2]}")
 
     def step(evs=None):
         if evs:
             evs[0].record(stream)
-        _native.check(lib.si_fb_run_passes(arr, 1, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
-                                           _native.SI_FB_PASS_CHAIN, sh), "chain pass")
+        _native.check(lib.si_fb_run_variant(arr, 1, var, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+                                            _native.SI_FB_PASS_CHAIN, sh), "chain pass")
         if evs:
             evs[1].record(stream)
-        _native.check(lib.si_fb_run_passes(arr, 1, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
-                                           _native.SI_FB_PASS_BLOCK, sh), "block pass")
+        _native.check(lib.si_fb_run_variant(arr, 1, var, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
+                                            _native.SI_FB_PASS_BLOCK, sh), "block pass")
         if evs:
             evs[2].record(stream)
 
@@ -449,6 +453,8 @@ def bench_plain(args):
         "all_marker_genotypes_per_s": units / (ms * 1e-3),
         "ms_per_step": ms,
         "passes_ms": {"fb_chain_kernel": chain_ms, "fb_block_kernel": block_ms},
+        "variant": {"chain_lanes_per_sample": variant[0], "block_lanes_per_sample": variant[1],
+                    "checkpoint_block": variant[2]},
         "roofline": {
             "bound": "valu",
             "kernel": dom + " (plain 20,000 x 100,000 workload)",
