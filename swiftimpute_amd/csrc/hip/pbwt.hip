@@ -47,7 +47,20 @@ struct PbwtArgs {
     uint32_t* gid;           // [nchunk][H] group id of each haplotype at the chunk end
     uint32_t* ngroups;       // [nchunk]
     uint32_t* xb;            // [2][H] phase B exchange
+    uint8_t* tp;             // [nchunk][NP][H] key byte p of each haplotype (radix digit planes)
+    uint8_t* th;             // [nchunk][H][TS] the same bytes, haplotype-major (key comparison)
+    uint32_t NP, TS;         // key bytes per chunk (C / 8), row stride of th (NP rounded to 16)
+    uint32_t* gseg;          // [nchunk][H] group id of the key over chunks [segment start, c]
+    uint32_t* nseg;          // [nchunk]   number of those groups
+    uint32_t SL;             // chunks per composition segment
+    uint32_t rows_div;       // phase C also writes the divergence rows of its chunk
 };
+
+// Divergence row m (every entry m), streaming stores by the whole workgroup.
+__device__ __forceinline__ void div_row(const PbwtArgs& a, uint32_t m) {
+    uint32_t* drow = a.div + size_t(m) * a.H;
+    for (uint32_t i = threadIdx.x; i < a.H; i += blockDim.x) __builtin_nontemporal_store(m, drow + i);
+}
 
 __device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
@@ -495,6 +508,8 @@ __global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
     __shared__ uint32_t wz[kPW];
     const uint32_t c = blockIdx.x;
     const uint32_t m_first = c * a.C, m_end = min(a.M, m_first + a.C);
+    const bool divs = a.rows_div && a.div;
+    if (divs) div_row(a, m_end - 1u);  // the chunk's last row is written by phase A/B
     if (m_end - m_first < 2) return;
     const WaveCtx q(a);
 
@@ -538,6 +553,7 @@ __global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
                 row[dst] = h;
             }
         }
+        if (divs) div_row(a, m);
         if (pre) nx.store(a, col[((m - m_first) & 1u) ^ 1u]);
         __syncthreads();
     }
@@ -662,6 +678,420 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_lds_kernel(PbwtArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Radix form of phases A and B (H <= 32768).
+//
+// A chunk's key of haplotype h is its alleles over the chunk, last marker
+// most significant, as NP = C/8 bytes (byte p = markers 8p..8p+7 of the
+// chunk, bit k = marker 8p+k).  Sorting the haplotypes by that key, ties by
+// index, is the chunk's sweep from the identity order, so phase A is an LSD
+// radix sort over the key (2·NP stable 4-bit counting passes instead of C
+// stable 1-bit partitions) followed by an exact comparison of adjacent keys
+// for the group starts.  Phase B's stable sort by group id uses the same
+// pass.  The order stays in registers, thread t holding positions
+// [t*K, t*K + K) (two 16-bit ids per register); a pass counts each thread's
+// digits in its own LDS counters (so the count is stable without any
+// cross-lane matching), scans the digit-major counter array, scatters
+// through an LDS exchange buffer and reads the new order back.
+// ---------------------------------------------------------------------------
+
+// Key bytes of a chunk: workgroup (chunk c, 16 column words = 1024
+// haplotypes), wave w the 64 haplotypes of word b0 + w.  The chunk's columns
+// are staged in LDS 512 markers at a time with coalesced loads; both output
+// layouts are written with coalesced stores; markers past M count as 0.
+constexpr uint32_t kKeyBlock = 512;  // markers staged per round (64 key bytes)
+__global__ __launch_bounds__(kPT) void pbwt_keys_kernel(PbwtArgs a) {
+    __shared__ uint64_t colw[kKeyBlock][kPW];
+    const uint32_t c = blockIdx.x;
+    const uint32_t t = threadIdx.x, lane = t & 63u;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t W = a.W2 / 2u, b0 = blockIdx.y * kPW, b = b0 + w;
+    const uint64_t* cols = reinterpret_cast<const uint64_t*>(a.cols32);
+    const uint32_t h = b * 64u + lane;
+    const bool valid = b < W && h < a.H;
+    const uint32_t m0 = c * a.C, len = min(a.M, m0 + a.C) - m0;
+    uint8_t* tp = a.tp + size_t(c) * a.NP * a.H;
+    uint8_t* th = a.th + (size_t(c) * a.H + (valid ? h : 0u)) * a.TS;
+    for (uint32_t mb = 0; mb < a.TS * 8u; mb += kKeyBlock) {
+        __syncthreads();
+        for (uint32_t i = t; i < kKeyBlock * kPW; i += kPT) {
+            const uint32_t mr = mb + i / kPW, bb = b0 + i % kPW;
+            colw[i / kPW][i % kPW] = (mr < len && bb < W) ? cols[size_t(m0 + mr) * W + bb] : 0ull;
+        }
+        __syncthreads();
+        for (uint32_t p0 = 0; p0 < kKeyBlock / 8u && (mb / 8u + p0) < a.TS; p0 += 16) {
+            uint32_t v[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (uint32_t q = 0; q < 16; ++q) {
+                const uint32_t pr = p0 + q;  // byte within the staged block
+                uint32_t byte = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < 8; ++k)
+                    byte |= uint32_t((colw[8u * pr + k][w] >> lane) & 1u) << k;
+                const uint32_t p = mb / 8u + pr;
+                if (valid && p < a.NP) tp[size_t(p) * a.H + h] = uint8_t(byte);
+                v[q >> 2] |= byte << (8u * (q & 3u));
+            }
+            if (valid) *reinterpret_cast<uint4*>(th + mb / 8u + p0) = make_uint4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+// The order of a radix kernel: thread t holds positions [t*K, t*K + K),
+// two 16-bit haplotype ids per register.
+template <int K>
+struct BOrd {
+    uint32_t v[(K + 1) / 2];
+    __device__ __forceinline__ uint32_t get(int j) const { return (v[j >> 1] >> (16 * (j & 1))) & 0xffffu; }
+    __device__ __forceinline__ void set(int j, uint32_t x) {
+        const uint32_t sh = 16u * uint32_t(j & 1);
+        v[j >> 1] = (v[j >> 1] & ~(0xffffu << sh)) | (x << sh);
+    }
+};
+
+// LDS of a radix pass: per-thread counters of the 16 digits (digit-major,
+// [16][kPT] 16-bit) and the exchange buffer (which also carries the 16
+// per-wave sums of the scan: it is idle between a pass's read-back and the
+// next pass's scatter).
+struct R4Lds {
+    uint16_t* cnt;
+    uint16_t* X;
+};
+
+// One stable counting-sort pass of the workgroup's H elements by the 4-bit
+// digit(h).  The first barrier also publishes LDS data the digit function
+// reads, written before the call.  If row is set, the new order is also
+// written there (coalesced, from the exchange buffer).
+template <int K, class Digit>
+__device__ __forceinline__ void radix4_pass(BOrd<K>& e, uint32_t H, const R4Lds& L, Digit digit,
+                                            uint32_t* row) {
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t p0 = t * uint32_t(K);
+    uint4* c4 = reinterpret_cast<uint4*>(L.cnt);  // 2048 uint4
+    c4[t] = make_uint4(0u, 0u, 0u, 0u);
+    c4[t + kPT] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (p0 + uint32_t(j) < H) {
+            uint16_t* cp = L.cnt + digit(e.get(j)) * kPT + t;
+            *cp = uint16_t(*cp + 1u);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the digit-major counters: thread t owns entries
+    // [16t, 16t + 16) = digit w, threads 16*lane .. 16*lane + 15
+    uint32_t* wsum = reinterpret_cast<uint32_t*>(L.X);
+    const uint4 A = c4[2 * t], B = c4[2 * t + 1];
+    const uint32_t wd[8] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w};
+    uint32_t ex[16];
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        ex[i] = s;
+        s += (wd[i >> 1] >> (16 * (i & 1))) & 0xffffu;
+    }
+    uint32_t incl = s;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t n = __shfl_up(incl, o, kWave);
+        if (lane >= uint32_t(o)) incl += n;
+    }
+    if (lane == 63u) wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = incl - s;
+    for (uint32_t v = 0; v < w; ++v) base += wsum[v];
+    uint32_t o[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) o[i] = (base + ex[2 * i]) | ((base + ex[2 * i + 1]) << 16);
+    c4[2 * t] = make_uint4(o[0], o[1], o[2], o[3]);
+    c4[2 * t + 1] = make_uint4(o[4], o[5], o[6], o[7]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (p0 + uint32_t(j) < H) {
+            const uint32_t h = e.get(j);
+            uint16_t* cp = L.cnt + digit(h) * kPT + t;
+            const uint32_t dst = *cp;
+            *cp = uint16_t(dst + 1u);
+            L.X[dst] = uint16_t(h);
+        }
+    }
+    __syncthreads();
+    if (row)
+        for (uint32_t i = t; i < H; i += kPT) row[i] = L.X[i];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (p0 + uint32_t(j) < H) e.set(j, L.X[p0 + uint32_t(j)]);
+}
+
+// Phase A, radix form: one workgroup per chunk.
+template <int K>
+__global__ __launch_bounds__(kPT) void pbwt_chunk_sort_kernel(PbwtArgs a) {
+    using G = LdsGeom<K>;
+    __shared__ __align__(16) uint16_t X[G::NPOS];
+    __shared__ __align__(16) uint16_t cnt[16 * kPT];
+    __shared__ __align__(4) uint8_t dig[G::NPOS];  // the current key byte of each haplotype
+    __shared__ uint32_t s_b[kPW];
+    const R4Lds L{cnt, X};
+    const uint32_t c = blockIdx.x, t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t m_first = c * a.C, len = min(a.M, m_first + a.C) - m_first;
+    const uint32_t np = (len + 7u) / 8u;
+    const uint32_t H = a.H, p0 = t * uint32_t(K);
+    BOrd<K> e;
+#pragma unroll
+    for (int j = 0; j < K; ++j) e.set(j, p0 + uint32_t(j));
+
+    const uint8_t* tpc = a.tp + size_t(c) * a.NP * H;
+    const uint32_t nw = (H + 3u) / 4u;  // 32-bit words of a plane
+    constexpr uint32_t kPF = (G::NPOS / 4u + kPT - 1) / kPT;
+    auto plane_word = [&](uint32_t p, uint32_t i) -> uint32_t {
+        // four key bytes; a plane is H bytes, so an unaligned or last word is assembled bytewise
+        const uint8_t* src = tpc + size_t(p) * H + 4u * i;
+        if (4u * i + 4u <= H && ((reinterpret_cast<uintptr_t>(src) & 3u) == 0u))
+            return *reinterpret_cast<const uint32_t*>(src);
+        uint32_t v = 0;
+        for (uint32_t k = 0; k < 4u && 4u * i + k < H; ++k) v |= uint32_t(src[k]) << (8u * k);
+        return v;
+    };
+    uint32_t pf[kPF];
+#pragma unroll
+    for (uint32_t k = 0; k < kPF; ++k) {
+        const uint32_t i = t + k * kPT;
+        pf[k] = i < nw ? plane_word(0, i) : 0u;
+    }
+    for (uint32_t p = 0; p < np; ++p) {
+#pragma unroll
+        for (uint32_t k = 0; k < kPF; ++k) {
+            const uint32_t i = t + k * kPT;
+            if (i < nw) reinterpret_cast<uint32_t*>(dig)[i] = pf[k];
+        }
+        if (p + 1 < np) {
+#pragma unroll
+            for (uint32_t k = 0; k < kPF; ++k) {
+                const uint32_t i = t + k * kPT;
+                pf[k] = i < nw ? plane_word(p + 1, i) : 0u;
+            }
+        }
+        // low nibble, then high nibble (the dig buffer is rewritten only after
+        // the second pass's scatter barrier)
+        radix4_pass<K>(e, H, L, [&](uint32_t h) { return uint32_t(dig[h]) & 15u; }, nullptr);
+        radix4_pass<K>(e, H, L, [&](uint32_t h) { return uint32_t(dig[h]) >> 4; }, nullptr);
+    }
+    // X holds the sorted order by position (also for np == 0, written here)
+    if (np == 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (p0 + uint32_t(j) < H) X[p0 + uint32_t(j)] = uint16_t(e.get(j));
+    }
+    __syncthreads();
+    if (c == 0) {
+        uint32_t* lastrow = a.prefix + size_t(m_first + len - 1u) * H;
+        for (uint32_t i = t; i < H; i += kPT) lastrow[i] = X[i];
+    }
+    // group starts: a key differs from its predecessor's (exact, all bytes)
+    const uint8_t* thc = a.th + size_t(c) * H * a.TS;
+    uint32_t flags = 0, cntf = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t pos = p0 + uint32_t(j);
+        if (pos >= H) continue;
+        bool start = pos == 0u;
+        if (!start) {
+            const uint4* ka = reinterpret_cast<const uint4*>(thc + size_t(e.get(j)) * a.TS);
+            const uint4* kb = reinterpret_cast<const uint4*>(thc + size_t(X[pos - 1u]) * a.TS);
+            for (uint32_t u = 0; u < a.TS / 16u && !start; ++u) {
+                const uint4 x = ka[u], y = kb[u];
+                start = (x.x != y.x) || (x.y != y.y) || (x.z != y.z) || (x.w != y.w);
+            }
+        }
+        flags |= uint32_t(start) << j;
+        cntf += uint32_t(start);
+    }
+    // dense rank of each group in position order: thread counts, block scan
+    uint32_t incl = cntf;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t n = __shfl_up(incl, o, kWave);
+        if (lane >= uint32_t(o)) incl += n;
+    }
+    if (lane == 63u) s_b[w] = incl;
+    __syncthreads();
+    uint32_t rk = incl - cntf, Gn = 0;
+#pragma unroll
+    for (int v = 0; v < kPW; ++v) {
+        rk += uint32_t(v) < w ? s_b[v] : 0u;
+        Gn += s_b[v];
+    }
+    uint32_t* gid = a.gid + size_t(c) * H;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (p0 + uint32_t(j) >= H) continue;
+        rk += (flags >> j) & 1u;
+        gid[e.get(j)] = rk - 1u;
+    }
+    if (t == 0) a.ngroups[c] = Gn;
+}
+
+// Phase B, radix form, as a two-level scan over chunk segments.  With
+// g_c the chunk-c group ids and G[a, c] the dense rank of the key over chunks
+// a..c (later chunk more significant), G[a, c] = rank of (g_c, G[a, c-1]),
+// and the order at the end of chunk c is the stable sort of the order at the
+// end of chunk a - 1 by G[a, c].  Chunks 1..n-1 are cut into segments of SL:
+//   B1  one workgroup per segment: G[a, c] for each chunk c of the segment
+//       (a = its first chunk), from the identity order, by stable sorts by g_c
+//       and the adjacent comparison of (g_c, G[a, c-1]);
+//   B2  one workgroup, segments in sequence: the order at each segment end;
+//   B3  one workgroup per other chunk: its end order from its segment's
+//       starting order.
+// The serial depth is about 2 sqrt(n) sorts instead of n.
+
+// Radix passes (4 bits each) needed for ids below Gn.
+__device__ __forceinline__ uint32_t id_passes(uint32_t Gn) {
+    uint32_t n = 0;
+    while (Gn > 1u && n < 4u && ((Gn - 1u) >> (4u * n)) != 0u) ++n;
+    return n;
+}
+
+// Boundary row of chunk c (its last marker).
+__device__ __forceinline__ uint32_t* chunk_end_row(const PbwtArgs& a, uint32_t c) {
+    return a.prefix + size_t(min(a.M, c * a.C + a.C) - 1u) * a.H;
+}
+
+// Load a row into the blocked order.
+template <int K>
+__device__ __forceinline__ void load_order(BOrd<K>& e, const uint32_t* row, uint32_t H) {
+    const uint32_t p0 = threadIdx.x * uint32_t(K);
+#pragma unroll
+    for (int j = 0; j < K; ++j) e.set(j, p0 + uint32_t(j) < H ? row[p0 + uint32_t(j)] : 0u);
+}
+
+// Sort the blocked order e stably by ids (global [H], below Gn), through the
+// LDS id array gl; the result also goes to row.
+template <int K>
+__device__ __forceinline__ void sort_by_ids(BOrd<K>& e, const uint32_t* ids, uint32_t Gn, uint32_t H,
+                                            const R4Lds& L, uint16_t* gl, uint32_t* row) {
+    for (uint32_t i = threadIdx.x; i < H; i += kPT) gl[i] = uint16_t(ids[i]);
+    const uint32_t np = id_passes(Gn);
+    for (uint32_t pass = 0; pass < np; ++pass) {
+        const uint32_t sh = 4u * pass;
+        radix4_pass<K>(e, H, L, [&](uint32_t h) { return (uint32_t(gl[h]) >> sh) & 15u; },
+                       pass + 1 == np ? row : nullptr);
+    }
+    if (np == 0) {  // one group: the order is unchanged
+        __syncthreads();
+        const uint32_t p0 = threadIdx.x * uint32_t(K);
+        if (row) {
+#pragma unroll
+            for (int j = 0; j < K; ++j)
+                if (p0 + uint32_t(j) < H) row[p0 + uint32_t(j)] = e.get(j);
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if (p0 + uint32_t(j) < H) L.X[p0 + uint32_t(j)] = uint16_t(e.get(j));
+        __syncthreads();
+    }
+}
+
+// B1: one workgroup per segment.
+template <int K>
+__global__ __launch_bounds__(kPT) void pbwt_seg_ranks_kernel(PbwtArgs a) {
+    using G = LdsGeom<K>;
+    __shared__ __align__(16) uint16_t X[G::NPOS];
+    __shared__ __align__(16) uint16_t cnt[16 * kPT];
+    __shared__ __align__(4) uint16_t gl[G::NPOS];
+    const R4Lds L{cnt, X};
+    uint32_t* s_b = reinterpret_cast<uint32_t*>(cnt);  // free between sorts
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6, H = a.H, p0 = t * uint32_t(K);
+    const uint32_t c0 = 1u + blockIdx.x * a.SL, c1 = min(a.nchunk, c0 + a.SL);
+    BOrd<K> e;
+#pragma unroll
+    for (int j = 0; j < K; ++j) e.set(j, p0 + uint32_t(j));
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t* g = a.gid + size_t(c) * H;
+        uint32_t* out = a.gseg + size_t(c) * H;
+        sort_by_ids<K>(e, g, a.ngroups[c], H, L, gl, nullptr);
+        if (c == c0) {  // G[a, a] = g_a
+            for (uint32_t i = t; i < H; i += kPT) out[i] = g[i];
+            if (t == 0) a.nseg[c] = a.ngroups[c];
+        } else {
+            // X holds the new order by position; gl the g_c of each haplotype
+            const uint32_t* prev = a.gseg + size_t(c - 1u) * H;
+            uint32_t flags = 0, cntf = 0;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                const uint32_t pos = p0 + uint32_t(j);
+                if (pos >= H) continue;
+                const uint32_t h = e.get(j);
+                bool start = pos == 0u;
+                if (!start) {
+                    const uint32_t hp = X[pos - 1u];
+                    start = gl[h] != gl[hp] || prev[h] != prev[hp];
+                }
+                flags |= uint32_t(start) << j;
+                cntf += uint32_t(start);
+            }
+            uint32_t incl = cntf;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t n = __shfl_up(incl, o, kWave);
+                if (lane >= uint32_t(o)) incl += n;
+            }
+            if (lane == 63u) s_b[w] = incl;
+            __syncthreads();
+            uint32_t rk = incl - cntf, Gn = 0;
+#pragma unroll
+            for (int v = 0; v < kPW; ++v) {
+                rk += uint32_t(v) < w ? s_b[v] : 0u;
+                Gn += s_b[v];
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                if (p0 + uint32_t(j) >= H) continue;
+                rk += (flags >> j) & 1u;
+                out[e.get(j)] = rk - 1u;
+            }
+            if (t == 0) a.nseg[c] = Gn;
+        }
+        // this step's ranks are read (from global memory) by the next step
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// B2: the order at each segment end, segments in sequence.
+template <int K>
+__global__ __launch_bounds__(kPT) void pbwt_seg_chain_kernel(PbwtArgs a) {
+    using G = LdsGeom<K>;
+    __shared__ __align__(16) uint16_t X[G::NPOS];
+    __shared__ __align__(16) uint16_t cnt[16 * kPT];
+    __shared__ __align__(4) uint16_t gl[G::NPOS];
+    const R4Lds L{cnt, X};
+    BOrd<K> e;
+    load_order<K>(e, chunk_end_row(a, 0), a.H);
+    for (uint32_t c0 = 1; c0 < a.nchunk; c0 += a.SL) {
+        const uint32_t ce = min(a.nchunk, c0 + a.SL) - 1u;
+        sort_by_ids<K>(e, a.gseg + size_t(ce) * a.H, a.nseg[ce], a.H, L, gl, chunk_end_row(a, ce));
+    }
+}
+
+// B3: one workgroup per chunk that does not end a segment.
+template <int K>
+__global__ __launch_bounds__(kPT) void pbwt_seg_fill_kernel(PbwtArgs a) {
+    using G = LdsGeom<K>;
+    __shared__ __align__(16) uint16_t X[G::NPOS];
+    __shared__ __align__(16) uint16_t cnt[16 * kPT];
+    __shared__ __align__(4) uint16_t gl[G::NPOS];
+    const R4Lds L{cnt, X};
+    const uint32_t c = 1u + blockIdx.x;
+    const uint32_t c0 = 1u + ((c - 1u) / a.SL) * a.SL;
+    const uint32_t ce = min(a.nchunk, c0 + a.SL) - 1u;
+    if (c == ce) return;  // done by B2 (the whole workgroup leaves)
+    BOrd<K> e;
+    load_order<K>(e, chunk_end_row(a, c0 - 1u), a.H);
+    sort_by_ids<K>(e, a.gseg + size_t(c) * a.H, a.nseg[c], a.H, L, gl, chunk_end_row(a, c));
+}
+
+// ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
 namespace {
@@ -678,11 +1108,37 @@ uint32_t pbwt_rounds(uint32_t H) {
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// H <= 49152: phase A and B in the partition form (A also writes the
+// divergence rows), phase C with the order in LDS.
 template <int K>
 void launch_lds(const PbwtArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(pbwt_groups_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
     if (a.nchunk > 1) hipLaunchKernelGGL(pbwt_compose_kernel, dim3(1), dim3(kPT), 0, st, a);
     hipLaunchKernelGGL(pbwt_rows_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
+}
+
+// H <= 32768: key bytes, phase A and B in the radix form, divergence fill,
+// phase C with the order in LDS.
+constexpr uint32_t kRadixMaxK = 32;
+template <int K>
+void launch_radix(const PbwtArgs& a, hipStream_t st) {
+    const uint32_t W = a.W2 / 2u;
+    hipLaunchKernelGGL(pbwt_keys_kernel, dim3(a.nchunk, (W + kPW - 1) / kPW), dim3(kPT), 0, st, a);
+    hipLaunchKernelGGL(pbwt_chunk_sort_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
+    if (a.nchunk > 1) {
+        const uint32_t nseg = (a.nchunk - 1u + a.SL - 1u) / a.SL;
+        hipLaunchKernelGGL(pbwt_seg_ranks_kernel<K>, dim3(nseg), dim3(kPT), 0, st, a);
+        hipLaunchKernelGGL(pbwt_seg_chain_kernel<K>, dim3(1), dim3(kPT), 0, st, a);
+        if (a.nchunk > 2)
+            hipLaunchKernelGGL(pbwt_seg_fill_kernel<K>, dim3(a.nchunk - 1u), dim3(kPT), 0, st, a);
+    }
+    hipLaunchKernelGGL(pbwt_rows_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
+}
+
+size_t key_bytes(uint32_t H, uint32_t M, uint32_t C) {
+    const size_t n = (size_t(M) + C - 1) / C;
+    const size_t np = (C + 7u) / 8u, ts = (np + 15u) & ~size_t(15);
+    return align256(n * np * H) + align256(n * H * ts);
 }
 
 }  // namespace
@@ -696,8 +1152,11 @@ uint32_t pbwt_default_chunk(uint32_t H, uint32_t M) {
 size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C) {
     if (H == 0 || M == 0 || C == 0) return 0;
     const size_t n = (size_t(M) + C - 1) / C, h = H;
-    const size_t npos = size_t(kPW) * 64u * pbwt_rounds(H);  // padded exchange length
-    return align256(n * 2 * h * 4) + align256(n * h * 4) + align256(n * 4) + align256(2 * npos * 4);
+    const uint32_t K = pbwt_rounds(H);
+    const size_t npos = size_t(kPW) * 64u * K;  // padded exchange length
+    const size_t base = align256(n * 2 * h * 4) + align256(n * h * 4) + align256(n * 4) +
+                        align256(2 * npos * 4);
+    return base + (K != 0 && K <= kRadixMaxK ? key_bytes(H, M, C) + align256(n * 4) : 0);
 }
 
 int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
@@ -725,17 +1184,32 @@ int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* pr
     a.ngroups = reinterpret_cast<uint32_t*>(p);
     p += align256(n * 4);
     a.xb = reinterpret_cast<uint32_t*>(p);
+    p += align256(2 * size_t(kPW) * 64u * K * 4);
+    a.NP = (C + 7u) / 8u;
+    a.TS = (a.NP + 15u) & ~15u;
+    if (K <= kRadixMaxK) {
+        a.tp = reinterpret_cast<uint8_t*>(p);
+        p += align256(n * a.NP * h);
+        a.th = reinterpret_cast<uint8_t*>(p);
+        p += align256(n * h * a.TS);
+        a.nseg = reinterpret_cast<uint32_t*>(p);
+        a.gseg = a.xa;  // the partition form's exchange, unused by the radix form
+        uint32_t sl = 1;
+        while (sl * sl < a.nchunk - 1u) ++sl;
+        a.SL = sl;
+        a.rows_div = 1;
+    }
     switch (K) {
-        case 1: launch_lds<1>(a, stream); break;
-        case 2: launch_lds<2>(a, stream); break;
-        case 4: launch_lds<4>(a, stream); break;
-        case 6: launch_lds<6>(a, stream); break;
-        case 8: launch_lds<8>(a, stream); break;
-        case 12: launch_lds<12>(a, stream); break;
-        case 16: launch_lds<16>(a, stream); break;
-        case 20: launch_lds<20>(a, stream); break;
-        case 24: launch_lds<24>(a, stream); break;
-        case 32: launch_lds<32>(a, stream); break;
+        case 1: launch_radix<1>(a, stream); break;
+        case 2: launch_radix<2>(a, stream); break;
+        case 4: launch_radix<4>(a, stream); break;
+        case 6: launch_radix<6>(a, stream); break;
+        case 8: launch_radix<8>(a, stream); break;
+        case 12: launch_radix<12>(a, stream); break;
+        case 16: launch_radix<16>(a, stream); break;
+        case 20: launch_radix<20>(a, stream); break;
+        case 24: launch_radix<24>(a, stream); break;
+        case 32: launch_radix<32>(a, stream); break;
         case 40: launch_lds<40>(a, stream); break;
         case 48: launch_lds<48>(a, stream); break;
         default:  // H > 49152: the order exchanges through global memory
