@@ -30,6 +30,14 @@ enum class Precision : uint32_t {
     LogF32 = 2      // log-space FP32 (three-term log-sum-exp per state)
 };
 
+// Which L-best state selection builds the state table.
+enum class SelectionMode : uint32_t {
+    CpuSort = 0,   // the reference's CPU path, std::sort over (divergence, haplotype)
+                   // (src/pbwt/pbwt_index.cpp:20-44)
+    GpuCompat = 1  // the reference's GPU kernels, launch_select_states
+                   // (src/pbwt/pbwt_selector.cu:400-463), which choose differently
+};
+
 enum class SamplingMode : uint32_t {
     Philox = 0,  // inverse-CDF draw with Philox4x32-10 (seed, sample, marker)
     Argmax = 1   // most probable state (src/kernels/sampling.cu:138-182)
@@ -57,6 +65,7 @@ struct ImputationConfig {
     TransitionMode transition_mode;
     SamplingMode sampling;
     Precision precision;
+    SelectionMode selection;
     uint64_t seed;               // Philox key (default 12345, sampling.hpp:74)
     bool keep_pbwt_arrays;       // keep prefix/divergence resident after build_index()
 
@@ -75,6 +84,7 @@ struct ImputationConfig {
           transition_mode(TransitionMode::FirstInterval),
           sampling(SamplingMode::Philox),
           precision(Precision::Auto),
+          selection(SelectionMode::CpuSort),
           seed(12345ull),
           keep_pbwt_arrays(true) {}
 };

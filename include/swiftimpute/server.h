@@ -41,6 +41,8 @@ typedef struct si_server_cfg {
     uint64_t hbm_budget_bytes;  /* device memory for admitted requests */
     int device_id;
     uint32_t max_queue_delay_us; /* a partial batch waits up to this long for company (0) */
+    uint32_t selection;         /* 0: CPU-path selection (pbwt_index.cpp:20-44);
+                                   1: the reference's GPU kernels (si_select_perm_gpu_compat) */
 } si_server_cfg;
 
 void si_server_defaults(si_server_cfg* cfg);

@@ -88,16 +88,36 @@ int si_pbwt_sweep_chunked(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t
  */
 int si_select_perm(uint32_t H, uint32_t L, uint32_t* perm);
 
+/* A perm entry with this bit set names a haplotype directly (the low 31 bits)
+ * instead of a position in the marker's prefix row. */
+#define SI_PERM_FIXED 0x80000000u
+
+/* Host: the selection the reference's GPU kernels make instead
+ * (launch_select_states, src/pbwt/pbwt_selector.cu:400-463), which differs from
+ * the CPU path above.  With every divergence of a row equal:
+ *   L <= 32 and H <= 10000 (select_states_warp_kernel, :212-312): prefix
+ *     positions 0, 32, 64, 96 (as many as lane 0 holds, min(4, ceil(H/32))),
+ *     then haplotypes 4, 5, ..., flagged SI_PERM_FIXED;
+ *   otherwise (select_states_bitonic_kernel, :320-394): positions 0..L-1.
+ * For H < 128 the reference's warp kernel shuffles across lanes that have
+ * left its loop (undefined); this function takes lane 0's entries.
+ * perm receives min(L, H) entries; L <= 64 (the reference's third kernel,
+ * for L > 64, is outside the supported state counts). */
+int si_select_perm_gpu_compat(uint32_t H, uint32_t L, uint32_t* perm);
+
 /* Device: L-best state selection + pairing + allele lookup, replacing the
  * reference's per-(sample, marker) CPU selection and pair expansion
  * (src/api/imputer.cpp:894-951) and the emission kernels' allele reads
  * (src/kernels/emission.cu:96-103).  The selection is sample-independent, so
  * one [M][L] table serves every sample.
  *   mode 0: PBWT selection, states[m][k] = prefix[m][perm[k]] (k < H), k % H
- *           otherwise; partner (h + 1) % H.
+ *           otherwise; partner (h + 1) % H.  An entry flagged SI_PERM_FIXED
+ *           gives the haplotype itself (perm[k] & 0x7fffffff).  Entries must
+ *           name a position or haplotype below H.
  *   mode 1: simple selection (use_pbwt_selection = false), pair (2k, 2k+1)
  *           (src/api/imputer.cpp:933-942); requires 2L <= H.
- * perm_host: min(L, H) entries from si_select_perm (ignored in mode 1).
+ * perm_host: min(L, H) entries from si_select_perm or
+ *   si_select_perm_gpu_compat (ignored in mode 1).
  * states [M][L] uint32 (h0) may be NULL.  sa/mmask as described above.
  */
 int si_state_table(const uint32_t* prefix, const uint64_t* cols, const uint64_t* miss,

@@ -47,6 +47,11 @@ def lib():
         L.or_select_states_sort.argtypes = [_u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
         L.or_state_table.argtypes = [_u8p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_uint32, ctypes.c_int, _u32p, _u32p, _u8p, _u8p]
+        for name in ("or_select_states_gpu_warp", "or_select_states_gpu_insert",
+                     "or_select_states_gpu"):
+            getattr(L, name).argtypes = [_u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        L.or_state_table_gpu.argtypes = [_u8p, _u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32, _u32p, _u32p, _u8p, _u8p]
         L.or_genetic_map.argtypes = [_u64p, _f64p, ctypes.c_uint32, _f64p]
         L.or_genetic_map.restype = None
         L.or_transitions.argtypes = [_f64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
@@ -137,6 +142,37 @@ def state_table(ref: np.ndarray, prefix: np.ndarray, L: int, simple_mode: bool =
     _chk(lib().or_state_table(_p(ref, _u8p), _p(prefix, _u32p), M, H, L, int(simple_mode),
                               _p(h0, _u32p), _p(h1, _u32p), _p(a0, _u8p), _p(a1, _u8p)),
          "or_state_table")
+    return h0, h1, a0, a1
+
+
+def select_states_gpu(prefix_row: np.ndarray, div_row: np.ndarray, L: int,
+                      kernel: str = "dispatch") -> np.ndarray:
+    """The reference's GPU selection at one marker (pbwt_selector.cu:212-463):
+    ``kernel`` is "warp", "insert" or "dispatch" (launch_select_states' pick)."""
+    prefix_row = np.ascontiguousarray(prefix_row, np.uint32)
+    div_row = np.ascontiguousarray(div_row, np.uint32)
+    out = np.empty(L, np.uint32)
+    fn = {"warp": "or_select_states_gpu_warp", "insert": "or_select_states_gpu_insert",
+          "dispatch": "or_select_states_gpu"}[kernel]
+    _chk(getattr(lib(), fn)(_p(prefix_row, _u32p), _p(div_row, _u32p), prefix_row.size, L,
+                            _p(out, _u32p)), fn)
+    return out
+
+
+def state_table_gpu(ref: np.ndarray, prefix: np.ndarray, div: np.ndarray, L: int):
+    """State table from the reference's GPU selection over the PBWT arrays,
+    pairs (h, (h+1) % H) as imputer.cpp:862-873.  Returns (h0, h1, a0, a1)."""
+    ref = np.ascontiguousarray(ref, np.uint8)
+    prefix = np.ascontiguousarray(prefix, np.uint32)
+    div = np.ascontiguousarray(div, np.uint32)
+    M, H = ref.shape
+    h0 = np.empty((M, L), np.uint32)
+    h1 = np.empty((M, L), np.uint32)
+    a0 = np.empty((M, L), np.uint8)
+    a1 = np.empty((M, L), np.uint8)
+    _chk(lib().or_state_table_gpu(_p(ref, _u8p), _p(prefix, _u32p), _p(div, _u32p), M, H, L,
+                                  _p(h0, _u32p), _p(h1, _u32p), _p(a0, _u8p), _p(a1, _u8p)),
+         "or_state_table_gpu")
     return h0, h1, a0, a1
 
 

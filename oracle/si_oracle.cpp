@@ -15,6 +15,10 @@
 //   or_select_perm         pinned: libstdc++ std::sort is the library routine the
 //                          reference calls; KATs of the permutation prefixes.
 //   or_select_states_sort  pinned: equals or_select_perm gather (div == m).
+//   or_select_states_gpu_* pinned: closed forms with div == m ([p0, p32, p64,
+//                          p96, 4, 5, ...] and p[0..L-1]), hand-traced rounds
+//                          on crafted divergences, a stable sort by divergence
+//                          for the insertion kernel on random rows.
 //   or_transitions         pinned: closed forms (rho = 1 limit, d = 0 limit).
 //   or_fb_f64 / _fast      pinned: brute-force path enumeration (M <= 6, L <= 3)
 //                          with the -999 floor folded in, uniform-T closed form,
@@ -144,6 +148,125 @@ int or_select_states_sort(const uint32_t* prefix_row, const uint32_t* div_row,
                  const std::pair<uint32_t, uint32_t>& b) { return a.first > b.first; });
     for (uint32_t i = 0; i < L && i < cand.size(); ++i) out[i] = cand[i].second;
     for (uint32_t i = uint32_t(cand.size()); i < L; ++i) out[i] = i % H;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// A3: select_states_warp_kernel (src/pbwt/pbwt_selector.cu:212-312) for one
+// (sample, marker), lane by lane.  Each of 32 lanes walks prefix positions
+// lane, lane+32, ... keeping a top-4 by divergence with the kernel's
+// insertion rule (:243-259); then round r = 0.. while r < L and r < the
+// lane's count (:269) takes the warp maximum of every lane's r-th entry, the
+// lowest lane holding it wins (ballot + ffs, :279-283), lane 0 writes the
+// winner's haplotype and the winner zeroes its r-th divergence (:292-294).
+// Slots from lane 0's count up to L are filled with i % H (:305-310).  The
+// loop bound is per lane, so for H < 128 lanes leave before lane 0 and the
+// kernel's full-mask shuffles are undefined; here only lanes still in the
+// loop take part (lane 0 always is, holding the most entries).
+// ---------------------------------------------------------------------------
+int or_select_states_gpu_warp(const uint32_t* prefix_row, const uint32_t* div_row,
+                              uint32_t H, uint32_t L, uint32_t* out) {
+    if (H == 0 || L == 0) return -1;
+    uint32_t bd[32][4] = {}, bh[32][4] = {}, cnt[32] = {};
+    for (uint32_t lane = 0; lane < 32; ++lane) {
+        for (uint32_t i = lane; i < H; i += 32) {
+            const uint32_t hap = prefix_row[i], dv = div_row[i];
+            for (uint32_t j = 0; j < 4; ++j) {
+                if (cnt[lane] <= j || dv > bd[lane][j]) {
+                    for (uint32_t k = 3; k > j; --k) {
+                        bd[lane][k] = bd[lane][k - 1];
+                        bh[lane][k] = bh[lane][k - 1];
+                    }
+                    bd[lane][j] = dv;
+                    bh[lane][j] = hap;
+                    cnt[lane] = std::min(cnt[lane] + 1, 4u);
+                    break;
+                }
+            }
+        }
+    }
+    for (uint32_t r = 0; r < L && r < cnt[0]; ++r) {
+        uint32_t mx = 0;
+        for (uint32_t lane = 0; lane < 32; ++lane)
+            if (r < cnt[lane]) mx = std::max(mx, bd[lane][r]);
+        uint32_t win = 0;
+        while (!(r < cnt[win] && bd[win][r] == mx)) ++win;
+        out[r] = bh[win][r];
+        bd[win][r] = 0;
+    }
+    for (uint32_t i = cnt[0]; i < L; ++i) out[i] = i % H;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// A3: select_states_bitonic_kernel (src/pbwt/pbwt_selector.cu:320-394) for
+// one marker: a linear scan keeping the top L by divergence, inserting before
+// the first strictly smaller entry (:349-379); slots past the count are
+// filled with i % H (:386-392).  L <= 64 (:340).
+// ---------------------------------------------------------------------------
+int or_select_states_gpu_insert(const uint32_t* prefix_row, const uint32_t* div_row,
+                                uint32_t H, uint32_t L, uint32_t* out) {
+    if (H == 0 || L == 0 || L > 64) return -1;
+    uint32_t td[64], th[64], n = 0;
+    for (uint32_t i = 0; i < H; ++i) {
+        const uint32_t hap = prefix_row[i], dv = div_row[i];
+        uint32_t pos;
+        if (n < L) {
+            pos = n;
+            for (uint32_t j = 0; j < n; ++j)
+                if (dv > td[j]) { pos = j; break; }
+            for (uint32_t j = n; j > pos; --j) { td[j] = td[j - 1]; th[j] = th[j - 1]; }
+            ++n;
+        } else if (dv > td[L - 1]) {
+            pos = L - 1;
+            for (uint32_t j = 0; j + 1 < L; ++j)
+                if (dv > td[j]) { pos = j; break; }
+            for (uint32_t j = L - 1; j > pos; --j) { td[j] = td[j - 1]; th[j] = th[j - 1]; }
+        } else {
+            continue;
+        }
+        td[pos] = dv;
+        th[pos] = hap;
+    }
+    for (uint32_t i = 0; i < L; ++i) out[i] = i < n ? th[i] : i % H;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// A3 dispatch: launch_select_states (src/pbwt/pbwt_selector.cu:400-463) picks
+// the warp kernel for L <= 32 and H <= 10000, else the insertion kernel for
+// L <= 64 (the L > 64 kernel is not restated: -1).
+// ---------------------------------------------------------------------------
+int or_select_states_gpu(const uint32_t* prefix_row, const uint32_t* div_row, uint32_t H,
+                         uint32_t L, uint32_t* out) {
+    if (L <= 32 && H <= 10000) return or_select_states_gpu_warp(prefix_row, div_row, H, L, out);
+    if (L <= 64) return or_select_states_gpu_insert(prefix_row, div_row, H, L, out);
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// State table from the reference's GPU selection on the given PBWT arrays,
+// expanded into pairs (h, (h+1) % H) as imputer.cpp:862-873 does, with the
+// alleles looked up as in or_state_table.
+// ---------------------------------------------------------------------------
+int or_state_table_gpu(const uint8_t* ref, const uint32_t* prefix, const uint32_t* div,
+                       uint32_t M, uint32_t H, uint32_t L, uint32_t* h0_out, uint32_t* h1_out,
+                       uint8_t* a0_out, uint8_t* a1_out) {
+    if (H == 0 || L == 0 || L > 64) return -1;
+    std::vector<uint32_t> sel(L);
+    for (uint32_t m = 0; m < M; ++m) {
+        const int rc = or_select_states_gpu(prefix + size_t(m) * H, div + size_t(m) * H, H, L,
+                                            sel.data());
+        if (rc) return rc;
+        for (uint32_t k = 0; k < L; ++k) {
+            const uint32_t h0 = sel[k], h1 = (h0 + 1) % H;
+            const size_t o = size_t(m) * L + k;
+            if (h0_out) h0_out[o] = h0;
+            if (h1_out) h1_out[o] = h1;
+            a0_out[o] = ref[size_t(m) * H + h0];
+            a1_out[o] = ref[size_t(m) * H + h1];
+        }
+    }
     return 0;
 }
 

@@ -32,6 +32,7 @@ SI_FB_ARGMAX = 2
 SI_FB_LOGSPACE = 4
 SI_FB_PASS_CHAIN = 1
 SI_FB_PASS_BLOCK = 2
+SI_PERM_FIXED = 0x80000000
 
 
 class StepCoef(ctypes.Structure):
@@ -92,6 +93,7 @@ HIP_SYMBOLS = [
     ("si_pbwt_sweep_chunked", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp,
                                              ctypes.c_uint32, _vp, ctypes.c_size_t, _vp]),
     ("si_select_perm", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p]),
+    ("si_select_perm_gpu_compat", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p]),
     ("si_state_table", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.c_uint32, ctypes.c_uint32, _u32p, _vp, _vp, _vp,
                                       _vp]),

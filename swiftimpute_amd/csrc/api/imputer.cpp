@@ -143,7 +143,9 @@ void Imputer::build_index() {
         if (wsb) hip_check(hipMalloc(&ws, wsb), "hipMalloc");
         si_check(si_pbwt_sweep(dev_->cols, H, M, dev_->prefix, dev_->div, ws, wsb, dev_->stream),
                  "PBWT sweep");
-        si_check(si_select_perm(H, L, perm.data()), "selection permutation");
+        si_check(config_.selection == SelectionMode::GpuCompat ? si_select_perm_gpu_compat(H, L, perm.data())
+                                                               : si_select_perm(H, L, perm.data()),
+                 "selection permutation");
         si_check(si_state_table(dev_->prefix, dev_->cols, dev_->miss, H, M, L, 0, perm.data(), dev_->states,
                                 dev_->sa, dev_->mmask, dev_->stream),
                  "state table");

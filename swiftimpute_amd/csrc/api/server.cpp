@@ -407,6 +407,7 @@ void si_server_defaults(si_server_cfg* c) {
     c->hbm_budget_bytes = 32ull << 30;
     c->device_id = 0;
     c->max_queue_delay_us = 0;
+    c->selection = 0;
 }
 
 int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H, uint32_t M,
@@ -414,6 +415,7 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
     if (!cfg || !cols || !pos || !out || H == 0 || M == 0)
         return fail(SI_ERR_ARG, "si_server_create: bad arguments");
     if (cfg->n_states == 0 || cfg->n_states > 64) return fail(SI_ERR_ARG, "n_states must be in 1..64");
+    if (cfg->selection > 1) return fail(SI_ERR_ARG, "selection must be 0 (CPU path) or 1 (GPU kernels)");
     auto s = std::make_unique<si_server>();
     s->cfg = *cfg;
     s->cfg.max_streams = std::max(1u, cfg->max_streams);
@@ -440,7 +442,9 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
     }
     std::vector<uint32_t> perm(std::min(L, H));
     int rc = si_pbwt_sweep(s->d_cols, H, M, prefix, nullptr, nullptr, 0, st);
-    if (rc == SI_OK) rc = si_select_perm(H, L, perm.data());
+    if (rc == SI_OK)
+        rc = cfg->selection == 1 ? si_select_perm_gpu_compat(H, L, perm.data())
+                                 : si_select_perm(H, L, perm.data());
     if (rc == SI_OK)
         rc = si_state_table(prefix, s->d_cols, nullptr, H, M, L, 0, perm.data(), nullptr, s->d_sa,
                             s->d_mmask, st);

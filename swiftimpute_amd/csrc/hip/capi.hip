@@ -112,6 +112,27 @@ int si_select_perm(uint32_t H, uint32_t L, uint32_t* perm) {
     return SI_OK;
 }
 
+int si_select_perm_gpu_compat(uint32_t H, uint32_t L, uint32_t* perm) {
+    if (H == 0 || L == 0 || L > 64 || !perm)
+        return fail(SI_ERR_ARG, "si_select_perm_gpu_compat: need H > 0, 1 <= L <= 64, perm");
+    // launch_select_states (src/pbwt/pbwt_selector.cu:400-463) with every
+    // divergence in a row equal.  Warp kernel (L <= 32, H <= 10000,
+    // :212-312): each lane keeps its first four strided entries, every round's
+    // maximum is tied and lane 0 wins it, so round r emits lane 0's r-th entry,
+    // prefix position 32r, for r below lane 0's count min(4, ceil(H/32)); the
+    // rest is filled with haplotype k (k % H, :305-310).  Insertion kernel
+    // (:320-394): ties never displace, so the first min(L, H) positions in
+    // prefix order.
+    const uint32_t n = std::min(L, H);
+    if (L <= 32 && H <= 10000) {
+        const uint32_t c0 = std::min(4u, (H + 31) / 32);
+        for (uint32_t k = 0; k < n; ++k) perm[k] = k < c0 ? 32 * k : (SI_PERM_FIXED | k);
+    } else {
+        for (uint32_t k = 0; k < n; ++k) perm[k] = k;
+    }
+    return SI_OK;
+}
+
 int si_state_table(const uint32_t* prefix, const uint64_t* cols, const uint64_t* miss, uint32_t H,
                    uint32_t M, uint32_t L, uint32_t mode, const uint32_t* perm_host,
                    uint32_t* states, uint8_t* sa, uint8_t* mmask, void* stream) {
@@ -123,6 +144,11 @@ int si_state_table(const uint32_t* prefix, const uint64_t* cols, const uint64_t*
     if (mode == 0 && (!prefix || !perm_host))
         return fail(SI_ERR_ARG, "si_state_table: PBWT selection needs prefix and perm");
     if (!cols || !sa || !mmask) return fail(SI_ERR_ARG, "si_state_table: cols, sa, mmask required");
+    if (mode == 0)
+        for (uint32_t k = 0; k < std::min(L, H); ++k)
+            if ((perm_host[k] & ~SI_PERM_FIXED) >= H)
+                return fail(SI_ERR_ARG, "si_state_table: perm[%u] = 0x%x is outside the panel (H=%u)",
+                            k, perm_host[k], H);
     if (M == 0) return SI_OK;
     const uint32_t Lp = si_padded_states(L);
     return device_status(si::launch_state_table(prefix, cols, miss, H, M, L, Lp, mode, perm_host,

@@ -3,7 +3,8 @@
 // State table (reference: PBWTIndex::select_states with all keys equal,
 // pbwt_index.cpp:20-44, pairing imputer.cpp:913-920, allele reads
 // emission.cu:96-103): states[m][k] = prefix[m][perm[k]] for k < H, k % H
-// beyond, partner (h + 1) % H.  Sample-independent, so it is built once.
+// beyond, partner (h + 1) % H; a perm entry flagged SI_PERM_FIXED names the
+// haplotype itself.  Sample-independent, so it is built once.
 #include "si_device.hpp"
 #include "si_internal.hpp"
 
@@ -38,7 +39,8 @@ __global__ __launch_bounds__(256) void state_table_kernel(
             h0 = 2 * k;
             h1 = 2 * k + 1;
         } else {
-            h0 = (k < H) ? prefix[size_t(m) * H + perm.v[k]] : (k % H);
+            const uint32_t e = (k < H) ? perm.v[k] : (SI_PERM_FIXED | (k % H));
+            h0 = (e & SI_PERM_FIXED) ? (e & ~SI_PERM_FIXED) : prefix[size_t(m) * H + e];
             h1 = (h0 + 1 == H) ? 0u : h0 + 1;
         }
         if (states) states[size_t(m) * L + k] = h0;

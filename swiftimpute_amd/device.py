@@ -33,9 +33,13 @@ def padded_states(L: int) -> int:
     return int(hip().si_padded_states(L))
 
 
-def select_perm(H: int, L: int) -> np.ndarray:
+def select_perm(H: int, L: int, gpu_compat: bool = False) -> np.ndarray:
+    """Selection permutation of the CPU path (std::sort over equal keys), or
+    with ``gpu_compat`` the one the reference's GPU selection kernels make
+    (entries flagged SI_PERM_FIXED name a haplotype directly)."""
     out = np.empty(min(L, H), np.uint32)
-    check(hip().si_select_perm(H, L, out.ctypes.data_as(_native._u32p)), "si_select_perm")
+    name = "si_select_perm_gpu_compat" if gpu_compat else "si_select_perm"
+    check(getattr(hip(), name)(H, L, out.ctypes.data_as(_native._u32p)), name)
     return out
 
 
@@ -139,7 +143,8 @@ class DeviceIndex:
     """Resident panel, PBWT index and state table (Imputer::build_index)."""
 
     def __init__(self, cols: np.ndarray, H: int, L: int, mode: int = 0, miss: np.ndarray | None = None,
-                 keep_prefix: bool = True, with_div: bool = True, stream=None, device=DEV):
+                 keep_prefix: bool = True, with_div: bool = True, stream=None, device=DEV,
+                 gpu_compat: bool = False):
         M = cols.shape[0]
         self.H, self.M, self.L, self.Lp = H, M, L, padded_states(L)
         self.cols = torch.from_numpy(np.ascontiguousarray(cols).view(np.int64)).to(device)
@@ -152,7 +157,7 @@ class DeviceIndex:
         if mode == 0:
             check(lib.si_pbwt_sweep(_ptr(self.cols), H, M, _ptr(self.prefix), _ptr(self.div), None, 0,
                                     sh), "si_pbwt_sweep")
-        self.perm = select_perm(H, L)
+        self.perm = select_perm(H, L, gpu_compat)
         self.states = torch.empty((M, L), dtype=torch.int32, device=device)
         self.sa = torch.empty((M, self.Lp), dtype=torch.uint8, device=device)
         self.mmask = torch.empty((M,), dtype=torch.uint8, device=device)
@@ -163,7 +168,7 @@ class DeviceIndex:
             self.prefix = None
             self.div = None
 
-    def derive(self, L: int, mode: int = 0, stream=None) -> "DeviceIndex":
+    def derive(self, L: int, mode: int = 0, stream=None, gpu_compat: bool = False) -> "DeviceIndex":
         """Another state table (L states) over this index's PBWT arrays (which
         must have been kept), sharing the resident panel and prefix rows."""
         if self.prefix is None and mode == 0:
@@ -172,7 +177,7 @@ class DeviceIndex:
         d.H, d.M, d.L, d.Lp = self.H, self.M, L, padded_states(L)
         d.cols, d.miss, d.prefix, d.div = self.cols, self.miss, self.prefix, self.div
         dev = self.cols.device
-        d.perm = select_perm(self.H, L)
+        d.perm = select_perm(self.H, L, gpu_compat)
         d.states = torch.empty((self.M, L), dtype=torch.int32, device=dev)
         d.sa = torch.empty((self.M, d.Lp), dtype=torch.uint8, device=dev)
         d.mmask = torch.empty((self.M,), dtype=torch.uint8, device=dev)

@@ -176,6 +176,7 @@ static int run(int argc, char** argv) {
     const uint32_t batch = std::stoul(argv[8]);
     const uint64_t seed = std::stoull(argv[9]);
     const bool use_pbwt = argc > 10 ? std::stoi(argv[10]) != 0 : true;
+    const bool gpu_compat = argc > 11 ? std::stoi(argv[11]) != 0 : false;
     uint32_t H, M, B;
     {
         std::ifstream meta(in + "/meta.txt");
@@ -200,6 +201,7 @@ static int run(int argc, char** argv) {
     cfg.hmm_params.num_states = L;
     cfg.hmm_params.ne = ne;
     cfg.hmm_params.use_pbwt_selection = use_pbwt;
+    cfg.selection = gpu_compat ? SelectionMode::GpuCompat : SelectionMode::CpuSort;
     cfg.transition_mode = per_marker ? TransitionMode::PerMarker : TransitionMode::FirstInterval;
     cfg.sampling = argmax ? SamplingMode::Argmax : SamplingMode::Philox;
     cfg.batch_size = batch;

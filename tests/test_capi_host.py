@@ -80,6 +80,30 @@ def test_select_perm_matches_oracle(H, L):
     assert np.array_equal(device.select_perm(H, L), oracle.select_perm(H, L))
 
 
+def _gather(prefix_row, perm, L):
+    H = prefix_row.size
+    out = []
+    for k in range(L):
+        e = int(perm[k]) if k < H else (_native.SI_PERM_FIXED | (k % H))
+        out.append(e & 0x7FFFFFFF if e & _native.SI_PERM_FIXED else int(prefix_row[e]))
+    return out
+
+
+@pytest.mark.parametrize("H,L", [(128, 1), (128, 8), (200, 32), (4432, 8), (10000, 16), (10001, 8),
+                                 (20000, 16), (300, 33), (300, 64), (40, 8), (70, 12), (5, 8),
+                                 (1, 3)])
+def test_gpu_compat_perm_matches_the_reference_kernels(H, L):
+    # the permutation gathered from each PBWT row equals the oracle's lane-by-lane
+    # run of the reference's GPU selection kernels on that row and its divergences
+    rng = np.random.default_rng(H * 7 + L)
+    ref = (rng.random((4, H)) < 0.45).astype(np.uint8)
+    prefix, div = oracle.pbwt_build(ref)
+    perm = device.select_perm(H, L, gpu_compat=True)
+    assert perm.size == min(L, H)
+    for m in range(ref.shape[0]):
+        assert _gather(prefix[m], perm, L) == oracle.select_states_gpu(prefix[m], div[m], L).tolist()
+
+
 @pytest.mark.parametrize("spacing_mode", [0, 1])
 @pytest.mark.parametrize("ne", [1000, 10000, 20000, 100000])
 def test_transition_constants_match_oracle(spacing_mode, ne):
@@ -174,6 +198,14 @@ def test_state_table_argument_errors():
     assert lib.si_state_table(None, None, None, 0, 10, 8, 0, p32, None, None, None, None) == -1
     assert lib.si_state_table(None, None, None, 10, 10, 8, 1, p32, None, None, None, None) == -1
     assert lib.si_state_table(None, None, None, 10, 10, 8, 0, None, None, None, None, None) == -1
+    fake = ctypes.c_void_p(256)  # never dereferenced: the perm check fails first
+    for bad in (10, _native.SI_PERM_FIXED | 10, 0xFFFFFFFF):
+        perm[3] = bad
+        assert lib.si_state_table(fake, fake, None, 10, 10, 8, 0, p32, None, fake, fake, None) == -1
+        assert b"outside the panel" in lib.si_last_error()
+    q = np.zeros(4, np.uint32)
+    assert lib.si_select_perm_gpu_compat(0, 8, q.ctypes.data_as(_native._u32p)) == -1
+    assert lib.si_select_perm_gpu_compat(10, 65, q.ctypes.data_as(_native._u32p)) == -1
 
 
 def test_variant_queries():

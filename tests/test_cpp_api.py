@@ -88,6 +88,29 @@ def test_cpp_api_against_oracle(tmp_path, per_marker, spacing, L, batch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("H,L", [(600, 8), (12000, 16)])
+def test_cpp_api_gpu_compat_selection(tmp_path, H, L):
+    # SelectionMode::GpuCompat: the states the reference's GPU kernels select,
+    # and the posteriors of the model over those states
+    pan = synth.make_panel(H, 300, seed=51)
+    tg = synth.make_targets(pan, 6, 4, seed=52)
+    write_inputs(tmp_path / "in", pan, tg, synth.HARDCALL_PALETTE)
+    os.makedirs(tmp_path / "out", exist_ok=True)
+    r = run_exe("run", tmp_path / "in", tmp_path / "out", L, 10000, 1, 0, 100, 5, 1, 1)
+    assert r.returncode == 0, r.stdout
+    o = read_outputs(tmp_path / "out", pan, tg.B, L)
+    ref = pan.alleles()
+    p, d = oracle.pbwt_build(ref)
+    h0, _, a0, a1 = oracle.state_table_gpu(ref, p, d, L)
+    assert np.array_equal(o["states"], h0)
+    assert not np.array_equal(h0, oracle.state_table(ref, p, L)[0])
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 10000, per_marker=True)
+    gl = synth.HARDCALL_PALETTE[tg.codes[:, : pan.M]]
+    fb = oracle.fb_f64(a0, a1, gl, tr.Td, tr.To, per_marker=True)
+    assert np.abs(o["ap"] - fb["ap"]).max() <= 1e-5
+
+
+@pytest.mark.gpu
 def test_cpp_api_simple_selection(tmp_path):
     pan = synth.make_panel(200, 300, seed=41)
     tg = synth.make_targets(pan, 5, 4, seed=42)

@@ -32,9 +32,9 @@ def pack_cols(ref):
     return pack(ref != 0), (pack(miss) if miss.any() else None)
 
 
-def gpu_index(ref, L, mode=0):
+def gpu_index(ref, L, mode=0, gpu_compat=False):
     cols, miss = pack_cols(ref)
-    idx = device.DeviceIndex(cols, ref.shape[1], L, mode=mode, miss=miss)
+    idx = device.DeviceIndex(cols, ref.shape[1], L, mode=mode, miss=miss, gpu_compat=gpu_compat)
     torch.cuda.synchronize()
     return idx
 
@@ -151,6 +151,23 @@ def test_state_table(H, L):
     k = np.arange(idx.Lp)
     want = np.where((k % 4)[None, :] < 3, (mm[:, None] >> (k % 4)[None, :]) & 1, 0)
     assert np.array_equal(sa >> 7, want)
+
+
+@pytest.mark.parametrize("H,L", [(4432, 8), (200, 32), (128, 3), (10001, 8), (300, 40), (300, 64),
+                                 (70, 12), (5, 8)])
+def test_state_table_gpu_compat(H, L):
+    # the reference's GPU selection kernels, run lane by lane in the oracle on
+    # the PBWT rows and divergences
+    rng = np.random.default_rng(H * 3 + L)
+    M = 40
+    ref = (rng.random((M, H)) < 0.5).astype(np.uint8)
+    ref[rng.random((M, H)) < 0.02] = 255
+    idx = gpu_index(ref, L, gpu_compat=True)
+    p, d = oracle.pbwt_build(ref)
+    h0, _, a0, a1 = oracle.state_table_gpu(ref, p, d, L)
+    assert np.array_equal(idx.states.cpu().numpy().view(np.uint32), h0)
+    g0, g1 = idx.state_alleles()
+    assert np.array_equal(g0, a0) and np.array_equal(g1, a1)
 
 
 def test_state_table_simple_mode():

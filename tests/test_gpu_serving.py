@@ -131,3 +131,20 @@ def test_request_above_budget_is_refused(world):
                         hbm_budget_gb=1e-4) as srv:
         with pytest.raises(_native.NativeError, match="budget"):
             srv.submit(0, pan.M, np.ascontiguousarray(tg.codes[:, : pan.M]))
+
+
+def test_gpu_compat_selection_equals_plain(world):
+    # a server built with the reference's GPU-kernel selection serves the same
+    # results as the plain path over a state table built that way
+    pan, tg = world
+    reqs = _requests(pan, tg, 5, seed=12)
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE, max_streams=2,
+                        max_batch=4, gpu_compat_selection=True) as srv:
+        ids = [srv.submit(*r) for r in reqs]
+        got = [srv.wait(i) for i in ids]
+    idx = device.DeviceIndex(pan.cols, pan.H, L, keep_prefix=False, gpu_compat=True)
+    tables = device.SharedTables(synth.HARDCALL_PALETTE)
+    for r, a in zip(reqs, got):
+        hap, ds = _plain(pan, idx, tables, *r)
+        assert np.array_equal(a.hap, hap), r[:2]
+        assert np.array_equal(a.ds, ds), r[:2]

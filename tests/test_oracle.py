@@ -114,6 +114,73 @@ def test_state_table_simple_mode():
     assert h0[0].tolist() == [0, 2, 4] and h1[0].tolist() == [1, 3, 5]
 
 
+# ------------------------------------------- the reference's GPU selection
+def _pbwt_rows(H, M=5, seed=0):
+    rng = np.random.default_rng(seed + H)
+    ref = (rng.random((M, H)) < 0.4).astype(np.uint8)
+    return ref, *oracle.pbwt_build(ref)
+
+
+@pytest.mark.parametrize("H,L", [(128, 1), (128, 3), (200, 8), (4432, 8), (10000, 32), (640, 4)])
+def test_gpu_warp_selection_closed_form(H, L):
+    # every divergence of row m is m, so [p0, p32, p64, p96, 4, 5, ..., L-1]
+    _, prefix, div = _pbwt_rows(H)
+    for m in range(prefix.shape[0]):
+        p = prefix[m]
+        want = ([int(p[0]), int(p[32]), int(p[64]), int(p[96])] + list(range(4, L)))[:L]
+        assert oracle.select_states_gpu(p, div[m], L, "warp").tolist() == want
+        assert oracle.select_states_gpu(p, div[m], L).tolist() == want
+
+
+@pytest.mark.parametrize("H,L", [(10001, 8), (20000, 16), (300, 33), (300, 64), (20, 40)])
+def test_gpu_insert_selection_closed_form(H, L):
+    # ties never displace: the first min(L, H) prefix positions, then i % H
+    _, prefix, div = _pbwt_rows(H)
+    for m in range(prefix.shape[0]):
+        p = prefix[m]
+        want = [int(p[i]) if i < H else i % H for i in range(L)]
+        assert oracle.select_states_gpu(p, div[m], L, "insert").tolist() == want
+        assert oracle.select_states_gpu(p, div[m], L).tolist() == want
+
+
+def test_gpu_warp_selection_hand_traced_rounds():
+    p = np.arange(1000, 1160, dtype=np.uint32)  # prefix row: position i holds haplotype 1000+i
+    # a larger divergence at lane 5's first entry wins round 0 only
+    d = np.ones(128, np.uint32)
+    d[5] = 100
+    assert oracle.select_states_gpu(p[:128], d, 6, "warp").tolist() == [1005, 1032, 1064, 1096, 4, 5]
+    # lane 0's third entry moves to the front of its list
+    d = np.ones(128, np.uint32)
+    d[64] = 50
+    assert oracle.select_states_gpu(p[:128], d, 5, "warp").tolist() == [1064, 1000, 1032, 1096, 4]
+    # a fifth entry with a larger divergence pushes lane 0's fourth out
+    d = np.ones(160, np.uint32)
+    d[128] = 9
+    assert oracle.select_states_gpu(p, d, 4, "warp").tolist() == [1128, 1000, 1032, 1064]
+
+
+def test_gpu_insert_selection_is_a_stable_sort_by_divergence():
+    rng = np.random.default_rng(5)
+    for H, L in [(50, 8), (300, 64), (1000, 17), (12, 20)]:
+        p = rng.permutation(H).astype(np.uint32)
+        d = rng.integers(0, 6, H).astype(np.uint32)
+        order = sorted(range(H), key=lambda i: -int(d[i]))  # Python's sort is stable
+        want = [int(p[order[i]]) if i < H else i % H for i in range(L)]
+        assert oracle.select_states_gpu(p, d, L, "insert").tolist() == want
+
+
+@pytest.mark.parametrize("H,L", [(200, 8), (12000, 8), (300, 40)])
+def test_state_table_gpu_pairs_and_alleles(H, L):
+    ref, prefix, div = _pbwt_rows(H)
+    h0, h1, a0, a1 = oracle.state_table_gpu(ref, prefix, div, L)
+    for m in range(ref.shape[0]):
+        sel = oracle.select_states_gpu(prefix[m], div[m], L)
+        assert h0[m].tolist() == sel.tolist()
+        assert h1[m].tolist() == [(int(h) + 1) % H for h in sel]
+        assert a0[m].tolist() == [ref[m, h] for h in sel]
+        assert a1[m].tolist() == [ref[m, (int(h) + 1) % H] for h in sel]
+
+
 # ---------------------------------------------------------------- transitions
 def test_transition_limits():
     # 50 kb at 1 cM/Mb, Ne = 1e4, rho_rate 4: exponent -80, rho == 1.0 in double,
