@@ -57,7 +57,8 @@ struct ImputationConfig {
     bool output_info_score;      // compute INFO scores
     bool deterministic;          // every draw is a function of (seed, sample, marker)
 
-    uint32_t window_size;        // accepted; the device kernel needs no windows
+    uint32_t window_size;        // M > window_size > 0: the reference's windowed model
+                                 // (imputer.cpp:965-971, 1044-1174); 0: the exact one
     uint32_t window_overlap;
     uint32_t checkpoint_interval;
 

@@ -196,6 +196,16 @@ typedef struct si_fb_job {
     uint8_t* hap;         /* [B][2][M] device or NULL */
     float* prob;          /* [B][M][L] device or NULL */
     double* loglik;       /* [B] device or NULL */
+    uint32_t window;      /* 0, or the reference's windowed model when M > window
+                             (>= 32): run_windowed_forward_backward,
+                             src/api/imputer.cpp:1044-1174.  Window w > 0 starts
+                             its forward from the previous window's last vector
+                             (no transition or emission into marker w*window),
+                             its posteriors use a forward re-run from the
+                             window's own last vector, and the backward step out
+                             of each window start is not taken
+                             (forward_backward.cu:406-626).  loglik is that of
+                             the first forward run. */
 } si_fb_job;
 
 /* Workspace (device bytes) si_fb_run needs for these jobs (enough for every

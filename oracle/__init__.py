@@ -60,6 +60,9 @@ def lib():
         L.or_fb_f64.argtypes = [_u8p, _u8p, _f32p, _f32p, _f32p, ctypes.c_uint32,
                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                 _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.or_fb_f64_windowed.argtypes = [_u8p, _u8p, _f32p, _f32p, _f32p, ctypes.c_uint32,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                         ctypes.c_uint32, _f64p, _f64p, _f64p, _f64p]
         L.or_fb_f64_fast.argtypes = [_u8p, _u8p, _f32p, _f32p, _f32p, ctypes.c_uint32,
                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
                                      _f64p, _f64p, _f64p, _f64p, ctypes.c_int]
@@ -222,6 +225,29 @@ def fb_f64(a0, a1, gl, Td, To, per_marker=False, want_logpost=False):
     if want_logpost:
         out["logpost"] = lp
     return out
+
+
+def fb_f64_windowed(a0, a1, gl, Td, To, window, per_marker=False):
+    """The reference's windowed forward-backward (imputer.cpp:1044-1174,
+    forward_backward.cu:406-626) in FP64, with the sampler normalisation and
+    dosage.  window = 0 or M <= window is fb_f64.  Returns dict(prob, ds, ap,
+    loglik)."""
+    a0 = np.ascontiguousarray(a0, np.uint8)
+    a1 = np.ascontiguousarray(a1, np.uint8)
+    gl = np.ascontiguousarray(gl, np.float32)
+    B, M = gl.shape[0], gl.shape[1]
+    L = a0.shape[1]
+    Td = np.ascontiguousarray(Td if len(Td) else np.zeros(1), np.float32)
+    To = np.ascontiguousarray(To if len(To) else np.zeros(1), np.float32)
+    prob = np.empty((B, M, L), np.float64)
+    ds = np.empty((B, M), np.float64)
+    ap = np.empty((B, M, 2), np.float64)
+    ll = np.empty(B, np.float64)
+    _chk(lib().or_fb_f64_windowed(_p(a0, _u8p), _p(a1, _u8p), _p(gl, _f32p), _p(Td, _f32p),
+                                  _p(To, _f32p), B, M, L, int(per_marker), int(window),
+                                  _p(prob, _f64p), _p(ds, _f64p), _p(ap, _f64p), _p(ll, _f64p)),
+         "or_fb_f64_windowed")
+    return dict(prob=prob, ds=ds, ap=ap, loglik=ll)
 
 
 def fb_f64_fast(a0, a1, gl, Td, To, per_marker=False, want_prob=True, nthreads=None):

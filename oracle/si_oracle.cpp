@@ -24,6 +24,10 @@
 //                          with the -999 floor folded in, uniform-T closed form,
 //                          the reference's own FB integration case.
 //   or_fb_f32_faithful     pinned against or_fb_f64 (drift bound only).
+//   or_fb_f64_windowed     pinned: equals or_fb_f64 when M <= W; with windows,
+//                          brute-force path enumeration of the model it defines
+//                          (tiny M), and window-local closed forms (a window
+//                          whose forward restarts at a fixed vector).
 //   or_philox4x32_10       pinned: Random123 published KAT vectors and rocRAND's
 //                          open-source header.
 //   or_sample_from_probs   pinned: closed form on hand-built vectors.
@@ -446,6 +450,121 @@ int or_fb_f64(const uint8_t* a0, const uint8_t* a1, const float* gl,
             double d, x0, x1;
             dosage_from_probs(p.data(), a0 + size_t(m) * L, a1 + size_t(m) * L, L, &d, &x0, &x1);
             size_t so = size_t(s) * M + m;
+            if (ds) ds[so] = d;
+            if (ap) { ap[2 * so] = x0; ap[2 * so + 1] = x1; }
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Windowed forward-backward, FP64 fold form: the reference's default path when
+// window_size > 0 and M > window_size (imputer.cpp:965-971), restated window
+// by window as run_windowed_forward_backward drives it (imputer.cpp:1044-1174)
+// with forward_pass_windowed_kernel (forward_backward.cu:406-491) and
+// backward_pass_windowed_kernel (:497-626):
+//   forward, windows in order: window w starts from the previous window's
+//     final alpha (w > 0; :432-438, so marker w*W gets neither a transition
+//     nor its emission) or from E at marker 0, rescales by the max, steps to
+//     its end; each window's final alpha is kept (imputer.cpp:1101-1106);
+//   backward, windows in reverse: the window's forward is re-run from that
+//     window's OWN final alpha (w > 0; imputer.cpp:1120-1142), beta starts at
+//     the next window's final beta or 0 (:530-533, so the step from marker
+//     (w+1)*W is not taken), walks down the window forming alpha + beta, and
+//     the window's posteriors are normalized by their log-sum-exp (:602-625).
+// The sampler normalisation and dosage follow as in or_fb_f64.  loglik (not
+// produced by the reference) is the sum of the first forward run's rescale
+// factors plus the log mass of its final alpha.  W = 0 or M <= W is the
+// non-windowed or_fb_f64.
+// ---------------------------------------------------------------------------
+int or_fb_f64_windowed(const uint8_t* a0, const uint8_t* a1, const float* gl,
+                       const float* Td, const float* To, uint32_t B, uint32_t M, uint32_t L,
+                       int per_marker, uint32_t W, double* prob, double* ds, double* ap,
+                       double* loglik) {
+    if (W == 0 || M <= W)
+        return or_fb_f64(a0, a1, gl, Td, To, B, M, L, per_marker, prob, nullptr, ds, ap, loglik);
+    if (L == 0 || !Td || !To) return -1;
+    const uint32_t nw = (M + W - 1) / W;
+    std::vector<double> E(size_t(M) * L), win(size_t(W) * L), fa(size_t(nw) * L);
+    std::vector<double> post(size_t(M) * L), bc(L), bp(L), x(L), p(L);
+    // one window's forward (:406-491) into win[0..n-1]; returns the rescale sum
+    auto fwd_window = [&](uint32_t start, uint32_t n, const double* init) {
+        double ll = 0.0;
+        for (uint32_t k = 0; k < L; ++k) win[k] = init ? init[k] : E[size_t(start) * L + k];
+        double mx = win[0];
+        for (uint32_t k = 1; k < L; ++k) mx = std::fmax(mx, win[k]);
+        for (uint32_t k = 0; k < L; ++k) win[k] -= mx;
+        ll += mx;
+        for (uint32_t w = 1; w < n; ++w) {
+            const uint32_t m = start + w, t = interval_for_step(m, per_marker);
+            const double td = Td[t], to = To[t];
+            const double* prev = &win[size_t(w - 1) * L];
+            double* cur = &win[size_t(w) * L];
+            for (uint32_t j = 0; j < L; ++j) {
+                double S = kLogZero;
+                for (uint32_t i = 0; i < L; ++i) S = lse2(S, prev[i] + (i == j ? td : to));
+                cur[j] = S + E[size_t(m) * L + j];
+            }
+            mx = cur[0];
+            for (uint32_t j = 1; j < L; ++j) mx = std::fmax(mx, cur[j]);
+            for (uint32_t j = 0; j < L; ++j) cur[j] -= mx;
+            ll += mx;
+        }
+        return ll;
+    };
+    for (uint32_t s = 0; s < B; ++s) {
+        const float* gls = gl + size_t(s) * M * 3;
+        for (uint32_t m = 0; m < M; ++m)
+            for (uint32_t k = 0; k < L; ++k)
+                E[size_t(m) * L + k] = emission_lookup(gls + size_t(m) * 3, a0[size_t(m) * L + k],
+                                                       a1[size_t(m) * L + k]);
+        double ll = 0.0;
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint32_t start = w * W, n = std::min(W, M - start);
+            ll += fwd_window(start, n, w > 0 ? &fa[size_t(w - 1) * L] : nullptr);
+            std::copy(&win[size_t(n - 1) * L], &win[size_t(n) * L], &fa[size_t(w) * L]);
+        }
+        {
+            double S = 0.0;
+            for (uint32_t k = 0; k < L; ++k) S += std::exp(fa[size_t(nw - 1) * L + k]);
+            ll += std::log(S);
+        }
+        if (loglik) loglik[s] = ll;
+        for (int w = int(nw) - 1; w >= 0; --w) {
+            const uint32_t start = uint32_t(w) * W, n = std::min(W, M - start);
+            fwd_window(start, n, w > 0 ? &fa[size_t(w) * L] : nullptr);
+            if (w == int(nw) - 1) std::fill(bc.begin(), bc.end(), 0.0);  // else: the next window's
+            for (uint32_t k = 0; k < L; ++k)
+                post[size_t(start + n - 1) * L + k] = win[size_t(n - 1) * L + k] + bc[k];
+            for (int ww = int(n) - 2; ww >= 0; --ww) {
+                const uint32_t m = start + uint32_t(ww), t = interval_for_step(m + 1, per_marker);
+                const double td = Td[t], to = To[t];
+                bp = bc;
+                const double* En = &E[size_t(m + 1) * L];
+                for (uint32_t i = 0; i < L; ++i) {
+                    double S = kLogZero;
+                    for (uint32_t j = 0; j < L; ++j) S = lse2(S, (i == j ? td : to) + En[j] + bp[j]);
+                    bc[i] = S;
+                }
+                for (uint32_t k = 0; k < L; ++k)
+                    post[size_t(m) * L + k] = win[size_t(ww) * L + k] + bc[k];
+            }
+            // bc now holds beta at the window's first marker: the next (lower)
+            // window starts from it
+        }
+        for (uint32_t m = 0; m < M; ++m) {
+            double Z = kLogZero;
+            for (uint32_t k = 0; k < L; ++k) Z = lse2(Z, post[size_t(m) * L + k]);
+            double mx = kLogZero;
+            for (uint32_t k = 0; k < L; ++k) { x[k] = post[size_t(m) * L + k] - Z; mx = std::fmax(mx, x[k]); }
+            double sum = 0.0;
+            for (uint32_t k = 0; k < L; ++k) { p[k] = std::exp(x[k] - mx); sum += p[k]; }
+            for (uint32_t k = 0; k < L; ++k) p[k] /= sum;
+            const size_t o = (size_t(s) * M + m) * L;
+            if (prob) for (uint32_t k = 0; k < L; ++k) prob[o + k] = p[k];
+            double d, x0, x1;
+            dosage_from_probs(p.data(), a0 + size_t(m) * L, a1 + size_t(m) * L, L, &d, &x0, &x1);
+            const size_t so = size_t(s) * M + m;
             if (ds) ds[so] = d;
             if (ap) { ap[2 * so] = x0; ap[2 * so + 1] = x1; }
         }

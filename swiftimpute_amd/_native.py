@@ -60,7 +60,7 @@ class FbJob(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("step", StepCoef),
         ("steps", _vp), ("sa", _vp), ("mmask", _vp), ("codes", _vp), ("eps", _vp),
         ("emax", _vp), ("palette", _vp), ("ds", _vp), ("ap", _vp), ("hap", _vp), ("prob", _vp),
-        ("loglik", _vp),
+        ("loglik", _vp), ("window", ctypes.c_uint32),
     ]
 
 

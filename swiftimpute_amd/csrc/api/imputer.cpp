@@ -292,6 +292,7 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     job.hap = dev_->hap;
     job.loglik = dev_->loglik;
     job.n_samples = batch;
+    job.window = config_.window_size;
     const size_t need = si_fb_workspace_bytes(&job, 1);
     if (need > dev_->ws_bytes) {
         if (dev_->ws) (void)hipFree(dev_->ws);
@@ -300,6 +301,9 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
         dev_->ws_bytes = need;
         dev_->bytes += need;
     }
+    if (job.window && M > job.window)
+        Logger::instance().info("windowed forward-backward (" + std::to_string(M) + " markers, window=" +
+                                std::to_string(job.window) + ")");
     const uint8_t* codes = targets.codes().data();
     for (uint32_t start = 0; start < B; start += batch) {
         const uint32_t end = std::min(start + batch, B), n = end - start;
@@ -341,6 +345,7 @@ size_t Imputer::estimate_memory_requirement(const ReferencePanel& reference, uin
     j.n_samples = batch;
     j.n_markers = uint32_t(M);
     j.n_states = L;
+    j.window = config.window_size;
     total += si_fb_workspace_bytes(&j, 1);
     total += size_t(batch) * ((M + 15) / 16 * 16 + M * 2 * 4 + 2 * M + 8);
     return total;

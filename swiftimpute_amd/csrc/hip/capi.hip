@@ -244,6 +244,9 @@ static int check_job(const si_fb_job& j, uint32_t idx) {
                     "(use SI_FB_LOGSPACE)", idx, double(j.step.t_o));
     if (!j.ds && !j.ap && !j.hap && !j.prob && !j.loglik)
         return fail(SI_ERR_ARG, "job %u: no output requested", idx);
+    if (si::fb_window(j) && j.window < si::kMinWindow)
+        return fail(SI_ERR_UNSUPPORTED, "job %u: window %u shorter than %u markers", idx, j.window,
+                    si::kMinWindow);
     return SI_OK;
 }
 
@@ -251,7 +254,8 @@ size_t si_fb_workspace_bytes(const si_fb_job* jobs, uint32_t n_jobs) {
     size_t mx = 0;
     for (uint32_t i = 0; jobs && i < n_jobs; ++i)
         mx = std::max(mx, si::fb_workspace_bytes_one(jobs[i].n_samples, jobs[i].n_markers,
-                                                     si_padded_states(jobs[i].n_states)));
+                                                     si_padded_states(jobs[i].n_states),
+                                                     si::fb_window(jobs[i])));
     return mx;
 }
 
@@ -611,7 +615,8 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
     for (uint32_t i = 0; i < n_jobs; ++i) {
         b->ws_off[i] = b->ws_bytes;
         const size_t w = si::fb_workspace_bytes_one(jobs[i].n_samples, jobs[i].n_markers,
-                                                    si_padded_states(jobs[i].n_states));
+                                                    si_padded_states(jobs[i].n_states),
+                                                    si::fb_window(jobs[i]));
         b->ws_bytes += (w + 255) & ~size_t(255);
     }
     // classes by (padded states, numeric space); jobs longest chain first

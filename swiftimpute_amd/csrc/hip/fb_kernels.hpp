@@ -482,6 +482,14 @@ __device__ __forceinline__ void wave_sync() {
 // every step is computed and committed by a select where it lies inside the
 // chain, so a load block's 16 steps are straight-line code.  J.nblk counts
 // checkpoint blocks of CB markers.
+//
+// Windows (J.W > 0, each window [lo, hi]): the forward chain of window w > 0
+// first runs from the carried vector (the previous window's last one; the
+// step into lo is not taken) to hi, adding to the log-likelihood, which gives
+// the window's last vector F_w; it then runs the window again from F_w,
+// storing the checkpoints, and carries F_w into window w + 1.  Window 0 runs
+// once.  The backward chain does not take the step out of a window start.
+// Without windows the whole job is window 0.
 // ---------------------------------------------------------------------------
 template <class S, bool LOG, bool PM, int CB>
 __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint32_t P, uint32_t s,
@@ -494,6 +502,8 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
     BlockIn<S, LOG, PM> cur;
     BlockEm<S, LOG, PM> em;
     if (fwd) {
+        const uint32_t W = J.W ? J.W : J.M;
+        const uint32_t nw = J.W ? J.nwin : 1u;
         cur.load(J, codes, r, 0);
         float a[S::R];
         float ls = 0.0f;
@@ -503,16 +513,30 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
             const float off = lane_emission<S>(tab, P, cur.code(0), cur.sa[0], sub, e);
             ll = fwd_init<S, LOG>(a, ls, e, off);
         }
-        for (uint32_t c = 0; c < nlb; ++c) {
+        // One flat walk over the load blocks of every (window, pass), so the
+        // next block's inputs are always in flight: pass 0 (windows after the
+        // first) is the carried chain, pass 1 the checkpointed run from the
+        // window's last vector.
+        uint32_t w = 0, pass = 1, lo = 0, hi = min(W, J.M) - 1u, c = 0;
+        for (;;) {
             em.fill(tab, P, cur, sub);
-            if (c + 1 < nlb) cur.load(J, codes, r, (c + 1) * kSub);
+            const uint32_t c1 = hi / kSub;
+            // the load block after this one
+            uint32_t nc = c + 1u;
+            const bool pass_end = c == c1;
+            const bool last = pass_end && pass == 1u && w + 1u == nw;
+            if (pass_end) nc = pass == 0u ? lo / kSub : (hi + 1u) / kSub;
+            if (!last) cur.load(J, codes, r, nc * kSub);
             const uint32_t m0 = c * kSub;
-            const bool ck = (c % NS) == 0;
+            const bool keep = pass == 1u;
+            const bool sum_ll = pass == 0u || w == 0u;
+            const bool ck = keep && (c % NS) == 0 && m0 >= lo;
             float* ckp = J.ck_alpha + (size_t(s) * J.nblk + m0 / CB) * S::Lp + vo;
-            if (m0 > 0 && m0 + kSub <= J.M) {  // every step inside the chain: no selects
+            if (lo < m0 && m0 + kSub - 1u <= hi) {  // every step inside: no selects
 #pragma unroll
                 for (uint32_t t = 0; t < kSub; ++t) {
-                    ll += fwd_step<S, LOG>(a, ls, em.e[t], em.off[t], em.coef(t));
+                    const double inc = fwd_step<S, LOG>(a, ls, em.e[t], em.off[t], em.coef(t));
+                    if (sum_ll) ll += inc;
                     if (t == 0 && ck) store_vec<S::R>(ckp, a);
                 }
             } else {
@@ -524,13 +548,31 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
                     for (int i = 0; i < S::R; ++i) an[i] = a[i];
                     float lsn = ls;
                     const double inc = fwd_step<S, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
-                    const bool live = m > 0 && m < J.M;
+                    const bool live = m > lo && m <= hi;
                     vsel(live, a, an);
                     ls = live ? lsn : ls;
-                    ll += live ? inc : 0.0;
+                    ll += (live && sum_ll) ? inc : 0.0;
                     if (t == 0 && ck) store_vec<S::R>(ckp, a);
                 }
             }
+            if (last) break;
+            if (pass_end) {
+                float* fwp = J.ck_win + (size_t(s) * J.nwin + w) * S::Lp + vo;
+                if (pass == 0u) {  // F_w: the restart of pass 1 and of the block pass
+                    store_vec<S::R>(fwp, a);
+                    pass = 1u;
+                } else {  // carry F_w into the next window
+                    if (w > 0u) {
+                        load_vec<S::R>(fwp, a);
+                        if (LOG) ls = log_mass<S>(a);
+                    }
+                    ++w;
+                    pass = 0u;
+                    lo = w * W;
+                    hi = min(lo + W, J.M) - 1u;
+                }
+            }
+            c = nc;
         }
         if (LOG) ll += double(ls);  // log mass of the last normalized vector
         if (r == 0 && J.loglik) J.loglik[s] = ll;
@@ -541,15 +583,19 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
         const size_t last = size_t(s) * J.nblk + (J.nblk - 1);
         store_vec<S::R>(J.ck_beta + last * S::Lp + vo, v);
         if (r == 0) J.ck_bl[last] = Bl;
+        // the highest window start at or below the current load block's end
+        // (0 without windows: the step out of marker 0 is never taken)
+        uint32_t ws = J.W ? (J.M - 1u) / J.W * J.W : 0u;
         cur.load(J, codes, r, (nlb - 1) * kSub);
         for (int c = int(nlb) - 1; c >= 0; --c) {
             em.fill(tab, P, cur, sub);
             if (c > 0) cur.load(J, codes, r, uint32_t(c - 1) * kSub);
             const uint32_t m0 = uint32_t(c) * kSub;
             const bool ck = c > 0 && (uint32_t(c) % NS) == 0;
+            const bool has_ws = ws >= m0;
             // v at marker m0 - 1 is the last of checkpoint block m0 / CB - 1
             const size_t b = size_t(s) * J.nblk + (m0 / CB) - 1u;
-            if (m0 > 0 && m0 + kSub <= J.M) {  // every step inside the chain: no selects
+            if (m0 > 0 && m0 + kSub <= J.M && !has_ws) {  // every step inside: no selects
 #pragma unroll
                 for (int t = int(kSub) - 1; t >= 0; --t) {
                     bwd_step<S, LOG>(v, Bl, em.e[t], em.off[t], em.coef(uint32_t(t)));
@@ -567,7 +613,7 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
                     for (int i = 0; i < S::R; ++i) vn[i] = v[i];
                     double Bln = Bl;
                     bwd_step<S, LOG>(vn, Bln, em.e[t], em.off[t], em.coef(uint32_t(t)));
-                    const bool live = n > 0 && n < J.M;
+                    const bool live = n > 0 && n < J.M && n != ws;
                     vsel(live, v, vn);
                     Bl = live ? Bln : Bl;
                     if (t == 0 && ck) {
@@ -576,6 +622,7 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
                     }
                 }
             }
+            if (has_ws && ws >= J.W) ws -= J.W;  // windows are longer than a load block
         }
     }
 }
@@ -589,6 +636,9 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
 // then sample: lane r takes markers r, r + G, ... of the sub-block and walks
 // the states in order with a sequential FP32 cumulative sum, which the host
 // replays exactly.  Every lane of the wavefront passes the wave_syncs.
+// With windows (at least CB markers long), a window start inside the block
+// restarts the forward vector from the window's last vector F_w (stored by
+// the chain pass) and the backward step out of it is not taken.
 // ---------------------------------------------------------------------------
 template <class S, bool LOG, bool PM, int CB>
 __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint32_t P, bool active,
@@ -606,6 +656,16 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
     const size_t ck = size_t(s) * J.nblk + c;
     const bool argmax = (J.flags & SI_FB_ARGMAX) != 0;
     const uint2 key = make_uint2(uint32_t(J.seed), uint32_t(J.seed >> 32));
+    // offset in the block of a window start after its first marker (or none)
+    uint32_t tws = 0xffffffffu;
+    const float* fw = nullptr;
+    if (J.W) {
+        const uint32_t wi = m0 / J.W + 1u;
+        if (wi * J.W < m0 + nm) {
+            tws = wi * J.W - m0;
+            fw = J.ck_win + (size_t(s) * J.nwin + wi) * S::Lp + size_t(r) * S::R;
+        }
+    }
     float v[S::R];
     double Bl = 0.0;
     if (active) {
@@ -637,6 +697,10 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                     const bool live = tt > 0 && tt < nm;
                     vsel(live, a, an);
                     ls = live ? lsn : ls;
+                    if (tt == tws) {
+                        load_vec<S::R>(fw, a);
+                        if (LOG) ls = log_mass<S>(a);
+                    }
                 }
             }
             em.fill(tab, P, in, sub);
@@ -667,6 +731,10 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                         const bool live = tt > 0 && tt < nm;
                         vsel(live, a, an);
                         ls = live ? lsn : ls;
+                        if (!FULL && tt == tws) {
+                            load_vec<S::R>(fw, a);
+                            if (LOG) ls = log_mass<S>(a);
+                        }
                     }
                     store_vec<S::R>(pb + t * T + r * S::R, a);
                 }
@@ -697,13 +765,13 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                         for (int i = 0; i < S::R; ++i) vn[i] = v[i];
                         double Bln = Bl;
                         bwd_step<S, LOG>(vn, Bln, em.e[t], em.off[t], em.coef(uint32_t(t)));
-                        const bool live = tt < nm;
+                        const bool live = tt < nm && tt != tws;
                         vsel(live, v, vn);
                         Bl = live ? Bln : Bl;
                     }
                 }
             };
-            if (nm == uint32_t(CB)) walk(std::true_type{});
+            if (nm == uint32_t(CB) && tws == 0xffffffffu) walk(std::true_type{});
             else walk(std::false_type{});
             for (uint32_t q = r; q < kSub / 4; q += S::G)
                 rng[q] = philox4x32_10(

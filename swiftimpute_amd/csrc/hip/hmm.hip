@@ -61,20 +61,29 @@ bool fb_variant_valid(uint32_t Lp, const FbVariant& v) {
 }
 
 // ------------------------------------------------------------ workspace --
-size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp) {
+namespace {
+size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
+}  // namespace
+
+// Checkpoint vectors (forward, backward), backward scales and, with windows,
+// one forward vector per window, each region 256-byte aligned.  Sized at the
+// smallest checkpoint block, so one workspace serves every variant.
+size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp, uint32_t W) {
     const size_t nblk = (size_t(M) + kMinCkptBlock - 1) / kMinCkptBlock;
-    const size_t vec = size_t(B) * nblk * Lp * sizeof(float);
-    const size_t vec_al = (vec + 255) & ~size_t(255);
-    const size_t bl = size_t(B) * nblk * sizeof(double);
-    return 2 * vec_al + ((bl + 255) & ~size_t(255));
+    const size_t vec = align256(size_t(B) * nblk * Lp * sizeof(float));
+    const size_t bl = align256(size_t(B) * nblk * sizeof(double));
+    const size_t nwin = W ? (size_t(M) + W - 1) / W : 0;
+    return 2 * vec + bl + align256(size_t(B) * nwin * Lp * sizeof(float));
 }
 
 void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws) {
-    const size_t vec = size_t(v.B) * v.nblk * Lp * sizeof(float);
-    const size_t vec_al = (vec + 255) & ~size_t(255);
+    const size_t nblk = (size_t(v.M) + kMinCkptBlock - 1) / kMinCkptBlock;
+    const size_t vec = align256(size_t(v.B) * nblk * Lp * sizeof(float));
+    const size_t bl = align256(size_t(v.B) * nblk * sizeof(double));
     v.ck_alpha = reinterpret_cast<float*>(ws);
-    v.ck_beta = reinterpret_cast<float*>(ws + vec_al);
-    v.ck_bl = reinterpret_cast<double*>(ws + 2 * vec_al);
+    v.ck_beta = reinterpret_cast<float*>(ws + vec);
+    v.ck_bl = reinterpret_cast<double*>(ws + 2 * vec);
+    v.ck_win = v.nwin ? reinterpret_cast<float*>(ws + 2 * vec + bl) : nullptr;
 }
 
 JobView fb_job_view(const si_fb_job& j, uint32_t cb) {
@@ -87,6 +96,8 @@ JobView fb_job_view(const si_fb_job& j, uint32_t cb) {
     v.rng_stream = j.rng_stream;
     v.sample_base = j.sample_base;
     v.nblk = (j.n_markers + cb - 1) / cb;
+    v.W = fb_window(j);
+    v.nwin = v.W ? (j.n_markers + v.W - 1) / v.W : 0u;
     v.seed = j.seed;
     v.step = make_float4(j.step.t_o, j.step.u, j.step.lto, j.step.lu);
     v.steps = reinterpret_cast<const float4*>(j.steps);

@@ -17,6 +17,7 @@ constexpr uint32_t kMaxPalette = 64;
 // One job as a lane group sees it (plain and batched forms alike).
 struct JobView {
     uint32_t B, M, L, flags, pitch, rng_stream, sample_base, nblk;  // nblk: checkpoint blocks
+    uint32_t W, nwin;        // window length and count (W = 0: one window, no restarts)
     uint64_t seed;
     float4 step;             // (t_o, u, lto, lu) when not per-marker
     const float4* steps;     // [M] per-marker coefficients
@@ -31,6 +32,7 @@ struct JobView {
     float* ck_alpha;  // [B][nblk][Lp]  forward vector at each block's first marker
     float* ck_beta;   // [B][nblk][Lp]  backward vector at each block's last marker
     double* ck_bl;    // [B][nblk]      its log scale
+    float* ck_win;    // [B][nwin][Lp]  forward vector at each window's last marker
 };
 
 // Emission tables shared by every job of a launch.
@@ -53,7 +55,13 @@ uint32_t fb_candidates(uint32_t Lp, int pass, uint32_t* lanes, uint32_t max);
 bool fb_variant_valid(uint32_t Lp, const FbVariant& v);
 
 // Workspace of one job at the smallest checkpoint block (enough for all).
-size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp);
+// Window length a job runs with: 0 unless windowing applies (window > 0 and
+// M > window, src/api/imputer.cpp:965-966).
+inline uint32_t fb_window(const si_fb_job& j) {
+    return (j.window > 0 && j.n_markers > j.window) ? j.window : 0u;
+}
+constexpr uint32_t kMinWindow = 32;
+size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp, uint32_t W);
 void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws);
 JobView fb_job_view(const si_fb_job& j, uint32_t cb);
 

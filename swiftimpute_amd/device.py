@@ -222,6 +222,7 @@ class Job:
     sample_base: int = 0
     argmax: bool = False
     logspace: bool = False
+    window: int = 0              # the reference's windowed model when M > window > 0
     out: JobOutputs = field(default_factory=JobOutputs)
     _keep: list = field(default_factory=list)
 
@@ -262,6 +263,7 @@ class Job:
         j.n_palette = self.palette.shape[0]
         j.rng_stream = self.rng_stream
         j.sample_base = self.sample_base
+        j.window = self.window
         j.seed = self.seed
         s = self.trans.step
         j.step = StepCoef(float(s[0]), float(s[1]), float(s[2]), float(s[3]))

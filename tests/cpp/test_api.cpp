@@ -177,6 +177,7 @@ static int run(int argc, char** argv) {
     const uint64_t seed = std::stoull(argv[9]);
     const bool use_pbwt = argc > 10 ? std::stoi(argv[10]) != 0 : true;
     const bool gpu_compat = argc > 11 ? std::stoi(argv[11]) != 0 : false;
+    const uint32_t window = argc > 12 ? std::stoul(argv[12]) : 10000u;
     uint32_t H, M, B;
     {
         std::ifstream meta(in + "/meta.txt");
@@ -202,6 +203,7 @@ static int run(int argc, char** argv) {
     cfg.hmm_params.ne = ne;
     cfg.hmm_params.use_pbwt_selection = use_pbwt;
     cfg.selection = gpu_compat ? SelectionMode::GpuCompat : SelectionMode::CpuSort;
+    cfg.window_size = window;
     cfg.transition_mode = per_marker ? TransitionMode::PerMarker : TransitionMode::FirstInterval;
     cfg.sampling = argmax ? SamplingMode::Argmax : SamplingMode::Philox;
     cfg.batch_size = batch;

@@ -208,6 +208,26 @@ def test_state_table_argument_errors():
     assert lib.si_select_perm_gpu_compat(10, 65, q.ctypes.data_as(_native._u32p)) == -1
 
 
+def test_short_window_is_refused():
+    lib = _native.hip()
+    fake = ctypes.c_void_p(256)  # never dereferenced: the job check fails first
+    j = _native.FbJob()
+    j.n_samples, j.n_markers, j.n_states, j.code_pitch, j.n_palette = 2, 100, 8, 112, 4
+    j.sa = j.mmask = j.codes = j.eps = j.emax = j.ds = fake
+    j.step = _native.StepCoef(1e-3, 0.9, -6.9, -0.1)
+    j.window = 20
+    arr = (_native.FbJob * 1)(j)
+    assert lib.si_fb_run(arr, 1, None, 0, None) == _native.SI_ERR_UNSUPPORTED
+    assert b"window 20 shorter than 32" in lib.si_last_error()
+    # the workspace grows by one vector per window and sample
+    arr[0].window = 0
+    base = lib.si_fb_workspace_bytes(arr, 1)
+    arr[0].window = 32
+    assert lib.si_fb_workspace_bytes(arr, 1) > base
+    arr[0].window = 100  # M <= window: no windows
+    assert lib.si_fb_workspace_bytes(arr, 1) == base
+
+
 def test_variant_queries():
     lib = _native.hip()
     v = _native.FbVariant()

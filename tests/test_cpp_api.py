@@ -111,6 +111,26 @@ def test_cpp_api_gpu_compat_selection(tmp_path, H, L):
 
 
 @pytest.mark.gpu
+def test_cpp_api_windowed(tmp_path):
+    # window_size below M: the reference's windowed model (imputer.cpp:965-971)
+    pan = synth.make_panel(500, 700, seed=61, spacing_mode=1)
+    tg = synth.make_targets(pan, 9, 4, seed=62)
+    write_inputs(tmp_path / "in", pan, tg, synth.HARDCALL_PALETTE)
+    os.makedirs(tmp_path / "out", exist_ok=True)
+    r = run_exe("run", tmp_path / "in", tmp_path / "out", 8, 10000, 1, 0, 4, 5, 1, 0, 150)
+    assert r.returncode == 0, r.stdout
+    o = read_outputs(tmp_path / "out", pan, tg.B, 8)
+    ref = pan.alleles()
+    p, _ = oracle.pbwt_build(ref)
+    _, _, a0, a1 = oracle.state_table(ref, p, 8)
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 10000, per_marker=True)
+    gl = synth.HARDCALL_PALETTE[tg.codes[:, : pan.M]]
+    fb = oracle.fb_f64_windowed(a0, a1, gl, tr.Td, tr.To, 150, per_marker=True)
+    assert np.abs(o["ap"] - fb["ap"]).max() <= 1e-5
+    assert np.abs(o["loglik"] - fb["loglik"]).max() <= 1e-5 * np.abs(fb["loglik"]).max()
+
+
+@pytest.mark.gpu
 def test_cpp_api_simple_selection(tmp_path):
     pan = synth.make_panel(200, 300, seed=41)
     tg = synth.make_targets(pan, 5, 4, seed=42)

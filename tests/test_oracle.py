@@ -319,6 +319,112 @@ def test_floor_changes_the_answer():
     assert np.isfinite(lb).all()
 
 
+def brute_windowed(a0, a1, gl_s, Td, To, per_marker, W, seed_prev=False):
+    """Path enumeration of the windowed model: a step into a window start
+    w*W (w > 0) carries the state unchanged with weight 1 (no transition, no
+    emission); window w > 0 restarts its forward at the first forward's
+    normalized vector at its own last marker; the backward may stop at
+    marker e with weight e^-999 unless the step out of e is a carry."""
+    M, L = a0.shape
+    E = emission_matrix(a0, a1, gl_s)
+
+    def step(r, i, j):  # log weight of the step into marker r
+        if r % W == 0:
+            return 0.0 if i == j else -math.inf
+        t = (r - 1) if per_marker else 0
+        return float(Td[t] if i == j else To[t]) + E[r, j]
+
+    def fwd(lo, m, init):  # log sum over paths lo..m ending in each state
+        out = []
+        for k in range(L):
+            terms = []
+            for path in itertools.product(range(L), repeat=m - lo):
+                z = list(path) + [k]
+                terms.append(init[z[0]] + sum(step(lo + r, z[r - 1], z[r]) for r in range(1, m - lo + 1)))
+            out.append(lse(terms) if max(terms) > -math.inf else -math.inf)
+        return np.array(out)
+
+    def bwd(m, k):
+        terms = []
+        for e in range(m, M):
+            if e < M - 1 and (e + 1) % W == 0:
+                continue  # no floor term before a carry
+            c = 0.0 if e == M - 1 else -999.0
+            for path in itertools.product(range(L), repeat=e - m):
+                z = [k] + list(path)
+                terms.append(c + sum(step(m + r, z[r - 1], z[r]) for r in range(1, e - m + 1)))
+        return lse([t for t in terms if t > -math.inf])
+
+    nw = (M + W - 1) // W
+    F = []
+    for w in range(nw):
+        end = min(M, (w + 1) * W) - 1
+        f = fwd(0, end, E[0])
+        F.append(f - f.max())
+    post = np.empty((M, L))
+    for m in range(M):
+        w = m // W
+        a = fwd(0, m, E[0]) if w == 0 else fwd(w * W, m, F[w - 1] if seed_prev else F[w])
+        x = a + np.array([bwd(m, k) for k in range(L)])
+        x = np.exp(x - x.max())
+        post[m] = x / x.sum()
+    return post
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("W", [1, 2, 3])
+def test_fb_windowed_matches_brute_force(seed, W):
+    rng = np.random.default_rng(300 + seed)
+    M, L = int(rng.integers(W + 1, 7)), int(rng.integers(1, 4))
+    a0, a1, gl, Td, To = random_instance(rng, M, L)
+    per_marker = bool(seed % 2)
+    ref = brute_windowed(a0, a1, gl[0], Td, To, per_marker, W)
+    out = oracle.fb_f64_windowed(a0, a1, gl, Td, To, W, per_marker=per_marker)
+    assert np.abs(out["prob"][0] - ref).max() < 1e-10
+
+
+def test_fb_windowed_seed_is_the_windows_own_final_vector():
+    # The backward re-run of window w starts from window w's own final forward
+    # vector (imputer.cpp:1120-1142); seeding it from window w-1's instead
+    # (the continuous chain) is a different answer the brute force tells apart.
+    rng = np.random.default_rng(77)
+    a0, a1, gl, Td, To = random_instance(rng, 6, 3, with_floor=False)
+    own = brute_windowed(a0, a1, gl[0], Td, To, True, 3)
+    prev = brute_windowed(a0, a1, gl[0], Td, To, True, 3, seed_prev=True)
+    out = oracle.fb_f64_windowed(a0, a1, gl, Td, To, 3, per_marker=True)
+    assert np.abs(out["prob"][0] - own).max() < 1e-10
+    assert np.abs(out["prob"][0] - prev).max() > 1e-3
+
+
+def test_fb_windowed_closed_forms():
+    rng = np.random.default_rng(21)
+    B, M, L = 2, 30, 4
+    a0 = rng.integers(0, 2, (M, L)).astype(np.uint8)
+    a1 = rng.integers(0, 2, (M, L)).astype(np.uint8)
+    gl = rng.uniform(-4, 0, (B, M, 3)).astype(np.float32)
+    To = rng.uniform(-6, -2, M - 1).astype(np.float32)
+    Td = np.maximum(To, rng.uniform(-0.5, 0, M - 1)).astype(np.float32)
+    # no windowing when M <= W
+    for W in (0, M, M + 5):
+        a = oracle.fb_f64_windowed(a0, a1, gl, Td, To, W, per_marker=True)
+        b = oracle.fb_f64(a0, a1, gl, Td, To, per_marker=True)
+        assert np.array_equal(a["prob"], b["prob"]) and np.array_equal(a["loglik"], b["loglik"])
+    # W = 1: every window is one marker, so every forward is marker 0's
+    # normalized emission and every backward is empty
+    a = oracle.fb_f64_windowed(a0, a1, gl, Td, To, 1, per_marker=True)
+    for s in range(B):
+        e = emission_matrix(a0, a1, gl[s])[0]
+        x = np.exp(e - e.max())
+        assert np.abs(a["prob"][s] - x / x.sum()).max() < 1e-12
+    # W = M - 1: the last window is its own first marker, carried from M - 2;
+    # the first window is the plain model on markers 0..M-2
+    a = oracle.fb_f64_windowed(a0, a1, gl, Td, To, M - 1, per_marker=True)
+    b = oracle.fb_f64(a0[:-1], a1[:-1], gl[:, :-1], Td[:-1], To[:-1], per_marker=True)
+    assert np.abs(a["prob"][:, :-1] - b["prob"]).max() < 1e-12
+    assert np.abs(a["prob"][:, -1] - a["prob"][:, -2]).max() < 1e-12
+    assert np.abs(a["loglik"] - b["loglik"]).max() < 1e-9
+
+
 def test_fb_uniform_transitions_closed_form():
     # Td == To (rho = 1, the reference's configuration at 50 kb spacing):
     # backward is the same for every state, so posterior = softmax(E_m).
