@@ -43,6 +43,12 @@ typedef struct si_server_cfg {
     uint32_t max_queue_delay_us; /* a partial batch waits up to this long for company (0) */
     uint32_t selection;         /* 0: CPU-path selection (pbwt_index.cpp:20-44);
                                    1: the reference's GPU kernels (si_select_perm_gpu_compat) */
+    uint32_t region_resweep;    /* 0: a request over [m0, m1) uses the resident panel-wide
+                                   PBWT rows; 1: its state table comes from a PBWT sweep
+                                   of the region alone, started from the identity order
+                                   at m0, as the reference's --region run does
+                                   (main.cpp:103, imputer.cpp:40-44); costs a sweep and
+                                   M*H*4 bytes per request */
 } si_server_cfg;
 
 void si_server_defaults(si_server_cfg* cfg);

@@ -93,6 +93,8 @@ struct Request {
     uint64_t untyped = 0;
     Clock::time_point arrival, done_at;
     size_t off_codes = 0, off_hap = 0, off_ds = 0, own_bytes = 0;
+    // region re-sweep: the request's own state table, prefix rows and sweep workspace
+    size_t off_sa = 0, off_mmask = 0, off_prefix = 0, off_pws = 0, pws_bytes = 0;
     std::vector<uint8_t> hap;
     std::vector<float> ds;
     uint8_t* dst_hap = nullptr;  // caller buffers (si_server_submit_into), else hap/ds above
@@ -126,6 +128,7 @@ struct si_server {
     uint8_t* d_sa = nullptr;
     uint8_t* d_mmask = nullptr;
     si_step_coef* d_steps = nullptr;  // [M], entry m = step into m
+    std::vector<uint32_t> perm;       // selection permutation (min(L, H) entries)
     std::vector<si_step_coef> steps;  // host copy
     std::vector<uint32_t> tiny_switch_prefix;  // intervals below the linear range, prefix count
     float* d_eps = nullptr;
@@ -175,10 +178,27 @@ struct si_server {
         return tiny_switch_prefix[b] - tiny_switch_prefix[a] > 0;
     }
 
-    // Device footprint of a request apart from its checkpoint workspace.
-    static size_t own_bytes(uint32_t B, uint32_t M, uint32_t pitch) {
-        return align256(size_t(B) * pitch) + align256(size_t(B) * 2 * M) +
-               align256(size_t(B) * M * sizeof(float));
+    // Device footprint of a request apart from its checkpoint workspace:
+    // codes, haplotypes, dosages and, with region re-sweeps, its state table,
+    // prefix rows and sweep workspace (offsets set here, relative to the start).
+    size_t own_bytes(Request& r) const {
+        size_t o = align256(size_t(r.B) * r.pitch);
+        r.off_hap = o;
+        o += align256(size_t(r.B) * 2 * r.M);
+        r.off_ds = o;
+        o += align256(size_t(r.B) * r.M * sizeof(float));
+        if (cfg.region_resweep) {
+            r.off_sa = o;
+            o += align256(size_t(r.M) * Lp);
+            r.off_mmask = o;
+            o += align256(r.M);
+            r.off_prefix = o;
+            o += align256(size_t(r.M) * H * 4);
+            r.off_pws = o;
+            r.pws_bytes = si_pbwt_workspace_bytes(H, r.M);
+            o += align256(r.pws_bytes);
+        }
+        return o;
     }
     size_t ws_bytes_of(const Request& r) const {
         si_fb_job j{};
@@ -200,14 +220,16 @@ struct si_server {
         j.seed = r.seed;
         j.step = (r.M >= 2) ? steps[r.m0 + 1] : si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY};
         j.steps = d_steps + r.m0;
-        j.sa = d_sa + size_t(r.m0) * Lp;
-        j.mmask = d_mmask + r.m0;
+        j.sa = cfg.region_resweep ? reinterpret_cast<const uint8_t*>(arena.ptr(r.off_codes + r.off_sa))
+                                  : d_sa + size_t(r.m0) * Lp;
+        j.mmask = cfg.region_resweep ? reinterpret_cast<const uint8_t*>(arena.ptr(r.off_codes + r.off_mmask))
+                                     : d_mmask + r.m0;
         j.codes = reinterpret_cast<const uint8_t*>(arena.ptr(r.off_codes));
         j.eps = d_eps;
         j.emax = d_emax;
         j.palette = d_palette;
-        j.hap = reinterpret_cast<uint8_t*>(arena.ptr(r.off_hap));
-        j.ds = reinterpret_cast<float*>(arena.ptr(r.off_ds));
+        j.hap = reinterpret_cast<uint8_t*>(arena.ptr(r.off_codes + r.off_hap));
+        j.ds = reinterpret_cast<float*>(arena.ptr(r.off_codes + r.off_ds));
         return j;
     }
 
@@ -270,7 +292,7 @@ struct si_server {
         size_t ws_total = 0;
         while (!queue.empty() && take.size() < cfg.max_batch) {
             auto& r = queue.front();
-            const size_t own = own_bytes(r->B, r->M, r->pitch), ws = ws_bytes_of(*r);
+            const size_t own = own_bytes(*r), ws = ws_bytes_of(*r);
             size_t off = 0;
             // the batch workspace is allocated after selection; keep room for it
             if (arena.size() - arena.used() < own + ws_total + ws + 4096 || !arena.alloc(own, &off)) {
@@ -279,8 +301,6 @@ struct si_server {
             }
             r->own_bytes = own;
             r->off_codes = off;
-            r->off_hap = off + align256(size_t(r->B) * r->pitch);
-            r->off_ds = r->off_hap + align256(size_t(r->B) * 2 * r->M);
             ws_total += ws;
             take.push_back(r);
             queue.pop_front();
@@ -329,14 +349,31 @@ struct si_server {
                 throw std::runtime_error("codes upload failed");
             o += r->codes.size();
         }
+        if (cfg.region_resweep) {
+            // each request's region swept from the identity order at its first marker
+            const size_t W = (H + 63) / 64;
+            for (auto& r : take_ref) {
+                const uint64_t* cols = d_cols + size_t(r->m0) * W;
+                auto* prefix = reinterpret_cast<uint32_t*>(arena.ptr(r->off_codes + r->off_prefix));
+                if (si_pbwt_sweep(cols, H, r->M, prefix, nullptr, arena.ptr(r->off_codes + r->off_pws),
+                                  r->pws_bytes, sl.stream) != SI_OK ||
+                    si_state_table(prefix, cols, nullptr, H, r->M, cfg.n_states, 0, perm.data(), nullptr,
+                                   reinterpret_cast<uint8_t*>(arena.ptr(r->off_codes + r->off_sa)),
+                                   reinterpret_cast<uint8_t*>(arena.ptr(r->off_codes + r->off_mmask)),
+                                   sl.stream) != SI_OK)
+                    throw std::runtime_error(std::string("region sweep failed: ") + si_last_error());
+            }
+        }
         if (si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch) != SI_OK ||
             si_fb_batch_run(sl.batch, arena.ptr(ws_off), ws_total, 0, sl.stream) != SI_OK)
             throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
         for (auto& r : take_ref) {
             const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
-            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_hap), hb, hipMemcpyDeviceToHost, sl.stream);
+            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_codes + r->off_hap), hb, hipMemcpyDeviceToHost,
+                                 sl.stream);
             o += hb;
-            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_ds), db, hipMemcpyDeviceToHost, sl.stream);
+            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_codes + r->off_ds), db, hipMemcpyDeviceToHost,
+                                 sl.stream);
             o += db;
         }
         if (hipEventRecord(sl.done, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
@@ -408,6 +445,7 @@ void si_server_defaults(si_server_cfg* c) {
     c->device_id = 0;
     c->max_queue_delay_us = 0;
     c->selection = 0;
+    c->region_resweep = 0;
 }
 
 int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H, uint32_t M,
@@ -416,6 +454,7 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
         return fail(SI_ERR_ARG, "si_server_create: bad arguments");
     if (cfg->n_states == 0 || cfg->n_states > 64) return fail(SI_ERR_ARG, "n_states must be in 1..64");
     if (cfg->selection > 1) return fail(SI_ERR_ARG, "selection must be 0 (CPU path) or 1 (GPU kernels)");
+    if (cfg->region_resweep > 1) return fail(SI_ERR_ARG, "region_resweep must be 0 or 1");
     auto s = std::make_unique<si_server>();
     s->cfg = *cfg;
     s->cfg.max_streams = std::max(1u, cfg->max_streams);
@@ -440,7 +479,8 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
         if (st) (void)hipStreamDestroy(st);
         return fail(SI_ERR_DEVICE, "si_server_create: device allocation failed");
     }
-    std::vector<uint32_t> perm(std::min(L, H));
+    std::vector<uint32_t>& perm = s->perm;
+    perm.resize(std::min(L, H));
     int rc = si_pbwt_sweep(s->d_cols, H, M, prefix, nullptr, nullptr, 0, st);
     if (rc == SI_OK)
         rc = cfg->selection == 1 ? si_select_perm_gpu_compat(H, L, perm.data())
@@ -516,7 +556,7 @@ int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
             r->untyped += src[m] == 3;
         }
     }
-    const size_t need = si_server::own_bytes(B, r->M, r->pitch) + s->ws_bytes_of(*r) + 8192;
+    const size_t need = s->own_bytes(*r) + s->ws_bytes_of(*r) + 8192;
     if (need > s->cfg.hbm_budget_bytes)
         return fail(SI_ERR_ARG, "si_server_submit: request needs " + std::to_string(need) +
                                     " bytes, above the server's HBM budget");

@@ -20,7 +20,7 @@ class ServerCfg(ctypes.Structure):
                 ("per_marker", ctypes.c_uint32), ("max_streams", ctypes.c_uint32),
                 ("max_batch", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
                 ("device_id", ctypes.c_int), ("max_queue_delay_us", ctypes.c_uint32),
-                ("selection", ctypes.c_uint32)]
+                ("selection", ctypes.c_uint32), ("region_resweep", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -80,7 +80,8 @@ class Server:
     def __init__(self, cols: np.ndarray, H: int, pos: np.ndarray, cM: np.ndarray | None = None,
                  L: int = 8, ne: int = 10000, per_marker: bool = False, max_streams: int = 4,
                  max_batch: int = 256, hbm_budget_gb: float = 32.0, rho_rate: float = 4.0,
-                 max_queue_delay_us: int = 0, gpu_compat_selection: bool = False):
+                 max_queue_delay_us: int = 0, gpu_compat_selection: bool = False,
+                 region_resweep: bool = False):
         lib = _lib()
         cfg = ServerCfg()
         lib.si_server_defaults(ctypes.byref(cfg))
@@ -90,6 +91,7 @@ class Server:
         cfg.rho_rate = rho_rate
         cfg.max_queue_delay_us = max_queue_delay_us
         cfg.selection = int(gpu_compat_selection)
+        cfg.region_resweep = int(region_resweep)
         cols = np.ascontiguousarray(cols, np.uint64)
         pos = np.ascontiguousarray(pos, np.uint64)
         cmv = None if cM is None else np.ascontiguousarray(cM, np.float64)

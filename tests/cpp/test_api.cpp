@@ -165,6 +165,19 @@ static int fixture(const std::string& golden, const std::string& out) {
     CHECK(st2[0] == 0 && st2[1] == 2 && st2[2] == 4 && st2[3] == 6);
     CHECK(r2->num_samples() == 2);
     write_bin(out + "/fixture_hap.bin", res->haplotype0(0), size_t(2) * 2 * 10);
+    // --region: the panel and targets of a region, the index swept from the
+    // identity order at its first marker (main.cpp:103, imputer.cpp:40-44)
+    auto rref = ReferencePanel::load_vcf(golden + "/test_reference.vcf", "chr1:4000-8000");
+    auto rtgt = TargetData::load_vcf(golden + "/test_target.vcf", "chr1:4000-8000");
+    CHECK(rref->num_markers() == 5 && rtgt->num_markers() == 5);
+    Imputer rimp(*rref, cfg);
+    rimp.build_index();
+    auto rp = rimp.pbwt_prefix();
+    auto rs = rimp.selected_states();
+    write_bin(out + "/region_prefix.bin", rp.data(), rp.size());
+    write_bin(out + "/region_states.bin", rs.data(), rs.size());
+    auto rres = rimp.impute(*rtgt);
+    CHECK(rres->num_samples() == 2 && rres->num_markers() == 5);
     std::printf("%s fixture checks (%d failures)\n", failures ? "FAIL" : "PASS", failures);
     return failures ? 1 : 0;
 }

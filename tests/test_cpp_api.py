@@ -11,6 +11,7 @@ import torch
 
 import oracle
 from conftest import GOLDEN, ROOT
+from test_oracle import fixture_panel
 from swiftimpute_amd import device, synth
 
 EXE = os.path.join(ROOT, "swiftimpute_amd", "_build", "test_api")
@@ -33,6 +34,14 @@ def test_cpp_api_fixture(tmp_path):
     assert r.returncode == 0 and "PASS fixture" in r.stdout
     lines = [l for l in open(tmp_path / "fixture_out.vcf") if not l.startswith("#")]
     assert len(lines) == 10
+    # the region run re-sweeps the PBWT from the identity at rs4 (marker 3)
+    _, ref = fixture_panel()
+    p, _ = oracle.pbwt_build(ref[3:8])
+    rp = np.fromfile(tmp_path / "region_prefix.bin", np.uint32).reshape(5, 8)
+    assert np.array_equal(rp, p)
+    assert not np.array_equal(rp, oracle.pbwt_build(ref)[0][3:8])
+    h0, _, _, _ = oracle.state_table(ref[3:8], p, 4)
+    assert np.array_equal(np.fromfile(tmp_path / "region_states.bin", np.uint32).reshape(5, 4), h0)
 
 
 def write_inputs(d, pan, tg, palette):

@@ -36,7 +36,10 @@ def world():
     return pan, tg
 
 
-def _plain(pan, idx, tables, m0, m1, codes, seed):
+def _plain(pan, idx, tables, m0, m1, codes, seed, idx_m0=0):
+    """The plain path over [m0, m1); idx holds the markers from idx_m0 on
+    (0: a panel-wide index)."""
+    base = m0 - idx_m0
     M = m1 - m0
     B = codes.shape[0]
     pitch = (M + 15) // 16 * 16
@@ -56,8 +59,8 @@ def _plain(pan, idx, tables, m0, m1, codes, seed):
     j.step = _native.StepCoef(float(s[0]), float(s[1]), float(s[2]), float(s[3]))
     j.steps = device._ptr(tr.steps_dev)
     import ctypes
-    j.sa = ctypes.c_void_p(idx.sa.data_ptr() + m0 * idx.Lp)
-    j.mmask = ctypes.c_void_p(idx.mmask.data_ptr() + m0)
+    j.sa = ctypes.c_void_p(idx.sa.data_ptr() + base * idx.Lp)
+    j.mmask = ctypes.c_void_p(idx.mmask.data_ptr() + base)
     j.codes = device._ptr(cd)
     tables.apply(j)
     j.ds, j.hap = device._ptr(ds), device._ptr(hap)
@@ -148,3 +151,28 @@ def test_gpu_compat_selection_equals_plain(world):
         hap, ds = _plain(pan, idx, tables, *r)
         assert np.array_equal(a.hap, hap), r[:2]
         assert np.array_equal(a.ds, ds), r[:2]
+
+
+def test_region_resweep_equals_a_panel_of_the_region(world):
+    # region_resweep: a request's states come from the PBWT of its region alone,
+    # swept from the identity order at m0 -- the plain path over an index built
+    # from the region's columns
+    pan, tg = world
+    reqs = _requests(pan, tg, 6, seed=21)
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE, max_streams=2,
+                        max_batch=3, region_resweep=True) as srv:
+        ids = [srv.submit(*r) for r in reqs]
+        got = [srv.wait(i) for i in ids]
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE, max_streams=2,
+                        max_batch=3) as srv:
+        resident = [srv.wait(srv.submit(*r)) for r in reqs]
+    tables = device.SharedTables(synth.HARDCALL_PALETTE)
+    differ = 0
+    for r, a, b in zip(reqs, got, resident):
+        m0, m1 = r[0], r[1]
+        idx = device.DeviceIndex(np.ascontiguousarray(pan.cols[m0:m1]), pan.H, L, keep_prefix=False)
+        hap, ds = _plain(pan, idx, tables, *r, idx_m0=m0)
+        assert np.array_equal(a.hap, hap), r[:2]
+        assert np.array_equal(a.ds, ds), r[:2]
+        differ += m0 > 0 and not np.array_equal(a.ds, b.ds)
+    assert differ > 0  # re-sweeping from m0 changes the states of most regions
