@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
     const uint32_t c = blockIdx.x;
     const uint32_t m_first = c * a.C, m_end = min(a.M, m_first + a.C);
     const bool divs = a.rows_div && a.div;
-    if (divs) div_row(a, m_end - 1u);  // the chunk's last row is written by phase A/B
+    if (divs) div_row(a, m_end - 1u);  // every divergence row of the chunk, its last one too
     if (m_end - m_first < 2) return;
     const WaveCtx q(a);
 
@@ -1092,9 +1092,82 @@ __global__ __launch_bounds__(kPT) void pbwt_seg_fill_kernel(PbwtArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Divergence rows alone (row m = m everywhere), for the radix form, which
+// runs it on a second stream beside the sort and composition launches: those
+// leave most of the HBM bandwidth idle.  Workgroups stride over rows; 16-byte
+// stores when rows are 16-byte aligned.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pbwt_div_fill_kernel(uint32_t* __restrict__ div, uint32_t H,
+                                                            uint32_t M) {
+    const bool vec = (H & 3u) == 0u;
+    for (uint32_t m = blockIdx.x; m < M; m += gridDim.x) {
+        uint32_t* row = div + size_t(m) * H;
+        if (vec) {
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4 v = {m, m, m, m};
+            u32x4* r4 = reinterpret_cast<u32x4*>(row);
+            for (uint32_t i = threadIdx.x; i < H / 4u; i += blockDim.x) __builtin_nontemporal_store(v, r4 + i);
+        } else {
+            for (uint32_t i = threadIdx.x; i < H; i += blockDim.x) __builtin_nontemporal_store(m, row + i);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
 namespace {
+
+// A second stream per (host thread, device) for work that runs beside a
+// call's main launches; created on first use with the default priority.
+struct SideStreams {
+    hipStream_t s[64] = {};
+    ~SideStreams() {
+        for (hipStream_t x : s)
+            if (x) (void)hipStreamDestroy(x);
+    }
+};
+
+hipStream_t side_stream() {
+    thread_local SideStreams t;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    if (!t.s[dev] && hipStreamCreateWithFlags(&t.s[dev], hipStreamNonBlocking) != hipSuccess)
+        t.s[dev] = nullptr;
+    return t.s[dev];
+}
+
+// Runs the divergence fill on the side stream, forked from and joined back
+// into `st` with events (so it also works under graph capture).  Returns
+// false if the fork could not be set up (the caller then fills in line).
+bool div_fill_beside(const PbwtArgs& a, hipStream_t st, hipEvent_t* join) {
+    hipStream_t side = side_stream();
+    hipEvent_t fork = nullptr;
+    if (!side || hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return false;
+    if (hipEventCreateWithFlags(join, hipEventDisableTiming) != hipSuccess) {
+        (void)hipEventDestroy(fork);
+        return false;
+    }
+    bool ok = hipEventRecord(fork, st) == hipSuccess && hipStreamWaitEvent(side, fork, 0) == hipSuccess;
+    if (ok) {
+        // a quarter of the CUs: enough stores in flight to finish during the
+        // sort, while the sort's one-workgroup-per-chunk launches still find
+        // room (a fill over every CU makes them wait and adds its whole time)
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        const uint32_t gmax = cus >= 8 ? uint32_t(cus) / 4u : 2u;
+        const uint32_t grid = a.M < gmax ? a.M : gmax;
+        hipLaunchKernelGGL(pbwt_div_fill_kernel, dim3(grid), dim3(256), 0, side, a.div, a.H, a.M);
+        ok = hipGetLastError() == hipSuccess && hipEventRecord(*join, side) == hipSuccess;
+    }
+    (void)hipEventDestroy(fork);
+    if (!ok) {
+        (void)hipEventDestroy(*join);
+        *join = nullptr;
+    }
+    return ok;
+}
 
 // Rounds per wave K (16 * 64 * K >= H): the smallest instantiated value.
 constexpr uint32_t kRounds[] = {1, 2, 4, 6, 8, 12, 16, 20, 24, 32, 40, 48, 64};
@@ -1117,13 +1190,17 @@ void launch_lds(const PbwtArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(pbwt_rows_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
 }
 
-// H <= 32768: key bytes, phase A and B in the radix form, divergence fill,
-// phase C with the order in LDS.
+// H <= 32768: key bytes, phase A and B in the radix form, divergence rows on a
+// second stream, phase C with the order in LDS.
 constexpr uint32_t kRadixMaxK = 32;
 template <int K>
-void launch_radix(const PbwtArgs& a, hipStream_t st) {
+void launch_radix(PbwtArgs a, hipStream_t st) {
     const uint32_t W = a.W2 / 2u;
+    hipEvent_t join = nullptr;
     hipLaunchKernelGGL(pbwt_keys_kernel, dim3(a.nchunk, (W + kPW - 1) / kPW), dim3(kPT), 0, st, a);
+    // the divergence rows are written beside the sort and composition, from
+    // the end of the key launch on (fill in the row re-sweep if that fails)
+    if (a.div) a.rows_div = div_fill_beside(a, st, &join) ? 0u : 1u;
     hipLaunchKernelGGL(pbwt_chunk_sort_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
     if (a.nchunk > 1) {
         const uint32_t nseg = (a.nchunk - 1u + a.SL - 1u) / a.SL;
@@ -1133,6 +1210,10 @@ void launch_radix(const PbwtArgs& a, hipStream_t st) {
             hipLaunchKernelGGL(pbwt_seg_fill_kernel<K>, dim3(a.nchunk - 1u), dim3(kPT), 0, st, a);
     }
     hipLaunchKernelGGL(pbwt_rows_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
+    if (join) {
+        (void)hipStreamWaitEvent(st, join, 0);
+        (void)hipEventDestroy(join);
+    }
 }
 
 size_t key_bytes(uint32_t H, uint32_t M, uint32_t C) {
