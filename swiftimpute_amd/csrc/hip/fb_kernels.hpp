@@ -656,14 +656,20 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
     const size_t ck = size_t(s) * J.nblk + c;
     const bool argmax = (J.flags & SI_FB_ARGMAX) != 0;
     const uint2 key = make_uint2(uint32_t(J.seed), uint32_t(J.seed >> 32));
-    // offset in the block of a window start after its first marker (or none)
+    // offset in the block of a window start after its first marker (or none),
+    // and the window's last forward vector, the restart there (in registers,
+    // so the step loops issue no loads of their own)
     uint32_t tws = 0xffffffffu;
-    const float* fw = nullptr;
+    float fwv[S::R];
+    float fls = 0.0f;
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) fwv[i] = 0.0f;
     if (J.W) {
         const uint32_t wi = m0 / J.W + 1u;
         if (wi * J.W < m0 + nm) {
             tws = wi * J.W - m0;
-            fw = J.ck_win + (size_t(s) * J.nwin + wi) * S::Lp + size_t(r) * S::R;
+            load_vec<S::R>(J.ck_win + (size_t(s) * J.nwin + wi) * S::Lp + size_t(r) * S::R, fwv);
+            if (LOG) fls = log_mass<S>(fwv);
         }
     }
     float v[S::R];
@@ -697,10 +703,8 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                     const bool live = tt > 0 && tt < nm;
                     vsel(live, a, an);
                     ls = live ? lsn : ls;
-                    if (tt == tws) {
-                        load_vec<S::R>(fw, a);
-                        if (LOG) ls = log_mass<S>(a);
-                    }
+                    vsel(tt == tws, a, fwv);
+                    ls = tt == tws ? fls : ls;
                 }
             }
             em.fill(tab, P, in, sub);
@@ -731,9 +735,9 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                         const bool live = tt > 0 && tt < nm;
                         vsel(live, a, an);
                         ls = live ? lsn : ls;
-                        if (!FULL && tt == tws) {
-                            load_vec<S::R>(fw, a);
-                            if (LOG) ls = log_mass<S>(a);
+                        if constexpr (!FULL) {
+                            vsel(tt == tws, a, fwv);
+                            ls = tt == tws ? fls : ls;
                         }
                     }
                     store_vec<S::R>(pb + t * T + r * S::R, a);
