@@ -523,6 +523,7 @@ struct si_fb_batch {
         uint64_t* d_toff = nullptr;
         unsigned long long* d_counters = nullptr;
         uint64_t chain_items = 0, block_items = 0;
+        uint64_t units = 0;           // sum of samples x markers over the class's jobs
         uint32_t off_cb = 0;          // checkpoint block the offsets were made for
         si::FbVariant var{};
         bool tuned = false, fixed = false;
@@ -533,6 +534,10 @@ struct si_fb_batch {
     size_t ws_bytes = 0;
     si::FbTables tables{};
     std::vector<Class> classes;
+    // with several classes: a stream per class, forked from and joined back
+    // into the caller's stream, so the classes' passes overlap
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;  // [0] fork, [1 + k] class k done
     void* bound_ws = nullptr;
     hipGraphExec_t exec = nullptr;
     void* graph_ws = nullptr;
@@ -548,6 +553,10 @@ static void batch_free(si_fb_batch* b) {
         if (c.d_counters) (void)hipFree(c.d_counters);
     }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    for (hipStream_t s : b->side)
+        if (s) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
     delete b;
 }
 
@@ -640,6 +649,7 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
         std::stable_sort(c.order.begin(), c.order.end(), [&](uint32_t x, uint32_t y) {
             return jobs[x].n_markers > jobs[y].n_markers;
         });
+        for (uint32_t i : c.order) c.units += uint64_t(jobs[i].n_samples) * jobs[i].n_markers;
         const size_t n = c.order.size();
         if (hipMalloc(&c.d_jobs, n * sizeof(si::JobView)) != hipSuccess ||
             hipMalloc(&c.d_off, 2 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
@@ -650,6 +660,24 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
             batch_free(b);
             return device_status(SI_ERR_DEVICE, "si_fb_batch_create");
         }
+    }
+    if (b->classes.size() > 1) {
+        b->side.assign(b->classes.size(), nullptr);
+        b->ev.assign(b->classes.size() + 1, nullptr);
+        for (auto& s : b->side)
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+        for (auto& e : b->ev)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+        for (auto s : b->side)
+            if (!s) {
+                batch_free(b);
+                return device_status(SI_ERR_DEVICE, "si_fb_batch_create (streams)");
+            }
+        for (auto e : b->ev)
+            if (!e) {
+                batch_free(b);
+                return device_status(SI_ERR_DEVICE, "si_fb_batch_create (events)");
+            }
     }
     *out = b;
     return SI_OK;
@@ -818,8 +846,29 @@ static int batch_bind(si_fb_batch* b, void* ws, hipStream_t stream) {
     return SI_OK;
 }
 
+// Every class's chain pass lasts about as long as its longest chains take one
+// wavefront, far below what its work needs of the GPU, so the classes run side
+// by side on their own streams, the persistent chain grids shared out by the
+// classes' chain work (lanes x samples x markers); each class's block pass
+// follows its chain pass on its stream with the full grid.
 static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
-    for (auto& c : b->classes) {
+    const bool fork = b->classes.size() > 1 && !b->side.empty();
+    double work_total = 0.0;
+    for (const auto& c : b->classes) work_total += double(c.units) * c.var.gc;
+    if (fork && hipEventRecord(b->ev[0], stream) != hipSuccess)
+        return device_status(SI_ERR_DEVICE, "si_fb_batch_run (fork)");
+    for (size_t k = 0; k < b->classes.size(); ++k) {
+        auto& c = b->classes[k];
+        hipStream_t st = stream;
+        uint32_t grid_chain = c.grid_chain;
+        if (fork) {
+            st = b->side[k];
+            if (hipStreamWaitEvent(st, b->ev[0], 0) != hipSuccess)
+                return device_status(SI_ERR_DEVICE, "si_fb_batch_run (fork)");
+            const double share = work_total > 0.0 ? double(c.units) * c.var.gc / work_total : 1.0;
+            grid_chain = std::max<uint32_t>(1u, std::min<uint32_t>(
+                c.grid_chain, uint32_t(std::ceil(double(c.grid_chain) * share))));
+        }
         si::BatchClass bc{};
         bc.Lp = c.Lp;
         bc.n_jobs = uint32_t(c.order.size());
@@ -830,12 +879,18 @@ static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
         bc.chain_items = c.chain_items;
         bc.block_items = c.block_items;
         bc.counters = c.d_counters;
-        bc.grid_chain = c.grid_chain;
+        bc.grid_chain = grid_chain;
         bc.grid_block = c.grid_block;
         const int rc = si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, c.var,
-                                           stream);
+                                           st);
         if (rc) return device_status(rc, "si_fb_batch_run");
+        if (fork && hipEventRecord(b->ev[1 + k], st) != hipSuccess)
+            return device_status(SI_ERR_DEVICE, "si_fb_batch_run (join)");
     }
+    if (fork)
+        for (size_t k = 0; k < b->classes.size(); ++k)
+            if (hipStreamWaitEvent(stream, b->ev[1 + k], 0) != hipSuccess)
+                return device_status(SI_ERR_DEVICE, "si_fb_batch_run (join)");
     return SI_OK;
 }
 

@@ -933,9 +933,17 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass b
     const Tab tab = load_tables<LOG>(tb);
     __syncthreads();
     const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
+    const uint32_t m_max = bc.n_jobs ? bc.jobs[0].M : 0u;  // jobs are sorted longest first
     for (;;) {
         const Chunk ch = wave_take(&bc.counters[0], bc.chain_items, S::GPW, 1);
         if (ch.base >= bc.chain_items) break;
+        // The pass lasts as long as its longest chains, which the queue hands
+        // out first: their wavefronts issue ahead of the short chains sharing
+        // their SIMD, which fill the remaining issue slots.
+        if (2u * bc.jobs[find_job(bc.chain_off, bc.n_jobs, ch.base)].M >= m_max)
+            __builtin_amdgcn_s_setprio(3);
+        else
+            __builtin_amdgcn_s_setprio(0);
         for (uint32_t k = 0; k < ch.n; k += S::GPW) {
             const uint64_t item = ch.base + k + gw;
             if (item < bc.chain_items) {
