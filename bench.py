@@ -11,8 +11,8 @@ swiftimpute_amd.synth; every input is resident in HBM before a timed region):
          uniform in {4, 8, 16}, typed density uniform in {1/10, 1/20, 1/50},
          Ne = 10,000.  One step = every job imputed once (forward-backward,
          posterior dosage, Philox sampling).  Batched form: one persistent
-         work-queue launch per pass and state class, replayed from a captured
-         hipGraph.  Plain form: si_fb_run once per job, one after another on
+         work-queue launch per pass and state class, the classes side by side
+         on forked streams, replayed from a captured hipGraph.  Plain form: si_fb_run once per job, one after another on
          one stream.  Both write the same outputs; the bench checks that they
          are equal bit for bit.
   serve  -- a resident 40,000 x 500,000 panel; Poisson request arrivals;
@@ -645,7 +645,8 @@ def main():
                 "workload": f"{b['n_jobs']} uneven jobs on one 20000 x 200000 panel: regions "
                             "log-uniform 2000-80000 markers, cohorts log-uniform 1-2000 samples, "
                             "L in {4,8,16}, typed 1/10, 1/20 or 1/50, Ne=10000; batched "
-                            "persistent work-queue launches replayed from a hipGraph",
+                            "persistent work-queue launches (state classes side by side) "
+                            "replayed from a hipGraph",
                 "jobs": b["n_jobs"], "haplotypes": 20000, "markers": 200000,
                 "sample_marker_units": b["units"], "untyped_genotypes": b["untyped"],
                 "jobs_per_L": b["L_counts"], "logspace_jobs": b["n_logspace_jobs"],
