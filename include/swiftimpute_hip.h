@@ -286,13 +286,17 @@ int si_fb_batch_run(si_fb_batch* batch, void* workspace, size_t workspace_bytes,
 int si_fb_batch_destroy(si_fb_batch* batch);
 
 /* Batch classes (one per padded-state count and numeric space) and their
- * variants.  A batch tunes each class with its own persistent kernels on its
- * own jobs (cut to max_markers markers, 0: whole jobs), outputs disabled:
- * every candidate is warmed on 16-marker cuts, then timed once per pass,
- * about five runs of the class in all.  Results are kept per process, keyed
- * by (device, Lp, numeric space, per-marker transitions, size bucket of the
- * class's chain count), so later batches of the same kind reuse them without
- * timing.  Synchronizes the stream. */
+ * variants.  The classes of a batch run side by side (a stream each, forked
+ * from and joined back into the caller's stream).  A batch tunes each class
+ * with its own persistent kernels on its own jobs (cut to max_markers
+ * markers, 0: whole jobs), outputs disabled: every candidate is warmed on
+ * 16-marker cuts, then timed once per pass, about five runs of the class in
+ * all.  Beside other classes a class keeps the default chain lanes (Lp / 4)
+ * and takes its block lanes and checkpoint block from that timing.  Results
+ * are kept per process, keyed by (device, Lp, numeric space, per-marker
+ * transitions, size bucket of the class's chain count, alone or beside
+ * others), so later batches of the same kind reuse them without timing.
+ * Synchronizes the stream. */
 uint32_t si_fb_batch_n_classes(const si_fb_batch* batch);
 int si_fb_batch_class_info(const si_fb_batch* batch, uint32_t k, uint32_t* padded_states,
                            uint32_t* flags, si_fb_variant* variant, int* tuned);

@@ -380,9 +380,10 @@ si::JobView tuning_view(const si_fb_job& j, uint32_t cap, uint32_t cb) {
 struct CacheKey {
     int device;
     uint32_t Lp, flags, pm, bucket;
+    uint32_t side;  // 1: a class of a batch whose classes run side by side
     bool operator==(const CacheKey& o) const {
         return device == o.device && Lp == o.Lp && flags == o.flags && pm == o.pm &&
-               bucket == o.bucket;
+               bucket == o.bucket && side == o.side;
     }
 };
 std::mutex g_tune_mu;
@@ -789,6 +790,7 @@ static int class_tune(si_fb_batch* b, si_fb_batch::Class& c, char* ws, uint32_t 
     *best = pick(chain_ms, gc, nc, block_ms, gb, nb, c.Lp);
     return SI_OK;
 }
+
 }  // extern "C++"
 
 void si_fb_tune_cache_clear(void) {
@@ -806,15 +808,11 @@ int si_fb_batch_tune(si_fb_batch* b, void* workspace, size_t workspace_bytes, ui
     const uint32_t cap = max_markers ? max_markers : 0xffffffffu;
     int dev = 0;
     (void)hipGetDevice(&dev);
-    for (auto& c : b->classes) {
-        if (c.fixed) continue;
-        const CacheKey key{dev, c.Lp, c.flags, c.pm, size_bucket(c.chain_items)};
-        si::FbVariant v{};
-        if (!cache_get(key, &v)) {
-            const int rc = class_tune(b, c, static_cast<char*>(workspace), cap, st, &v);
-            if (rc) return device_status(rc, "si_fb_batch_tune");
-            cache_put(key, v);
-        }
+    const bool side_by_side = b->classes.size() > 1 && !b->side.empty();
+    auto key_of = [&](const si_fb_batch::Class& c) {
+        return CacheKey{dev, c.Lp, c.flags, c.pm, size_bucket(c.chain_items), side_by_side ? 1u : 0u};
+    };
+    auto set_var = [&](si_fb_batch::Class& c, const si::FbVariant& v) {
         if (v.gc != c.var.gc || v.gb != c.var.gb || v.cb != c.var.cb) {
             c.var = v;
             if (b->exec) {
@@ -822,9 +820,26 @@ int si_fb_batch_tune(si_fb_batch* b, void* workspace, size_t workspace_bytes, ui
                 b->exec = nullptr;
             }
         }
+        return class_prepare(b, c);
+    };
+    // Several classes run side by side, so a class's chain pass is neither the
+    // latency-bound pass it is alone nor a throughput-bound one: its lanes per
+    // sample stay at the default (Lp / 4), and only the block pass and the
+    // checkpoint block, throughput-bound either way, are taken from the
+    // class's own timing.
+    for (auto& c : b->classes) {
+        if (c.fixed) continue;
+        si::FbVariant v{};
+        if (!cache_get(key_of(c), &v)) {
+            const int rc = class_tune(b, c, static_cast<char*>(workspace), cap, st, &v);
+            if (rc) return device_status(rc, "si_fb_batch_tune");
+            if (side_by_side) v.gc = si::fb_default_variant(c.Lp).gc;
+        }
         c.tuned = true;
-        if (class_prepare(b, c) != SI_OK) return device_status(SI_ERR_DEVICE, "si_fb_batch_tune");
+        if (set_var(c, v) != SI_OK) return device_status(SI_ERR_DEVICE, "si_fb_batch_tune");
     }
+    for (const auto& c : b->classes)
+        if (!c.fixed && c.tuned) cache_put(key_of(c), c.var);
     return SI_OK;
 }
 
@@ -850,26 +865,42 @@ static int batch_bind(si_fb_batch* b, void* ws, hipStream_t stream) {
 // wavefront, far below what its work needs of the GPU, so the classes run side
 // by side on their own streams, the persistent chain grids shared out by the
 // classes' chain work (lanes x samples x markers); each class's block pass
-// follows its chain pass on its stream with the full grid.
-static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
+// follows its chain pass on its stream with the full grid.  `bcs` holds each
+// class's queues (job tables, item counts, full persistent grids for its
+// variant in `vars`).
+static int enqueue_classes(si_fb_batch* b, std::vector<si::BatchClass> bcs,
+                           const std::vector<si::FbVariant>& vars, hipStream_t stream) {
     const bool fork = b->classes.size() > 1 && !b->side.empty();
     double work_total = 0.0;
-    for (const auto& c : b->classes) work_total += double(c.units) * c.var.gc;
-    if (fork && hipEventRecord(b->ev[0], stream) != hipSuccess)
-        return device_status(SI_ERR_DEVICE, "si_fb_batch_run (fork)");
+    for (size_t k = 0; k < b->classes.size(); ++k) work_total += double(b->classes[k].units) * vars[k].gc;
+    if (fork && hipEventRecord(b->ev[0], stream) != hipSuccess) return SI_ERR_DEVICE;
     for (size_t k = 0; k < b->classes.size(); ++k) {
-        auto& c = b->classes[k];
+        si::BatchClass& bc = bcs[k];
         hipStream_t st = stream;
-        uint32_t grid_chain = c.grid_chain;
         if (fork) {
             st = b->side[k];
-            if (hipStreamWaitEvent(st, b->ev[0], 0) != hipSuccess)
-                return device_status(SI_ERR_DEVICE, "si_fb_batch_run (fork)");
-            const double share = work_total > 0.0 ? double(c.units) * c.var.gc / work_total : 1.0;
-            grid_chain = std::max<uint32_t>(1u, std::min<uint32_t>(
-                c.grid_chain, uint32_t(std::ceil(double(c.grid_chain) * share))));
+            if (hipStreamWaitEvent(st, b->ev[0], 0) != hipSuccess) return SI_ERR_DEVICE;
+            const double share =
+                work_total > 0.0 ? double(b->classes[k].units) * vars[k].gc / work_total : 1.0;
+            bc.grid_chain = std::max<uint32_t>(1u, std::min<uint32_t>(
+                bc.grid_chain, uint32_t(std::ceil(double(bc.grid_chain) * share))));
         }
-        si::BatchClass bc{};
+        const int rc = si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, vars[k], st);
+        if (rc) return rc;
+        if (fork && hipEventRecord(b->ev[1 + k], st) != hipSuccess) return SI_ERR_DEVICE;
+    }
+    if (fork)
+        for (size_t k = 0; k < b->classes.size(); ++k)
+            if (hipStreamWaitEvent(stream, b->ev[1 + k], 0) != hipSuccess) return SI_ERR_DEVICE;
+    return SI_OK;
+}
+
+static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
+    std::vector<si::BatchClass> bcs(b->classes.size());
+    std::vector<si::FbVariant> vars(b->classes.size());
+    for (size_t k = 0; k < b->classes.size(); ++k) {
+        const auto& c = b->classes[k];
+        si::BatchClass& bc = bcs[k];
         bc.Lp = c.Lp;
         bc.n_jobs = uint32_t(c.order.size());
         bc.flags = c.flags;
@@ -879,19 +910,12 @@ static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
         bc.chain_items = c.chain_items;
         bc.block_items = c.block_items;
         bc.counters = c.d_counters;
-        bc.grid_chain = grid_chain;
+        bc.grid_chain = c.grid_chain;
         bc.grid_block = c.grid_block;
-        const int rc = si::launch_fb_batch(bc, b->tables, SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, c.var,
-                                           st);
-        if (rc) return device_status(rc, "si_fb_batch_run");
-        if (fork && hipEventRecord(b->ev[1 + k], st) != hipSuccess)
-            return device_status(SI_ERR_DEVICE, "si_fb_batch_run (join)");
+        vars[k] = c.var;
     }
-    if (fork)
-        for (size_t k = 0; k < b->classes.size(); ++k)
-            if (hipStreamWaitEvent(stream, b->ev[1 + k], 0) != hipSuccess)
-                return device_status(SI_ERR_DEVICE, "si_fb_batch_run (join)");
-    return SI_OK;
+    const int rc = enqueue_classes(b, bcs, vars, stream);
+    return rc ? device_status(rc, "si_fb_batch_run") : SI_OK;
 }
 
 int si_fb_batch_run(si_fb_batch* b, void* workspace, size_t workspace_bytes, int use_graph,
