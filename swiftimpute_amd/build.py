@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
     f"-I{INCLUDE}",
 ]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
               f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
 HIP_SOURCES = ["hip/capi.hip", "hip/index.hip", "hip/pbwt.hip", "hip/hmm.hip"]

@@ -34,6 +34,32 @@ using Clock = std::chrono::steady_clock;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// fn(lo, hi) over [0, n) in up to 8 host threads when the work is large
+// (requests of up to ~1e9 genotypes move gigabytes through host memory, which
+// one thread copies at a fraction of the memory bandwidth).
+template <class F>
+void host_parallel(size_t n, size_t bytes, F&& fn) {
+    const size_t nt = std::min<size_t>({8, n, bytes / (32u << 20) + 1});
+    if (nt <= 1) {
+        fn(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (size_t t = 1; t < nt; ++t) th.emplace_back([&, t] { fn(n * t / nt, n * (t + 1) / nt); });
+    fn(size_t(0), n / nt);
+    for (auto& x : th) x.join();
+}
+
+void par_memcpy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kPiece = 4u << 20;
+    const size_t pieces = (bytes + kPiece - 1) / kPiece;
+    host_parallel(pieces, bytes, [&](size_t lo, size_t hi) {
+        const size_t a = lo * kPiece, b = std::min(bytes, hi * kPiece);
+        if (b > a) std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+    });
+}
+
 // First-fit allocator over one device buffer (host-side bookkeeping only).
 class Arena {
   public:
@@ -260,12 +286,13 @@ struct si_server {
             for (auto& r : sl->reqs) {
                 const size_t hb = size_t(r->B) * 2 * r->M, n = size_t(r->B) * r->M;
                 if (r->into) {
-                    if (r->dst_hap) std::memcpy(r->dst_hap, sl->pinned + o, hb);
-                    if (r->dst_ds) std::memcpy(r->dst_ds, sl->pinned + o + hb, n * sizeof(float));
+                    if (r->dst_hap) par_memcpy(r->dst_hap, sl->pinned + o, hb);
+                    if (r->dst_ds) par_memcpy(r->dst_ds, sl->pinned + o + hb, n * sizeof(float));
                 } else {
-                    r->hap.assign(sl->pinned + o, sl->pinned + o + hb);
+                    r->hap.resize(hb);
                     r->ds.resize(n);
-                    std::memcpy(r->ds.data(), sl->pinned + o + hb, n * sizeof(float));
+                    par_memcpy(r->hap.data(), sl->pinned + o, hb);
+                    par_memcpy(r->ds.data(), sl->pinned + o + hb, n * sizeof(float));
                 }
                 o += hb + n * sizeof(float);
             }
@@ -343,7 +370,7 @@ struct si_server {
         }
         size_t o = 0;
         for (auto& r : take_ref) {
-            std::memcpy(sl.pinned + o, r->codes.data(), r->codes.size());
+            par_memcpy(sl.pinned + o, r->codes.data(), r->codes.size());
             if (hipMemcpyAsync(arena.ptr(r->off_codes), sl.pinned + o, r->codes.size(),
                                hipMemcpyHostToDevice, sl.stream) != hipSuccess)
                 throw std::runtime_error("codes upload failed");
@@ -547,14 +574,36 @@ int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
     r->into = hap != nullptr || ds != nullptr;
     r->dst_hap = hap;
     r->dst_ds = ds;
-    r->codes.assign(size_t(B) * r->pitch, 3);
-    for (uint32_t b = 0; b < B; ++b) {
-        const uint8_t* src = codes + size_t(b) * r->M;
-        std::memcpy(&r->codes[size_t(b) * r->pitch], src, r->M);
-        for (uint32_t m = 0; m < r->M; ++m) {
-            if (src[m] > 3) return fail(SI_ERR_ARG, "si_server_submit: code outside 0..3");
-            r->untyped += src[m] == 3;
+    r->codes.resize(size_t(B) * r->pitch);
+    // rows copied into the padded layout while checking the codes and counting
+    // the untyped ones (branch-free, so the scan vectorizes)
+    std::vector<uint64_t> part_untyped(8, 0);
+    std::vector<uint8_t> part_max(8, 0);
+    std::atomic<uint32_t> slot{0};
+    host_parallel(B, size_t(B) * r->M, [&](size_t lo, size_t hi) {
+        const uint32_t me = slot.fetch_add(1);
+        uint64_t un = 0;
+        uint8_t mx = 0;
+        for (size_t b = lo; b < hi; ++b) {
+            const uint8_t* src = codes + b * r->M;
+            uint8_t* dst = &r->codes[b * r->pitch];
+            std::memcpy(dst, src, r->M);
+            std::memset(dst + r->M, 3, r->pitch - r->M);
+            uint32_t row_un = 0;
+            uint8_t row_mx = 0;
+            for (uint32_t m = 0; m < r->M; ++m) {
+                row_mx = std::max(row_mx, src[m]);
+                row_un += src[m] == 3;
+            }
+            un += row_un;
+            mx = std::max(mx, row_mx);
         }
+        part_untyped[me] = un;
+        part_max[me] = mx;
+    });
+    for (size_t t = 0; t < part_untyped.size(); ++t) {
+        if (part_max[t] > 3) return fail(SI_ERR_ARG, "si_server_submit: code outside 0..3");
+        r->untyped += part_untyped[t];
     }
     const size_t need = s->own_bytes(*r) + s->ws_bytes_of(*r) + 8192;
     if (need > s->cfg.hbm_budget_bytes)

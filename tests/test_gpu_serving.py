@@ -176,3 +176,15 @@ def test_region_resweep_equals_a_panel_of_the_region(world):
         assert np.array_equal(a.ds, ds), r[:2]
         differ += m0 > 0 and not np.array_equal(a.ds, b.ds)
     assert differ > 0  # re-sweeping from m0 changes the states of most regions
+
+
+def test_bad_codes_are_refused(world):
+    pan, tg = world
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE) as srv:
+        c = np.ascontiguousarray(tg.codes[:3, 100:400])
+        c[2, 17] = 4
+        with pytest.raises(_native.NativeError, match="outside 0..3"):
+            srv.submit(100, 400, c)
+        c[2, 17] = 3
+        r = srv.wait(srv.submit(100, 400, c))
+        assert r.hap.shape == (3, 2, 300)
