@@ -48,7 +48,8 @@ struct ImputationConfig {
     HMMParameters hmm_params;
 
     int device_id;               // GPU device (-1: device 0)
-    uint32_t batch_size;         // samples per device batch (one job per launch)
+    uint32_t batch_size;         // progress is reported every batch_size samples; launches take
+                                 // as many multiples of it as fit a 16 GiB device budget
     uint32_t num_threads;        // host threads for input conversion
     bool use_pinned_memory;      // pinned host staging for transfers
 

@@ -49,6 +49,28 @@ void dev_free(T*& p) {
     p = nullptr;
 }
 
+// Device memory a launch may take for its samples (checkpoints, codes,
+// outputs).  Every draw depends only on (seed, sample, marker), so results do
+// not depend on how samples are grouped into launches; one launch per cohort
+// (up to this budget) keeps the chain pass, which takes one wavefront's walk
+// over all markers however many samples run beside it, from repeating per
+// batch_size samples.
+constexpr size_t kLaunchBudget = size_t(16) << 30;
+
+// Samples per launch: a multiple of batch_size (so progress is reported at
+// the same sample counts as one launch per batch) within kLaunchBudget.
+uint32_t launch_samples(uint32_t batch, uint32_t B, uint32_t M, uint32_t L, uint32_t window, uint32_t pitch) {
+    si_fb_job j{};
+    j.n_samples = 1;
+    j.n_markers = M;
+    j.n_states = L;
+    j.window = window;
+    const size_t per = si_fb_workspace_bytes(&j, 1) + pitch + size_t(M) * (2 * 4 + 2) + 8;
+    const size_t groups = std::max<size_t>(1, kLaunchBudget / (per * batch + 1));
+    const size_t want = std::min<size_t>(size_t(batch) * groups, (size_t(B) + batch - 1) / batch * batch);
+    return uint32_t(std::max<size_t>(batch, std::min<size_t>(want, 0x7fffffffu)));
+}
+
 }  // namespace
 
 struct Imputer::Device {
@@ -72,6 +94,8 @@ struct Imputer::Device {
     double* loglik = nullptr;
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    bool tuned = false;          // variant picked by si_fb_tune for this panel
+    si_fb_variant variant{};
     si_step_coef* steps = nullptr;
     float* eps = nullptr;
     float* emax = nullptr;
@@ -256,8 +280,9 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     hip_check(hipMemcpyAsync(dev_->emax, emax.data(), emax.size() * 4, hipMemcpyHostToDevice, dev_->stream),
               "upload");
 
-    const uint32_t batch = std::max<uint32_t>(1, std::min(config_.batch_size, B));
     const uint32_t pitch = targets.code_pitch();
+    const uint32_t report = std::max<uint32_t>(1, std::min(config_.batch_size, B));  // progress step
+    const uint32_t batch = std::min(B, launch_samples(report, B, M, L, config_.window_size, pitch));
     if (dev_->cap < batch || dev_->pitch != pitch) {
         dev_free(dev_->codes);
         dev_free(dev_->ap);
@@ -305,6 +330,7 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
         Logger::instance().info("windowed forward-backward (" + std::to_string(M) + " markers, window=" +
                                 std::to_string(job.window) + ")");
     const uint8_t* codes = targets.codes().data();
+    uint32_t reported = 0;
     for (uint32_t start = 0; start < B; start += batch) {
         const uint32_t end = std::min(start + batch, B), n = end - start;
         hip_check(hipMemcpyAsync(dev_->codes, codes + size_t(start) * pitch, size_t(n) * pitch,
@@ -312,7 +338,14 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
                   "codes upload");
         job.n_samples = n;
         job.sample_base = start;
-        si_check(si_fb_run(&job, 1, dev_->ws, dev_->ws_bytes, dev_->stream), "forward-backward");
+        if (!dev_->tuned) {  // lane shapes and checkpoint block for this panel's jobs (cached per process)
+            si_check(si_fb_tune(&job, 1, dev_->ws, dev_->ws_bytes, 0, dev_->stream, &dev_->variant),
+                     "variant tuning");
+            dev_->tuned = true;
+        }
+        si_check(si_fb_run_variant(&job, 1, &dev_->variant, dev_->ws, dev_->ws_bytes,
+                                   SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, dev_->stream),
+                 "forward-backward");
         hip_check(hipMemcpyAsync(result->mutable_haplotypes() + size_t(start) * 2 * M, dev_->hap,
                                  size_t(n) * 2 * M, hipMemcpyDeviceToHost, dev_->stream),
                   "haplotype download");
@@ -324,7 +357,10 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
                                  size_t(n) * sizeof(double), hipMemcpyDeviceToHost, dev_->stream),
                   "log-likelihood download");
         hip_check(hipStreamSynchronize(dev_->stream), "batch");
-        if (progress_callback) progress_callback(end, B);
+        while (progress_callback && reported < end) {  // once per batch_size samples
+            reported = std::min(reported + report, B);
+            progress_callback(reported, B);
+        }
     }
     result->compute_info_scores();
     return result;
@@ -340,7 +376,11 @@ size_t Imputer::estimate_memory_requirement(const ReferencePanel& reference, uin
     size_t total = M * W * 8 * (reference.packed_missing().empty() ? 1 : 2);
     if (config.hmm_params.use_pbwt_selection) total += 2 * M * H * 4;
     total += M * (size_t(L) * 4 + Lp + 1);
-    const uint32_t batch = std::max<uint32_t>(1, std::min(config.batch_size, num_target_samples));
+    const uint32_t pitch = (uint32_t(M) + 15) / 16 * 16;
+    const uint32_t report = std::max<uint32_t>(1, std::min(config.batch_size, num_target_samples));
+    const uint32_t batch = std::max<uint32_t>(
+        1, std::min(num_target_samples,
+                    launch_samples(report, num_target_samples, uint32_t(M), L, config.window_size, pitch)));
     si_fb_job j{};
     j.n_samples = batch;
     j.n_markers = uint32_t(M);
