@@ -19,9 +19,9 @@
  *   prefix   [M][H] uint32 -- PBWT order after marker m.
  *   div      [M][H] uint32 -- PBWT divergence (every entry of row m is m).
  *   sa       [M][Lp] uint8  -- state-allele table, Lp = si_padded_states(L):
- *              bit0 allele(h0)==1, bit1 allele(h1)==1, bit2 h0 missing,
- *              bit3 h1 missing, bits4-5 emission class min(a0+a1,2),
- *              bit6 padding state (k >= L), bit7 of byte k (k % 4 < 3) =
+ *              bit0 allele(h0)==1, bit1 allele(h1)==1, bits2-3 emission
+ *              class min(a0+a1,2) (3 for padding), bit4 h0 missing, bit5 h1
+ *              missing, bit6 padding state (k >= L), bit7 of byte k (k % 4 < 3) =
  *              bit (k % 4) of the marker's mmask (so every 4-byte word of
  *              the row carries the mask).
  *   mmask    [M] uint8      -- bit c set iff some real state has class c.

@@ -148,6 +148,11 @@ struct LaneWords {
         if constexpr (S::R >= 4) return (w[i >> 2] >> (8 * (i & 3))) & 0xffu;
         else return (w[0] >> (8 * (sub + uint32_t(i)))) & 0xffu;
     }
+    // The emission class of state i (bits 2-3 of its byte).
+    __device__ __forceinline__ uint32_t cls(int i, uint32_t sub) const {
+        if constexpr (S::R >= 4) return (w[i >> 2] >> (8 * (i & 3) + 2)) & 3u;
+        else return (w[0] >> (8u * (sub + uint32_t(i)) + 2u)) & 3u;
+    }
     // Bit b of state byte i as an all-ones / all-zeros mask.
     __device__ __forceinline__ uint32_t bitmask(int i, uint32_t sub, int b) const {
         const uint32_t x = S::R >= 4 ? w[i >> 2] : w[0];
@@ -170,7 +175,7 @@ __device__ __forceinline__ float lane_emission(const Tab& tab, uint32_t P, uint3
                                                float (&e)[S::R]) {
     const float* rp = tab.f + (min(code, P - 1u) * 8u + class_mask(sw.w[0])) * kTabRow;
 #pragma unroll
-    for (int i = 0; i < S::R; ++i) e[i] = rp[(sw.byte(i, sub) >> 4) & 3u];
+    for (int i = 0; i < S::R; ++i) e[i] = rp[sw.cls(i, sub)];
     return rp[4];
 }
 
@@ -826,8 +831,8 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                 if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0r[q], ap1r[q]);
                 if (J.hap) {
                     const uint32_t b = reinterpret_cast<const uint8_t*>(sb)[t * S::Lp + ks];
-                    J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 4u) ? 255u : (b & 1u));
-                    J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 8u) ? 255u : ((b >> 1) & 1u));
+                    J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 16u) ? 255u : (b & 1u));
+                    J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 32u) ? 255u : ((b >> 1) & 1u));
                 }
                 if (J.prob)
                     for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pt[k];

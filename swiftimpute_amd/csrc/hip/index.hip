@@ -48,8 +48,10 @@ __global__ __launch_bounds__(256) void state_table_kernel(
         const uint32_t a1 = allele_raw(cols, miss, W, m, h1);
         const uint32_t sum = a0 + a1;
         cls = sum >= 2 ? 2u : sum;
-        return uint8_t((a0 == 1) | ((a1 == 1) << 1) | ((a0 == 255) << 2) | ((a1 == 255) << 3) |
-                       (cls << 4));
+        // the class sits at bits 2-3, so a byte-select and yields class * 4, the
+        // state's byte offset into an emission row
+        return uint8_t((a0 == 1) | ((a1 == 1) << 1) | (cls << 2) | ((a0 == 255) << 4) |
+                       ((a1 == 255) << 5));
     };
     uint32_t present = 0;
     for (uint32_t k = 0; k < L; ++k) {
@@ -59,7 +61,7 @@ __global__ __launch_bounds__(256) void state_table_kernel(
     }
     for (uint32_t k = 0; k < Lp; ++k) {
         uint32_t cls = 3;
-        uint8_t b = (k < L) ? slot(k, cls) : uint8_t(0x40 | (3u << 4));  // padding state
+        uint8_t b = (k < L) ? slot(k, cls) : uint8_t(0x40 | (3u << 2));  // padding state
         // every lane's 4-byte word carries the marker's class-present mask in
         // bit 7 of its bytes 0-2
         if ((k & 3u) < 3u) b |= uint8_t(((present >> (k & 3u)) & 1u) << 7);

@@ -195,8 +195,8 @@ class DeviceIndex:
         """(a0, a1) [M][L] uint8 raw alleles of the selected state pairs, from
         the device state table (for the oracle)."""
         sa = self.sa.cpu().numpy()[:, : self.L]
-        a0 = np.where(sa & 4, 255, sa & 1).astype(np.uint8)
-        a1 = np.where(sa & 8, 255, (sa >> 1) & 1).astype(np.uint8)
+        a0 = np.where(sa & 16, 255, sa & 1).astype(np.uint8)
+        a1 = np.where(sa & 32, 255, (sa >> 1) & 1).astype(np.uint8)
         return a0, a1
 
 

@@ -142,8 +142,8 @@ def test_state_table(H, L):
     # class-present mask and padding bytes
     sa = idx.sa.cpu().numpy()
     cls = np.minimum(a0.astype(int) + a1.astype(int), 2)
-    assert np.array_equal((sa[:, :L] >> 4) & 3, cls)
-    assert ((sa[:, L:] & 0x7f) == 0x70).all()
+    assert np.array_equal((sa[:, :L] >> 2) & 3, cls)
+    assert ((sa[:, L:] & 0x7f) == 0x4c).all()
     mm = idx.mmask.cpu().numpy()
     for m in range(M):
         assert mm[m] == sum(1 << c for c in set(cls[m].tolist()))
