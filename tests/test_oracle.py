@@ -145,7 +145,7 @@ def test_gpu_insert_selection_closed_form(H, L):
 
 def test_gpu_warp_selection_hand_traced_rounds():
     p = np.arange(1000, 1160, dtype=np.uint32)  # prefix row: position i holds haplotype 1000+i
-    # a larger divergence at lane 5's first entry wins round 0 only
+    # a larger divergence at lane 5's first entry wins only the first selection pass
     d = np.ones(128, np.uint32)
     d[5] = 100
     assert oracle.select_states_gpu(p[:128], d, 6, "warp").tolist() == [1005, 1032, 1064, 1096, 4, 5]
