@@ -290,13 +290,13 @@ int si_fb_batch_destroy(si_fb_batch* batch);
  * from and joined back into the caller's stream).  A batch tunes each class
  * with its own persistent kernels on its own jobs (cut to max_markers
  * markers, 0: whole jobs), outputs disabled: every candidate is warmed on
- * 16-marker cuts, then timed once per pass, about five runs of the class in
- * all.  Beside other classes a class keeps the default chain lanes (Lp / 4)
- * and takes its block lanes and checkpoint block from that timing.  Results
- * are kept per process, keyed by (device, Lp, numeric space, per-marker
- * transitions, size bucket of the class's chain count, alone or beside
- * others), so later batches of the same kind reuse them without timing.
- * Synchronizes the stream. */
+ * 16-marker cuts, then timed twice per pass (the faster run counts), about
+ * ten runs of the class in all.  Beside other classes a class keeps the
+ * default chain lanes (Lp / 4), and other block lanes only when they are at
+ * least 10 % faster alone.  Results are kept per process, keyed by (device,
+ * Lp, numeric space, per-marker transitions, size bucket of the class's
+ * chain count, alone or beside others), so later batches of the same kind
+ * reuse them without timing.  Synchronizes the stream. */
 uint32_t si_fb_batch_n_classes(const si_fb_batch* batch);
 int si_fb_batch_class_info(const si_fb_batch* batch, uint32_t k, uint32_t* padded_states,
                            uint32_t* flags, si_fb_variant* variant, int* tuned);
