@@ -198,7 +198,8 @@ def bench_batch(args):
         "default_variant_ms_per_step": default_ms,
         "candidates": "lanes per sample Lp/8, Lp/4, Lp/2 (chain), Lp/4, Lp/2 (block); "
                       "checkpoint block 16 or 32; each candidate warmed on 16-marker cuts, "
-                      "then timed once on each class's whole jobs",
+                      "then timed twice on each class's whole jobs; beside other classes the "
+                      "chain keeps Lp/4 lanes and another block shape needs a 10% gain",
     }
     return {
         "cpu_baseline": cpu,
