@@ -157,7 +157,7 @@ struct LaneWords {
     __device__ __forceinline__ uint32_t bitmask(int i, uint32_t sub, int b) const {
         const uint32_t x = S::R >= 4 ? w[i >> 2] : w[0];
         const uint32_t pos = S::R >= 4 ? uint32_t(8 * (i & 3) + b) : 8u * (sub + uint32_t(i)) + b;
-        return uint32_t(int32_t(x << (31u - pos)) >> 31);
+        return uint32_t(__builtin_amdgcn_sbfe(int32_t(x), pos, 1u));  // one signed bit-field extract
     }
 };
 
@@ -223,6 +223,19 @@ __device__ __forceinline__ float gmax(float v) {
 template <class S>
 __device__ __forceinline__ float vsum(const float (&x)[S::R]) {
     return gsum<S::G>(tree_sum<S::R>(x));
+}
+// Two independent group sums, level by level, so each cross-lane add has the
+// other to hide its latency behind (the same tree as gsum).
+template <int G>
+__device__ __forceinline__ void gsum2(float& a, float& b) {
+    if constexpr (G >= 2) { a += dpp<0xB1>(a); b += dpp<0xB1>(b); }
+    if constexpr (G >= 4) { a += dpp<0x4E>(a); b += dpp<0x4E>(b); }
+    if constexpr (G >= 8) { a += dpp<0x141>(a); b += dpp<0x141>(b); }
+    if constexpr (G >= 16) { a += dpp<0x140>(a); b += dpp<0x140>(b); }
+    if constexpr (G >= 32) {
+        a += __shfl_xor(a, 16, kWave);
+        b += __shfl_xor(b, 16, kWave);
+    }
 }
 template <class S>
 __device__ __forceinline__ float vmax(const float (&x)[S::R]) {
@@ -760,7 +773,8 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                         x0[i] = __uint_as_float(pi & in.sa[t].bitmask(i, sub, 0));
                         x1[i] = __uint_as_float(pi & in.sa[t].bitmask(i, sub, 1));
                     }
-                    const float ap0 = vsum<S>(x0), ap1 = vsum<S>(x1);
+                    float ap0 = tree_sum<S::R>(x0), ap1 = tree_sum<S::R>(x1);
+                    gsum2<S::G>(ap0, ap1);
                     if (FULL || tt < nm) store_vec<S::R>(pb + t * T + r * S::R, p);
                     if (uint32_t(t) % S::G == r) {
                         ap0r[uint32_t(t) / S::G] = ap0;
