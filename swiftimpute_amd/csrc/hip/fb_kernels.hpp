@@ -953,26 +953,36 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass b
     __syncthreads();
     const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
     const uint32_t m_max = bc.n_jobs ? bc.jobs[0].M : 0u;  // jobs are sorted longest first
+    // chain_off counts two items (forward, backward) per sample.  The queue
+    // hands out rounds of GPW samples (the class's samples in job order) in
+    // one direction: round 2i runs the forward chains of samples
+    // [i*GPW, (i+1)*GPW), round 2i + 1 their backward chains, so every group
+    // of a wavefront runs the same direction's code.
+    const uint64_t n_samples = bc.chain_items >> 1;
+    const uint64_t total = 2 * ((n_samples + S::GPW - 1) / S::GPW) * S::GPW;
     for (;;) {
-        const Chunk ch = wave_take(&bc.counters[0], bc.chain_items, S::GPW, 1);
-        if (ch.base >= bc.chain_items) break;
+        const Chunk ch = wave_take(&bc.counters[0], total, S::GPW, 1);
+        if (ch.base >= total) break;
         // The pass lasts as long as its longest chains, which the queue hands
         // out first: their wavefronts issue ahead of the short chains sharing
         // their SIMD, which fill the remaining issue slots.
-        if (2u * bc.jobs[find_job(bc.chain_off, bc.n_jobs, ch.base)].M >= m_max)
+        const uint64_t first = (ch.base / S::GPW >> 1) * S::GPW;  // < n_samples
+        if (2u * bc.jobs[find_job(bc.chain_off, bc.n_jobs, 2 * first)].M >= m_max)
             __builtin_amdgcn_s_setprio(3);
         else
             __builtin_amdgcn_s_setprio(0);
         for (uint32_t k = 0; k < ch.n; k += S::GPW) {
-            const uint64_t item = ch.base + k + gw;
-            if (item < bc.chain_items) {
-                const uint32_t j = find_job(bc.chain_off, bc.n_jobs, item);
+            const uint64_t round = (ch.base + k) / S::GPW;
+            const uint64_t gs = (round >> 1) * S::GPW + gw;  // sample within the class
+            const bool fwd = (round & 1) == 0;
+            if (gs < n_samples) {
+                const uint32_t j = find_job(bc.chain_off, bc.n_jobs, 2 * gs);
                 const JobView J = bc.jobs[j];
-                const uint64_t q = item - bc.chain_off[j];
+                const uint32_t q = uint32_t(gs - (bc.chain_off[j] >> 1));
                 if (J.flags & SI_FB_PER_MARKER)
-                    run_chain<S, LOG, true, CB>(J, tab, tb.P, uint32_t(q >> 1), (q & 1) == 0, r);
+                    run_chain<S, LOG, true, CB>(J, tab, tb.P, q, fwd, r);
                 else
-                    run_chain<S, LOG, false, CB>(J, tab, tb.P, uint32_t(q >> 1), (q & 1) == 0, r);
+                    run_chain<S, LOG, false, CB>(J, tab, tb.P, q, fwd, r);
             }
         }
     }
