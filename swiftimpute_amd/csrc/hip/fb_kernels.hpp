@@ -73,21 +73,23 @@ struct Shape {
     static_assert(G_ >= 1 && G_ <= 32 && R_ >= 2 && R_ <= 8, "lane shape");
 };
 
-// Block-pass LDS of one lane group: posteriors [16][T] floats (T = Lp + pad,
-// group stride S floats) and the state bytes [16][Lp] (group stride SW
-// words).  The paddings make the vector writes of the backward walk and the
-// 16-byte reads of the sampling phase free of bank conflicts (searched over
-// 32- and 64-bank models for every lane shape).
-__host__ __device__ constexpr int pbuf_T(int Lp) { return Lp == 4 ? 4 : Lp == 8 ? 12 : Lp + 4; }
-__host__ __device__ constexpr int pbuf_E(int Lp, int G) {
-    return Lp == 4 ? 4 : Lp == 8 ? 8 : Lp == 16 ? (G == 2 ? 8 : 16) : Lp == 32 ? (G == 4 ? 4 : 32)
-                                                                              : (G == 8 ? 4 : 0);
+// Block-pass LDS of one lane group: posteriors [16][T] floats (row t at
+// t*T, group stride PS = 16 T + E floats) and the state bytes [16][Lp]
+// (group stride SW words).  Up to 16 states the rows carry no padding
+// (T = Lp), which keeps more workgroups resident; the group padding E makes
+// the vector writes and reads of the walks free of bank conflicts on 32- and
+// 64-bank models, and the sampling phase's 16-byte row reads keep 2- to
+// 4-way conflicts (measured, cheaper than padding or rotating the rows).
+// With 32 or 64 states T = Lp + 4, conflict-free for the sampling reads too.
+__host__ __device__ constexpr int pbuf_T(int Lp) { return Lp <= 16 ? Lp : Lp + 4; }
+__host__ __device__ constexpr int pbuf_E(int Lp) {
+    return Lp == 4 ? 4 : Lp == 8 ? 8 : Lp == 16 ? 16 : Lp == 32 ? 32 : 0;
 }
 template <class S>
 struct BlockLds {
     static constexpr int T = pbuf_T(S::Lp);
-    static constexpr int PS = int(kSub) * T + pbuf_E(S::Lp, S::G);  // floats per group
-    static constexpr int SW = int(kSub) * S::Lp / 4 + 1;            // words per group
+    static constexpr int PS = int(kSub) * T + pbuf_E(S::Lp);  // floats per group
+    static constexpr int SW = int(kSub) * S::Lp / 4 + 1;      // words per group
     static_assert(T % 4 == 0 && PS % 4 == 0, "16-byte aligned rows");
 };
 
@@ -664,9 +666,9 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                                           uint32_t* sb, uint4* rng) {
     using Lds = BlockLds<S>;
     constexpr uint32_t NS = uint32_t(CB) / kSub;
+    float* const prow = pb + r * S::R;  // the lane's R states of row 0
     // markers per lane in a sampling phase (with 32 lanes, half the lanes sample)
     constexpr uint32_t kPer = (kSub + uint32_t(S::G) - 1) / uint32_t(S::G);
-    constexpr int T = Lds::T;
     const uint32_t m0 = c * CB;
     const uint32_t nm = active ? min(uint32_t(CB), J.M - m0) : 0u;
     const uint32_t sub = lane_sub<S>(r);
@@ -758,13 +760,13 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                             ls = tt == tws ? fls : ls;
                         }
                     }
-                    store_vec<S::R>(pb + t * T + r * S::R, a);
+                    store_vec<S::R>(prow + t * Lds::T, a);
                 }
 #pragma unroll
                 for (int t = int(kSub) - 1; t >= 0; --t) {
                     const uint32_t tt = base + uint32_t(t);
                     float af[S::R], p[S::R];
-                    load_vec<S::R>(pb + t * T + r * S::R, af);
+                    load_vec<S::R>(prow + t * Lds::T, af);
                     posterior<S, LOG>(af, v, p);
                     float x0[S::R], x1[S::R];
 #pragma unroll
@@ -775,7 +777,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                     }
                     float ap0 = tree_sum<S::R>(x0), ap1 = tree_sum<S::R>(x1);
                     gsum2<S::G>(ap0, ap1);
-                    if (FULL || tt < nm) store_vec<S::R>(pb + t * T + r * S::R, p);
+                    if (FULL || tt < nm) store_vec<S::R>(prow + t * Lds::T, p);
                     if (uint32_t(t) % S::G == r) {
                         ap0r[uint32_t(t) / S::G] = ap0;
                         ap1r[uint32_t(t) / S::G] = ap1;
@@ -809,7 +811,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                 const uint32_t tt = base + t;
                 if (t >= kSub || tt >= nm) continue;
                 const uint32_t m = m0 + tt;
-                const float* pt = pb + t * T;
+                const float* pt = pb + t * Lds::T;
                 uint32_t ks = 0;
                 if (argmax) {  // first maximum
                     float best = -1.0f;
@@ -849,6 +851,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                     J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 32u) ? 255u : ((b >> 1) & 1u));
                 }
                 if (J.prob)
+                    #pragma unroll 1
                     for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pt[k];
             }
         }
