@@ -74,13 +74,14 @@ struct Shape {
 };
 
 // Block-pass LDS of one lane group: posteriors [16][T] floats (row t at
-// t*T, group stride PS = 16 T + E floats) and the state bytes [16][Lp]
-// (group stride SW words).  Up to 16 states the rows carry no padding
-// (T = Lp), which keeps more workgroups resident; the group padding E makes
-// the vector writes and reads of the walks free of bank conflicts on 32- and
-// 64-bank models, and the sampling phase's 16-byte row reads keep 2- to
-// 4-way conflicts (measured, cheaper than padding or rotating the rows).
-// With 32 or 64 states T = Lp + 4, conflict-free for the sampling reads too.
+// t*T, group stride PS = 16 T + E floats), 16 Philox outputs and (see
+// below) the state bytes.  Up to 16
+// states the rows carry no padding (T = Lp), which keeps more workgroups
+// resident; the group padding E makes the vector writes and reads of the
+// walks free of bank conflicts on 32- and 64-bank models, and the sampling
+// phase's 16-byte row reads keep 2- to 4-way conflicts (measured, cheaper
+// than padding or rotating the rows).  With 32 or 64 states T = Lp + 4,
+// conflict-free for the sampling reads too.
 __host__ __device__ constexpr int pbuf_T(int Lp) { return Lp <= 16 ? Lp : Lp + 4; }
 __host__ __device__ constexpr int pbuf_E(int Lp) {
     return Lp == 4 ? 4 : Lp == 8 ? 8 : Lp == 16 ? 16 : Lp == 32 ? 32 : 0;
@@ -89,7 +90,14 @@ template <class S>
 struct BlockLds {
     static constexpr int T = pbuf_T(S::Lp);
     static constexpr int PS = int(kSub) * T + pbuf_E(S::Lp);  // floats per group
-    static constexpr int SW = int(kSub) * S::Lp / 4 + 1;      // words per group
+    // State bytes [16][Lp] of the group in LDS for the sampler (SW words per
+    // group).  The plain kernel drops them where that lets one more
+    // workgroup be resident (16 states on 4 lanes: 3 -> 4 per CU); there the
+    // sampler reads the sampled state's byte from the state table, which the
+    // item has just read.  Elsewhere, and in the batched kernel (whose
+    // classes share the caches), the load latency costs more than it gains.
+    static constexpr bool kSbPlain = !(S::Lp == 16 && S::G == 4);
+    static constexpr int SW = int(kSub) * S::Lp / 4 + 1;  // words per group
     static_assert(T % 4 == 0 && PS % 4 == 0, "16-byte aligned rows");
 };
 
@@ -660,7 +668,7 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
 // restarts the forward vector from the window's last vector F_w (stored by
 // the chain pass) and the backward step out of it is not taken.
 // ---------------------------------------------------------------------------
-template <class S, bool LOG, bool PM, int CB>
+template <class S, bool LOG, bool PM, int CB, bool SB>
 __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint32_t P, bool active,
                                           uint32_t s, uint32_t c, uint32_t r, float* pb,
                                           uint32_t* sb, uint4* rng) {
@@ -735,7 +743,8 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                 constexpr bool FULL = decltype(full_c)::value;
 #pragma unroll
                 for (uint32_t t = 0; t < kSub; ++t) {
-                    if constexpr (S::R >= 4) {
+                    if constexpr (!SB) {
+                    } else if constexpr (S::R >= 4) {
 #pragma unroll
                         for (int w = 0; w < S::NW; ++w)
                             sb[t * (S::Lp / 4) + r * S::NW + w] = in.sa[t].w[w];
@@ -846,7 +855,8 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                 if (J.ds) J.ds[o] = ap0r[q] + ap1r[q];
                 if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0r[q], ap1r[q]);
                 if (J.hap) {
-                    const uint32_t b = reinterpret_cast<const uint8_t*>(sb)[t * S::Lp + ks];
+                    const uint32_t b = SB ? uint32_t(reinterpret_cast<const uint8_t*>(sb)[t * S::Lp + ks])
+                                          : uint32_t(J.sa[size_t(m) * S::Lp + ks]);
                     J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 16u) ? 255u : (b & 1u));
                     J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 32u) ? 255u : ((b >> 1) & 1u));
                 }
@@ -887,7 +897,8 @@ __global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_kernel(Jo
     using S = Shape<G, R>;
     using Lds = BlockLds<S>;
     __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
-    __shared__ uint32_t sbuf[S::GPB * Lds::SW];
+    constexpr bool SB = Lds::kSbPlain;
+    __shared__ uint32_t sbuf[S::GPB * (SB ? Lds::SW : 1)];
     __shared__ uint4 rng[S::GPB * (kSub / 4)];
     const Tab tab = load_tables<LOG>(tb);
     __syncthreads();
@@ -897,10 +908,12 @@ __global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_kernel(Jo
     const uint32_t s = active ? grp / J.nblk : 0u;
     const uint32_t c = active ? grp % J.nblk : 0u;
     float* pb = pbuf + gl * Lds::PS;
-    uint32_t* sb = sbuf + gl * Lds::SW;
+    uint32_t* sb = sbuf + (SB ? gl * Lds::SW : 0u);
     uint4* rg = rng + gl * (kSub / 4);
-    if (J.flags & SI_FB_PER_MARKER) run_block<S, LOG, true, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
-    else run_block<S, LOG, false, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+    if (J.flags & SI_FB_PER_MARKER)
+        run_block<S, LOG, true, CB, SB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+    else
+        run_block<S, LOG, false, CB, SB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
 }
 
 // ---------------------------------------------------------------------------
@@ -1056,9 +1069,9 @@ __global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_batch_ker
             const uint32_t s = active ? uint32_t(q / J.nblk) : 0u;
             const uint32_t c = active ? uint32_t(q % J.nblk) : 0u;
             if (J.flags & SI_FB_PER_MARKER)
-                run_block<S, LOG, true, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+                run_block<S, LOG, true, CB, true>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
             else
-                run_block<S, LOG, false, CB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+                run_block<S, LOG, false, CB, true>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
         }
     }
 }
