@@ -401,45 +401,60 @@ def bench_plain(args):
     torch.cuda.empty_cache()
 
     tr = device.Transitions.build(pan.pos, pan.genetic_cM(), ne)
-    job = device.Job(idx, torch.from_numpy(tg.codes).cuda(), synth.HARDCALL_PALETTE, tr, seed=12345)
-    job.alloc_outputs(ds=True, ap=False, hap=True, prob=False, loglik=False)
-    arr = (_native.FbJob * 1)(job.c_job())
-    ws_bytes = int(lib.si_fb_workspace_bytes(arr, 1))
-    ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
-    variant = device.tune_jobs([arr[0]], stream=stream, workspace=ws)
-    var = ctypes.byref(_native.FbVariant(*variant))
-    log(f"[plain] tuned variant: chain {variant[0]} lanes/sample, block {variant[1]} "
-        f"lanes/sample, checkpoint block {variant[2]}")
+    codes = torch.from_numpy(tg.codes).cuda()
 
-    def step(evs=None):
-        if evs:
-            evs[0].record(stream)
-        _native.check(lib.si_fb_run_variant(arr, 1, var, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
-                                            _native.SI_FB_PASS_CHAIN, sh), "chain pass")
-        if evs:
-            evs[1].record(stream)
-        _native.check(lib.si_fb_run_variant(arr, 1, var, ctypes.c_void_p(ws.data_ptr()), ws_bytes,
-                                            _native.SI_FB_PASS_BLOCK, sh), "block pass")
-        if evs:
-            evs[2].record(stream)
+    def time_form(logspace):
+        """Tune, warm up and time the plain two-pass step of one numeric form;
+        returns (ms/step, chain ms, block ms, variant)."""
+        name = "log-space" if logspace else "linear"
+        job = device.Job(idx, codes, synth.HARDCALL_PALETTE, tr, seed=12345, logspace=logspace)
+        job.alloc_outputs(ds=True, ap=False, hap=True, prob=False, loglik=False)
+        arr = (_native.FbJob * 1)(job.c_job())
+        ws_bytes = int(lib.si_fb_workspace_bytes(arr, 1))
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device="cuda")
+        variant = device.tune_jobs([arr[0]], stream=stream, workspace=ws)
+        var = ctypes.byref(_native.FbVariant(*variant))
+        log(f"[plain] {name}: tuned variant: chain {variant[0]} lanes/sample, block {variant[1]} "
+            f"lanes/sample, checkpoint block {variant[2]}")
 
-    for _ in range(args.warmup):
-        step()
-    sync()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
-    log(f"[plain] timing {args.steps} steps")
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(evs[k])
-    sync()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    chain_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
-    block_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
-    log(f"[plain] {ms:.3f} ms/step (chain {chain_ms:.3f} ms, block {block_ms:.3f} ms)")
-    ds = job.out.ds
-    assert torch.isfinite(ds).all().item() and float(ds.min()) >= -1e-6 and float(ds.max()) <= 2 + 1e-5
-    assert int(job.out.hap.max()) <= 1
-    del ws
+        def step(evs=None):
+            if evs:
+                evs[0].record(stream)
+            _native.check(lib.si_fb_run_variant(arr, 1, var, ctypes.c_void_p(ws.data_ptr()),
+                                                ws_bytes, _native.SI_FB_PASS_CHAIN, sh),
+                          "chain pass")
+            if evs:
+                evs[1].record(stream)
+            _native.check(lib.si_fb_run_variant(arr, 1, var, ctypes.c_void_p(ws.data_ptr()),
+                                                ws_bytes, _native.SI_FB_PASS_BLOCK, sh),
+                          "block pass")
+            if evs:
+                evs[2].record(stream)
+
+        for _ in range(args.warmup):
+            step()
+        sync()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+        log(f"[plain] {name}: timing {args.steps} steps")
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            step(evs[k])
+        sync()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        chain_ms = sum(e[0].elapsed_time(e[1]) for e in evs) / args.steps
+        block_ms = sum(e[1].elapsed_time(e[2]) for e in evs) / args.steps
+        log(f"[plain] {name}: {ms:.3f} ms/step (chain {chain_ms:.3f} ms, block {block_ms:.3f} ms)")
+        ds = job.out.ds
+        assert torch.isfinite(ds).all().item() and float(ds.min()) >= -1e-6 and float(ds.max()) <= 2 + 1e-5
+        assert int(job.out.hap.max()) <= 1
+        del ws, job
+        torch.cuda.empty_cache()
+        return ms, chain_ms, block_ms, variant
+
+    # the default form (linear when every switch probability is >= 1e-30), then
+    # the log-space form the workload also names
+    ms, chain_ms, block_ms, variant = time_form(False)
+    lms, lchain_ms, lblock_ms, lvariant = time_form(True)
 
     units = B * M
     block_flops = 12.0 * L * units   # forward replay 4L + backward 4L + posterior/dosage 4L
@@ -456,6 +471,15 @@ def bench_plain(args):
         "passes_ms": {"fb_chain_kernel": chain_ms, "fb_block_kernel": block_ms},
         "variant": {"chain_lanes_per_sample": variant[0], "block_lanes_per_sample": variant[1],
                     "checkpoint_block": variant[2]},
+        "form": "linear FP32 (per-marker sum-normalised forward, FP64 backward scale)",
+        "log_space": {
+            "form": "log-space FP32 (max-normalised, per-state log-sum-exp)",
+            "genotypes_per_s": untyped / (lms * 1e-3),
+            "ms_per_step": lms,
+            "passes_ms": {"fb_chain_kernel": lchain_ms, "fb_block_kernel": lblock_ms},
+            "variant": {"chain_lanes_per_sample": lvariant[0],
+                        "block_lanes_per_sample": lvariant[1], "checkpoint_block": lvariant[2]},
+        },
         "roofline": {
             "bound": "valu",
             "kernel": dom + " (plain 20,000 x 100,000 workload)",
