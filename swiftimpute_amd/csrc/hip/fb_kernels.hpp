@@ -19,12 +19,13 @@
 //     the floor enters exactly as f = exp(-999 - Bl_m) once b reaches it.
 //     exp/log run once per (sample, marker), not per state.
 //   * log-space FP32: forward log vector normalized to maximum 0 (as the
-//     reference normalizes, so its -999 floor weighs the same), its log mass
-//     ls = LSE(alpha) carried beside it; backward log vector relative to an
-//     FP64 offset; every step is a three-term log-sum-exp
-//     LSE(-999, To + ls, U + alpha_j) per state, U = log(e^Td - e^To), whose
-//     first two terms are shared by all states (one exp and one log per state).
-//     Valid for any To, including the -999 of coincident markers.
+//     reference normalizes, so its -999 floor weighs the same); backward log
+//     vector relative to an FP64 offset; every step is a three-term
+//     log-sum-exp LSE(-999, To + ls, U + alpha_j) per state (ls = LSE(alpha),
+//     U = log(e^Td - e^To)), whose first two terms are shared by all states:
+//     with the exponentials of the step's own log-mass sum it costs one log
+//     per state (an exp and a log where the shared terms underflow).  Valid
+//     for any To, including the -999 of coincident markers.
 // Both take emissions relative to the best class present at the marker (so a
 // marker where every state mismatches does not push FP32 values towards
 // -1000, where their resolution is 6e-5) and carry that offset in FP64.
@@ -287,17 +288,13 @@ __device__ __forceinline__ void vsel(bool c, float (&dst)[R], const float (&src)
 }
 
 // ---------------------------------------------------------------- forward --
-// Marker 0.  Returns the log normalizer (log-likelihood increment); ls is the
-// log mass of the normalized log-space vector (unused by the linear form,
-// whose vector sums to 1).
+// Marker 0.  Returns the log normalizer (log-likelihood increment).
 template <class S, bool LOG>
-__device__ __forceinline__ double fwd_init(float (&a)[S::R], float& ls, const float (&e)[S::R],
-                                           float off) {
+__device__ __forceinline__ double fwd_init(float (&a)[S::R], const float (&e)[S::R], float off) {
     if constexpr (LOG) {
         const float c = vmax<S>(e);
 #pragma unroll
         for (int i = 0; i < S::R; ++i) a[i] = e[i] - c;
-        ls = log_mass<S>(a);
         return double(off) + double(c);
     } else {
         const float Sm = vsum<S>(e);
@@ -308,20 +305,50 @@ __device__ __forceinline__ double fwd_init(float (&a)[S::R], float& ls, const fl
     }
 }
 
-// Step into a marker with emission e / offset off and coefficients k:
-// k = (t_o, u) for the linear form, (lto, lu) for the log form.
+// Step coefficients: (t_o, u) for the linear form, (t_o, u, lto, lu) for the
+// log form (t_o = e^lto, u = e^lu).
+template <bool LOG>
+struct CoefT {
+    using type = float2;
+};
+template <>
+struct CoefT<true> {
+    using type = float4;
+};
+
+// The log form's three-term LSE(floor, switch term, u e^{a_i}) over the
+// states is ln(F + K w_i), with w_i = e^{a_i - max} the exponentials the
+// step forms anyway for its log mass, whenever the state-independent part F
+// is at least 2^-30: the two terms are then added in linear space (no
+// cancellation; a w_i that underflows (a_i < -87) drops a part below
+// 2^-96 < e^-66 F) and each state costs one log instead of an exp and a log.
+// Below that (coincident markers, or a floor-dominated scale whose switch
+// term underflows) the exact three-term form is used.
+constexpr float kLseLinMin = 9.3132257e-10f;  // 2^-30
+
+// Step into a marker with emission e / offset off and coefficients k.  In
+// the log form the vector a is max-normalized (its largest entry is 0).
 template <class S, bool LOG>
-__device__ __forceinline__ double fwd_step(float (&a)[S::R], float& ls, const float (&e)[S::R],
-                                           float off, float2 k) {
+__device__ __forceinline__ double fwd_step(float (&a)[S::R], const float (&e)[S::R], float off,
+                                           typename CoefT<LOG>::type k) {
     float x[S::R];
     if constexpr (LOG) {
-        const LsePair q = lse_pair(kFloor, k.x + ls);  // LSE_i(a_i + To) = To + ls
+        float w[S::R];
 #pragma unroll
-        for (int i = 0; i < S::R; ++i) x[i] = lse_with(q, k.y + a[i]) + e[i];
+        for (int i = 0; i < S::R; ++i) w[i] = fast_exp(a[i]);
+        const float Wm = vsum<S>(w);  // e^{log mass}
+        const float F = k.x * Wm;     // LSE_i(a_i + To), in linear space
+        if (F >= kLseLinMin) {
+#pragma unroll
+            for (int i = 0; i < S::R; ++i) x[i] = fast_ln(fmaf(k.y, w[i], F)) + e[i];
+        } else {
+            const LsePair q = lse_pair(kFloor, k.z + fast_ln(Wm));
+#pragma unroll
+            for (int i = 0; i < S::R; ++i) x[i] = lse_with(q, k.w + a[i]) + e[i];
+        }
         const float c = vmax<S>(x);
 #pragma unroll
         for (int i = 0; i < S::R; ++i) a[i] = x[i] - c;
-        ls = log_mass<S>(a);
         return double(off) + double(c);
     } else {
         // a sums to 1, so sum_i a_i t_o = t_o
@@ -343,7 +370,7 @@ __device__ __forceinline__ double fwd_step(float (&a)[S::R], float& ls, const fl
 // shifted down by sh = -999 - nat, so no value ever leaves a safe range.
 template <class S, bool LOG>
 __device__ __forceinline__ void bwd_step(float (&v)[S::R], double& Bl, const float (&e)[S::R],
-                                         float off, float2 k) {
+                                         float off, typename CoefT<LOG>::type k) {
     float y[S::R];
     if constexpr (LOG) {
 #pragma unroll
@@ -352,17 +379,29 @@ __device__ __forceinline__ void bwd_step(float (&v)[S::R], double& Bl, const flo
         float w[S::R];
 #pragma unroll
         for (int i = 0; i < S::R; ++i) w[i] = fast_exp(y[i] - mx);
-        const float lw = mx + fast_ln(vsum<S>(w));
+        const float Sw = vsum<S>(w);
+        const float lw = mx + fast_ln(Sw);
         const double nat = Bl + double(off) + double(lw);
         const double arg = -999.0 - nat;
         const bool floor_dominates = arg > 0.0;
-        const float fl = floor_dominates ? 0.0f : float(arg);
-        const float sh = floor_dominates ? float(arg) : 0.0f;
         Bl = floor_dominates ? -999.0 : nat;
-        const LsePair q = lse_pair(fl, k.x - sh);
-        const float bu = k.y - lw - sh;
+        // v_i = LSE(fl, To - sh, U - sh + y_i - lw) with (fl, sh) = (arg, 0),
+        // or (0, arg) when the floor dominates; with z = e^{-|arg|} that is
+        // ln(F + K w_i), F = z + t_o or 1 + t_o z, K = u / Sw or u z / Sw.
+        const float z = fast_exp(float(-fabs(arg)));
+        const float F = floor_dominates ? fmaf(k.x, z, 1.0f) : z + k.x;
+        if (F >= kLseLinMin) {
+            const float K = (floor_dominates ? k.y * z : k.y) * __builtin_amdgcn_rcpf(Sw);
 #pragma unroll
-        for (int i = 0; i < S::R; ++i) v[i] = lse_with(q, bu + y[i]);
+            for (int i = 0; i < S::R; ++i) v[i] = fast_ln(fmaf(K, w[i], F));
+        } else {
+            const float fl = floor_dominates ? 0.0f : float(arg);
+            const float sh = floor_dominates ? float(arg) : 0.0f;
+            const LsePair q = lse_pair(fl, k.z - sh);
+            const float bu = k.w - lw - sh;
+#pragma unroll
+            for (int i = 0; i < S::R; ++i) v[i] = lse_with(q, bu + y[i]);
+        }
     } else {
 #pragma unroll
         for (int i = 0; i < S::R; ++i) y[i] = e[i] * v[i];
@@ -418,7 +457,7 @@ template <class S, bool LOG, bool PM>
 struct BlockIn {
     uint4 cw;
     LaneWords<S> sa[kSub];
-    float2 k[PM ? kSub : 1];
+    typename CoefT<LOG>::type k[PM ? kSub : 1];
     __device__ __forceinline__ void load(const JobView& J, const uint8_t* codes, uint32_t r,
                                          uint32_t m0) {
         const uint32_t ml = J.M - 1u;
@@ -437,10 +476,12 @@ struct BlockIn {
 #pragma unroll
             for (uint32_t t = 0; t < kSub; ++t) {
                 const float4 q = J.steps[min(m0 + t, ml)];
-                k[t] = LOG ? make_float2(q.z, q.w) : make_float2(q.x, q.y);
+                if constexpr (LOG) k[t] = q;
+                else k[t] = make_float2(q.x, q.y);
             }
         } else {
-            k[0] = LOG ? make_float2(J.step.z, J.step.w) : make_float2(J.step.x, J.step.y);
+            if constexpr (LOG) k[0] = J.step;
+            else k[0] = make_float2(J.step.x, J.step.y);
         }
     }
     __device__ __forceinline__ uint32_t code(uint32_t t) const {
@@ -455,7 +496,7 @@ template <class S, bool LOG, bool PM>
 struct BlockEm {
     float e[kSub][S::R];
     float off[kSub];
-    float2 k[PM ? kSub : 1];
+    typename CoefT<LOG>::type k[PM ? kSub : 1];
     __device__ __forceinline__ void fill(const Tab& tab, uint32_t P, const BlockIn<S, LOG, PM>& in,
                                          uint32_t sub) {
 #pragma unroll
@@ -464,7 +505,9 @@ struct BlockEm {
 #pragma unroll
         for (uint32_t t = 0; t < (PM ? kSub : 1); ++t) k[t] = in.k[t];
     }
-    __device__ __forceinline__ float2 coef(uint32_t t) const { return k[PM ? t : 0]; }
+    __device__ __forceinline__ typename CoefT<LOG>::type coef(uint32_t t) const {
+        return k[PM ? t : 0];
+    }
 };
 
 // R consecutive floats of a checkpoint vector.
@@ -534,12 +577,11 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
         const uint32_t nw = J.W ? J.nwin : 1u;
         cur.load(J, codes, r, 0);
         float a[S::R];
-        float ls = 0.0f;
         double ll;
         {
             float e[S::R];
             const float off = lane_emission<S>(tab, P, cur.code(0), cur.sa[0], sub, e);
-            ll = fwd_init<S, LOG>(a, ls, e, off);
+            ll = fwd_init<S, LOG>(a, e, off);
         }
         // One flat walk over the load blocks of every (window, pass), so the
         // next block's inputs are always in flight: pass 0 (windows after the
@@ -563,7 +605,7 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
             if (lo < m0 && m0 + kSub - 1u <= hi) {  // every step inside: no selects
 #pragma unroll
                 for (uint32_t t = 0; t < kSub; ++t) {
-                    const double inc = fwd_step<S, LOG>(a, ls, em.e[t], em.off[t], em.coef(t));
+                    const double inc = fwd_step<S, LOG>(a, em.e[t], em.off[t], em.coef(t));
                     if (sum_ll) ll += inc;
                     if (t == 0 && ck) store_vec<S::R>(ckp, a);
                 }
@@ -574,11 +616,9 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
                     float an[S::R];
 #pragma unroll
                     for (int i = 0; i < S::R; ++i) an[i] = a[i];
-                    float lsn = ls;
-                    const double inc = fwd_step<S, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
+                    const double inc = fwd_step<S, LOG>(an, em.e[t], em.off[t], em.coef(t));
                     const bool live = m > lo && m <= hi;
                     vsel(live, a, an);
-                    ls = live ? lsn : ls;
                     ll += (live && sum_ll) ? inc : 0.0;
                     if (t == 0 && ck) store_vec<S::R>(ckp, a);
                 }
@@ -590,10 +630,7 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
                     store_vec<S::R>(fwp, a);
                     pass = 1u;
                 } else {  // carry F_w into the next window
-                    if (w > 0u) {
-                        load_vec<S::R>(fwp, a);
-                        if (LOG) ls = log_mass<S>(a);
-                    }
+                    if (w > 0u) load_vec<S::R>(fwp, a);
                     ++w;
                     pass = 0u;
                     lo = w * W;
@@ -602,7 +639,7 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
             }
             c = nc;
         }
-        if (LOG) ll += double(ls);  // log mass of the last normalized vector
+        if (LOG) ll += double(log_mass<S>(a));  // log mass of the last normalized vector
         if (r == 0 && J.loglik) J.loglik[s] = ll;
     } else {
         float v[S::R];
@@ -689,7 +726,6 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
     // so the step loops issue no loads of their own)
     uint32_t tws = 0xffffffffu;
     float fwv[S::R];
-    float fls = 0.0f;
 #pragma unroll
     for (int i = 0; i < S::R; ++i) fwv[i] = 0.0f;
     if (J.W) {
@@ -697,7 +733,6 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
         if (wi * J.W < m0 + nm) {
             tws = wi * J.W - m0;
             load_vec<S::R>(J.ck_win + (size_t(s) * J.nwin + wi) * S::Lp + size_t(r) * S::R, fwv);
-            if (LOG) fls = log_mass<S>(fwv);
         }
     }
     float v[S::R];
@@ -714,7 +749,6 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
         if (active) {
             float a[S::R];
             load_vec<S::R>(J.ck_alpha + ck * S::Lp + size_t(r) * S::R, a);
-            float ls = LOG ? log_mass<S>(a) : 0.0f;  // as the chain pass derived it
             in.load(J, codes, r, m0);
             // replay through the sub-blocks before j
             for (uint32_t u = 0; u < uint32_t(j); ++u) {
@@ -726,13 +760,10 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                     float an[S::R];
 #pragma unroll
                     for (int i = 0; i < S::R; ++i) an[i] = a[i];
-                    float lsn = ls;
-                    fwd_step<S, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
+                    fwd_step<S, LOG>(an, em.e[t], em.off[t], em.coef(t));
                     const bool live = tt > 0 && tt < nm;
                     vsel(live, a, an);
-                    ls = live ? lsn : ls;
                     vsel(tt == tws, a, fwv);
-                    ls = tt == tws ? fls : ls;
                 }
             }
             em.fill(tab, P, in, sub);
@@ -754,19 +785,16 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                     }
                     const uint32_t tt = base + t;
                     if (FULL && t > 0) {
-                        fwd_step<S, LOG>(a, ls, em.e[t], em.off[t], em.coef(t));
+                        fwd_step<S, LOG>(a, em.e[t], em.off[t], em.coef(t));
                     } else if (!(FULL && NS == 1)) {  // with one sub-block, step 0 is never taken
                         float an[S::R];
 #pragma unroll
                         for (int i = 0; i < S::R; ++i) an[i] = a[i];
-                        float lsn = ls;
-                        fwd_step<S, LOG>(an, lsn, em.e[t], em.off[t], em.coef(t));
+                        fwd_step<S, LOG>(an, em.e[t], em.off[t], em.coef(t));
                         const bool live = tt > 0 && tt < nm;
                         vsel(live, a, an);
-                        ls = live ? lsn : ls;
                         if constexpr (!FULL) {
                             vsel(tt == tws, a, fwv);
-                            ls = tt == tws ? fls : ls;
                         }
                     }
                     store_vec<S::R>(prow + t * Lds::T, a);
