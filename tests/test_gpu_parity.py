@@ -296,6 +296,29 @@ def test_fb_coincident_markers_logspace():
         run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=False)
 
 
+def test_fb_logspace_switch_terms_around_linear_bound():
+    # The log-space steps add the shared switch (and floor) terms in linear
+    # space when they are at least 2^-30 (e^-20.8) and use the exact
+    # three-term form below that.  Switch terms spread across that bound, per
+    # marker, make one chain alternate between the two forms; both must stay
+    # within the FP64 oracle's tolerance.
+    rng = np.random.default_rng(31)
+    M = 400
+    pan = synth.make_panel(256, M, seed=31)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = hardcall_codes(rng, 6, M, typed_every=3)
+    To = rng.uniform(-24.0, -17.0, M - 1).astype(np.float32)
+    Td = np.log1p(-8.0 * np.exp(To.astype(np.float64))).astype(np.float32)
+    co = device.step_coefs(Td, To)
+    full = np.zeros((M, 4), np.float32)
+    full[1:] = co
+    tr = device.Transitions(True, co[0], torch.from_numpy(full).cuda(), Td, To)
+    assert (np.exp(To.astype(np.float64)) < 2.0 ** -30).any()
+    assert (np.exp(To.astype(np.float64)) > 2.0 ** -30).any()
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=True)
+    check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, True)
+
+
 @pytest.mark.parametrize("M", [1, 2, 15, 16, 17, 33])
 def test_fb_marker_count_edges(M):
     rng = np.random.default_rng(M)
