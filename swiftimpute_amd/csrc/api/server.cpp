@@ -391,8 +391,21 @@ struct si_server {
                     throw std::runtime_error(std::string("region sweep failed: ") + si_last_error());
             }
         }
-        if (si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch) != SI_OK ||
-            si_fb_batch_run(sl.batch, arena.ptr(ws_off), ws_total, 0, sl.stream) != SI_OK)
+        if (si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch) != SI_OK)
+            throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
+        // Default lanes per sample, not the autotuner: a batch of a cohort-size
+        // bucket not seen before would otherwise time every candidate inside
+        // its requests' latency, and on these shapes the tuner keeps the
+        // defaults anyway.  Outputs do not depend on the variant.
+        for (uint32_t k = 0; k < si_fb_batch_n_classes(sl.batch); ++k) {
+            uint32_t lp = 0, flags = 0;
+            int tuned = 0;
+            si_fb_variant v{};
+            if (si_fb_batch_class_info(sl.batch, k, &lp, &flags, &v, &tuned) != SI_OK ||
+                si_fb_default_variant(lp, &v) != SI_OK || si_fb_batch_set_variant(sl.batch, k, &v) != SI_OK)
+                throw std::runtime_error(std::string("batch variant failed: ") + si_last_error());
+        }
+        if (si_fb_batch_run(sl.batch, arena.ptr(ws_off), ws_total, 0, sl.stream) != SI_OK)
             throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
         for (auto& r : take_ref) {
             const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
