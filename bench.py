@@ -215,6 +215,7 @@ def bench_batch(args):
 # --------------------------------------------------------------------------
 def bench_serve(args):
     import numpy as np
+    import torch
 
     from swiftimpute_amd import serving, synth
 
@@ -228,13 +229,16 @@ def bench_serve(args):
     codes = [synth.region_codes(pool.codes, sp) for _, sp in reqs]
     untyped = sum(int((c == synth.CODE_MISSING).sum()) for c in codes)
     del pool
-    # the client's result arrays, allocated and touched before the clock starts
-    outs = []
+    # the client's result arrays, page-locked (the server copies into them
+    # from the device directly), allocated and touched before the clock starts
+    outs, pinned = [], []
     for _, sp in reqs:
-        h = np.empty((sp.B, 2, sp.M), np.uint8)
-        d = np.empty((sp.B, sp.M), np.float32)
+        ht = torch.empty((sp.B, 2, sp.M), dtype=torch.uint8, pin_memory=True)
+        dt = torch.empty((sp.B, sp.M), dtype=torch.float32, pin_memory=True)
+        h, d = ht.numpy(), dt.numpy()
         h.fill(0)
         d.fill(0)
+        pinned.append((ht, dt))
         outs.append((h, d))
     log("[serve] starting the server (PBWT index build on the device)")
     t0 = time.perf_counter()

@@ -67,7 +67,9 @@ int si_server_submit(si_server* s, uint32_t m0, uint32_t m1, uint32_t B, const u
 
 /* The same, with the results written straight into caller buffers hap
  * [B][2][m1-m0] and ds [B][m1-m0] (either may be NULL), which must stay valid
- * until si_server_wait returns for this request; saves one host copy. */
+ * until si_server_wait returns for this request; saves one host copy.  A
+ * buffer in page-locked memory (hipHostMalloc) receives the device copy
+ * itself, without passing through the server's staging. */
 int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
                           const uint8_t* codes, uint64_t seed, uint8_t* hap, float* ds,
                           uint64_t* request_id);

@@ -51,6 +51,17 @@ void host_parallel(size_t n, size_t bytes, F&& fn) {
     for (auto& x : th) x.join();
 }
 
+// Page-locked host memory (hipHostMalloc, torch's pin_memory): results are
+// copied from the device into it directly, without the staging copy.
+bool host_pinned(const void* p) {
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory: clear the error
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 void par_memcpy(void* dst, const void* src, size_t bytes) {
     constexpr size_t kPiece = 4u << 20;
     const size_t pieces = (bytes + kPiece - 1) / kPiece;
@@ -126,6 +137,7 @@ struct Request {
     uint8_t* dst_hap = nullptr;  // caller buffers (si_server_submit_into), else hap/ds above
     float* dst_ds = nullptr;
     bool into = false;
+    bool hap_pinned = false, ds_pinned = false;  // caller buffers in page-locked memory
     bool done = false;
     int status = 0;
 };
@@ -286,8 +298,9 @@ struct si_server {
             for (auto& r : sl->reqs) {
                 const size_t hb = size_t(r->B) * 2 * r->M, n = size_t(r->B) * r->M;
                 if (r->into) {
-                    if (r->dst_hap) par_memcpy(r->dst_hap, sl->pinned + o, hb);
-                    if (r->dst_ds) par_memcpy(r->dst_ds, sl->pinned + o + hb, n * sizeof(float));
+                    if (r->dst_hap && !r->hap_pinned) par_memcpy(r->dst_hap, sl->pinned + o, hb);
+                    if (r->dst_ds && !r->ds_pinned)
+                        par_memcpy(r->dst_ds, sl->pinned + o + hb, n * sizeof(float));
                 } else {
                     r->hap.resize(hb);
                     r->ds.resize(n);
@@ -409,11 +422,12 @@ struct si_server {
             throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
         for (auto& r : take_ref) {
             const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
-            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_codes + r->off_hap), hb, hipMemcpyDeviceToHost,
-                                 sl.stream);
+            // into the staging, or straight into page-locked caller buffers
+            void* dh = r->hap_pinned ? static_cast<void*>(r->dst_hap) : static_cast<void*>(sl.pinned + o);
+            (void)hipMemcpyAsync(dh, arena.ptr(r->off_codes + r->off_hap), hb, hipMemcpyDeviceToHost, sl.stream);
             o += hb;
-            (void)hipMemcpyAsync(sl.pinned + o, arena.ptr(r->off_codes + r->off_ds), db, hipMemcpyDeviceToHost,
-                                 sl.stream);
+            void* dd = r->ds_pinned ? static_cast<void*>(r->dst_ds) : static_cast<void*>(sl.pinned + o);
+            (void)hipMemcpyAsync(dd, arena.ptr(r->off_codes + r->off_ds), db, hipMemcpyDeviceToHost, sl.stream);
             o += db;
         }
         if (hipEventRecord(sl.done, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
@@ -587,6 +601,8 @@ int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
     r->into = hap != nullptr || ds != nullptr;
     r->dst_hap = hap;
     r->dst_ds = ds;
+    r->hap_pinned = hap && host_pinned(hap);
+    r->ds_pinned = ds && host_pinned(ds);
     r->codes.resize(size_t(B) * r->pitch);
     // rows copied into the padded layout while checking the codes and counting
     // the untyped ones (branch-free, so the scan vectorizes)
