@@ -320,10 +320,10 @@ struct CoefT<true> {
 // states is ln(F + K w_i), with w_i = e^{a_i - max} the exponentials the
 // step forms anyway for its log mass, whenever the state-independent part F
 // is at least 2^-30: the two terms are then added in linear space (no
-// cancellation; a w_i that underflows (a_i < -87) drops a part below
-// 2^-96 < e^-66 F) and each state costs one log instead of an exp and a log.
-// Below that (coincident markers, or a floor-dominated scale whose switch
-// term underflows) the exact three-term form is used.
+// cancellation; a w_i that underflows, a_i < -87, drops a term smaller than
+// 2^-96 F) and each state costs one log instead of an exp and a log.  Below
+// that (coincident markers, where t_o underflows) the exact three-term form
+// is used.
 constexpr float kLseLinMin = 9.3132257e-10f;  // 2^-30
 
 // Step into a marker with emission e / offset off and coefficients k.  In
