@@ -75,17 +75,19 @@ struct Shape {
 };
 
 // Block-pass LDS of one lane group: posteriors [16][T] floats (row t at
-// t*T, group stride PS = 16 T + E floats), 16 Philox outputs and (see
-// below) the state bytes.  Up to 16
-// states the rows carry no padding (T = Lp), which keeps more workgroups
-// resident; the group padding E makes the vector writes and reads of the
-// walks free of bank conflicts on 32- and 64-bank models, and the sampling
-// phase's 16-byte row reads keep 2- to 4-way conflicts (measured, cheaper
-// than padding or rotating the rows).  With 32 or 64 states T = Lp + 4,
-// conflict-free for the sampling reads too.
+// t*T, group stride PS = 16 T + E floats), 16 Philox outputs (in registers
+// for one-lane groups) and the sampler's state bits (below).  Its size sets
+// how many workgroups a CU holds, which bounds these latency-bound kernels,
+// so up to 16 states the rows carry no padding (T = Lp).  The group padding
+// E makes the walks' vector writes and reads free of bank conflicts on 32-
+// and 64-bank models for 4 and 8 states; 16 states go without it (2-way
+// conflicts) so that 4 workgroups of 16 states on 4 lanes fit a CU.  The
+// sampling phase's 16-byte row reads keep 2- to 4-way conflicts (measured,
+// cheaper than padding or rotating the rows).  With 32 or 64 states
+// T = Lp + 4, conflict-free for the sampling reads too.
 __host__ __device__ constexpr int pbuf_T(int Lp) { return Lp <= 16 ? Lp : Lp + 4; }
 __host__ __device__ constexpr int pbuf_E(int Lp) {
-    return Lp == 4 ? 4 : Lp == 8 ? 8 : Lp == 16 ? 16 : Lp == 32 ? 32 : 0;
+    return Lp == 4 ? 4 : Lp == 8 ? 8 : Lp == 32 ? 32 : 0;
 }
 template <class S>
 struct BlockLds {
@@ -98,7 +100,10 @@ struct BlockLds {
     // item has just read.  Elsewhere, and in the batched kernel (whose
     // classes share the caches), the load latency costs more than it gains.
     static constexpr bool kSbPlain = !(S::Lp == 16 && S::G == 4);
-    static constexpr int SW = int(kSub) * S::Lp / 4 + 1;  // words per group
+    // 4 bits per state (alleles, missing flags): 16 Lp nibbles = 2 Lp words
+    static constexpr int SW = int(kSub) * S::Lp / 8;
+    // Philox outputs of a one-lane group stay in its registers
+    static constexpr bool kRngLds = S::G > 1;
     static_assert(T % 4 == 0 && PS % 4 == 0, "16-byte aligned rows");
 };
 
@@ -692,6 +697,21 @@ __device__ __forceinline__ void run_chain(const JobView& J, const Tab& tab, uint
     }
 }
 
+// The sampler's 4 bits of a state byte (bits 0-1 alleles, 4-5 missing
+// flags) as a nibble: alleles in bits 0-1, missing flags in bits 2-3.  nib4
+// packs the 4 state bytes of a word into 16 bits, nib2 the 2 of a half-word
+// into 8 (state k's nibble at bit 4k).
+__device__ __forceinline__ uint32_t nib4(uint32_t w) {
+    uint32_t x = (w & 0x03030303u) | ((w >> 2) & 0x0C0C0C0Cu);
+    x |= x >> 4;
+    return (x & 0xFFu) | ((x >> 8) & 0xFF00u);
+}
+__device__ __forceinline__ uint32_t nib2(uint32_t h) {
+    uint32_t x = (h & 0x0303u) | ((h >> 2) & 0x0C0Cu);
+    x |= x >> 4;
+    return x & 0xFFu;
+}
+
 // ---------------------------------------------------------------------------
 // Checkpoint block c of sample s of job J, by one lane group.  For each
 // 16-marker sub-block j (last first): replay the forward vector from the
@@ -743,6 +763,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
     }
     BlockIn<S, LOG, PM> in;
     BlockEm<S, LOG, PM> em;
+    uint4 rl[Lds::kRngLds ? 1 : kSub / 4];  // a one-lane group's Philox outputs
     for (int j = int(NS) - 1; j >= 0; --j) {
         const uint32_t base = uint32_t(j) * kSub;  // sub-block start within the block
         float ap0r[kPer], ap1r[kPer];
@@ -775,13 +796,12 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
 #pragma unroll
                 for (uint32_t t = 0; t < kSub; ++t) {
                     if constexpr (!SB) {
-                    } else if constexpr (S::R >= 4) {
-#pragma unroll
-                        for (int w = 0; w < S::NW; ++w)
-                            sb[t * (S::Lp / 4) + r * S::NW + w] = in.sa[t].w[w];
+                    } else if constexpr (S::R == 4) {
+                        reinterpret_cast<uint16_t*>(sb)[t * (S::Lp / 4) + r] = uint16_t(nib4(in.sa[t].w[0]));
                     } else {
-                        reinterpret_cast<uint16_t*>(sb)[t * (S::Lp / 2) + r] =
-                            uint16_t(in.sa[t].w[0] >> (8 * sub));
+                        static_assert(S::R == 2, "block pass lane shapes");
+                        reinterpret_cast<uint8_t*>(sb)[t * (S::Lp / 2) + r] =
+                            uint8_t(nib2(in.sa[t].w[0] >> (8 * sub)));
                     }
                     const uint32_t tt = base + t;
                     if (FULL && t > 0) {
@@ -835,9 +855,16 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
             };
             if (nm == uint32_t(CB) && tws == 0xffffffffu) walk(std::true_type{});
             else walk(std::false_type{});
-            for (uint32_t q = r; q < kSub / 4; q += S::G)
-                rng[q] = philox4x32_10(
-                    make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
+            if constexpr (Lds::kRngLds) {
+                for (uint32_t q = r; q < kSub / 4; q += S::G)
+                    rng[q] = philox4x32_10(
+                        make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
+            } else {
+#pragma unroll
+                for (uint32_t q = 0; q < kSub / 4; ++q)
+                    rl[q] = philox4x32_10(
+                        make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
+            }
         }
         wave_sync();
         if (active) {
@@ -864,7 +891,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                         }
                     }
                 } else {  // first k with u < cumsum, else 0 (padding states add 0)
-                    const float uu = u24((&rng[t >> 2].x)[t & 3]);
+                    const float uu = u24(Lds::kRngLds ? (&rng[t >> 2].x)[t & 3] : (&rl[t >> 2].x)[t & 3]);
                     float cs = 0.0f;
                     uint32_t kd = 0xffffffffu;
 #pragma unroll
@@ -883,8 +910,15 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                 if (J.ds) J.ds[o] = ap0r[q] + ap1r[q];
                 if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0r[q], ap1r[q]);
                 if (J.hap) {
-                    const uint32_t b = SB ? uint32_t(reinterpret_cast<const uint8_t*>(sb)[t * S::Lp + ks])
-                                          : uint32_t(J.sa[size_t(m) * S::Lp + ks]);
+                    // bits: alleles at 0-1, missing flags at 4-5 (the state byte's)
+                    uint32_t b;
+                    if (SB) {
+                        const uint32_t nb =
+                            (reinterpret_cast<const uint8_t*>(sb)[t * (S::Lp / 2) + (ks >> 1)] >> (4 * (ks & 1))) & 15u;
+                        b = (nb & 3u) | ((nb & 12u) << 2);
+                    } else {
+                        b = J.sa[size_t(m) * S::Lp + ks];
+                    }
                     J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 16u) ? 255u : (b & 1u));
                     J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 32u) ? 255u : ((b >> 1) & 1u));
                 }
@@ -927,7 +961,7 @@ __global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_kernel(Jo
     __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
     constexpr bool SB = Lds::kSbPlain;
     __shared__ uint32_t sbuf[S::GPB * (SB ? Lds::SW : 1)];
-    __shared__ uint4 rng[S::GPB * (kSub / 4)];
+    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
     const Tab tab = load_tables<LOG>(tb);
     __syncthreads();
     const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
@@ -937,7 +971,7 @@ __global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_kernel(Jo
     const uint32_t c = active ? grp % J.nblk : 0u;
     float* pb = pbuf + gl * Lds::PS;
     uint32_t* sb = sbuf + (SB ? gl * Lds::SW : 0u);
-    uint4* rg = rng + gl * (kSub / 4);
+    uint4* rg = rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u);
     if (J.flags & SI_FB_PER_MARKER)
         run_block<S, LOG, true, CB, SB>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
     else
@@ -1066,14 +1100,14 @@ __global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_batch_ker
     using Lds = BlockLds<S>;
     __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
     __shared__ uint32_t sbuf[S::GPB * Lds::SW];
-    __shared__ uint4 rng[S::GPB * (kSub / 4)];
+    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
     const Tab tab = load_tables<LOG>(tb);
     __syncthreads();
     const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
     const uint32_t gl = threadIdx.x / G;
     float* pb = pbuf + gl * Lds::PS;
     uint32_t* sb = sbuf + gl * Lds::SW;
-    uint4* rg = rng + gl * (kSub / 4);
+    uint4* rg = rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u);
     uint64_t lo = 0, hi = 0;  // item range of the job in J (wavefront-uniform)
     JobView J{};
     for (;;) {
