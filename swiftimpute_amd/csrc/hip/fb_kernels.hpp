@@ -102,8 +102,10 @@ struct BlockLds {
     static constexpr bool kSbPlain = !(S::Lp == 16 && S::G == 4);
     // 4 bits per state (alleles, missing flags): 16 Lp nibbles = 2 Lp words
     static constexpr int SW = int(kSub) * S::Lp / 8;
-    // Philox outputs of a one-lane group stay in its registers
-    static constexpr bool kRngLds = S::G > 1;
+    // Philox outputs of a one- or two-lane group of 4 states per lane stay in
+    // its registers (two lanes swap theirs by DPP); with 2 states per lane
+    // the register budget is the tighter limit
+    static constexpr bool kRngLds = S::G > 2 || S::R < 4;
     static_assert(T % 4 == 0 && PS % 4 == 0, "16-byte aligned rows");
 };
 
@@ -763,7 +765,7 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
     }
     BlockIn<S, LOG, PM> in;
     BlockEm<S, LOG, PM> em;
-    uint4 rl[Lds::kRngLds ? 1 : kSub / 4];  // a one-lane group's Philox outputs
+    uint4 rl[Lds::kRngLds ? 1 : kSub / 4];  // Philox outputs of a one- or two-lane group
     for (int j = int(NS) - 1; j >= 0; --j) {
         const uint32_t base = uint32_t(j) * kSub;  // sub-block start within the block
         float ap0r[kPer], ap1r[kPer];
@@ -859,11 +861,24 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                 for (uint32_t q = r; q < kSub / 4; q += S::G)
                     rng[q] = philox4x32_10(
                         make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
-            } else {
+            } else if constexpr (S::G == 1) {
 #pragma unroll
                 for (uint32_t q = 0; q < kSub / 4; ++q)
                     rl[q] = philox4x32_10(
                         make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
+            } else {
+                // lane r forms blocks r and r + 2, then takes its partner's
+                // (blocks 1 - r and 3 - r): rl[q] holds block q on both lanes
+#pragma unroll
+                for (uint32_t h = 0; h < 2; ++h) {
+                    const uint4 own = philox4x32_10(
+                        make_uint4(((m0 + base) >> 2) + r + 2u * h, J.sample_base + s, J.rng_stream, 0u),
+                        key);
+                    const uint4 oth = make_uint4(dpp_u32<0xB1>(own.x), dpp_u32<0xB1>(own.y),
+                                                 dpp_u32<0xB1>(own.z), dpp_u32<0xB1>(own.w));
+                    rl[2 * h] = r == 0 ? own : oth;
+                    rl[2 * h + 1] = r == 0 ? oth : own;
+                }
             }
         }
         wave_sync();
@@ -891,7 +906,15 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
                         }
                     }
                 } else {  // first k with u < cumsum, else 0 (padding states add 0)
-                    const float uu = u24(Lds::kRngLds ? (&rng[t >> 2].x)[t & 3] : (&rl[t >> 2].x)[t & 3]);
+                    uint32_t rv;
+                    if constexpr (Lds::kRngLds) {
+                        rv = (&rng[t >> 2].x)[t & 3];
+                    } else {  // block t >> 2, with t = q (one lane) or r + 2q (two lanes)
+                        const uint4 rb = rl[S::G == 1 ? (q >> 2) : (q >> 1)];
+                        const uint32_t c = t & 3u;
+                        rv = c == 0u ? rb.x : c == 1u ? rb.y : c == 2u ? rb.z : rb.w;
+                    }
+                    const float uu = u24(rv);
                     float cs = 0.0f;
                     uint32_t kd = 0xffffffffu;
 #pragma unroll

@@ -39,6 +39,10 @@ __device__ __forceinline__ float u24(uint32_t x) {
 // quad_perm [2,3,0,1] (xor 2), row_half_mirror (i <-> 7-i within 8) and
 // row_mirror (i <-> 15-i within 16).
 template <int CTRL>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
 __device__ __forceinline__ float dpp(float v) {
     return __int_as_float(
         __builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
