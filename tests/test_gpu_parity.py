@@ -296,6 +296,37 @@ def test_fb_coincident_markers_logspace():
         run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=False)
 
 
+@pytest.mark.parametrize("L", [4, 8, 16])
+def test_fb_panel_with_missing_alleles(L):
+    # Missing panel alleles reach the sampled haplotypes as 255 through the
+    # block pass's packed state bits (or, for 16 states on 4 lanes, the state
+    # table read by the sampler): the host replay over the same state
+    # alleles, and every variant, must agree bit for bit.
+    rng = np.random.default_rng(L + 71)
+    M, H = 200, 120
+    ref = (rng.random((M, H)) < 0.4).astype(np.uint8)
+    ref[rng.random((M, H)) < 0.05] = 255
+    idx = gpu_index(ref, L)
+    a0, a1 = idx.state_alleles()
+    assert (a0 == 255).any() and (a1 == 255).any()
+    codes = hardcall_codes(rng, 8, M, typed_every=3)
+    g = np.arange(1, M + 1, dtype=np.uint64) * 2000
+    tr = device.Transitions.build(g, None, 10000, per_marker=True)
+    res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
+    check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, True)
+    assert (res["hap"] == 255).any()
+    pitch = (M + 15) // 16 * 16
+    c = np.full((8, pitch), 3, np.uint8)
+    c[:, :M] = codes
+    for v in device.variants(L):
+        job = device.Job(idx, torch.from_numpy(c).cuda(), synth.HARDCALL_PALETTE, tr, seed=12345)
+        job.alloc_outputs(ds=True, ap=False, hap=True, prob=False, loglik=False)
+        device.run_jobs([job.c_job()], variant=v)
+        torch.cuda.synchronize()
+        assert np.array_equal(job.out.hap.cpu().numpy(), res["hap"]), v
+        assert np.array_equal(job.out.ds.cpu().numpy(), res["ds"]), v
+
+
 def test_fb_logspace_switch_terms_around_linear_bound():
     # The log-space steps add the shared switch (and floor) terms in linear
     # space when they are at least 2^-30 (e^-20.8) and use the exact
