@@ -81,10 +81,12 @@ int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
 int si_server_wait(si_server* s, uint64_t request_id, uint8_t* hap, float* ds, double* latency_s);
 
 /* Counters since creation: completed requests, batches, imputed genotypes
- * (untyped (sample, marker) pairs), largest batch, peak admitted bytes. */
+ * (untyped (sample, marker) pairs), largest batch, peak admitted bytes,
+ * admissions deferred for memory, and requests whose results the device
+ * copied straight into page-locked caller buffers (si_server_submit_into). */
 typedef struct si_server_stats {
     uint64_t requests, batches, genotypes, untyped_genotypes, max_batch_requests;
-    uint64_t peak_bytes, deferred_admissions;
+    uint64_t peak_bytes, deferred_admissions, direct_copies;
 } si_server_stats;
 int si_server_get_stats(si_server* s, si_server_stats* out);
 

@@ -92,6 +92,14 @@ class DeviceError : public ImputationError {
 };
 using CUDAError = DeviceError;
 
+// A shape or mode this build does not handle (SI_ERR_UNSUPPORTED from the C
+// ABI).  No counterpart in the reference; it derives from ImputationError so
+// that callers catching the reference's hierarchy still see it.
+class UnsupportedError : public ImputationError {
+  public:
+    explicit UnsupportedError(const std::string& msg) : ImputationError(msg) {}
+};
+
 // Minimal leveled logger (the reference's Logger, src/core/types.hpp:253-283).
 class Logger {
   public:

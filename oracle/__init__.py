@@ -43,6 +43,8 @@ def lib():
             build()
         L = ctypes.CDLL(_LIB_PATH)
         L.or_pbwt_build.argtypes = [_u8p, ctypes.c_uint32, ctypes.c_uint32, _u32p, _u32p]
+        L.or_pbwt_sweep_window.argtypes = [_u8p, ctypes.c_uint32, ctypes.c_uint32,
+                                           ctypes.c_uint32, _u32p, _u32p, _u32p]
         L.or_select_perm.argtypes = [ctypes.c_uint32, ctypes.c_uint32, _u32p]
         L.or_select_states_sort.argtypes = [_u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
         L.or_state_table.argtypes = [_u8p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
@@ -111,6 +113,30 @@ def pbwt_build(ref: np.ndarray):
     _chk(lib().or_pbwt_build(_p(ref, _u8p), M, H, _p(prefix, _u32p), _p(div, _u32p)),
          "or_pbwt_build")
     return prefix, div
+
+
+class PbwtStream:
+    """or_pbwt_build over a panel fed window by window (pbwt_index.cpp:153-208):
+    ``next(ref_window)`` returns the window's (prefix, div) rows, carrying the
+    order into the next window, so full-size panels are checked without their
+    [M][H] arrays on the host."""
+
+    def __init__(self, H: int):
+        self.H = H
+        self.m = 0
+        self.order = np.arange(H, dtype=np.uint32)
+
+    def next(self, ref: np.ndarray, with_div: bool = True):
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        M, H = ref.shape
+        assert H == self.H
+        prefix = np.empty((M, H), np.uint32)
+        div = np.empty((M, H), np.uint32) if with_div else None
+        _chk(lib().or_pbwt_sweep_window(_p(ref, _u8p), M, H, self.m, _p(self.order, _u32p),
+                                        _p(prefix, _u32p), _p(div, _u32p)),
+             "or_pbwt_sweep_window")
+        self.m += M
+        return prefix, div
 
 
 # --------------------------------------------------------------------------- A2/A4

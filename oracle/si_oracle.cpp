@@ -118,6 +118,36 @@ int or_pbwt_build(const uint8_t* ref, uint32_t M, uint32_t H,
 }
 
 // ---------------------------------------------------------------------------
+// A1, streamed: the same sweep over markers [m_base, m_base + M) of a panel,
+// continuing from `order` (the prefix row after marker m_base - 1, or the
+// identity when m_base == 0), which is updated to the row after the window's
+// last marker.  Divergence entries are the global marker index
+// (pbwt_index.cpp:173-192 reset p/q to m after every push).  ref holds the
+// window's M rows [M][H]; prefix/div receive the window's rows (div may be
+// null).  Splitting a panel into windows gives or_pbwt_build's rows exactly.
+// ---------------------------------------------------------------------------
+int or_pbwt_sweep_window(const uint8_t* ref, uint32_t M, uint32_t H, uint32_t m_base,
+                         uint32_t* order, uint32_t* prefix, uint32_t* div) {
+    if (H == 0 || !order) return -1;
+    std::vector<uint32_t> z, o;
+    z.reserve(H); o.reserve(H);
+    for (uint32_t m = 0; m < M; ++m) {
+        z.clear(); o.clear();
+        const uint8_t* col = ref + size_t(m) * H;
+        for (uint32_t i = 0; i < H; ++i) {
+            const uint32_t hap = order[i];
+            if (col[hap] == 0) z.push_back(hap);
+            else o.push_back(hap);
+        }
+        std::memcpy(order, z.data(), sizeof(uint32_t) * z.size());
+        std::memcpy(order + z.size(), o.data(), sizeof(uint32_t) * o.size());
+        std::memcpy(prefix + size_t(m) * H, order, sizeof(uint32_t) * H);
+        if (div) std::fill(div + size_t(m) * H, div + size_t(m + 1) * H, m_base + m);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // A2: the permutation libstdc++'s std::sort applies to H candidates with equal
 // keys under the reference comparator (a.first > b.first,
 // src/pbwt/pbwt_index.cpp:30-33).  perm[k] = original position of the element

@@ -31,6 +31,7 @@ void si_check(int rc, const char* what) {
     if (rc == SI_OK) return;
     const std::string msg = std::string(what) + ": " + si_last_error();
     if (rc == SI_ERR_DEVICE) throw DeviceError(msg, rc);
+    if (rc == SI_ERR_UNSUPPORTED) throw UnsupportedError(msg);
     throw ImputationError(msg);
 }
 
@@ -43,8 +44,20 @@ T* dev_alloc(size_t count, size_t& total) {
     return static_cast<T*>(p);
 }
 
+// Frees p (allocated by dev_alloc with `count` elements) and takes its bytes
+// off the running total.
 template <class T>
-void dev_free(T*& p) {
+void dev_free(T*& p, size_t count, size_t& total) {
+    if (p) {
+        (void)hipFree(p);
+        total -= std::min(total, count * sizeof(T));
+    }
+    p = nullptr;
+}
+
+// Destructor-time release (no accounting).
+template <class T>
+void dev_drop(T*& p) {
     if (p) (void)hipFree(p);
     p = nullptr;
 }
@@ -88,6 +101,7 @@ struct Imputer::Device {
     // per-batch
     uint32_t cap = 0;
     uint32_t pitch = 0;
+    uint32_t cap_markers = 0;    // markers the per-batch buffers were sized for
     uint8_t* codes = nullptr;
     float* ap = nullptr;
     uint8_t* hap = nullptr;
@@ -101,23 +115,25 @@ struct Imputer::Device {
     float* emax = nullptr;
     float* palette = nullptr;
 
+    uint32_t n_palette = 0;      // palette entries of eps/emax/palette
+
     ~Device() {
-        dev_free(cols);
-        dev_free(miss);
-        dev_free(prefix);
-        dev_free(div);
-        dev_free(states);
-        dev_free(sa);
-        dev_free(mmask);
-        dev_free(codes);
-        dev_free(ap);
-        dev_free(hap);
-        dev_free(loglik);
+        dev_drop(cols);
+        dev_drop(miss);
+        dev_drop(prefix);
+        dev_drop(div);
+        dev_drop(states);
+        dev_drop(sa);
+        dev_drop(mmask);
+        dev_drop(codes);
+        dev_drop(ap);
+        dev_drop(hap);
+        dev_drop(loglik);
         if (ws) (void)hipFree(ws);
-        dev_free(steps);
-        dev_free(eps);
-        dev_free(emax);
-        dev_free(palette);
+        dev_drop(steps);
+        dev_drop(eps);
+        dev_drop(emax);
+        dev_drop(palette);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -160,8 +176,9 @@ void Imputer::build_index() {
     const int mode = config_.hmm_params.use_pbwt_selection ? 0 : 1;
     std::vector<uint32_t> perm(std::min(L, H));
     if (mode == 0) {
-        dev_->prefix = dev_alloc<uint32_t>(size_t(M) * H, dev_->bytes);
-        dev_->div = dev_alloc<uint32_t>(size_t(M) * H, dev_->bytes);
+        // a rebuild reuses arrays a previous build kept
+        if (!dev_->prefix) dev_->prefix = dev_alloc<uint32_t>(size_t(M) * H, dev_->bytes);
+        if (!dev_->div) dev_->div = dev_alloc<uint32_t>(size_t(M) * H, dev_->bytes);
         const size_t wsb = si_pbwt_workspace_bytes(H, M);
         void* ws = nullptr;
         if (wsb) hip_check(hipMalloc(&ws, wsb), "hipMalloc");
@@ -176,9 +193,8 @@ void Imputer::build_index() {
         hip_check(hipStreamSynchronize(dev_->stream), "build_index");
         if (ws) (void)hipFree(ws);
         if (!config_.keep_pbwt_arrays) {
-            dev_free(dev_->prefix);
-            dev_free(dev_->div);
-            dev_->bytes -= 2 * size_t(M) * H * 4;
+            dev_free(dev_->prefix, size_t(M) * H, dev_->bytes);
+            dev_free(dev_->div, size_t(M) * H, dev_->bytes);
         }
     } else {
         si_check(si_state_table(nullptr, dev_->cols, dev_->miss, H, M, L, 1, perm.data(), dev_->states, dev_->sa,
@@ -267,12 +283,15 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
         palf[3 * p + 2] = pal[p].ll_11;
     }
     si_check(si_emission_table(palf.data(), P, eps.data(), emax.data()), "emission table");
-    dev_free(dev_->eps);
-    dev_free(dev_->emax);
-    dev_free(dev_->palette);
-    dev_->eps = dev_alloc<float>(eps.size(), dev_->bytes);
-    dev_->emax = dev_alloc<float>(emax.size(), dev_->bytes);
-    dev_->palette = dev_alloc<float>(palf.size(), dev_->bytes);
+    if (dev_->n_palette != P) {  // tables sized for this palette, reused across calls
+        dev_free(dev_->eps, size_t(dev_->n_palette) * 32, dev_->bytes);
+        dev_free(dev_->emax, size_t(dev_->n_palette) * 8, dev_->bytes);
+        dev_free(dev_->palette, size_t(dev_->n_palette) * 3, dev_->bytes);
+        dev_->eps = dev_alloc<float>(eps.size(), dev_->bytes);
+        dev_->emax = dev_alloc<float>(emax.size(), dev_->bytes);
+        dev_->palette = dev_alloc<float>(palf.size(), dev_->bytes);
+        dev_->n_palette = P;
+    }
     hip_check(hipMemcpyAsync(dev_->palette, palf.data(), palf.size() * 4, hipMemcpyHostToDevice,
                              dev_->stream),
               "upload");
@@ -283,17 +302,18 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     const uint32_t pitch = targets.code_pitch();
     const uint32_t report = std::max<uint32_t>(1, std::min(config_.batch_size, B));  // progress step
     const uint32_t batch = std::min(B, launch_samples(report, B, M, L, config_.window_size, pitch));
-    if (dev_->cap < batch || dev_->pitch != pitch) {
-        dev_free(dev_->codes);
-        dev_free(dev_->ap);
-        dev_free(dev_->hap);
-        dev_free(dev_->loglik);
+    if (dev_->cap < batch || dev_->pitch != pitch || dev_->cap_markers != M) {
+        dev_free(dev_->codes, size_t(dev_->cap) * dev_->pitch, dev_->bytes);
+        dev_free(dev_->ap, size_t(dev_->cap) * dev_->cap_markers * 2, dev_->bytes);
+        dev_free(dev_->hap, size_t(dev_->cap) * 2 * dev_->cap_markers, dev_->bytes);
+        dev_free(dev_->loglik, size_t(dev_->cap), dev_->bytes);
         dev_->codes = dev_alloc<uint8_t>(size_t(batch) * pitch, dev_->bytes);
         dev_->ap = dev_alloc<float>(size_t(batch) * M * 2, dev_->bytes);
         dev_->hap = dev_alloc<uint8_t>(size_t(batch) * 2 * M, dev_->bytes);
         dev_->loglik = dev_alloc<double>(batch, dev_->bytes);
         dev_->cap = batch;
         dev_->pitch = pitch;
+        dev_->cap_markers = M;
     }
     si_fb_job job{};
     job.n_markers = M;
@@ -422,6 +442,8 @@ std::unique_ptr<ImputationResult> impute(const std::string& reference_vcf, const
     Imputer imputer(*reference, config);
     try {
         imputer.build_index();
+    } catch (const UnsupportedError&) {
+        throw;  // a panel this build cannot index is an error, not a fallback
     } catch (const ImputationError& e) {
         Logger::instance().warning(std::string("PBWT index not available, proceeding without it: ") + e.what());
     }

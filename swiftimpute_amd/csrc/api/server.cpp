@@ -7,6 +7,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -194,6 +195,13 @@ struct si_server {
     std::vector<si_fb_batch*> finished;
     si_server_stats stats{};
     std::string worker_error;
+    // Testing aid: when nonzero, the launch of this batch (1-based count)
+    // raises an error on the host, as a failing launch would
+    // (SI_SERVER_FAIL_AT_BATCH, read at creation).  No device work is affected.
+    uint64_t fail_at_batch = 0;
+    // requests whose results the device copied straight into page-locked
+    // caller buffers (counted outside the lock)
+    std::atomic<uint64_t> n_direct{0};
 
     ~si_server() {
         for (auto* b : finished) si_fb_batch_destroy(b);
@@ -404,6 +412,8 @@ struct si_server {
                     throw std::runtime_error(std::string("region sweep failed: ") + si_last_error());
             }
         }
+        if (fail_at_batch && stats.batches == fail_at_batch)
+            throw std::runtime_error("injected launch failure (SI_SERVER_FAIL_AT_BATCH)");
         if (si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch) != SI_OK)
             throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
         // Default lanes per sample, not the autotuner: a batch of a cohort-size
@@ -422,6 +432,7 @@ struct si_server {
             throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
         for (auto& r : take_ref) {
             const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
+            if (r->hap_pinned || r->ds_pinned) n_direct.fetch_add(1, std::memory_order_relaxed);
             // into the staging, or straight into page-locked caller buffers
             void* dh = r->hap_pinned ? static_cast<void*>(r->dst_hap) : static_cast<void*>(sl.pinned + o);
             (void)hipMemcpyAsync(dh, arena.ptr(r->off_codes + r->off_hap), hb, hipMemcpyDeviceToHost, sl.stream);
@@ -474,8 +485,20 @@ struct si_server {
             }
         } catch (const std::exception& e) {
             worker_error = e.what();
+            // Batches still on the device may be copying results straight into
+            // page-locked caller buffers: wait for their streams before any of
+            // their requests is reported, so that no copy lands after
+            // si_server_wait has returned.
+            for (auto& sl : slots)
+                if (sl.state == SlotState::Running) (void)hipStreamSynchronize(sl.stream);
+            // Requests of batches already handed to a copier complete normally.
+            std::vector<const Request*> copying;
+            for (auto& sl : slots)
+                if (sl.state == SlotState::Copying)
+                    for (auto& r : sl.reqs) copying.push_back(r.get());
             for (auto& kv : reqs)
-                if (!kv.second->done) {
+                if (!kv.second->done &&
+                    std::find(copying.begin(), copying.end(), kv.second.get()) == copying.end()) {
                     kv.second->done = true;
                     kv.second->status = SI_ERR_DEVICE;
                 }
@@ -578,6 +601,7 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
             return fail(SI_ERR_DEVICE, "si_server_create: stream creation failed");
+    if (const char* f = std::getenv("SI_SERVER_FAIL_AT_BATCH")) s->fail_at_batch = std::strtoull(f, nullptr, 10);
     si_server* raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
     for (uint32_t i = 0; i < raw->cfg.max_streams; ++i)
@@ -674,6 +698,7 @@ int si_server_get_stats(si_server* s, si_server_stats* out) {
     if (!s || !out) return fail(SI_ERR_ARG, "si_server_get_stats: bad arguments");
     std::lock_guard<std::mutex> g(s->mu);
     *out = s->stats;
+    out->direct_copies = s->n_direct.load(std::memory_order_relaxed);
     return SI_OK;
 }
 

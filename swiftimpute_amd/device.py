@@ -97,6 +97,9 @@ class Transitions:
     steps_dev: torch.Tensor | None   # [M][4] when per_marker
     Td: np.ndarray
     To: np.ndarray
+    # what the constants were built from: (pos, cM, ne, rho_rate), or None
+    # when a caller supplied Td/To directly
+    source: tuple | None = None
 
     @staticmethod
     def build(pos, cM, ne: int, rho_rate: float = 4.0, per_marker: bool = False,
@@ -110,7 +113,7 @@ class Transitions:
             full = np.zeros((len(g), 4), np.float32)
             full[1:] = co
             steps_dev = torch.from_numpy(full).to(device)
-        return Transitions(per_marker, step, steps_dev, Td, To)
+        return Transitions(per_marker, step, steps_dev, Td, To, (pos, cM, ne, rho_rate))
 
     def min_switch(self) -> float:
         """Smallest e^To over the steps the kernel will use."""

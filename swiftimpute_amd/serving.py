@@ -26,7 +26,7 @@ class ServerCfg(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("requests", "batches", "genotypes", "untyped_genotypes",
                                                "max_batch_requests", "peak_bytes",
-                                               "deferred_admissions")]
+                                               "deferred_admissions", "direct_copies")]
 
 
 _SYMBOLS = [

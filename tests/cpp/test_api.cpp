@@ -230,6 +230,15 @@ static int run(int argc, char** argv) {
         CHECK(total == B);
     });
     CHECK(calls == (B + batch - 1) / batch && last == B);
+    // a second call reuses the device buffers: the reported device memory
+    // stays the same and the results are identical
+    const size_t mem1 = imp.device_memory_usage();
+    auto res2 = imp.impute(*tgt);
+    CHECK(imp.device_memory_usage() == mem1);
+    CHECK(std::memcmp(res2->haplotype0(0), res->haplotype0(0), size_t(B) * 2 * M) == 0);
+    // a rebuild of the index keeps the accounting (arrays kept or reused)
+    imp.build_index();
+    CHECK(imp.device_memory_usage() == mem1);
     auto prefix = imp.pbwt_prefix();
     auto div = imp.pbwt_divergence();
     auto st = imp.selected_states();

@@ -185,10 +185,18 @@ def check_fb(idx, codes, palette, trans, res, per_marker, fold=True, sample_base
              argmax=False, tol=DS_TOL):
     a0, a1 = idx.state_alleles()
     gl = palette[codes[:, : idx.M]]
+    Td, To = trans.Td, trans.To
+    if trans.source is not None:
+        # the oracle's own restatement of the map and constants
+        # (imputer.cpp:596-605, transition.cu:83-121), which the library's
+        # must equal bit for bit
+        pos, cM, ne, rho = trans.source
+        Td, To = oracle.transitions(oracle.genetic_map(pos, cM), ne, rho)
+        assert np.array_equal(Td, trans.Td) and np.array_equal(To, trans.To)
     if fold:
-        ref = oracle.fb_f64(a0, a1, gl, trans.Td, trans.To, per_marker=per_marker)
+        ref = oracle.fb_f64(a0, a1, gl, Td, To, per_marker=per_marker)
     else:
-        ref = oracle.fb_f64_fast(a0, a1, gl, trans.Td, trans.To, per_marker=per_marker)
+        ref = oracle.fb_f64_fast(a0, a1, gl, Td, To, per_marker=per_marker)
     err = np.abs(res["ds"] - ref["ds"]).max()
     assert err <= tol, f"max |DS - DS_fp64| = {err:.3g}"
     assert np.abs(res["ap"] - ref["ap"]).max() <= tol
@@ -361,6 +369,7 @@ def test_fb_marker_count_edges(M):
         tr.step = device.step_coefs(np.array([-0.2], np.float32), np.array([-3.0], np.float32))[0]
         tr.Td = np.full(M - 1, -0.2, np.float32)
         tr.To = np.full(M - 1, -3.0, np.float32)
+        tr.source = None  # constants chosen here, not from the map
     res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr)
     check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, False)
 
@@ -390,15 +399,21 @@ def test_fb_batch_split_invariance():
         assert np.array_equal(whole[k], np.concatenate([a[k], b[k]])), k
 
 
-def test_fb_config1_full():
+@pytest.mark.parametrize("logspace", [True, False])
+def test_fb_config1_full(logspace):
     """Config 1 at full size: 4432 haplotypes x 1644 markers, 444 targets,
-    L = 8, Ne = 1e4 (reference semantics: first interval's transitions)."""
+    L = 8, Ne = 1e4 (reference semantics: first interval's transitions), in
+    the log-space FP32 form the workload names and in linear FP32."""
     w = synth.WORKLOADS["cfg1"]
     pan = synth.make_panel(w["H"], w["M"], seed=12345)
     tg = synth.make_targets(pan, w["B"], w["typed_every"], seed=12346)
     idx = device.DeviceIndex(pan.cols, pan.H, w["L"])
+    p, d = oracle.pbwt_build(pan.alleles())
+    assert np.array_equal(idx.prefix.cpu().numpy().view(np.uint32), p)
+    h0, _, _, _ = oracle.state_table(pan.alleles(), p, w["L"])
+    assert np.array_equal(idx.states.cpu().numpy().view(np.uint32), h0)
     tr = device.Transitions.build(pan.pos, None, w["ne"], per_marker=False)
-    res = run_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr)
+    res = run_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr, logspace=logspace)
     check_fb(idx, tg.codes, synth.HARDCALL_PALETTE, tr, res, False, fold=False)
 
 

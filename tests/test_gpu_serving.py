@@ -108,6 +108,10 @@ def test_alone_equals_mix_equals_plain(world):
                 for k, r in enumerate(reqs)]
         ids = [srv.submit_into(r[0], r[1], r[2], h, d, seed=r[3]) for r, (h, d) in zip(reqs, outs)]
         into = [srv.wait(i) for i in ids]
+        st3 = srv.stats()
+    # the page-locked requests (odd k) took the direct device copy, the others
+    # the staging copy
+    assert st3["direct_copies"] == len(keep) == len(reqs) // 2, st3
     for r, a, b, (h, d) in zip(reqs, alone, into, outs):
         assert b.hap is h and b.ds is d
         assert np.array_equal(a.hap, h) and np.array_equal(a.ds, d), r[:2]
@@ -200,3 +204,34 @@ def test_bad_codes_are_refused(world):
         c[2, 17] = 3
         r = srv.wait(srv.submit(100, 400, c))
         assert r.hap.shape == (3, 2, 300)
+
+
+def test_failed_launch_waits_for_in_flight_copies(world, monkeypatch):
+    # The second batch's launch fails on the host (injected); the first batch
+    # may still be on the device, copying into page-locked caller arrays.  No
+    # wait may return before that copy has landed: the arrays must not change
+    # after wait() returns, whatever the request's status.
+    pan, tg = world
+    monkeypatch.setenv("SI_SERVER_FAIL_AT_BATCH", "2")
+    big = np.ascontiguousarray(np.repeat(tg.codes[:, : pan.M], 40, axis=0))  # 1,920 samples
+    B, M = big.shape
+    ht = torch.full((B, 2, M), 9, dtype=torch.uint8, pin_memory=True)
+    dt = torch.full((B, M), -1.0, dtype=torch.float32, pin_memory=True)
+    with serving.Server(pan.cols, pan.H, pan.pos, pan.genetic_cM(), L=L, ne=NE, max_streams=2,
+                        max_batch=1) as srv:
+        first = srv.submit_into(0, M, big, ht.numpy(), dt.numpy(), seed=3)
+        second = srv.submit(0, 500, np.ascontiguousarray(tg.codes[:4, :500]), seed=4)
+        statuses = []
+        for rid in (first, second):
+            try:
+                srv.wait(rid)
+                statuses.append(0)
+            except _native.NativeError:
+                statuses.append(1)
+        snap_h, snap_d = ht.numpy().copy(), dt.numpy().copy()
+        import time
+        time.sleep(0.3)
+        assert np.array_equal(ht.numpy(), snap_h) and np.array_equal(dt.numpy(), snap_d)
+        assert statuses[1] == 1  # the injected failure is reported
+        with pytest.raises(_native.NativeError, match="injected"):
+            srv.submit(0, 100, np.ascontiguousarray(tg.codes[:2, :100]))

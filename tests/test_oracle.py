@@ -63,6 +63,23 @@ def test_pbwt_brute_force(seed, M, H):
         assert sorted(prefix[m].tolist()) == list(range(H))
 
 
+@pytest.mark.parametrize("windows", [[1, 1, 28], [5, 7, 18], [30], [3, 0, 27]])
+def test_pbwt_stream_equals_whole_build(windows):
+    # the streamed sweep (window by window, order carried) gives the rows of
+    # the whole-panel build, with global divergence indices
+    rng = np.random.default_rng(sum(windows) + len(windows))
+    ref = (rng.random((30, 77)) < 0.45).astype(np.uint8)
+    ref[rng.random((30, 77)) < 0.03] = 255
+    p, d = oracle.pbwt_build(ref)
+    st = oracle.PbwtStream(77)
+    m = 0
+    for w in windows:
+        pw, dw = st.next(ref[m:m + w])
+        assert np.array_equal(pw, p[m:m + w]) and np.array_equal(dw, d[m:m + w])
+        m += w
+    assert st.m == 30 and np.array_equal(st.order, p[-1])
+
+
 # ------------------------------------------------------------------ selection
 def test_select_perm_known_answers():
     # Prefixes of the permutation libstdc++'s std::sort applies to H equal keys.
@@ -541,3 +558,33 @@ def test_sample_from_probs_closed_form():
     assert top < 1.0
     assert oracle.sample_from_probs(q, np.nextafter(np.float32(top), np.float32(1))) == 0
     assert oracle.argmax(np.array([0.1, 0.4, 0.4, 0.1], np.float32)) == 1
+
+
+# ------------------------------------------------------------ CPU baseline
+@pytest.mark.parametrize("seed,L", [(3, 4), (4, 8)])
+def test_cpu_reference_impute_is_the_reference_loop(seed, L):
+    # The timed CPU baseline (PBWT sweep, per-(sample, marker) std::sort
+    # selection, L^2 FP32 fold with sqrt(M) checkpoint replay, inverse-CDF
+    # draw) computes what the reference computes: its haplotypes equal the
+    # draw replayed over the FP32 reference-order forward-backward (which keeps
+    # every forward vector) on the state table the PBWT selects.
+    rng = np.random.default_rng(seed)
+    M, H, B = 60, 40, 5
+    ref = (rng.random((M, H)) < 0.4).astype(np.uint8)
+    gl = np.zeros((B, M, 3), np.float32)
+    typed = rng.random((B, M)) < 0.3
+    call = rng.integers(0, 3, (B, M))
+    for g in range(3):
+        sel = typed & (call == g)
+        gl[sel] = -999.0
+        gl[sel, g] = 0.0
+    pos = (np.arange(M, dtype=np.uint64) + 1) * 7000
+    Td, To = oracle.transitions(oracle.genetic_map(pos), 10000)
+    hap = oracle.cpu_reference_impute(ref, gl, Td, To, L, 99, nthreads=2)
+    p, _ = oracle.pbwt_build(ref)
+    _, _, a0, a1 = oracle.state_table(ref, p, L)
+    f = oracle.fb_f32_faithful(a0, a1, gl, Td, To, per_marker=False)
+    want = oracle.sample_replay(f["prob"], a0, a1, 99)
+    assert np.array_equal(hap, want)
+    # and the draw is not trivially constant: the posteriors spread over states
+    assert (f["prob"].max(axis=2) < 0.999).any()
