@@ -253,6 +253,21 @@ int si_fb_run_passes(const si_fb_job* jobs, uint32_t n_jobs, void* workspace,
 int si_fb_run_variant(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_variant* variant,
                       void* workspace, size_t workspace_bytes, uint32_t passes, void* stream);
 
+/* Allocation-free form of si_fb_run_variant (both passes), for callers that
+ * cannot allocate device memory: code under stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  No counterpart in the reference.
+ * The checkpoint workspace is a static device array of
+ * SI_FB_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until the
+ * process ends; jobs whose si_fb_workspace_bytes exceed it are refused with
+ * SI_ERR_WORKSPACE.  The call allocates nothing and runs on the current
+ * device.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (other streams, other host threads) race, so serialize them.
+ * Nothing in the array needs to start cleared.  variant may be NULL (the
+ * default).  Outputs are identical to si_fb_run's. */
+#define SI_FB_NOALLOC_SCRATCH_BYTES (64u << 20)
+int si_fb_run_noalloc(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_variant* variant,
+                      void* stream);
+
 /* Autotuner for the plain form: times every candidate variant on these jobs
  * (one padded-state class, one numeric space), each job cut to its first
  * max_markers markers (0: 2048) with every output disabled, so the jobs'

@@ -33,6 +33,7 @@ SI_FB_LOGSPACE = 4
 SI_FB_PASS_CHAIN = 1
 SI_FB_PASS_BLOCK = 2
 SI_PERM_FIXED = 0x80000000
+SI_FB_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 class StepCoef(ctypes.Structure):
@@ -118,6 +119,8 @@ HIP_SYMBOLS = [
     ("si_fb_run_variant", ctypes.c_int, [ctypes.POINTER(FbJob), ctypes.c_uint32,
                                          ctypes.POINTER(FbVariant), _vp, ctypes.c_size_t,
                                          ctypes.c_uint32, _vp]),
+    ("si_fb_run_noalloc", ctypes.c_int, [ctypes.POINTER(FbJob), ctypes.c_uint32,
+                                         ctypes.POINTER(FbVariant), _vp]),
     ("si_fb_tune", ctypes.c_int, [ctypes.POINTER(FbJob), ctypes.c_uint32, _vp, ctypes.c_size_t,
                                   ctypes.c_uint32, _vp, ctypes.POINTER(FbVariant)]),
     ("si_fb_batch_n_classes", ctypes.c_uint32, [_vp]),

@@ -335,6 +335,63 @@ int si_fb_run_variant(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_varian
 
 }  // extern "C"
 
+// --------------------------------------------- forward-backward, no allocation --
+// The checkpoint workspace of si_fb_run_noalloc: one array per device, resident
+// from the library's load until the process ends.  In host code `scratch`
+// names a host-side stand-in, not device memory: only the address returned
+// by hipGetSymbolAddress for the current device may reach a kernel or a copy.
+namespace {
+static __device__ __align__(256) char scratch[SI_FB_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// This device's address of `scratch`, looked up once per device (with that
+// device current) and kept.
+int scratch_address(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(SI_ERR_UNSUPPORTED, "si_fb_run_noalloc: device %d outside 0..%d", dev,
+                    kMaxDevices - 1);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SI_ERR_DEVICE, "si_fb_run_noalloc: hipGetSymbolAddress failed on device %d: %s",
+                        dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SI_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int si_fb_run_noalloc(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_variant* variant,
+                      void* stream) {
+    if (n_jobs && !jobs) return fail(SI_ERR_ARG, "si_fb_run_noalloc: jobs is null");
+    const size_t need = si_fb_workspace_bytes(jobs, n_jobs);
+    if (need > SI_FB_NOALLOC_SCRATCH_BYTES)
+        return fail(SI_ERR_WORKSPACE,
+                    "si_fb_run_noalloc: these jobs need %zu bytes of checkpoint workspace, above the "
+                    "%u-byte static scratch; use si_fb_run with a caller workspace",
+                    need, unsigned(SI_FB_NOALLOC_SCRATCH_BYTES));
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess)
+        return fail(SI_ERR_DEVICE, "si_fb_run_noalloc: hipGetDevice failed");
+    void* ws = nullptr;
+    const int rc = scratch_address(dev, &ws);
+    if (rc) return rc;
+    // every checkpoint the block pass reads is written by the chain pass of
+    // the same call, so nothing needs clearing between calls
+    return si_fb_run_variant(jobs, n_jobs, variant, ws, SI_FB_NOALLOC_SCRATCH_BYTES,
+                             SI_FB_PASS_CHAIN | SI_FB_PASS_BLOCK, stream);
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------- autotuner --
 namespace {
 
