@@ -24,7 +24,13 @@ swiftimpute_amd.synth; every input is resident in HBM before a timed region):
   plain  -- 20,000 x 100,000 panel, 10,000 targets typed every 20th marker,
          L = 16, Ne = 20,000, one job per launch: the per-kernel roofline
          (chain and block passes timed with HIP events on their stream), the
-         PBWT index build, and the CPU baseline.
+         log-space step's transcendental rate, and the PBWT index build.
+  cfg1   -- 4,432 x 1,644 panel, 444 targets typed every 4th marker, L = 8,
+         FP32 log-space: end to end through the C++ API (VCF in, VCF out,
+         timed per stage) beside the reference's published 24.8 s, the
+         kernel alone, and the CPU baseline on the whole cohort.
+  precision -- 8,000 x 100,000 panel with heavy-tailed spacing: linear and
+         log-space FP32 against the FP64 oracle for Ne in {1e3, 1e4, 1e5}.
 
 Imputed genotype = one (target sample, untyped marker) pair.
 """
@@ -43,6 +49,27 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 vector (= FP32 matrix) peak
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+# v_exp_f32 / v_log_f32 issue at a quarter of the FP32 lane rate:
+# 256 CUs x 64 lanes x 2.4 GHz / 4 per second
+PEAK_TRANSCENDENTAL_TPS = 256 * 64 * 2.4e9 / 4 / 1e12
+
+
+def host_threads():
+    """Worker threads for the CPU baselines: the CPUs this process may run on
+    (its affinity mask), at most 16 -- a one-GPU share of the GPU box."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def host_info():
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        aff = None
+    return {"os_cpu_count": os.cpu_count(), "affinity_cpus": aff}
 
 
 def log(msg):
@@ -103,7 +130,7 @@ def bench_batch(args):
                                      sp.seed, tables))
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         log(f"[batch] CPU baseline: reference CPU path on a sample of the jobs, {threads} threads")
         v, dt, n, parts = cpu_baseline_jobs(pan, pool.codes, specs, ne, args.cpu_jobs,
                                             args.cpu_samples, args.cpu_markers, threads)
@@ -112,8 +139,26 @@ def bench_batch(args):
                          f"{args.cpu_samples} samples x {args.cpu_markers} markers "
                          f"({', '.join(parts)}; {n} imputed genotypes in {dt:.1f} s): PBWT sweep "
                          "+ per-(sample, marker) std::sort selection + L^2 FP32 forward-backward "
-                         "with sqrt(M) replay + sampling, threaded over samples"}
+                         "with sqrt(M) replay + sampling, threaded over samples",
+               "host": host_info()}
         log(f"[batch] CPU baseline {v:.4g} genotypes/s")
+        # the reference runs single-threaded: the same path on one thread, on
+        # the first job cut to 4 samples
+        v1, dt1, n1, parts1 = cpu_baseline_jobs(pan, pool.codes, specs, ne, 1, 4,
+                                                args.cpu_markers, 1)
+        cpu["single_thread"] = {"value": v1, "cores": 1,
+                                "sample": f"{parts1[0]} ({n1} imputed genotypes in {dt1:.1f} s)"}
+        log(f"[batch] CPU baseline, one thread: {v1:.4g} genotypes/s")
+        # the stronger CPU form: selection once per marker (it is
+        # sample-independent) and the O(M L) forward-backward
+        vh, dth, nh, _ = cpu_baseline_jobs(pan, pool.codes, specs, ne, args.cpu_jobs,
+                                           args.cpu_samples, args.cpu_markers, threads,
+                                           hoisted=True)
+        cpu["hoisted"] = {"value": vh, "cores": threads,
+                          "sample": f"the same sample ({nh} imputed genotypes in {dth:.2f} s): "
+                                    "PBWT sweep + one state table per marker + rank-1 FP64 "
+                                    "forward-backward + sampling"}
+        log(f"[batch] CPU baseline, hoisted: {vh:.4g} genotypes/s")
     del pool
     cjobs = [j.c_job() for j in jobs]
     n_log = sum(j.logspace for j in jobs)
@@ -177,6 +222,31 @@ def bench_batch(args):
         f"(workspace {batch.ws_bytes / 2**30:.2f} GiB)")
     batch_ms = time_graph(args.steps)
     log(f"[batch] batched {batch_ms:.1f} ms/step")
+    # the graph launch timed with HIP events on its own stream
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    evs[0].record(stream)
+    for _ in range(args.steps):
+        batch.run(use_graph=True, stream=stream)
+    evs[1].record(stream)
+    sync()
+    graph_ms = evs[0].elapsed_time(evs[1]) / args.steps
+    flops = sum(12.0 * sp.L * sp.B * sp.M for sp in specs)
+    per_class = {}
+    for sp in specs:
+        per_class.setdefault(device.padded_states(sp.L), 0.0)
+        per_class[device.padded_states(sp.L)] += 12.0 * sp.L * sp.B * sp.M
+    step_roofline = {
+        "bound": "valu", "kernel": "batched step: one graph launch holding every state class's "
+                                   "chain and block kernels",
+        "achieved": flops / (graph_ms * 1e-3) / 1e12, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": flops / (graph_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+        "graph_ms": graph_ms, "flops_per_step": flops,
+        "flops_per_class": {str(k): v for k, v in sorted(per_class.items())},
+        "algorithmic": "sum over jobs of 12*L*B*M FLOP (forward 4L, backward 4L, posterior and "
+                       "dosage 4L per (sample, marker))",
+    }
+    log(f"[batch] graph launch {graph_ms:.1f} ms: {step_roofline['achieved']:.2f} TFLOP/s "
+        f"algorithmic ({100 * step_roofline['frac']:.1f} % of FP32 peak)")
     equal = all(torch.equal(j.out[k], r[k]) for j, r in zip(jobs, ref) for k in r)
     if not equal:
         raise SystemExit("batched outputs differ from the plain path")
@@ -205,7 +275,7 @@ def bench_batch(args):
         "cpu_baseline": cpu,
         "units": units, "untyped": untyped, "batch_ms": batch_ms, "plain_ms": plain_ms,
         "plain_steps": plain_steps, "n_jobs": n_jobs, "n_logspace_jobs": n_log,
-        "autotune": autotune,
+        "autotune": autotune, "step_roofline": step_roofline,
         "L_counts": {str(L): sum(sp.L == L for sp in specs) for L in (4, 8, 16)},
     }
 
@@ -333,10 +403,13 @@ def cpu_baseline(pan, tg, L, ne, n_samples, n_markers, threads):
     return untyped / dt, dt, untyped
 
 
-def cpu_baseline_jobs(pan, pool_codes, specs, ne, n_jobs, max_samples, max_markers, threads):
+def cpu_baseline_jobs(pan, pool_codes, specs, ne, n_jobs, max_samples, max_markers, threads,
+                      hoisted=False):
     """The reference CPU path on a bounded sample of a job set: the first
     n_jobs jobs, each cut to its first max_samples samples and first
-    max_markers markers of its region."""
+    max_markers markers of its region.  hoisted: the stronger CPU form
+    (PBWT sweep, one state table per marker, rank-1 FP64 forward-backward,
+    sampling) on the same sample."""
     import numpy as np
 
     import oracle
@@ -354,7 +427,13 @@ def cpu_baseline_jobs(pan, pool_codes, specs, ne, n_jobs, max_samples, max_marke
         g = oracle.genetic_map(pan.pos[sp.m0:sp.m0 + Mc], None if cm is None else cm[sp.m0:sp.m0 + Mc])
         Td, To = oracle.transitions(g, ne)
         t0 = time.perf_counter()
-        oracle.cpu_reference_impute(ref, gl, Td, To, sp.L, sp.seed, nthreads=threads)
+        if hoisted:
+            p, _ = oracle.pbwt_build(ref)
+            _, _, a0, a1 = oracle.state_table(ref, p, sp.L)
+            o = oracle.fb_f64_fast(a0, a1, gl, Td, To, want_prob=True, nthreads=threads)
+            oracle.sample_replay(o["prob"].astype(np.float32), a0, a1, sp.seed)
+        else:
+            oracle.cpu_reference_impute(ref, gl, Td, To, sp.L, sp.seed, nthreads=threads)
         total_dt += time.perf_counter() - t0
         untyped += int((c == synth.CODE_MISSING).sum())
         parts.append(f"{Bc}x{Mc} L={sp.L}")
@@ -461,11 +540,16 @@ def bench_plain(args):
     lms, lchain_ms, lblock_ms, lvariant = time_form(True)
 
     units = B * M
-    block_flops = 12.0 * L * units   # forward replay 4L + backward 4L + posterior/dosage 4L
-    chain_flops = 8.0 * L * units    # forward 4L + backward 4L
+    # The HMM's algorithmic count is 12*L FLOP per (sample, marker): forward
+    # 4L and backward 4L (the chain pass) and posterior + dosage 4L (the block
+    # pass, whose forward replay and backward re-walk are not counted).
+    chain_flops = 8.0 * L * units
+    block_flops = 4.0 * L * units
     dom, dom_ms, dom_flops = (("fb_block_kernel", block_ms, block_flops) if block_ms >= chain_ms
                               else ("fb_chain_kernel", chain_ms, chain_flops))
     achieved = dom_flops / (dom_ms * 1e-3) / 1e12
+    # log-space: one exp per state per pass, 3L transcendentals per unit
+    trans_tps = 3.0 * L * units / (lms * 1e-3) / 1e12
     out = {
         "workload": f"{H} haplotypes x {M} markers panel, {B} targets typed every {te}th marker, "
                     f"L={L}, Ne={ne}, plain one-job-per-launch",
@@ -483,6 +567,24 @@ def bench_plain(args):
             "passes_ms": {"fb_chain_kernel": lchain_ms, "fb_block_kernel": lblock_ms},
             "variant": {"chain_lanes_per_sample": lvariant[0],
                         "block_lanes_per_sample": lvariant[1], "checkpoint_block": lvariant[2]},
+            "transcendental": {
+                "achieved": trans_tps, "peak": PEAK_TRANSCENDENTAL_TPS, "unit": "T/s",
+                "frac": trans_tps / PEAK_TRANSCENDENTAL_TPS,
+                "algorithmic": "3*L exp per (sample, marker) over the whole step; peak = 256 CUs "
+                               "x 64 lanes x 2.4 GHz / 4 (quarter-rate v_exp_f32)"},
+        },
+        "pass_rooflines": {
+            "fb_chain_kernel": {"flops": chain_flops, "ms": chain_ms,
+                                "achieved": chain_flops / (chain_ms * 1e-3) / 1e12,
+                                "frac": chain_flops / (chain_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                                "algorithmic": "8*L FLOP per unit (forward 4L, backward 4L)"},
+            "fb_block_kernel": {"flops": block_flops, "ms": block_ms,
+                                "achieved": block_flops / (block_ms * 1e-3) / 1e12,
+                                "frac": block_flops / (block_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                                "algorithmic": "4*L FLOP per unit (posterior and dosage)"},
+            "whole_step": {"flops": 12.0 * L * units, "ms": ms,
+                           "achieved": 12.0 * L * units / (ms * 1e-3) / 1e12,
+                           "frac": 12.0 * L * units / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS},
         },
         "roofline": {
             "bound": "valu",
@@ -492,7 +594,7 @@ def bench_plain(args):
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": load_traffic(dom),
-            "algorithmic": f"{12 if dom == 'fb_block_kernel' else 8}*L FLOP per (sample, marker)"
+            "algorithmic": f"{4 if dom == 'fb_block_kernel' else 8}*L FLOP per (sample, marker)"
                            f" x {units} units per launch",
         },
         "pbwt": {
@@ -508,7 +610,7 @@ def bench_plain(args):
         },
     }
     if not args.no_cpu_baseline and "batch" not in args.sections.split(","):
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         ns, nm = min(args.cpu_samples, B), min(args.cpu_markers, M)
         log(f"[plain] CPU baseline: reference CPU path on {ns} samples x {nm} markers, "
             f"{threads} threads")
@@ -601,6 +703,123 @@ def bench_precision(args):
     return out
 
 
+# --------------------------------------------------------------------------
+# cfg1: the reference's published run's shape, end to end and kernel-only
+# --------------------------------------------------------------------------
+REF_PUBLISHED_S = 24.789          # TEST_RESULTS.md:32 (NVIDIA GB10, load -> write)
+REF_PUBLISHED_GPS = 14700.0       # README.md:165 (genotypes/s)
+
+
+def bench_cfg1(args):
+    """4,432 haplotypes x 1,644 markers, 444 targets typed every 4th marker,
+    L = 8, Ne = 1e4, FP32 log-space, Philox seed 12345.  End to end: the
+    synthetic panel and cohort written as VCF text (untimed), then the C++
+    API's impute(ref_vcf, tgt_vcf, out_vcf, config) path timed per stage
+    (load, index, impute, write), as the reference's executable reports it.
+    Kernel-only: the same job on device-resident inputs, timed with HIP events.
+    CPU: the reference loop structure on the whole cohort."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from swiftimpute_amd import _native, device, synth
+
+    w = synth.WORKLOADS["cfg1"]
+    H, M, B, L, ne, te = w["H"], w["M"], w["B"], w["L"], w["ne"], w["typed_every"]
+    log(f"[cfg1] generating panel H={H} M={M} and {B} targets typed every {te}")
+    pan = synth.make_panel(H, M, seed=12345)
+    tg = synth.make_targets(pan, B, te, seed=12346, with_truth=False)
+    untyped = int((tg.codes[:, :M] == synth.CODE_MISSING).sum())
+    out = {"workload": f"{H} haplotypes x {M} markers, {B} targets typed every {te}th marker, "
+                       f"L={L}, Ne={ne}, FP32 log-space, Philox seed 12345",
+           "untyped_genotypes": untyped}
+    lib = _native.api()
+    with tempfile.TemporaryDirectory(prefix="si_cfg1_") as d:
+        rv, tv, ov = (os.path.join(d, n) for n in ("panel.vcf", "targets.vcf", "out.vcf"))
+        synth.write_panel_vcf(rv, pan)
+        synth.write_targets_vcf(tv, pan, tg)
+        opts = _native.ImputeOpts()
+        lib.si_impute_defaults(ctypes.byref(opts))
+        opts.num_states, opts.ne, opts.precision = L, ne, 2
+        runs = []
+        for k in range(2):
+            t = _native.ImputeTiming()
+            rc = lib.si_impute_vcf(rv.encode(), tv.encode(), ov.encode(), ctypes.byref(opts),
+                                   ctypes.byref(t))
+            if rc != 0:
+                raise SystemExit(f"si_impute_vcf failed ({rc}): {lib.si_impute_last_error().decode()}")
+            runs.append({n: getattr(t, n) for n, _ in _native.ImputeTiming._fields_})
+            log(f"[cfg1] end to end ({'first' if k == 0 else 'second'} call): "
+                f"{t.total_s * 1e3:.1f} ms (load {1e3 * (t.reference_load_s + t.target_load_s):.1f}, "
+                f"index {1e3 * t.index_s:.1f}, impute {1e3 * t.impute_s:.1f}, write {1e3 * t.write_s:.1f})")
+        assert runs[0]["untyped_genotypes"] == untyped
+        out_lines = sum(1 for l in open(ov) if not l.startswith("#"))
+        assert out_lines == M
+    e2e = runs[0]
+    out["end_to_end"] = {
+        "stages_s_first_call": {k: e2e[k] for k in ("reference_load_s", "target_load_s", "index_s",
+                                                   "impute_s", "write_s", "total_s")},
+        "stages_s_second_call": {k: runs[1][k] for k in ("reference_load_s", "target_load_s",
+                                                        "index_s", "impute_s", "write_s", "total_s")},
+        "genotypes_per_s": untyped / e2e["total_s"],
+        "reference_published": {"total_s": REF_PUBLISHED_S, "genotypes_per_s": REF_PUBLISHED_GPS,
+                                "imputation_stage_s": 24.133,
+                                "note": "NVIDIA GB10, Phylos panel of the same shape with 50% of "
+                                        "genotypes masked at random (this cohort masks 75%); "
+                                        "TEST_RESULTS.md:26-40"},
+        "speedup_vs_published_total": REF_PUBLISHED_S / e2e["total_s"],
+        "speedup_vs_published_imputation_stage": 24.133 / e2e["impute_s"],
+        "what": "the C++ API's impute(ref_vcf, tgt_vcf, out_vcf, config) via si_impute_vcf: VCF "
+                "parsing, device index build, impute() with its uploads and downloads, VCF "
+                "writing (GT, DS, GP, AP, INFO)",
+    }
+    # kernel-only: the plain job on resident inputs
+    idx = device.DeviceIndex(pan.cols, H, L, keep_prefix=False, with_div=False)
+    tr = device.Transitions.build(pan.pos, None, ne)
+    job = device.Job(idx, torch.from_numpy(tg.codes).cuda(), synth.HARDCALL_PALETTE, tr,
+                     seed=12345, logspace=True)
+    job.alloc_outputs(ds=True, ap=True, hap=True, prob=False, loglik=False)
+    arr = (_native.FbJob * 1)(job.c_job())
+    wsb = int(_native.hip().si_fb_workspace_bytes(arr, 1))
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = ctypes.c_void_p(stream.cuda_stream)
+    hip = _native.hip()
+
+    def step():
+        _native.check(hip.si_fb_run(arr, 1, ctypes.c_void_p(ws.data_ptr()), wsb, sh), "si_fb_run")
+
+    for _ in range(3):
+        step()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    reps = 20
+    ev[0].record(stream)
+    for _ in range(reps):
+        step()
+    ev[1].record(stream)
+    sync()
+    kms = ev[0].elapsed_time(ev[1]) / reps
+    out["kernel_only"] = {"ms": kms, "genotypes_per_s": untyped / (kms * 1e-3),
+                          "what": "si_fb_run (log-space FP32, both passes) on device-resident "
+                                  "inputs and outputs"}
+    log(f"[cfg1] kernel-only {kms:.3f} ms")
+    if not args.no_cpu_baseline:
+        threads = host_threads()
+        log(f"[cfg1] CPU baseline: reference loop structure on the whole cohort, {threads} threads")
+        v, dt, n = cpu_baseline(pan, tg, L, ne, B, M, threads)
+        v1, dt1, n1 = cpu_baseline(pan, tg, L, ne, 16, M, 1)
+        out["cpu_baseline"] = {
+            "value": v, "unit": "genotypes/s", "cores": threads, "kind": "port",
+            "sample": f"the whole cohort ({n} imputed genotypes in {dt:.1f} s)",
+            "single_thread": {"value": v1, "cores": 1,
+                              "sample": f"first 16 targets, all markers ({n1} imputed genotypes "
+                                        f"in {dt1:.1f} s)"},
+            "host": host_info()}
+        log(f"[cfg1] CPU baseline {v:.4g} genotypes/s ({threads} threads), {v1:.4g} (1 thread)")
+    return out
+
+
 def copy_calibration():
     """A plain device copy of 1 GiB (three launches), the reference for
     scaling the FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc."""
@@ -619,7 +838,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--sections", default="batch,serve,plain,precision")
+    ap.add_argument("--sections", default="batch,serve,plain,cfg1,precision")
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="timed steps of the one-job-per-launch comparison")
@@ -650,7 +869,7 @@ def main():
     res = {}
     for s in sections:
         res[s] = {"batch": bench_batch, "serve": bench_serve, "plain": bench_plain,
-                  "precision": bench_precision}[s](args)
+                  "cfg1": bench_cfg1, "precision": bench_precision}[s](args)
         torch.cuda.empty_cache()
 
     out = {
@@ -689,6 +908,7 @@ def main():
                 "outputs_bit_equal": True,
             },
             "autotune": b["autotune"],
+            "roofline_batched_step": b["step_roofline"],
         })
     elif "plain" in res:
         p = res["plain"]
@@ -696,6 +916,8 @@ def main():
                     "config": {"workload": p["workload"], "form": "plain"}})
     if "precision" in res:
         out["precision"] = res["precision"]
+    if "cfg1" in res:
+        out["config1_4432x1644"] = res["cfg1"]
     if "batch" in res and res["batch"]["cpu_baseline"]:
         out["cpu_baseline"] = res["batch"]["cpu_baseline"]
     if "serve" in res:

@@ -206,8 +206,13 @@ class ImputationResult {
     void compute_info_scores();
     size_t memory_usage() const;
 
-    // added: posterior dosage AP0 + AP1 and raw access
+    // added: posterior dosage AP0 + AP1 and raw access.  has_posteriors()
+    // says whether the allele probabilities were filled (set_posterior, or an
+    // Imputer run with output_dosages or output_probabilities); INFO and the
+    // VCF's DS/GP/AP fields use them when they are.
     prob_t get_posterior_dosage(sample_t s, marker_t m) const;
+    bool has_posteriors() const { return has_posteriors_; }
+    void set_has_posteriors(bool v) { has_posteriors_ = v; }
     allele_t* mutable_haplotypes() { return haplotypes_.get(); }
     prob_t* mutable_posteriors() { return posteriors_.get(); }
     double log_likelihood(sample_t s) const { return s < loglik_.size() ? loglik_[s] : 0.0; }
@@ -220,6 +225,7 @@ class ImputationResult {
     std::unique_ptr<prob_t[]> posteriors_;    // [B][M][2] (AP0, AP1)
     std::unique_ptr<prob_t[]> info_scores_;   // [M]
     std::vector<double> loglik_;
+    bool has_posteriors_ = false;
 };
 
 // src/api/imputer.hpp:197-287

@@ -144,7 +144,26 @@ API_SYMBOLS = [
     ("si_synth_targets", ctypes.c_int, [ctypes.POINTER(SynthTargetCfg), _u64p, ctypes.c_uint32,
                                         ctypes.c_uint32, _f64p, _u8p, _u8p, ctypes.c_int]),
     ("si_hardcall_palette", None, [_f32p]),
+    ("si_impute_defaults", None, [ctypes.c_void_p]),
+    ("si_impute_vcf", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                     ctypes.c_void_p, ctypes.c_void_p]),
+    ("si_impute_last_error", ctypes.c_char_p, []),
 ]
+
+
+class ImputeOpts(ctypes.Structure):
+    """Mirror of ``si_impute_opts`` (include/swiftimpute/impute_c.h)."""
+    _fields_ = [("num_states", ctypes.c_uint32), ("ne", ctypes.c_uint32),
+                ("batch_size", ctypes.c_uint32), ("window_size", ctypes.c_uint32),
+                ("precision", ctypes.c_uint32), ("per_marker", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64)]
+
+
+class ImputeTiming(ctypes.Structure):
+    """Mirror of ``si_impute_timing``."""
+    _fields_ = [(n, ctypes.c_double) for n in ("reference_load_s", "target_load_s", "index_s",
+                                              "impute_s", "write_s", "total_s")] + \
+               [(n, ctypes.c_uint64) for n in ("samples", "markers", "untyped_genotypes")]
 
 _hip = None
 _api = None

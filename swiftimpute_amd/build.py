@@ -87,7 +87,7 @@ def build(force: bool = True, verbose: bool = False) -> dict:
     jobs.sort(key=lambda c: 0 if any("fb_inst" in x for x in c) else 1)
     api_objs = []
     api_hdrs = hdrs + [os.path.join(INCLUDE, "swiftimpute", f)
-                       for f in ("types.hpp", "imputer.hpp", "synth.h", "server.h")]
+                       for f in ("types.hpp", "imputer.hpp", "synth.h", "server.h", "impute_c.h")]
     for src in API_SOURCES:
         s = os.path.join(CSRC, src)
         if not os.path.exists(s):

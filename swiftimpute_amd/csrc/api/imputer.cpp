@@ -11,11 +11,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "../../../include/swiftimpute/impute_c.h"
 #include "../../../include/swiftimpute/imputer.hpp"
 #include "../../../include/swiftimpute_hip.h"
 
@@ -382,7 +384,8 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
             progress_callback(reported, B);
         }
     }
-    result->compute_info_scores();
+    result->set_has_posteriors(want_ap);
+    if (config_.output_info_score) result->compute_info_scores();
     return result;
 }
 
@@ -453,3 +456,96 @@ std::unique_ptr<ImputationResult> impute(const std::string& reference_vcf, const
 }
 
 }  // namespace swiftimpute
+
+// ------------------------------------------------------------ C entry point --
+namespace {
+thread_local std::string g_impute_err;
+}
+
+extern "C" {
+
+void si_impute_defaults(si_impute_opts* o) {
+    if (!o) return;
+    const swiftimpute::ImputationConfig c;
+    o->num_states = c.hmm_params.num_states;
+    o->ne = c.hmm_params.ne;
+    o->batch_size = c.batch_size;
+    o->window_size = c.window_size;
+    o->precision = 0;
+    o->per_marker = 0;
+    o->seed = c.seed;
+}
+
+const char* si_impute_last_error(void) { return g_impute_err.c_str(); }
+
+int si_impute_vcf(const char* reference_vcf, const char* target_vcf, const char* output_vcf,
+                  const si_impute_opts* opts, si_impute_timing* t) {
+    using namespace swiftimpute;
+    using Clock = std::chrono::steady_clock;
+    auto secs = [](Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    if (!reference_vcf || !target_vcf || !output_vcf || !opts || opts->precision > 2) {
+        g_impute_err = "si_impute_vcf: bad arguments";
+        return SI_ERR_ARG;
+    }
+    try {
+        ImputationConfig cfg;
+        cfg.hmm_params.num_states = opts->num_states;
+        cfg.hmm_params.ne = opts->ne;
+        cfg.batch_size = opts->batch_size;
+        cfg.window_size = opts->window_size;
+        cfg.precision = static_cast<Precision>(opts->precision);
+        cfg.transition_mode = opts->per_marker ? TransitionMode::PerMarker : TransitionMode::FirstInterval;
+        cfg.seed = opts->seed;
+        const auto t0 = Clock::now();
+        auto reference = ReferencePanel::load_vcf(reference_vcf);
+        const auto t1 = Clock::now();
+        auto targets = TargetData::load_vcf(target_vcf);
+        const auto t2 = Clock::now();
+        Imputer imputer(*reference, cfg);
+        try {
+            imputer.build_index();
+        } catch (const UnsupportedError&) {
+            throw;
+        } catch (const DeviceError&) {
+            throw;
+        } catch (const ImputationError& e) {  // as the reference: proceed without the index
+            Logger::instance().warning(std::string("PBWT index not available, proceeding without it: ") + e.what());
+        }
+        const auto t3 = Clock::now();
+        auto result = imputer.impute(*targets);
+        const auto t4 = Clock::now();
+        result->write_vcf(output_vcf, *targets, *reference, cfg);
+        const auto t5 = Clock::now();
+        if (t) {
+            t->reference_load_s = secs(t0, t1);
+            t->target_load_s = secs(t1, t2);
+            t->index_s = secs(t2, t3);
+            t->impute_s = secs(t3, t4);
+            t->write_s = secs(t4, t5);
+            t->total_s = secs(t0, t5);
+            t->samples = targets->num_samples();
+            t->markers = targets->num_markers();
+            uint64_t untyped = 0;
+            const auto& pal = targets->palette();
+            const auto& codes = targets->codes();
+            for (size_t s = 0; s < targets->num_samples(); ++s)
+                for (size_t m = 0; m < targets->num_markers(); ++m) {
+                    const auto& g = pal[codes[s * targets->code_pitch() + m]];
+                    untyped += g.ll_00 == 0.0f && g.ll_01 == 0.0f && g.ll_11 == 0.0f;
+                }
+            t->untyped_genotypes = untyped;
+        }
+        return SI_OK;
+    } catch (const UnsupportedError& e) {
+        g_impute_err = e.what();
+        return SI_ERR_UNSUPPORTED;
+    } catch (const DeviceError& e) {
+        g_impute_err = e.what();
+        return SI_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        g_impute_err = e.what();
+        return SI_ERR_ARG;
+    }
+}
+
+}  // extern "C"

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <fstream>
 #include <iostream>
 #include <map>
@@ -390,27 +391,71 @@ ImputationResult::ImputationResult(uint32_t B, uint32_t M) : num_samples_(B), nu
     loglik_.assign(B, 0.0);
 }
 
+// The reference's writer (VCFWriter, src/io/vcf_writer.hpp:97-210; called
+// from ImputationResult::write_vcf, src/api/imputer.cpp:354-406) declares
+// DS/GP/AP but writes only the phased GT.  Here the FORMAT carries what the
+// config asks for and the result holds: GT (the sampled haplotypes, "." for a
+// missing panel allele), DS = AP0 + AP1 when output_dosages, GP and AP when
+// output_probabilities (4 decimals, the reference's format_dosage /
+// format_genotype_probs); INFO carries the score when output_info_score.
+// Only the fields written are declared in the header.
 void ImputationResult::write_vcf(const std::string& filename, const TargetData& targets,
-                                 const ReferencePanel& reference, const ImputationConfig&) const {
+                                 const ReferencePanel& reference, const ImputationConfig& config) const {
     std::ofstream out(filename);
     if (!out) throw ImputationError("cannot open output VCF: " + filename);
+    const bool ds = has_posteriors_ && config.output_dosages;
+    const bool gp = has_posteriors_ && config.output_probabilities;
+    const bool info = config.output_info_score;
     out << "##fileformat=VCFv4.2\n##source=SwiftImpute\n";
     if (!reference.markers().empty()) out << "##contig=<ID=" << reference.markers()[0].chrom << ">\n";
-    out << "##INFO=<ID=IMP,Number=0,Type=Flag,Description=\"Imputed variant\">\n"
-        << "##FORMAT=<ID=GT,Number=1,Type=String,Description=\"Genotype\">\n"
-        << "##FORMAT=<ID=DS,Number=1,Type=Float,Description=\"Dosage (expected ALT allele count)\">\n"
-        << "##FORMAT=<ID=GP,Number=G,Type=Float,Description=\"Genotype posterior probabilities\">\n"
-        << "##FORMAT=<ID=AP,Number=2,Type=Float,Description=\"Allelic probabilities\">\n"
-        << "#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT";
+    out << "##INFO=<ID=IMP,Number=0,Type=Flag,Description=\"Imputed variant\">\n";
+    if (info)
+        out << "##INFO=<ID=INFO,Number=1,Type=Float,Description=\"Imputation quality: 1 - Var(DS) / "
+               "(2p(1-p)), DS the "
+            << (has_posteriors_ ? "posterior" : "sampled") << " dosage\">\n";
+    out << "##FORMAT=<ID=GT,Number=1,Type=String,Description=\"Genotype\">\n";
+    if (ds) out << "##FORMAT=<ID=DS,Number=1,Type=Float,Description=\"Dosage (expected ALT allele count)\">\n";
+    if (gp) {
+        out << "##FORMAT=<ID=GP,Number=G,Type=Float,Description=\"Genotype posterior probabilities\">\n";
+        out << "##FORMAT=<ID=AP,Number=2,Type=Float,Description=\"Allelic probabilities\">\n";
+    }
+    out << "#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT";
     for (const auto& s : targets.samples()) out << "\t" << s.id;
     out << "\n";
+    std::string fmt = "GT";
+    if (ds) fmt += ":DS";
+    if (gp) fmt += ":GP:AP";
+    char buf[96];
     for (marker_t m = 0; m < num_markers_; ++m) {
         const auto& mk = reference.markers()[m];
         out << mk.chrom << "\t" << mk.pos << "\t" << (mk.id.empty() ? "." : mk.id) << "\t" << mk.ref << "\t"
-            << mk.alt << "\t.\tPASS\tIMP\tGT";
+            << mk.alt << "\t.\tPASS\tIMP";
+        if (info) {
+            std::snprintf(buf, sizeof(buf), ";INFO=%.4f", double(info_scores_[m]));
+            out << buf;
+        }
+        out << "\t" << fmt;
         for (sample_t s = 0; s < num_samples_; ++s) {
             const size_t b = size_t(s) * 2 * num_markers_;
-            out << "\t" << int(haplotypes_[b + m]) << "|" << int(haplotypes_[b + num_markers_ + m]);
+            const allele_t a0 = haplotypes_[b + m], a1 = haplotypes_[b + num_markers_ + m];
+            out << "\t";
+            if (a0 == 255) out << ".";
+            else out << int(a0);
+            out << "|";
+            if (a1 == 255) out << ".";
+            else out << int(a1);
+            if (ds) {
+                std::snprintf(buf, sizeof(buf), ":%.4f", double(get_posterior_dosage(s, m)));
+                out << buf;
+            }
+            if (gp) {
+                prob_t g[3];
+                get_probabilities(s, m, g);
+                const size_t i = (size_t(s) * num_markers_ + m) * 2;
+                std::snprintf(buf, sizeof(buf), ":%.4f,%.4f,%.4f:%.4f,%.4f", double(g[0]), double(g[1]),
+                              double(g[2]), double(posteriors_[i]), double(posteriors_[i + 1]));
+                out << buf;
+            }
         }
         out << "\n";
     }
@@ -460,14 +505,18 @@ void ImputationResult::set_posterior(sample_t s, marker_t m, const prob_t* p) {
     const size_t i = (size_t(s) * num_markers_ + m) * 2;
     posteriors_[i] = p[0];
     posteriors_[i + 1] = p[1];
+    has_posteriors_ = true;
 }
 
-// INFO = 1 - Var(d) / (2p(1-p)) over sampled dosages (src/api/imputer.cpp:467-496)
+// INFO = 1 - Var(d) / (2p(1-p)), p = mean(d) / 2, clamped to [0, 1], 1 for a
+// monomorphic marker (src/api/imputer.cpp:467-496).  d is the posterior
+// dosage AP0 + AP1 when the allele probabilities are filled, else (as the
+// reference, whose posteriors stay empty) the sampled dosage h0 + h1.
 void ImputationResult::compute_info_scores() {
     for (marker_t m = 0; m < num_markers_; ++m) {
         double sum = 0.0, sq = 0.0;
         for (sample_t s = 0; s < num_samples_; ++s) {
-            const double d = get_dosage(s, m);
+            const double d = has_posteriors_ ? double(get_posterior_dosage(s, m)) : double(get_dosage(s, m));
             sum += d;
             sq += d * d;
         }

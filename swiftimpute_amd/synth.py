@@ -183,3 +183,46 @@ def region_codes(pool: np.ndarray, spec: JobSpec, pitch: int | None = None) -> n
     c = np.full((spec.B, pitch), CODE_MISSING, np.uint8)
     c[:, :M:spec.typed_every] = pool[spec.s0:spec.s0 + spec.B, spec.m0:spec.m0 + M:spec.typed_every]
     return c
+
+
+# --------------------------------------------------------------------------
+# VCF text of a synthetic panel and cohort (the inputs an end-to-end run of
+# the C++ API reads): biallelic markers on chr1 at the panel's positions,
+# REF A / ALT G, panel genotypes phased "a|b" (haplotypes 2s, 2s+1), target
+# genotypes unphased from the codes ("./." for untyped markers).
+# --------------------------------------------------------------------------
+_GT_PHASED = np.array([b"0|0", b"0|1", b"1|0", b"1|1"])
+_GT_CODES = np.array([b"0/0", b"0/1", b"1/1", b"./."])
+
+
+def _vcf_header(sample_names) -> str:
+    return ("##fileformat=VCFv4.2\n##source=swiftimpute_amd.synth\n##contig=<ID=chr1>\n"
+            "##FORMAT=<ID=GT,Number=1,Type=String,Description=\"Genotype\">\n"
+            "#CHROM\tPOS\tID\tREF\tALT\tQUAL\tFILTER\tINFO\tFORMAT\t" + "\t".join(sample_names) + "\n")
+
+
+def _write_rows(f, pos, fields):
+    """fields: [M][n] array of byte strings (one genotype field each)."""
+    for m in range(fields.shape[0]):
+        f.write(f"chr1\t{int(pos[m])}\trs{m + 1}\tA\tG\t.\tPASS\t.\tGT\t".encode())
+        f.write(b"\t".join(fields[m].tolist()))
+        f.write(b"\n")
+
+
+def write_panel_vcf(path: str, panel: Panel) -> None:
+    """The panel as a VCF of H/2 diploid samples (H must be even)."""
+    if panel.H % 2:
+        raise ValueError("a VCF panel needs an even haplotype count")
+    a = panel.alleles().astype(np.int64)               # [M][H]
+    fields = _GT_PHASED[2 * a[:, 0::2] + a[:, 1::2]]   # [M][H/2]
+    with open(path, "wb") as f:
+        f.write(_vcf_header([f"REF{s}" for s in range(panel.H // 2)]).encode())
+        _write_rows(f, panel.pos, fields)
+
+
+def write_targets_vcf(path: str, panel: Panel, targets: Targets) -> None:
+    """The cohort's hard calls (codes 0/1/2, 3 = untyped) as a VCF."""
+    fields = _GT_CODES[targets.codes[:, : panel.M].T]  # [M][B]
+    with open(path, "wb") as f:
+        f.write(_vcf_header([f"TGT{s}" for s in range(targets.B)]).encode())
+        _write_rows(f, panel.pos, fields)

@@ -82,31 +82,75 @@ static int cpu_checks(const std::string& golden) {
     r.set_haplotype(1, 0, h1.data());
     r.set_haplotype(1, 1, h1.data());
     CHECK(r.get_dosage(0, 3) == 1.0f && r.get_dosage(1, 3) == 2.0f);
-    const prob_t ap[2] = {0.25f, 0.5f};
-    r.set_posterior(1, 2, ap);
-    prob_t gp[3];
-    r.get_probabilities(1, 2, gp);
-    CHECK(std::fabs(gp[0] - 0.375f) < 1e-7 && std::fabs(gp[1] - 0.5f) < 1e-7 && std::fabs(gp[2] - 0.125f) < 1e-7);
-    CHECK(std::fabs(r.get_posterior_dosage(1, 2) - 0.75f) < 1e-7);
-    r.compute_info_scores();
+    // without posteriors INFO comes from the sampled dosages, as the reference:
     // dosages 1 and 2: mean 1.5, p 0.75, var 0.25, 2p(1-p) 0.375 -> INFO 1/3
+    CHECK(!r.has_posteriors());
+    r.compute_info_scores();
     CHECK(std::fabs(r.get_info_score(0) - (1.0f - 0.25f / 0.375f)) < 1e-6);
     ImputationConfig cfg;
     CHECK(cfg.batch_size == 100 && cfg.window_size == 10000 && cfg.hmm_params.num_states == 8 &&
           cfg.hmm_params.ne == 10000 && cfg.hmm_params.rho_rate == 4.0f);
     const std::string out = "/tmp/si_test_output.vcf";
-    r.write_vcf(out, *tgt, *ref, cfg);
-    std::ifstream f(out);
-    std::string line;
+    auto first_line = [&](const std::string& path, int* n) {
+        std::ifstream f(path);
+        std::string line, first;
+        int variants = 0;
+        while (std::getline(f, line))
+            if (!line.empty() && line[0] != '#') {
+                if (!variants) first = line;
+                ++variants;
+            }
+        *n = variants;
+        return first;
+    };
+    auto header_has = [&](const std::string& path, const std::string& what) {
+        std::ifstream f(path);
+        std::string line;
+        while (std::getline(f, line))
+            if (line.rfind("##", 0) == 0 && line.find(what) != std::string::npos) return true;
+        return false;
+    };
     int variants = 0;
-    std::string first;
-    while (std::getline(f, line))
-        if (!line.empty() && line[0] != '#') {
-            if (!variants) first = line;
-            ++variants;
+    // no posteriors: GT only, and only GT declared
+    r.write_vcf(out, *tgt, *ref, cfg);
+    CHECK(first_line(out, &variants) == "chr1\t1000\trs1\tA\tG\t.\tPASS\tIMP;INFO=0.3333\tGT\t0|1\t1|1");
+    CHECK(variants == 10 && !header_has(out, "ID=DS") && header_has(out, "ID=INFO"));
+    // posterior allele probabilities at every (sample, marker)
+    for (sample_t s = 0; s < 2; ++s)
+        for (marker_t m = 0; m < 10; ++m) {
+            const prob_t ap[2] = {s == 0 ? 0.25f : 0.5f, 0.5f};
+            r.set_posterior(s, m, ap);
         }
-    CHECK(variants == 10);
-    CHECK(first == "chr1\t1000\trs1\tA\tG\t.\tPASS\tIMP\tGT\t0|1\t1|1");
+    CHECK(r.has_posteriors());
+    prob_t gp[3];
+    r.get_probabilities(0, 2, gp);
+    CHECK(std::fabs(gp[0] - 0.375f) < 1e-7 && std::fabs(gp[1] - 0.5f) < 1e-7 && std::fabs(gp[2] - 0.125f) < 1e-7);
+    CHECK(std::fabs(r.get_posterior_dosage(0, 2) - 0.75f) < 1e-7);
+    // INFO now from the posterior dosages 0.75 and 1.0: mean 0.875, p 0.4375,
+    // var 0.015625, 2p(1-p) 0.4921875
+    r.compute_info_scores();
+    CHECK(std::fabs(r.get_info_score(0) - float(1.0 - 0.015625 / 0.4921875)) < 1e-6);
+    r.write_vcf(out, *tgt, *ref, cfg);
+    CHECK(first_line(out, &variants) ==
+          "chr1\t1000\trs1\tA\tG\t.\tPASS\tIMP;INFO=0.9683\tGT:DS:GP:AP"
+          "\t0|1:0.7500:0.3750,0.5000,0.1250:0.2500,0.5000\t1|1:1.0000:0.2500,0.5000,0.2500:0.5000,0.5000");
+    CHECK(variants == 10 && header_has(out, "ID=DS") && header_has(out, "ID=GP") && header_has(out, "ID=AP"));
+    // the output flags select the fields; with all off the line is the reference's
+    ImputationConfig c3;
+    c3.output_dosages = false;
+    r.write_vcf(out, *tgt, *ref, c3);
+    CHECK(first_line(out, &variants).find("\tGT:GP:AP\t0|1:0.3750,0.5000,0.1250:0.2500,0.5000\t") !=
+          std::string::npos);
+    c3.output_probabilities = false;
+    c3.output_info_score = false;
+    r.write_vcf(out, *tgt, *ref, c3);
+    CHECK(first_line(out, &variants) == "chr1\t1000\trs1\tA\tG\t.\tPASS\tIMP\tGT\t0|1\t1|1");
+    CHECK(!header_has(out, "ID=DS") && !header_has(out, "ID=GP") && !header_has(out, "ID=INFO"));
+    // a missing panel allele reaches GT as "."
+    std::vector<allele_t> hm(10, 255);
+    r.set_haplotype(1, 0, hm.data());
+    r.write_vcf(out, *tgt, *ref, c3);
+    CHECK(first_line(out, &variants) == "chr1\t1000\trs1\tA\tG\t.\tPASS\tIMP\tGT\t0|1\t.|1");
     CHECK(Imputer::estimate_memory_requirement(*ref, 2, cfg) > 0);
     bool bad = false;
     try {

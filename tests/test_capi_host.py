@@ -34,6 +34,22 @@ def test_api_library_exports_synth_symbols():
         assert hasattr(lib, n), n
 
 
+def test_api_library_exports_impute_symbols():
+    lib = _native.api()
+    names = declared(os.path.join("swiftimpute", "impute_c.h"))
+    assert set(names) == {"si_impute_defaults", "si_impute_vcf", "si_impute_last_error"}
+    for n in names:
+        assert hasattr(lib, n), n
+    o = _native.ImputeOpts()
+    lib.si_impute_defaults(ctypes.byref(o))
+    # the reference's defaults (imputer.hpp:44-55, types.hpp:77-90, sampling.hpp:74)
+    assert (o.num_states, o.ne, o.batch_size, o.window_size, o.precision, o.seed) == \
+        (8, 10000, 100, 10000, 0, 12345)
+    # a missing argument is refused before anything is loaded
+    assert lib.si_impute_vcf(None, b"t.vcf", b"o.vcf", ctypes.byref(o), None) == _native.SI_ERR_ARG
+    assert b"bad arguments" in lib.si_impute_last_error()
+
+
 def test_api_library_exports_server_symbols():
     from swiftimpute_amd import serving
     lib = serving._lib()
