@@ -169,8 +169,17 @@ enum si_fb_flags {
                                    marker, as the reference (interval 0 only,
                                    forward_backward.cu:108)                   */
     SI_FB_ARGMAX = 1u << 1,     /* argmax instead of the Philox draw           */
-    SI_FB_LOGSPACE = 1u << 2    /* log-space FP32 kernel instead of the
+    SI_FB_LOGSPACE = 1u << 2,   /* log-space FP32 kernel instead of the
                                    rescaled linear-space one                   */
+    SI_FB_UNTYPED = 1u << 3     /* markers whose code is `untyped_code` (a
+                                   palette entry with all three likelihoods
+                                   0: a missing genotype, every state's
+                                   emission 1) take the untyped step: the
+                                   same affine map for every sample, composed
+                                   over runs of such markers in O(1) per
+                                   marker instead of O(L).  Same model, FP32
+                                   rounding only; ignored when that palette
+                                   entry is not (0, 0, 0)                     */
 };
 
 typedef struct si_fb_job {
@@ -206,6 +215,8 @@ typedef struct si_fb_job {
                              of each window start is not taken
                              (forward_backward.cu:406-626).  loglik is that of
                              the first forward run. */
+    uint32_t untyped_code; /* with SI_FB_UNTYPED: the palette code of an
+                              untyped marker (< 256)                          */
 } si_fb_job;
 
 /* Workspace (device bytes) si_fb_run needs for these jobs (enough for every
@@ -214,11 +225,11 @@ size_t si_fb_workspace_bytes(const si_fb_job* jobs, uint32_t n_jobs);
 
 /* Kernel variant: lanes of a wavefront per sample in the chain pass and in
  * the block pass (states per lane = Lp / lanes, Lp = si_padded_states(L)),
- * and the checkpoint block length in markers (16 or 32; 32 halves checkpoint
- * memory for 16 extra forward steps per block).  Every sum over states is
- * the same binary tree whatever the variant, so outputs are bit-identical
- * across variants: the choice is performance only.  No counterpart in the
- * reference (one thread per (sample, state), forward_backward.cu:41-317). */
+ * and the checkpoint block length in markers (16; other lengths are refused
+ * with SI_ERR_UNSUPPORTED).  Every sum over states is the same binary tree
+ * whatever the variant, so outputs are bit-identical across variants: the
+ * choice is performance only.  No counterpart in the reference (one thread
+ * per (sample, state), forward_backward.cu:41-317). */
 typedef struct si_fb_variant {
     uint32_t chain_lanes;
     uint32_t block_lanes;

@@ -30,6 +30,7 @@ SI_ERR_WORKSPACE = -4
 SI_FB_PER_MARKER = 1
 SI_FB_ARGMAX = 2
 SI_FB_LOGSPACE = 4
+SI_FB_UNTYPED = 8
 SI_FB_PASS_CHAIN = 1
 SI_FB_PASS_BLOCK = 2
 SI_PERM_FIXED = 0x80000000
@@ -61,7 +62,7 @@ class FbJob(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("step", StepCoef),
         ("steps", _vp), ("sa", _vp), ("mmask", _vp), ("codes", _vp), ("eps", _vp),
         ("emax", _vp), ("palette", _vp), ("ds", _vp), ("ap", _vp), ("hap", _vp), ("prob", _vp),
-        ("loglik", _vp), ("window", ctypes.c_uint32),
+        ("loglik", _vp), ("window", ctypes.c_uint32), ("untyped_code", ctypes.c_uint32),
     ]
 
 

@@ -325,6 +325,12 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
                 (logspace ? uint32_t(SI_FB_LOGSPACE) : 0u);
     job.code_pitch = pitch;
     job.n_palette = P;
+    for (uint32_t p = 0; p < P; ++p)  // untyped markers: likelihoods (0, 0, 0)
+        if (palf[3 * p] == 0.0f && palf[3 * p + 1] == 0.0f && palf[3 * p + 2] == 0.0f) {
+            job.flags |= SI_FB_UNTYPED;
+            job.untyped_code = p;
+            break;
+        }
     job.seed = config_.seed;
     job.step = step;
     job.steps = dev_->steps;

@@ -263,6 +263,8 @@ struct si_server {
                   (uses_logspace(r.m0, r.m0 + r.M) ? uint32_t(SI_FB_LOGSPACE) : 0u);
         j.code_pitch = r.pitch;
         j.n_palette = 4;
+        j.flags |= SI_FB_UNTYPED;  // code 3 of the hard-call palette: (0, 0, 0)
+        j.untyped_code = 3;
         j.seed = r.seed;
         j.step = (r.M >= 2) ? steps[r.m0 + 1] : si_step_coef{1.0f, 0.0f, 0.0f, -INFINITY};
         j.steps = d_steps + r.m0;

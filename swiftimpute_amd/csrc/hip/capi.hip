@@ -247,6 +247,9 @@ static int check_job(const si_fb_job& j, uint32_t idx) {
     if (si::fb_window(j) && j.window < si::kMinWindow)
         return fail(SI_ERR_UNSUPPORTED, "job %u: window %u shorter than %u markers", idx, j.window,
                     si::kMinWindow);
+    if ((j.flags & SI_FB_UNTYPED) && j.untyped_code >= j.n_palette)
+        return fail(SI_ERR_ARG, "job %u: untyped_code %u outside the palette (%u entries)", idx,
+                    j.untyped_code, j.n_palette);
     return SI_OK;
 }
 
@@ -326,7 +329,7 @@ int si_fb_run_variant(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_varian
         const si::FbVariant var = variant_or_default(Lp, variant);
         si::JobView v = si::fb_job_view(j, var.cb);
         si::fb_bind_workspace(v, Lp, static_cast<char*>(workspace));
-        const si::FbTables t{j.eps, j.emax, j.palette, j.n_palette};
+        const si::FbTables t{j.eps, j.emax, j.palette, j.n_palette, si::fb_untyped_code(j)};
         const int rc = si::launch_fb(v, t, Lp, passes, var, static_cast<hipStream_t>(stream));
         if (rc) return device_status(rc, "si_fb_run");
     }
@@ -395,7 +398,10 @@ int si_fb_run_noalloc(const si_fb_job* jobs, uint32_t n_jobs, const si_fb_varian
 // ------------------------------------------------------------- autotuner --
 namespace {
 
-constexpr uint32_t kCkptBlocks[2] = {16, 32};
+// Checkpoint blocks the tuners consider: 16 markers only (32-marker blocks
+// were dropped: their block kernels spilled to scratch).
+constexpr int kNumCkpt = 1;
+constexpr uint32_t kCkptBlocks[kNumCkpt] = {16};
 
 // Event pair timing one launch sequence on a stream (milliseconds).
 struct Timer {
@@ -490,11 +496,11 @@ bool tune_log() {
 // block shape 7 % faster alone was 2 % slower in the batch) -- the chain keeps
 // the default lanes, the block takes another shape only if it is at least
 // 10 % faster alone.
-si::FbVariant pick(const float chain_ms[2][8], const uint32_t* gc, uint32_t nc,
-                   const float block_ms[2][8], const uint32_t* gb, uint32_t nb, uint32_t Lp,
+si::FbVariant pick(const float chain_ms[kNumCkpt][8], const uint32_t* gc, uint32_t nc,
+                   const float block_ms[kNumCkpt][8], const uint32_t* gb, uint32_t nb, uint32_t Lp,
                    bool side = false) {
     if (tune_log())
-        for (int k = 0; k < 2; ++k) {
+        for (int k = 0; k < kNumCkpt; ++k) {
             for (uint32_t i = 0; i < nc; ++i)
                 std::fprintf(stderr, "[si_fb_tune] Lp=%u ckpt=%u chain lanes=%u: %.3f ms\n", Lp,
                              kCkptBlocks[k], gc[i], double(chain_ms[k][i]));
@@ -505,7 +511,7 @@ si::FbVariant pick(const float chain_ms[2][8], const uint32_t* gc, uint32_t nc,
     si::FbVariant best{gc[0], gb[0], kCkptBlocks[0]};
     float best_ms = 3.4e38f;
     const si::FbVariant def = si::fb_default_variant(Lp);
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kNumCkpt; ++k) {
         uint32_t ic = 0, ib = 0;
         for (uint32_t i = 1; i < nc; ++i)
             if (chain_ms[k][i] < chain_ms[k][ic]) ic = i;
@@ -554,17 +560,17 @@ int si_fb_tune(const si_fb_job* jobs, uint32_t n_jobs, void* workspace, size_t w
     const uint32_t nc = std::min(8u, si::fb_candidates(Lp, 0, gc, 8));
     const uint32_t nb = std::min(8u, si::fb_candidates(Lp, 1, gb, 8));
     if (!nc || !nb) return fail(SI_ERR_UNSUPPORTED, "si_fb_tune: no kernels for %u states", Lp);
-    float chain_ms[2][8] = {}, block_ms[2][8] = {};
+    float chain_ms[kNumCkpt][8] = {}, block_ms[kNumCkpt][8] = {};
     Timer tm;
     if (!tm.ok) return device_status(SI_ERR_DEVICE, "si_fb_tune (events)");
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kNumCkpt; ++k) {
         const uint32_t cb = kCkptBlocks[k];
         for (uint32_t i = 0; i < n_jobs; ++i) {
             const si_fb_job& j = jobs[i];
             if (j.n_samples == 0 || j.n_markers == 0) continue;
             si::JobView v = tuning_view(j, cap, cb);
             si::fb_bind_workspace(v, Lp, static_cast<char*>(workspace));
-            const si::FbTables t{j.eps, j.emax, j.palette, j.n_palette};
+            const si::FbTables t{j.eps, j.emax, j.palette, j.n_palette, si::fb_untyped_code(j)};
             for (uint32_t a = 0; a < nc; ++a) {
                 const si::FbVariant var{gc[a], gb[0], cb};
                 float ms = 0.0f;
@@ -592,11 +598,11 @@ struct si_fb_batch {
         uint32_t Lp = 0, flags = 0, pm = 0;
         std::vector<uint32_t> order;  // job indices, longest first
         si::JobView* d_jobs = nullptr;
-        uint64_t* d_off = nullptr;    // chain offsets [n+1] then block offsets [n+1]
+        uint64_t* d_off = nullptr;    // chain, block and mask-pass offsets, [n+1] each
         si::JobView* d_tjobs = nullptr;  // tuning copies
         uint64_t* d_toff = nullptr;
         unsigned long long* d_counters = nullptr;
-        uint64_t chain_items = 0, block_items = 0;
+        uint64_t chain_items = 0, block_items = 0, mask_groups = 0;
         uint64_t units = 0;           // sum of samples x markers over the class's jobs
         uint32_t off_cb = 0;          // checkpoint block the offsets were made for
         si::FbVariant var{};
@@ -639,14 +645,17 @@ extern "C++" {
 static void class_offsets(const si_fb_batch& b, const si_fb_batch::Class& c, uint32_t cb,
                           uint32_t cap, std::vector<uint64_t>& off) {
     const size_t n = c.order.size();
-    off.assign(2 * (n + 1), 0);
+    off.assign(3 * (n + 1), 0);
     for (size_t k = 0; k < n; ++k) {
         const si_fb_job& j = b.jobs[c.order[k]];
         const uint64_t M = std::min(j.n_markers, cap);
         const uint64_t nblk = (M + cb - 1) / cb;
-        off[k + 1] = off[k] + 2ull * j.n_samples;
-        // block items padded to whole wavefront rounds (see fb_block_batch_kernel)
-        off[n + 1 + k + 1] = off[n + 1 + k] + ((uint64_t(j.n_samples) * nblk + 63) & ~uint64_t(63));
+        // samples padded to whole wavefront rounds (see fb_chain_batch_kernel,
+        // fb_block_batch_kernel)
+        const uint64_t Bp = si::fb_batch_padded_samples(j.n_samples);
+        off[k + 1] = off[k] + 2ull * Bp;
+        off[n + 1 + k + 1] = off[n + 1 + k] + Bp * nblk;
+        off[2 * (n + 1) + k + 1] = off[2 * (n + 1) + k] + si::fb_tmask_groups(uint32_t(M));
     }
 }
 
@@ -661,6 +670,7 @@ static int class_prepare(si_fb_batch* b, si_fb_batch::Class& c) {
         const size_t n = c.order.size();
         c.chain_items = off[n];
         c.block_items = off[2 * n + 1];
+        c.mask_groups = off[3 * n + 2];
         c.off_cb = c.var.cb;
         b->bound_ws = nullptr;  // job views carry the block count: rebind
         if (b->exec) {
@@ -688,12 +698,14 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
         const int rc = check_job(jobs[i], i);
         if (rc) return rc;
         if (jobs[i].eps != jobs[0].eps || jobs[i].emax != jobs[0].emax ||
-            jobs[i].palette != jobs[0].palette || jobs[i].n_palette != jobs[0].n_palette)
+            jobs[i].palette != jobs[0].palette || jobs[i].n_palette != jobs[0].n_palette ||
+            si::fb_untyped_code(jobs[i]) != si::fb_untyped_code(jobs[0]))
             return fail(SI_ERR_ARG, "si_fb_batch_create: job %u uses a different emission table", i);
     }
     auto* b = new si_fb_batch();
     b->jobs.assign(jobs, jobs + n_jobs);
-    b->tables = si::FbTables{jobs[0].eps, jobs[0].emax, jobs[0].palette, jobs[0].n_palette};
+    b->tables = si::FbTables{jobs[0].eps, jobs[0].emax, jobs[0].palette, jobs[0].n_palette,
+                             si::fb_untyped_code(jobs[0])};
     b->ws_off.resize(n_jobs);
     for (uint32_t i = 0; i < n_jobs; ++i) {
         b->ws_off[i] = b->ws_bytes;
@@ -702,11 +714,12 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
                                                     si::fb_window(jobs[i]));
         b->ws_bytes += (w + 255) & ~size_t(255);
     }
-    // classes by (padded states, numeric space); jobs longest chain first
+    // classes by (padded states, numeric space, untyped-step kernels); jobs
+    // longest chain first
     for (uint32_t i = 0; i < n_jobs; ++i) {
         if (jobs[i].n_samples == 0 || jobs[i].n_markers == 0) continue;
         const uint32_t Lp = si_padded_states(jobs[i].n_states);
-        const uint32_t fl = jobs[i].flags & SI_FB_LOGSPACE;
+        const uint32_t fl = (jobs[i].flags & SI_FB_LOGSPACE) | (si::fb_fast_job(jobs[i]) ? si::kFbFastClass : 0u);
         auto it = std::find_if(b->classes.begin(), b->classes.end(),
                                [&](const si_fb_batch::Class& c) { return c.Lp == Lp && c.flags == fl; });
         if (it == b->classes.end()) {
@@ -726,9 +739,9 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
         for (uint32_t i : c.order) c.units += uint64_t(jobs[i].n_samples) * jobs[i].n_markers;
         const size_t n = c.order.size();
         if (hipMalloc(&c.d_jobs, n * sizeof(si::JobView)) != hipSuccess ||
-            hipMalloc(&c.d_off, 2 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc(&c.d_off, 3 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
             hipMalloc(&c.d_tjobs, n * sizeof(si::JobView)) != hipSuccess ||
-            hipMalloc(&c.d_toff, 2 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
+            hipMalloc(&c.d_toff, 3 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
             hipMalloc(&c.d_counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
             class_prepare(b, c) != SI_OK) {
             batch_free(b);
@@ -801,7 +814,7 @@ static int class_tune(si_fb_batch* b, si_fb_batch::Class& c, char* ws, uint32_t 
     const uint32_t nc = std::min(8u, si::fb_candidates(c.Lp, 0, gc, 8));
     const uint32_t nb = std::min(8u, si::fb_candidates(c.Lp, 1, gb, 8));
     if (!nc || !nb) return SI_ERR_UNSUPPORTED;
-    float chain_ms[2][8] = {}, block_ms[2][8] = {};
+    float chain_ms[kNumCkpt][8] = {}, block_ms[kNumCkpt][8] = {};
     Timer tm;
     if (!tm.ok) return SI_ERR_DEVICE;
     const size_t n = c.order.size();
@@ -825,12 +838,14 @@ static int class_tune(si_fb_batch* b, si_fb_batch::Class& c, char* ws, uint32_t 
         bc.jobs = c.d_tjobs;
         bc.chain_off = c.d_toff;
         bc.block_off = c.d_toff + n + 1;
+        bc.mask_off = c.d_toff + 2 * (n + 1);
         bc.chain_items = off[n];
         bc.block_items = off[2 * n + 1];
+        bc.mask_groups = off[3 * n + 2];
         bc.counters = c.d_counters;
         return 0;
     };
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kNumCkpt; ++k) {
         const uint32_t cb = kCkptBlocks[k];
         si::BatchClass bc{};
         // warm every candidate (code objects, caches) on 16-marker cuts
@@ -986,8 +1001,10 @@ static int batch_enqueue(si_fb_batch* b, hipStream_t stream) {
         bc.jobs = c.d_jobs;
         bc.chain_off = c.d_off;
         bc.block_off = c.d_off + c.order.size() + 1;
+        bc.mask_off = c.d_off + 2 * (c.order.size() + 1);
         bc.chain_items = c.chain_items;
         bc.block_items = c.block_items;
+        bc.mask_groups = c.mask_groups;
         bc.counters = c.d_counters;
         bc.grid_chain = c.grid_chain;
         bc.grid_block = c.grid_block;

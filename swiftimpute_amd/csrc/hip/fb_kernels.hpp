@@ -116,8 +116,12 @@ struct BlockLds {
 //   linear: (exp(gl_c - off), ..., 0, off), absent classes 0
 //   log:    (gl_c - off, ..., -inf, off),   absent classes -inf
 // A state's value is one LDS read at row * 5 + class.
+// The table's last word holds the untyped code (below), checked against the
+// table itself: kNoUntyped unless that code's row for all three classes is
+// emission 1 (log: 0) with offset 0, i.e. its likelihoods are (0, 0, 0).
 struct Tab {
     const float* f;
+    const uint32_t* uc;  // LDS word, read after the table's barrier
 };
 constexpr uint32_t kTabRow = 5;
 
@@ -125,7 +129,10 @@ template <bool LOG>
 __device__ __forceinline__ Tab load_tables(const FbTables& tb) {
     extern __shared__ float4 fb_dyn_lds[];
     float* f = reinterpret_cast<float*>(fb_dyn_lds);
+    uint32_t* uc = reinterpret_cast<uint32_t*>(f + tb.P * 8 * kTabRow);
     const uint32_t n = tb.P * 8;
+    const uint32_t urow = tb.ucode < tb.P ? tb.ucode * 8 + 7 : 0xffffffffu;
+    if (urow == 0xffffffffu && threadIdx.x == 0) *uc = kNoUntyped;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         float* row = f + i * kTabRow;
         const float o = tb.emax[i];
@@ -136,17 +143,20 @@ __device__ __forceinline__ Tab load_tables(const FbTables& tb) {
             row[1] = (c & 2u) ? g[1] - o : -INFINITY;
             row[2] = (c & 4u) ? g[2] - o : -INFINITY;
             row[3] = -INFINITY;
+            if (i == urow) *uc = (g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && o == 0.0f) ? tb.ucode : kNoUntyped;
         } else {
 #pragma unroll
             for (int k = 0; k < 4; ++k) row[k] = tb.eps[4 * i + k];
+            if (i == urow)
+                *uc = (row[0] == 1.0f && row[1] == 1.0f && row[2] == 1.0f && o == 0.0f) ? tb.ucode : kNoUntyped;
         }
         row[4] = o;
     }
-    return Tab{f};
+    return Tab{f, uc};
 }
 
 __host__ __device__ constexpr size_t fb_table_lds_bytes(uint32_t P) {
-    return size_t(P) * 8 * kTabRow * sizeof(float);
+    return size_t(P) * 8 * kTabRow * sizeof(float) + 16;
 }
 
 // The class-present mask of a marker rides in bit 7 of the first three state
@@ -453,6 +463,223 @@ __device__ __forceinline__ void posterior(const float (&a)[S::R], const float (&
     for (int i = 0; i < S::R; ++i) p[i] *= rq;
 }
 
+// ---------------------------------------------------------- untyped steps --
+// At an untyped marker (genotype likelihoods (0, 0, 0): every real state's
+// emission is 1, offset 0) a step is the same affine map for every sample:
+//   forward, sum-normalized:  a -> (t_o 1_L + u a) / S,  S = L t_o + u,
+//                             log-likelihood + ln S
+//   backward:                 b -> e^-999 1 + t_o (sum_L b) 1 + u b
+// (1_L: ones on the L real states).  Under constant transitions k of them in
+// a row have a closed form (c = u / S, geometric sums):
+//   forward   a -> c^k a + (1 - c^k)/L 1_L,    log-likelihood + k ln S
+//   backward  b = e^B v -> e^B' (x 1 + y v), with D = B + k ln S + 999 and
+//             f = e^(-999 - B), W = sum_L v:
+//     D > 0 (mass above the floor):  B' = B + k ln S,  y = c^k,
+//                                    x = f (S^-k - 1)/(1 - S) + (W/L)(1 - c^k)
+//     D <= 0 (the floor dominates):  B' = -999,  y = c^k e^D,
+//                                    x = (1 - S^k)/(1 - S) + (W/L)(1 - c^k) e^D
+//   (differences of 1 and a power as expm1, exact near 1).
+// The chain pass therefore does nothing at an untyped marker of a fast load
+// block: the vector stays at an anchor marker and is brought forward in one
+// closed-form run just before every other step and at each 16-aligned marker
+// (the checkpoints).  Where runs end depends on the markers only, so the
+// chain and block passes carry the same values and every lane shape and
+// checkpoint block gives the same bits.  The block pass also needs the
+// vectors at untyped markers for their posteriors: it forms them with a
+// per-marker recurrence from the anchor that is never carried.  Fast load
+// blocks: constant transitions (URun::ok), no windows, the block's whole
+// 32-marker super-block inside the job and past marker 31, so a checkpoint
+// block is fast or not as a whole.  The log form takes untyped steps only
+// where t_o >= 2^-30: its own step then adds the terms in linear space too
+// (the forward's -999 floor is below resolution).
+
+__device__ __forceinline__ bool fast_block(const JobView& J, uint32_t q) {
+    return J.ur.ok != 0u && J.tmask != nullptr && J.W == 0u && q >= 32u && (q | 31u) < J.M;
+}
+
+// Untyped markers of load block c of a job (0 unless fast), wavefront-uniform.
+__device__ __forceinline__ uint32_t block_untyped(const JobView& J, uint32_t c, bool fast) {
+    return fast ? __builtin_amdgcn_readfirstlane(~J.tmask[c] & 0xffffu) : 0u;
+}
+
+// Bit t set where code byte t of a load block equals uc (< 256).
+__device__ __forceinline__ uint32_t untyped_bits(const uint4& cw, uint32_t uc) {
+    const uint32_t rep = uc * 0x01010101u;
+    const uint32_t wd[4] = {cw.x, cw.y, cw.z, cw.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t x = wd[k] ^ rep;
+        // bit 7 of each byte set iff the byte is 0 (no carries between bytes)
+        const uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+        // bits 7, 15, 23 -> 21, 22, 23 (distinct partial products, no carries); bit 31 -> 3
+        const uint32_t b = ((__umul24((z >> 7) & 0x10101u, 0x204081u) >> 21) & 7u) | ((z >> 28) & 8u);
+        m |= b << (4 * k);
+    }
+    return m;
+}
+
+// The job's typed-marker mask: bit t of tmask[c] set where marker 16c + t
+// (< M) has a code other than uc in some sample, so a marker is untyped for
+// the job when no sample has a genotype there; the decision is the same for
+// every sample of the job (wavefront-uniform control in both passes) and
+// does not depend on which samples share a wavefront.  A workgroup covers
+// kMaskBlocks load blocks with kMaskLanes lanes each striding over the
+// samples (consecutive lanes read consecutive 16-byte code words of a row).
+constexpr uint32_t kMaskBlocks = kTmaskBlocksPerGroup, kMaskLanes = 8,
+                   kMaskThreads = kMaskBlocks * kMaskLanes;
+__device__ __forceinline__ void tmask_tile(const uint8_t* codes, uint32_t B, uint32_t M, uint32_t pitch,
+                                           uint32_t uc, uint32_t c0, uint32_t* tmask) {
+    __shared__ uint32_t red[kMaskThreads];
+    const uint32_t ci = threadIdx.x % kMaskBlocks, sl = threadIdx.x / kMaskBlocks;
+    const uint32_t c = c0 + ci;
+    const uint32_t nlb = (M + kSub - 1) / kSub;
+    uint32_t m = 0;
+    if (c < nlb) {
+        for (uint32_t s = sl; s < B; s += kMaskLanes)
+            m |= untyped_bits(*reinterpret_cast<const uint4*>(codes + size_t(s) * pitch + kSub * c), uc) ^ 0xffffu;
+        if (kSub * c + kSub > M) m &= (1u << (M - kSub * c)) - 1u;  // markers past the job
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    if (sl == 0 && c < nlb) {
+#pragma unroll
+        for (uint32_t k = 1; k < kMaskLanes; ++k) m |= red[k * kMaskBlocks + ci];
+        tmask[c] = m;
+    }
+}
+
+// Real-state mask of the lane's R states (1 on states < L).
+template <class S>
+__device__ __forceinline__ void real_mask(uint32_t r, uint32_t L, float (&rm)[S::R]) {
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) rm[i] = (r * uint32_t(S::R) + uint32_t(i) < L) ? 1.0f : 0.0f;
+}
+
+// e^x - 1, accurate near 0 (series below 1/8: truncation < 1e-10 relative).
+__device__ __forceinline__ float em1(float x) {
+    const float p =
+        x * (1.0f + x * (0.5f + x * (0.16666667f + x * (0.041666668f + x * (0.0083333338f + x * 0.0013888889f)))));
+    return fabsf(x) < 0.125f ? p : fast_exp(x) - 1.0f;
+}
+
+// The forward brought k untyped steps past its anchor (no-op for k = 0);
+// returns the log-likelihood increment.  Log form: the max-normalized log
+// vector, taken to linear space for the run.
+template <class S, bool LOG>
+__device__ __forceinline__ double fwd_run(float (&a)[S::R], const float (&rm)[S::R], uint32_t k,
+                                          const URun& q) {
+    if (k == 0u) return 0.0;
+    const float kf = float(k);
+    const float al = __builtin_amdgcn_exp2f(kf * q.lc1);  // c^k
+    const float be = -em1(kf * q.lnc1) * q.rL;           // (1 - c^k) / L
+    double inc = double(k) * q.lnSd;
+    if constexpr (LOG) {
+        float w[S::R];
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) w[i] = fast_exp(a[i]);
+        const float Wm = vsum<S>(w);
+        const float aw = al * __builtin_amdgcn_rcpf(Wm);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) a[i] = fast_ln(fmaf(aw, w[i], be * rm[i]));
+        const float c = vmax<S>(a);
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) a[i] -= c;
+        inc += double(fast_ln(Wm)) + double(c);
+    } else {
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) a[i] = fmaf(al, a[i], be * rm[i]);
+    }
+    return inc;
+}
+
+// The backward's anchor as a linear base: v itself (linear) or e^(v - max)
+// over the real states with the max folded into the scale (log; a scale
+// below -999 is rescaled to the floor, dropping less than 2^-24 of it).
+// Returns the base's sum over the real states; B = its scale.
+template <class S, bool LOG>
+__device__ __forceinline__ float bwd_base(const float (&v)[S::R], double Bl, const float (&rm)[S::R],
+                                          float (&base)[S::R], double& B) {
+    B = Bl;
+    if constexpr (LOG) {
+        float vr[S::R];
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) vr[i] = rm[i] != 0.0f ? v[i] : -INFINITY;
+        const float mv = vmax<S>(vr);
+        B = Bl + double(mv);
+        float sh = 0.0f;
+        if (B < -999.0) {
+            sh = float(B + 999.0);
+            B = -999.0;
+        }
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) base[i] = fast_exp(v[i] - mv + sh);
+    } else {
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) base[i] = v[i];
+    }
+    float t[S::R];
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) t[i] = base[i] * rm[i];
+    return vsum<S>(t);
+}
+
+// The backward brought k untyped steps below its anchor (no-op for k = 0).
+template <class S, bool LOG>
+__device__ __forceinline__ void bwd_run(float (&v)[S::R], double& Bl, const float (&rm)[S::R], uint32_t k,
+                                        const URun& q) {
+    if (k == 0u) return;
+    float base[S::R];
+    double B;
+    const float W0 = bwd_base<S, LOG>(v, Bl, rm, base, B);
+    const float kf = float(k);
+    const double Bn = fma(double(k), q.lnSd, B);
+    const float D = float(Bn + 999.0);
+    const float ck = __builtin_amdgcn_exp2f(kf * q.lc1);
+    const float wl = W0 * q.rL * -em1(kf * q.lnc1);  // (W/L)(1 - c^k)
+    float x, y;
+    if (D > 0.0f) {
+        const float f0 = fast_exp(float(-999.0 - B));
+        const float g = -kf * q.lnS;  // f (S^-k - 1) = e^-D - f
+        const float fl = g < 0.125f ? f0 * em1(g) : fast_exp(-D) - f0;
+        x = fmaf(fl, q.gS, wl);
+        y = ck;
+        Bl = Bn;
+    } else {
+        const float E = fast_exp(D);
+        x = fmaf(-em1(kf * q.lnS), q.gS, wl * E);
+        y = ck * E;
+        Bl = -999.0;
+    }
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) v[i] = LOG ? fast_ln(fmaf(y, base[i], x)) : fmaf(y, base[i], x);
+}
+
+// Block pass: the backward vector at untyped markers below the anchor, for
+// posteriors only.  V = X 1 + Y base up to a scale: per step
+// V' = (u V + t_o W)/S + F with the floor term F growing by 1/S, all three
+// rescaled by a power of two whenever F passes 1.
+struct BwdShow {
+    float X, Y, F, W0;
+    __device__ __forceinline__ void reset(float W0_, float f0) {
+        X = 0.0f;
+        Y = 1.0f;
+        F = f0;
+        W0 = W0_;
+    }
+    __device__ __forceinline__ void step(const URun& q, float Lf) {
+        const float W = fmaf(X, Lf, Y * W0);
+        const float Fn = F * q.rS;
+        const float Xn = fmaf(q.c1, X, fmaf(q.c2, W, Fn));
+        const float Yn = q.c1 * Y;
+        const int e = __builtin_amdgcn_frexp_expf(Fn);
+        const int sh = e > 0 ? -e : 0;
+        X = __builtin_amdgcn_ldexpf(Xn, sh);
+        Y = __builtin_amdgcn_ldexpf(Yn, sh);
+        F = __builtin_amdgcn_ldexpf(Fn, sh);
+    }
+};
+
 // ------------------------------------------------------------ block inputs --
 // One 16-marker load block of a lane's inputs: the sample's 16 genotype
 // codes, per marker the lane's state-table words and (per-marker
@@ -714,6 +941,116 @@ __device__ __forceinline__ uint32_t nib2(uint32_t h) {
     return x & 0xFFu;
 }
 
+// Sampling phase of a sub-block (both block-pass forms): Philox outputs for
+// its 16 markers, then lane r samples markers r, r + G, ... from the group's
+// LDS posterior rows and writes dosage, allele probabilities, haplotypes and
+// (tests) probabilities.  Every lane of the wavefront passes the wave_syncs.
+template <class S, bool SB, uint32_t kPer>
+__device__ __forceinline__ void block_sample(const JobView& J, bool active, uint32_t s, uint32_t m0,
+                                             uint32_t base, uint32_t nm, uint32_t r, const float* pb,
+                                             const uint32_t* sb, uint4* rng, const float (&ap0r)[kPer],
+                                             const float (&ap1r)[kPer]) {
+    using Lds = BlockLds<S>;
+    const bool argmax = (J.flags & SI_FB_ARGMAX) != 0;
+    const uint2 key = make_uint2(uint32_t(J.seed), uint32_t(J.seed >> 32));
+    uint4 rl[Lds::kRngLds ? 1 : kSub / 4];  // Philox outputs of a one- or two-lane group
+    if (active) {
+        if constexpr (Lds::kRngLds) {
+            for (uint32_t q = r; q < kSub / 4; q += S::G)
+                rng[q] = philox4x32_10(
+                    make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
+        } else if constexpr (S::G == 1) {
+#pragma unroll
+            for (uint32_t q = 0; q < kSub / 4; ++q)
+                rl[q] = philox4x32_10(
+                    make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
+        } else {
+            // lane r forms blocks r and r + 2, then takes its partner's
+            // (blocks 1 - r and 3 - r): rl[q] holds block q on both lanes
+#pragma unroll
+            for (uint32_t h = 0; h < 2; ++h) {
+                const uint4 own = philox4x32_10(
+                    make_uint4(((m0 + base) >> 2) + r + 2u * h, J.sample_base + s, J.rng_stream, 0u),
+                    key);
+                const uint4 oth = make_uint4(dpp_u32<0xB1>(own.x), dpp_u32<0xB1>(own.y),
+                                             dpp_u32<0xB1>(own.z), dpp_u32<0xB1>(own.w));
+                rl[2 * h] = r == 0 ? own : oth;
+                rl[2 * h + 1] = r == 0 ? oth : own;
+            }
+        }
+    }
+    wave_sync();
+    if (active) {
+        const uint32_t L = J.L;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; ++q) {
+            const uint32_t t = r + q * S::G;
+            const uint32_t tt = base + t;
+            if (t >= kSub || tt >= nm) continue;
+            const uint32_t m = m0 + tt;
+            const float* pt = pb + t * Lds::T;
+            uint32_t ks = 0;
+            if (argmax) {  // first maximum
+                float best = -1.0f;
+#pragma unroll
+                for (int k4 = 0; k4 < S::Lp; k4 += 4) {
+                    const float4 x = *reinterpret_cast<const float4*>(pt + k4);
+                    const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const bool gt = xs[i] > best;
+                        best = gt ? xs[i] : best;
+                        ks = gt ? uint32_t(k4 + i) : ks;
+                    }
+                }
+            } else {  // first k with u < cumsum, else 0 (padding states add 0)
+                uint32_t rv;
+                if constexpr (Lds::kRngLds) {
+                    rv = (&rng[t >> 2].x)[t & 3];
+                } else {  // block t >> 2, with t = q (one lane) or r + 2q (two lanes)
+                    const uint4 rb = rl[S::G == 1 ? (q >> 2) : (q >> 1)];
+                    const uint32_t c = t & 3u;
+                    rv = c == 0u ? rb.x : c == 1u ? rb.y : c == 2u ? rb.z : rb.w;
+                }
+                const float uu = u24(rv);
+                float cs = 0.0f;
+                uint32_t kd = 0xffffffffu;
+#pragma unroll
+                for (int k4 = 0; k4 < S::Lp; k4 += 4) {
+                    const float4 x = *reinterpret_cast<const float4*>(pt + k4);
+                    const float xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        cs += xs[i];
+                        kd = (kd == 0xffffffffu && uu < cs) ? uint32_t(k4 + i) : kd;
+                    }
+                }
+                ks = kd == 0xffffffffu ? 0u : kd;
+            }
+            const size_t o = size_t(s) * J.M + m;
+            if (J.ds) J.ds[o] = ap0r[q] + ap1r[q];
+            if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0r[q], ap1r[q]);
+            if (J.hap) {
+                // bits: alleles at 0-1, missing flags at 4-5 (the state byte's)
+                uint32_t b;
+                if (SB) {
+                    const uint32_t nb =
+                        (reinterpret_cast<const uint8_t*>(sb)[t * (S::Lp / 2) + (ks >> 1)] >> (4 * (ks & 1))) & 15u;
+                    b = (nb & 3u) | ((nb & 12u) << 2);
+                } else {
+                    b = J.sa[size_t(m) * S::Lp + ks];
+                }
+                J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 16u) ? 255u : (b & 1u));
+                J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 32u) ? 255u : ((b >> 1) & 1u));
+            }
+            if (J.prob)
+                #pragma unroll 1
+                for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pt[k];
+        }
+    }
+    wave_sync();  // the group's LDS rows are reused by the next sub-block
+}
+
 // ---------------------------------------------------------------------------
 // Checkpoint block c of sample s of job J, by one lane group.  For each
 // 16-marker sub-block j (last first): replay the forward vector from the
@@ -741,8 +1078,6 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
     const uint32_t sub = lane_sub<S>(r);
     const uint8_t* codes = J.codes + size_t(s) * J.pitch;
     const size_t ck = size_t(s) * J.nblk + c;
-    const bool argmax = (J.flags & SI_FB_ARGMAX) != 0;
-    const uint2 key = make_uint2(uint32_t(J.seed), uint32_t(J.seed >> 32));
     // offset in the block of a window start after its first marker (or none),
     // and the window's last forward vector, the restart there (in registers,
     // so the step loops issue no loads of their own)
@@ -765,7 +1100,6 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
     }
     BlockIn<S, LOG, PM> in;
     BlockEm<S, LOG, PM> em;
-    uint4 rl[Lds::kRngLds ? 1 : kSub / 4];  // Philox outputs of a one- or two-lane group
     for (int j = int(NS) - 1; j >= 0; --j) {
         const uint32_t base = uint32_t(j) * kSub;  // sub-block start within the block
         float ap0r[kPer], ap1r[kPer];
@@ -857,100 +1191,8 @@ __device__ __forceinline__ void run_block(const JobView& J, const Tab& tab, uint
             };
             if (nm == uint32_t(CB) && tws == 0xffffffffu) walk(std::true_type{});
             else walk(std::false_type{});
-            if constexpr (Lds::kRngLds) {
-                for (uint32_t q = r; q < kSub / 4; q += S::G)
-                    rng[q] = philox4x32_10(
-                        make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
-            } else if constexpr (S::G == 1) {
-#pragma unroll
-                for (uint32_t q = 0; q < kSub / 4; ++q)
-                    rl[q] = philox4x32_10(
-                        make_uint4(((m0 + base) >> 2) + q, J.sample_base + s, J.rng_stream, 0u), key);
-            } else {
-                // lane r forms blocks r and r + 2, then takes its partner's
-                // (blocks 1 - r and 3 - r): rl[q] holds block q on both lanes
-#pragma unroll
-                for (uint32_t h = 0; h < 2; ++h) {
-                    const uint4 own = philox4x32_10(
-                        make_uint4(((m0 + base) >> 2) + r + 2u * h, J.sample_base + s, J.rng_stream, 0u),
-                        key);
-                    const uint4 oth = make_uint4(dpp_u32<0xB1>(own.x), dpp_u32<0xB1>(own.y),
-                                                 dpp_u32<0xB1>(own.z), dpp_u32<0xB1>(own.w));
-                    rl[2 * h] = r == 0 ? own : oth;
-                    rl[2 * h + 1] = r == 0 ? oth : own;
-                }
-            }
         }
-        wave_sync();
-        if (active) {
-            const uint32_t L = J.L;
-#pragma unroll
-            for (uint32_t q = 0; q < kPer; ++q) {
-                const uint32_t t = r + q * S::G;
-                const uint32_t tt = base + t;
-                if (t >= kSub || tt >= nm) continue;
-                const uint32_t m = m0 + tt;
-                const float* pt = pb + t * Lds::T;
-                uint32_t ks = 0;
-                if (argmax) {  // first maximum
-                    float best = -1.0f;
-#pragma unroll
-                    for (int k4 = 0; k4 < S::Lp; k4 += 4) {
-                        const float4 x = *reinterpret_cast<const float4*>(pt + k4);
-                        const float xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const bool gt = xs[i] > best;
-                            best = gt ? xs[i] : best;
-                            ks = gt ? uint32_t(k4 + i) : ks;
-                        }
-                    }
-                } else {  // first k with u < cumsum, else 0 (padding states add 0)
-                    uint32_t rv;
-                    if constexpr (Lds::kRngLds) {
-                        rv = (&rng[t >> 2].x)[t & 3];
-                    } else {  // block t >> 2, with t = q (one lane) or r + 2q (two lanes)
-                        const uint4 rb = rl[S::G == 1 ? (q >> 2) : (q >> 1)];
-                        const uint32_t c = t & 3u;
-                        rv = c == 0u ? rb.x : c == 1u ? rb.y : c == 2u ? rb.z : rb.w;
-                    }
-                    const float uu = u24(rv);
-                    float cs = 0.0f;
-                    uint32_t kd = 0xffffffffu;
-#pragma unroll
-                    for (int k4 = 0; k4 < S::Lp; k4 += 4) {
-                        const float4 x = *reinterpret_cast<const float4*>(pt + k4);
-                        const float xs[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            cs += xs[i];
-                            kd = (kd == 0xffffffffu && uu < cs) ? uint32_t(k4 + i) : kd;
-                        }
-                    }
-                    ks = kd == 0xffffffffu ? 0u : kd;
-                }
-                const size_t o = size_t(s) * J.M + m;
-                if (J.ds) J.ds[o] = ap0r[q] + ap1r[q];
-                if (J.ap) reinterpret_cast<float2*>(J.ap)[o] = make_float2(ap0r[q], ap1r[q]);
-                if (J.hap) {
-                    // bits: alleles at 0-1, missing flags at 4-5 (the state byte's)
-                    uint32_t b;
-                    if (SB) {
-                        const uint32_t nb =
-                            (reinterpret_cast<const uint8_t*>(sb)[t * (S::Lp / 2) + (ks >> 1)] >> (4 * (ks & 1))) & 15u;
-                        b = (nb & 3u) | ((nb & 12u) << 2);
-                    } else {
-                        b = J.sa[size_t(m) * S::Lp + ks];
-                    }
-                    J.hap[size_t(s) * 2 * J.M + m] = uint8_t((b & 16u) ? 255u : (b & 1u));
-                    J.hap[size_t(s) * 2 * J.M + J.M + m] = uint8_t((b & 32u) ? 255u : ((b >> 1) & 1u));
-                }
-                if (J.prob)
-                    #pragma unroll 1
-                    for (uint32_t k = 0; k < L; ++k) J.prob[o * L + k] = pt[k];
-            }
-        }
-        wave_sync();  // the group's LDS rows are reused by the next sub-block
+        block_sample<S, SB, kPer>(J, active, s, m0, base, nm, r, pb, sb, rng, ap0r, ap1r);
     }
 }
 
@@ -1047,6 +1289,30 @@ __device__ __forceinline__ Chunk wave_take(unsigned long long* counter, uint64_t
     return Chunk{(uint64_t(hi) << 32) | lo, n};
 }
 
+// The job a whole wavefront works on, in scalar registers.
+__device__ __forceinline__ JobView uniform_job(const JobView* p) {
+    static_assert(sizeof(JobView) % 4 == 0, "JobView words");
+    JobView J;
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&J);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(JobView) / 4; ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return J;
+}
+
+__device__ __forceinline__ uint32_t find_job_uniform(const uint64_t* off, uint32_t n, uint64_t item) {
+    uint32_t lo = 0, hi = n;  // off[lo] <= item < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint64_t o = off[mid];
+        const uint64_t ou = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(o >> 32))) << 32) |
+                            __builtin_amdgcn_readfirstlane(uint32_t(o));
+        if (ou <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 template <int G, int R, bool LOG, int CB>
 __global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass bc, FbTables tb) {
     using S = Shape<G, R>;
@@ -1080,6 +1346,7 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass b
                 const uint32_t j = find_job(bc.chain_off, bc.n_jobs, 2 * gs);
                 const JobView J = bc.jobs[j];
                 const uint32_t q = uint32_t(gs - (bc.chain_off[j] >> 1));
+                if (q >= J.B) continue;  // a job's samples are padded to a multiple of 64
                 if (J.flags & SI_FB_PER_MARKER)
                     run_chain<S, LOG, true, CB>(J, tab, tb.P, q, fwd, r);
                 else
@@ -1087,30 +1354,6 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_batch_kernel(BatchClass b
             }
         }
     }
-}
-
-// The job a whole wavefront works on, in scalar registers.
-__device__ __forceinline__ JobView uniform_job(const JobView* p) {
-    static_assert(sizeof(JobView) % 4 == 0, "JobView words");
-    JobView J;
-    const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
-    uint32_t* dst = reinterpret_cast<uint32_t*>(&J);
-#pragma unroll
-    for (uint32_t i = 0; i < sizeof(JobView) / 4; ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
-    return J;
-}
-
-__device__ __forceinline__ uint32_t find_job_uniform(const uint64_t* off, uint32_t n, uint64_t item) {
-    uint32_t lo = 0, hi = n;  // off[lo] <= item < off[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        const uint64_t o = off[mid];
-        const uint64_t ou = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(o >> 32))) << 32) |
-                            __builtin_amdgcn_readfirstlane(uint32_t(o));
-        if (ou <= item) lo = mid;
-        else hi = mid;
-    }
-    return lo;
 }
 
 // Block items of each job are padded to whole wavefront rounds (the block
@@ -1157,6 +1400,492 @@ __global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_batch_ker
                 run_block<S, LOG, true, CB, true>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
             else
                 run_block<S, LOG, false, CB, true>(J, tab, tb.P, active, s, c, r, pb, sb, rg);
+        }
+    }
+}
+
+// ===========================================================================
+// Fast jobs: constant transitions, no windows, an untyped code.  The chain
+// pass walks only the job's typed markers and its 16-aligned anchors; the
+// block pass takes an untyped marker in O(1) per lane apart from the
+// posterior itself.  Checkpoints are block-major ([block][sample][Lp]) and
+// block-pass items run samples fastest, so the groups of a wavefront share a
+// checkpoint block and every branch on the job's typed markers is
+// wavefront-uniform.
+// ===========================================================================
+
+// The job's typed markers 1..M-1 in order, from its mask: one workgroup of
+// kListThreads per job, a workgroup-wide scan of the blocks' popcounts.
+constexpr uint32_t kListThreads = 256;
+__device__ __forceinline__ void tlist_build(const uint32_t* tmask, uint32_t M, uint32_t* tlist,
+                                            uint32_t* tcount) {
+    __shared__ uint32_t sc[kListThreads];
+    const uint32_t nlb = (M + kSub - 1) / kSub;
+    uint32_t base = 0;
+    for (uint32_t c0 = 0; c0 < nlb; c0 += kListThreads) {
+        const uint32_t c = c0 + threadIdx.x;
+        uint32_t bits = c < nlb ? tmask[c] : 0u;
+        if (c == 0) bits &= ~1u;  // marker 0 starts the forward, it is not a step
+        const uint32_t n = __popc(bits);
+        sc[threadIdx.x] = n;
+        __syncthreads();
+        for (uint32_t d = 1; d < kListThreads; d <<= 1) {
+            const uint32_t x = threadIdx.x >= d ? sc[threadIdx.x - d] : 0u;
+            __syncthreads();
+            sc[threadIdx.x] += x;
+            __syncthreads();
+        }
+        uint32_t o = base + sc[threadIdx.x] - n;
+        while (bits) {
+            tlist[o++] = kSub * c + uint32_t(__ffs(bits)) - 1u;
+            bits &= bits - 1u;
+        }
+        base += sc[kListThreads - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *tcount = base;
+}
+
+// A typed marker's inputs for one lane: the sample's code and the lane's
+// state-table words.
+template <class S>
+struct MarkIn {
+    uint32_t code;
+    LaneWords<S> sw;
+};
+template <class S>
+__device__ __forceinline__ void mark_load(const JobView& J, const uint8_t* codes, uint32_t r, uint32_t m,
+                                          MarkIn<S>& x) {
+    x.code = codes[m];
+    const uint32_t* p =
+        reinterpret_cast<const uint32_t*>(J.sa) + size_t(m) * (uint32_t(S::Lp) / 4u) + (r * uint32_t(S::R)) / 4u;
+#pragma unroll
+    for (int w = 0; w < S::NW; ++w) x.sw.w[w] = p[w];
+}
+
+template <bool LOG>
+__device__ __forceinline__ typename CoefT<LOG>::type const_coef(const JobView& J) {
+    if constexpr (LOG) return J.step;
+    else return make_float2(J.step.x, J.step.y);
+}
+
+// One chain of a fast job.  The vector stays at an anchor marker; a
+// closed-form run brings it up to each typed marker (whose step is exact)
+// and to each 16-aligned marker (forward: the marker itself; backward: the
+// marker before it), where checkpoints are stored.  Inputs of the next K
+// typed markers are in flight.
+template <class S, bool LOG, int CB>
+__device__ __forceinline__ void run_chain_fast(const JobView& J, const Tab& tab, uint32_t P, uint32_t s,
+                                               bool fwd, uint32_t r) {
+    constexpr uint32_t K = 8;
+    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
+    const uint32_t sub = lane_sub<S>(r);
+    const size_t vo = size_t(r) * S::R;
+    const URun& q = J.ur;
+    float rm[S::R];
+    real_mask<S>(r, J.L, rm);
+    const typename CoefT<LOG>::type k = const_coef<LOG>(J);
+    const uint32_t T = __builtin_amdgcn_readfirstlane(*J.tcount);
+    const uint32_t* tl = J.tlist;
+    const uint32_t last = J.M - 1u;
+    MarkIn<S> ring[K];
+    if (fwd) {
+#pragma unroll
+        for (uint32_t j = 0; j < K; ++j)
+            if (j < T) mark_load<S>(J, codes, r, tl[j], ring[j]);
+        float a[S::R];
+        double ll;
+        {
+            MarkIn<S> x0;
+            mark_load<S>(J, codes, r, 0, x0);
+            float e[S::R];
+            const float off = lane_emission<S>(tab, P, x0.code, x0.sw, sub, e);
+            ll = fwd_init<S, LOG>(a, e, off);
+        }
+        store_vec<S::R>(J.ck_alpha + size_t(s) * S::Lp + vo, a);  // block 0
+        uint32_t pa = 0, q16 = kSub;
+        auto anchors_below = [&](uint32_t lim) {
+            while (q16 < lim) {
+                ll += fwd_run<S, LOG>(a, rm, q16 - pa, q);
+                pa = q16;
+                if (q16 % uint32_t(CB) == 0u)
+                    store_vec<S::R>(J.ck_alpha + (size_t(q16 / CB) * J.B + s) * S::Lp + vo, a);
+                q16 += kSub;
+            }
+        };
+        for (uint32_t i0 = 0; i0 < T; i0 += K) {
+#pragma unroll
+            for (uint32_t j = 0; j < K; ++j) {
+                const uint32_t i = i0 + j;
+                if (i < T) {
+                    const uint32_t mt = tl[i];
+                    anchors_below(mt);
+                    ll += fwd_run<S, LOG>(a, rm, mt - 1u - pa, q);
+                    float e[S::R];
+                    const float off = lane_emission<S>(tab, P, ring[j].code, ring[j].sw, sub, e);
+                    ll += fwd_step<S, LOG>(a, e, off, k);
+                    pa = mt;
+                    if (i + K < T) mark_load<S>(J, codes, r, tl[i + K], ring[j]);
+                }
+            }
+        }
+        anchors_below(J.M);
+        ll += fwd_run<S, LOG>(a, rm, last - pa, q);
+        if (LOG) ll += double(log_mass<S>(a));  // log mass of the last normalized vector
+        if (r == 0 && J.loglik) J.loglik[s] = ll;
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < K; ++j)
+            if (j < T) mark_load<S>(J, codes, r, tl[T - 1u - j], ring[j]);
+        float v[S::R];
+        bwd_init<S, LOG>(v);
+        double Bl = 0.0;
+        {
+            const size_t b = size_t(J.nblk - 1u) * J.B + s;
+            store_vec<S::R>(J.ck_beta + b * S::Lp + vo, v);
+            if (r == 0) J.ck_bl[b] = Bl;
+        }
+        uint32_t pb = last, q16 = last / kSub * kSub;
+        // anchors at q16 - 1 (after the step out of q16) for 16-aligned q16 > lim
+        auto anchors_above = [&](uint32_t lim) {
+            while (q16 > lim && q16 >= kSub) {
+                bwd_run<S, LOG>(v, Bl, rm, pb - (q16 - 1u), q);
+                pb = q16 - 1u;
+                if (q16 % uint32_t(CB) == 0u) {
+                    const size_t b = size_t(q16 / CB - 1u) * J.B + s;
+                    store_vec<S::R>(J.ck_beta + b * S::Lp + vo, v);
+                    if (r == 0) J.ck_bl[b] = Bl;
+                }
+                q16 -= kSub;
+            }
+        };
+        for (uint32_t i0 = 0; i0 < T; i0 += K) {
+#pragma unroll
+            for (uint32_t j = 0; j < K; ++j) {
+                const uint32_t i = i0 + j;
+                if (i < T) {
+                    const uint32_t n = tl[T - 1u - i];
+                    anchors_above(n);
+                    bwd_run<S, LOG>(v, Bl, rm, pb - n, q);
+                    float e[S::R];
+                    const float off = lane_emission<S>(tab, P, ring[j].code, ring[j].sw, sub, e);
+                    bwd_step<S, LOG>(v, Bl, e, off, k);
+                    pb = n - 1u;
+                    if (i + K < T) mark_load<S>(J, codes, r, tl[T - 1u - (i + K)], ring[j]);
+                }
+            }
+        }
+        anchors_above(0u);
+    }
+}
+
+// Checkpoint block c of sample s of a fast job, by one lane group: as
+// run_block, with the forward and backward vectors carried at anchors (the
+// same runs as the chain pass) and, at untyped markers past an anchor, formed
+// for the posterior by per-marker recurrences that are never carried.  Typed
+// markers read their emissions when they step.
+template <class S, bool LOG, int CB, bool SB>
+__device__ __forceinline__ void run_block_fast(const JobView& J, const Tab& tab, uint32_t P, bool active,
+                                               uint32_t s, uint32_t c, uint32_t r, float* pb, uint32_t* sb,
+                                               uint4* rng) {
+    using Lds = BlockLds<S>;
+    constexpr uint32_t NS = uint32_t(CB) / kSub;
+    float* const prow = pb + r * S::R;
+    constexpr uint32_t kPer = (kSub + uint32_t(S::G) - 1) / uint32_t(S::G);
+    const uint32_t m0 = c * CB;
+    const uint32_t nm = active ? min(uint32_t(CB), J.M - m0) : 0u;
+    const uint32_t sub = lane_sub<S>(r);
+    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
+    const size_t ck = size_t(c) * J.B + s;  // block-major
+    const uint32_t last = J.M - 1u;
+    const URun& q = J.ur;
+    const float Lf = float(J.L);
+    float rm[S::R];
+    real_mask<S>(r, J.L, rm);
+    const typename CoefT<LOG>::type k = const_coef<LOG>(J);
+    float v[S::R];
+    double Bl = 0.0;
+    if (active) {
+        load_vec<S::R>(J.ck_beta + ck * S::Lp + size_t(r) * S::R, v);
+        Bl = J.ck_bl[ck];
+    }
+    for (int j = int(NS) - 1; j >= 0; --j) {
+        const uint32_t base = uint32_t(j) * kSub;
+        float ap0r[kPer], ap1r[kPer];
+        BlockIn<S, LOG, false> cur;
+        auto post_out = [&](uint32_t t, const float (&p)[S::R]) {
+            float x0[S::R], x1[S::R];
+#pragma unroll
+            for (int i = 0; i < S::R; ++i) {
+                const uint32_t pi = __float_as_uint(p[i]);
+                x0[i] = __uint_as_float(pi & cur.sa[t].bitmask(i, sub, 0));
+                x1[i] = __uint_as_float(pi & cur.sa[t].bitmask(i, sub, 1));
+            }
+            float ap0 = tree_sum<S::R>(x0), ap1 = tree_sum<S::R>(x1);
+            gsum2<S::G>(ap0, ap1);
+            store_vec<S::R>(prow + t * Lds::T, p);
+            if (t % S::G == r) {
+                ap0r[t / S::G] = ap0;
+                ap1r[t / S::G] = ap1;
+            }
+        };
+        if (active) {
+            float a[S::R];
+            load_vec<S::R>(J.ck_alpha + ck * S::Lp + size_t(r) * S::R, a);
+            uint32_t pa = m0;
+            // forward through sub-blocks 0..j-1 (carried only), then j
+            cur.load(J, codes, r, m0);
+            for (uint32_t u = 0; u < uint32_t(j); ++u) {
+                BlockIn<S, LOG, false> nxt;
+                nxt.load(J, codes, r, m0 + (u + 1) * kSub);
+                const uint32_t tm = __builtin_amdgcn_readfirstlane(J.tmask[m0 / kSub + u]);
+#pragma unroll
+                for (uint32_t t = 0; t < kSub; ++t) {
+                    const uint32_t n = m0 + u * kSub + t;
+                    if (n == m0 || n > last) continue;  // the checkpoint itself; past the job
+                    if ((tm >> t) & 1u) {
+                        fwd_run<S, LOG>(a, rm, n - 1u - pa, q);
+                        float e[S::R];
+                        const float off = lane_emission<S>(tab, P, cur.code(t), cur.sa[t], sub, e);
+                        fwd_step<S, LOG>(a, e, off, k);
+                        pa = n;
+                    }
+                    if (t == 0) {
+                        fwd_run<S, LOG>(a, rm, n - pa, q);
+                        pa = n;
+                    }
+                }
+                cur = nxt;
+            }
+            const uint32_t tm = __builtin_amdgcn_readfirstlane(J.tmask[m0 / kSub + uint32_t(j)]);
+            // Sub-block j into LDS rows: the vector at an anchor; past one, the
+            // composition al a + be 1_L off it (log form: of e^a / sum e^a, in
+            // linear space, flagged in fpm).
+            float dal = 1.0f, dbe = 0.0f;
+            float dw[LOG ? S::R : 1];
+            uint32_t fpm = 0;
+#pragma unroll
+            for (uint32_t t = 0; t < kSub; ++t) {
+                if constexpr (!SB) {
+                } else if constexpr (S::R == 4) {
+                    reinterpret_cast<uint16_t*>(sb)[t * (S::Lp / 4) + r] = uint16_t(nib4(cur.sa[t].w[0]));
+                } else {
+                    static_assert(S::R == 2, "block pass lane shapes");
+                    reinterpret_cast<uint8_t*>(sb)[t * (S::Lp / 2) + r] =
+                        uint8_t(nib2(cur.sa[t].w[0] >> (8 * sub)));
+                }
+                const uint32_t n = m0 + base + t;
+                if (n > last) continue;
+                if (n != m0) {
+                    if ((tm >> t) & 1u) {
+                        fwd_run<S, LOG>(a, rm, n - 1u - pa, q);
+                        float e[S::R];
+                        const float off = lane_emission<S>(tab, P, cur.code(t), cur.sa[t], sub, e);
+                        fwd_step<S, LOG>(a, e, off, k);
+                        pa = n;
+                    }
+                    if (t == 0) {
+                        fwd_run<S, LOG>(a, rm, n - pa, q);
+                        pa = n;
+                    }
+                }
+                float row[S::R];
+                if (pa == n) {
+#pragma unroll
+                    for (int i = 0; i < S::R; ++i) row[i] = a[i];
+                } else {
+                    if (pa + 1u == n) {
+                        dal = 1.0f;
+                        dbe = 0.0f;
+                        if constexpr (LOG) {
+#pragma unroll
+                            for (int i = 0; i < S::R; ++i) dw[i] = fast_exp(a[i]);
+                            const float rw = __builtin_amdgcn_rcpf(vsum<S>(dw));
+#pragma unroll
+                            for (int i = 0; i < S::R; ++i) dw[i] *= rw;
+                        }
+                    }
+                    dal *= q.c1;
+                    dbe = fmaf(dbe, q.c1, q.c2);
+#pragma unroll
+                    for (int i = 0; i < S::R; ++i) row[i] = fmaf(dal, LOG ? dw[i] : a[i], dbe * rm[i]);
+                    if constexpr (LOG) fpm |= 1u << t;
+                }
+                store_vec<S::R>(prow + t * Lds::T, row);
+            }
+            // Backward walk down sub-block j: the anchor pb; at an untyped
+            // marker below it the posterior's vector comes from BwdShow.
+            uint32_t pbk = min(m0 + base + kSub - 1u, last);
+            BwdShow bs;
+            float bb[S::R];
+#pragma unroll
+            for (int t = int(kSub) - 1; t >= 0; --t) {
+                const uint32_t n = m0 + base + uint32_t(t);
+                if (n > last) continue;
+                const bool typed = (tm >> t) & 1u;
+                if (typed) {
+                    bwd_run<S, LOG>(v, Bl, rm, pbk - n, q);
+                    pbk = n;
+                }
+                float af[S::R], vb[S::R], p[S::R];
+                load_vec<S::R>(prow + t * Lds::T, af);
+                const bool bshow = pbk != n;
+                if constexpr (LOG) {
+                    const bool fshow = (fpm >> t) & 1u;
+                    if (bshow) {
+#pragma unroll
+                        for (int i = 0; i < S::R; ++i) vb[i] = fmaf(bs.Y, bb[i], bs.X);
+                    }
+                    if (fshow && bshow) {
+                        posterior<S, false>(af, vb, p);
+                    } else {
+                        if (fshow) {
+#pragma unroll
+                            for (int i = 0; i < S::R; ++i) af[i] = fast_ln(af[i]);
+                        }
+#pragma unroll
+                        for (int i = 0; i < S::R; ++i) vb[i] = bshow ? fast_ln(vb[i]) : v[i];
+                        posterior<S, true>(af, vb, p);
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < S::R; ++i) vb[i] = bshow ? fmaf(bs.Y, bb[i], bs.X) : v[i];
+                    posterior<S, false>(af, vb, p);
+                }
+                post_out(uint32_t(t), p);
+                if (n > m0) {  // the step out of the block's first marker belongs to the block before
+                    if (typed) {
+                        float e[S::R];
+                        const float off = lane_emission<S>(tab, P, cur.code(uint32_t(t)), cur.sa[t], sub, e);
+                        bwd_step<S, LOG>(v, Bl, e, off, k);
+                        pbk = n - 1u;
+                    } else {
+                        if (pbk == n) {
+                            double B;
+                            const float W0 = bwd_base<S, LOG>(v, Bl, rm, bb, B);
+                            bs.reset(W0, fast_exp(float(-999.0 - B)));
+                        }
+                        bs.step(q, Lf);
+                    }
+                    if (t == 0) {
+                        bwd_run<S, LOG>(v, Bl, rm, pbk - (n - 1u), q);
+                        pbk = n - 1u;
+                    }
+                }
+            }
+        }
+        block_sample<S, SB, kPer>(J, active, s, m0, base, nm, r, pb, sb, rng, ap0r, ap1r);
+    }
+}
+
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_fast_kernel(JobView J, FbTables tb, uint32_t Bpad) {
+    using S = Shape<G, R>;
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / G, r = gtid % G;
+    const bool fwd = grp < Bpad;
+    const uint32_t s = fwd ? grp : grp - Bpad;
+    if (s >= J.B) return;  // whole lane groups leave together
+    run_chain_fast<S, LOG, CB>(J, tab, tb.P, s, fwd, r);
+}
+
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_fast_kernel(JobView J, FbTables tb) {
+    using S = Shape<G, R>;
+    using Lds = BlockLds<S>;
+    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
+    constexpr bool SB = Lds::kSbPlain;
+    __shared__ uint32_t sbuf[S::GPB * (SB ? Lds::SW : 1)];
+    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / G, r = gtid % G, gl = threadIdx.x / G;
+    // samples fastest, padded to whole wavefronts: the groups of a wavefront
+    // share a checkpoint block
+    const uint32_t Bp = (J.B + S::GPW - 1) / S::GPW * S::GPW;
+    const uint32_t s0 = grp % Bp, c0 = grp / Bp;
+    const bool active = s0 < J.B && c0 < J.nblk;
+    run_block_fast<S, LOG, CB, SB>(J, tab, tb.P, active, active ? s0 : 0u, c0 < J.nblk ? c0 : 0u, r,
+                                   pbuf + gl * Lds::PS, sbuf + (SB ? gl * Lds::SW : 0u),
+                                   rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u));
+}
+
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_fast_batch_kernel(BatchClass bc, FbTables tb) {
+    using S = Shape<G, R>;
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
+    const uint32_t m_max = bc.n_jobs ? bc.jobs[0].M : 0u;  // jobs are sorted longest first
+    // as fb_chain_batch_kernel; a job's samples are padded to a multiple of
+    // 64, so a round of GPW groups never spans two jobs
+    const uint64_t n_samples = bc.chain_items >> 1;
+    const uint64_t total = 2 * n_samples;
+    for (;;) {
+        const Chunk ch = wave_take(&bc.counters[0], total, S::GPW, 1);
+        if (ch.base >= total) break;
+        const uint64_t first = (ch.base / S::GPW >> 1) * S::GPW;
+        const uint32_t j0 = find_job_uniform(bc.chain_off, bc.n_jobs, 2 * first);
+        if (2u * bc.jobs[j0].M >= m_max)
+            __builtin_amdgcn_s_setprio(3);
+        else
+            __builtin_amdgcn_s_setprio(0);
+        for (uint32_t kk = 0; kk < ch.n; kk += S::GPW) {
+            const uint64_t round = (ch.base + kk) / S::GPW;
+            const uint64_t gs0 = (round >> 1) * S::GPW;
+            const bool fwd = (round & 1) == 0;
+            if (gs0 >= n_samples) continue;
+            const uint32_t j = find_job_uniform(bc.chain_off, bc.n_jobs, 2 * gs0);
+            const JobView J = uniform_job(bc.jobs + j);
+            const uint32_t q = uint32_t(gs0 - (bc.chain_off[j] >> 1)) + gw;
+            if (q < J.B) run_chain_fast<S, LOG, CB>(J, tab, tb.P, q, fwd, r);
+        }
+    }
+}
+
+template <int G, int R, bool LOG, int CB>
+__global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_fast_batch_kernel(BatchClass bc,
+                                                                                         FbTables tb) {
+    using S = Shape<G, R>;
+    using Lds = BlockLds<S>;
+    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
+    __shared__ uint32_t sbuf[S::GPB * Lds::SW];
+    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
+    const Tab tab = load_tables<LOG>(tb);
+    __syncthreads();
+    const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
+    const uint32_t gl = threadIdx.x / G;
+    float* pb = pbuf + gl * Lds::PS;
+    uint32_t* sb = sbuf + gl * Lds::SW;
+    uint4* rg = rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u);
+    uint64_t lo = 0, hi = 0;  // item range of the job in J (wavefront-uniform)
+    JobView J{};
+    for (;;) {
+        const Chunk ch = wave_take(&bc.counters[1], bc.block_items, S::GPW, 32);
+        if (ch.base >= bc.block_items) break;
+        for (uint32_t kk = 0; kk < ch.n; kk += S::GPW) {
+            const uint64_t round = ch.base + kk;
+            if (round >= bc.block_items) break;
+            if (round < lo || round >= hi) {
+                const uint32_t j = find_job_uniform(bc.block_off, bc.n_jobs, round);
+                J = uniform_job(bc.jobs + j);
+                lo = bc.block_off[j];
+                hi = bc.block_off[j + 1];
+                lo = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(lo >> 32))) << 32) |
+                     __builtin_amdgcn_readfirstlane(uint32_t(lo));
+                hi = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(hi >> 32))) << 32) |
+                     __builtin_amdgcn_readfirstlane(uint32_t(hi));
+            }
+            // samples fastest within the job, padded to a multiple of 64
+            const uint64_t qq = round - lo + gw;
+            const uint32_t Bp = uint32_t(fb_batch_padded_samples(J.B));
+            const uint32_t s0 = uint32_t(qq % Bp), c0 = uint32_t(qq / Bp);
+            const bool active = s0 < J.B && c0 < J.nblk;
+            run_block_fast<S, LOG, CB, true>(J, tab, tb.P, active, active ? s0 : 0u, c0 < J.nblk ? c0 : 0u, r,
+                                             pb, sb, rg);
         }
     }
 }

@@ -2,6 +2,7 @@
 // lane shape in fb_inst.hip): the kernel registry, workspace layout, job
 // views, variants and the plain and batched launch sequences.
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -54,7 +55,7 @@ uint32_t fb_candidates(uint32_t Lp, int pass, uint32_t* lanes, uint32_t max) {
 }
 
 bool fb_variant_valid(uint32_t Lp, const FbVariant& v) {
-    if (v.cb != 16 && v.cb != 32) return false;
+    if (v.cb != 16) return false;  // 16-marker checkpoint blocks only
     const FbImpl* c = fb_impl(Lp, v.gc);
     const FbImpl* b = fb_impl(Lp, v.gb);
     return c && b && b->has_block;
@@ -68,22 +69,78 @@ size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 // Checkpoint vectors (forward, backward), backward scales and, with windows,
 // one forward vector per window, each region 256-byte aligned.  Sized at the
 // smallest checkpoint block, so one workspace serves every variant.
+// Checkpoint vectors (forward, backward), backward scales, with windows one
+// forward vector per window, and the typed-marker mask, each region 256-byte
+// aligned.  Sized at the smallest checkpoint block, so one workspace serves
+// every variant.
 size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp, uint32_t W) {
     const size_t nblk = (size_t(M) + kMinCkptBlock - 1) / kMinCkptBlock;
     const size_t vec = align256(size_t(B) * nblk * Lp * sizeof(float));
     const size_t bl = align256(size_t(B) * nblk * sizeof(double));
     const size_t nwin = W ? (size_t(M) + W - 1) / W : 0;
-    return 2 * vec + bl + align256(size_t(B) * nwin * Lp * sizeof(float));
+    const size_t mask = align256((size_t(M) + 15) / 16 * sizeof(uint32_t));
+    const size_t list = align256((size_t(M) + 1) * sizeof(uint32_t));
+    return 2 * vec + bl + align256(size_t(B) * nwin * Lp * sizeof(float)) + mask + list;
 }
 
 void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws) {
     const size_t nblk = (size_t(v.M) + kMinCkptBlock - 1) / kMinCkptBlock;
     const size_t vec = align256(size_t(v.B) * nblk * Lp * sizeof(float));
     const size_t bl = align256(size_t(v.B) * nblk * sizeof(double));
+    const size_t win = align256(size_t(v.B) * v.nwin * Lp * sizeof(float));
     v.ck_alpha = reinterpret_cast<float*>(ws);
     v.ck_beta = reinterpret_cast<float*>(ws + vec);
     v.ck_bl = reinterpret_cast<double*>(ws + 2 * vec);
     v.ck_win = v.nwin ? reinterpret_cast<float*>(ws + 2 * vec + bl) : nullptr;
+    const size_t mask = align256((size_t(v.M) + 15) / 16 * sizeof(uint32_t));
+    v.tmask = reinterpret_cast<uint32_t*>(ws + 2 * vec + bl + win);
+    v.tcount = reinterpret_cast<uint32_t*>(ws + 2 * vec + bl + win + mask);
+    v.tlist = v.tcount + 1;
+}
+
+// ------------------------------------------------------ typed-marker mask --
+// Before a fast job's chain pass: its mask of typed markers per load block,
+// then the list of those markers.
+__global__ __launch_bounds__(kMaskThreads) void fb_tmask_kernel(JobView J, uint32_t uc) {
+    tmask_tile(J.codes, J.B, J.M, J.pitch, uc, blockIdx.x * kMaskBlocks, J.tmask);
+}
+
+__global__ __launch_bounds__(kListThreads) void fb_tlist_kernel(JobView J) {
+    tlist_build(J.tmask, J.M, J.tlist, J.tcount);
+}
+
+__global__ __launch_bounds__(kMaskThreads) void fb_tmask_batch_kernel(const JobView* jobs,
+                                                                     const uint64_t* off, uint32_t n,
+                                                                     uint32_t uc) {
+    const uint32_t j = find_job_uniform(off, n, blockIdx.x);
+    const JobView* J = jobs + j;
+    tmask_tile(J->codes, J->B, J->M, J->pitch, uc, uint32_t(blockIdx.x - off[j]) * kMaskBlocks, J->tmask);
+}
+
+__global__ __launch_bounds__(kListThreads) void fb_tlist_batch_kernel(const JobView* jobs) {
+    const JobView* J = jobs + blockIdx.x;
+    tlist_build(J->tmask, J->M, J->tlist, J->tcount);
+}
+
+URun fb_urun(const si_fb_job& j) {
+    URun q{};
+    if (j.flags & SI_FB_PER_MARKER) return q;
+    const double to = j.step.t_o, u = j.step.u, L = double(j.n_states);
+    const double S = L * to + u;
+    const bool log = (j.flags & SI_FB_LOGSPACE) != 0;
+    if (!(to > 0.0 && u >= 0.0 && S > 0.0 && S < 1.0 && (!log || to >= 0x1p-30))) return q;
+    const double c = u / S;
+    q.c1 = float(c);
+    q.c2 = float(to / S);
+    q.rS = float(1.0 / S);
+    q.rL = float(1.0 / L);
+    q.lc1 = float(std::log2(c));
+    q.lnc1 = float(std::log(c));
+    q.lnS = float(std::log(S));
+    q.gS = float(1.0 / (1.0 - S));
+    q.lnSd = std::log(S);
+    q.ok = 1u;
+    return q;
 }
 
 JobView fb_job_view(const si_fb_job& j, uint32_t cb) {
@@ -109,7 +166,15 @@ JobView fb_job_view(const si_fb_job& j, uint32_t cb) {
     v.hap = j.hap;
     v.prob = j.prob;
     v.loglik = j.loglik;
+    v.ur = fb_urun(j);
     return v;
+}
+
+// Whether a launch runs the untyped-step kernels: a fast job (untyped code
+// in the tables, constant transitions, no windows) at 16-marker checkpoint
+// blocks; otherwise the exact kernels.
+static bool fast_launch(const JobView& v, const FbTables& t, const FbVariant& var) {
+    return v.ur.ok != 0u && v.W == 0u && t.ucode < t.P && t.ucode < 256u && var.cb == 16u;
 }
 
 // ----------------------------------------------------------- plain form --
@@ -118,6 +183,13 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
     if (v.B == 0 || v.M == 0) return 0;
     if (!fb_variant_valid(Lp, var)) return SI_ERR_UNSUPPORTED;
     const bool log = (v.flags & SI_FB_LOGSPACE) != 0;
+    const bool fast = fast_launch(v, t, var);
+    if ((passes & SI_FB_PASS_CHAIN) && fast) {
+        hipLaunchKernelGGL(fb_tmask_kernel, dim3(uint32_t(fb_tmask_groups(v.M))), dim3(kMaskThreads), 0,
+                           stream, v, t.ucode);
+        hipLaunchKernelGGL(fb_tlist_kernel, dim3(1), dim3(kListThreads), 0, stream, v);
+        if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
+    }
     if (passes & SI_FB_PASS_CHAIN) {
         const FbImpl* f = fb_impl(Lp, var.gc);
         const uint32_t gpw = kWave / var.gc;  // groups per wavefront
@@ -125,16 +197,19 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
         const size_t threads = size_t(Bpad + v.B) * var.gc;
         const size_t grid = (threads + kFbThreads - 1) / kFbThreads;
         if (grid > 0x7fffffffu) return SI_ERR_UNSUPPORTED;
-        FbLaunch L{kFbPlainChain, log, var.cb, &v, Bpad, nullptr, t, uint32_t(grid), stream};
+        FbLaunch L{kFbPlainChain, log, var.cb, fast, &v, Bpad, nullptr, t, uint32_t(grid), stream};
         const int rc = f->launch(L);
         if (rc) return rc;
     }
     if (passes & SI_FB_PASS_BLOCK) {
         const FbImpl* f = fb_impl(Lp, var.gb);
-        const size_t threads = size_t(v.B) * v.nblk * var.gb;
+        // fast: samples fastest, padded to whole wavefronts (fb_block_fast_kernel)
+        const uint32_t gpw = kWave / var.gb;
+        const size_t Bp = fast ? (size_t(v.B) + gpw - 1) / gpw * gpw : size_t(v.B);
+        const size_t threads = Bp * v.nblk * var.gb;
         const size_t grid = (threads + kFbThreads - 1) / kFbThreads;
         if (grid > 0x7fffffffu) return SI_ERR_UNSUPPORTED;
-        FbLaunch L{kFbPlainBlock, log, var.cb, &v, 0, nullptr, t, uint32_t(grid), stream};
+        FbLaunch L{kFbPlainBlock, log, var.cb, fast, &v, 0, nullptr, t, uint32_t(grid), stream};
         const int rc = f->launch(L);
         if (rc) return rc;
     }
@@ -147,16 +222,21 @@ __global__ void zero_words_kernel(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0u;
 }
 
+static bool fast_class(const BatchClass& c, const FbTables& t, const FbVariant& var) {
+    return (c.flags & kFbFastClass) && t.ucode < t.P && t.ucode < 256u && var.cb == 16u;
+}
+
 void fb_batch_grid(BatchClass& c, const FbVariant& var, uint32_t P) {
     int dev = 0, cus = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const uint64_t ncu = uint64_t(cus > 0 ? cus : 1);
     const bool log = (c.flags & SI_FB_LOGSPACE) != 0;
+    const bool fast = (c.flags & kFbFastClass) && var.cb == 16u;
     const FbImpl* fc = fb_impl(c.Lp, var.gc);
     const FbImpl* fbk = fb_impl(c.Lp, var.gb);
-    const uint64_t occ_c = fc ? uint64_t(fc->occupancy(kFbBatchChain, log, var.cb, P)) : 1;
-    const uint64_t occ_b = fbk ? uint64_t(fbk->occupancy(kFbBatchBlock, log, var.cb, P)) : 1;
+    const uint64_t occ_c = fc ? uint64_t(fc->occupancy(kFbBatchChain, log, var.cb, P, fast)) : 1;
+    const uint64_t occ_b = fbk ? uint64_t(fbk->occupancy(kFbBatchBlock, log, var.cb, P, fast)) : 1;
     const uint64_t gpc = kFbThreads / var.gc, gpb = kFbThreads / var.gb;  // groups per workgroup
     const uint64_t need_c = (c.chain_items + gpc - 1) / gpc, need_b = (c.block_items + gpb - 1) / gpb;
     c.grid_chain = uint32_t(std::max<uint64_t>(1, std::min(need_c, ncu * occ_c)));
@@ -167,16 +247,23 @@ int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, con
                     hipStream_t stream) {
     if (!fb_variant_valid(c.Lp, var)) return SI_ERR_UNSUPPORTED;
     const bool log = (c.flags & SI_FB_LOGSPACE) != 0;
+    const bool fast = fast_class(c, t, var);
     hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream,
                        reinterpret_cast<uint32_t*>(c.counters), 4u);
     if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
+    if ((passes & SI_FB_PASS_CHAIN) && c.chain_items && fast) {
+        hipLaunchKernelGGL(fb_tmask_batch_kernel, dim3(uint32_t(c.mask_groups)), dim3(kMaskThreads), 0,
+                           stream, c.jobs, c.mask_off, c.n_jobs, t.ucode);
+        hipLaunchKernelGGL(fb_tlist_batch_kernel, dim3(c.n_jobs), dim3(kListThreads), 0, stream, c.jobs);
+        if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
+    }
     if ((passes & SI_FB_PASS_CHAIN) && c.chain_items) {
-        FbLaunch L{kFbBatchChain, log, var.cb, nullptr, 0, &c, t, c.grid_chain, stream};
+        FbLaunch L{kFbBatchChain, log, var.cb, fast, nullptr, 0, &c, t, c.grid_chain, stream};
         const int rc = fb_impl(c.Lp, var.gc)->launch(L);
         if (rc) return rc;
     }
     if ((passes & SI_FB_PASS_BLOCK) && c.block_items) {
-        FbLaunch L{kFbBatchBlock, log, var.cb, nullptr, 0, &c, t, c.grid_block, stream};
+        FbLaunch L{kFbBatchBlock, log, var.cb, fast, nullptr, 0, &c, t, c.grid_block, stream};
         const int rc = fb_impl(c.Lp, var.gb)->launch(L);
         if (rc) return rc;
     }

@@ -14,6 +14,19 @@ constexpr uint32_t kMinCkptBlock = 16;
 // Largest genotype-likelihood palette the kernels stage in LDS.
 constexpr uint32_t kMaxPalette = 64;
 
+// Constants of the untyped step (every real state's emission 1) under the
+// job's constant transitions, computed on the host in FP64: S = L t_o + u,
+// c = u / S.  ok = 0 disables untyped steps (per-marker transitions, or a
+// shape outside 0 < S < 1; log form: t_o < 2^-30).
+struct URun {
+    float c1, c2;        // u / S, t_o / S
+    float rS, rL;        // 1 / S, 1 / L
+    float lc1, lnc1;     // log2 c, ln c
+    float lnS, gS;       // ln S, 1 / (1 - S)
+    uint32_t ok;
+    double lnSd;         // ln S
+};
+
 // One job as a lane group sees it (plain and batched forms alike).
 struct JobView {
     uint32_t B, M, L, flags, pitch, rng_stream, sample_base, nblk;  // nblk: checkpoint blocks
@@ -33,6 +46,10 @@ struct JobView {
     float* ck_beta;   // [B][nblk][Lp]  backward vector at each block's last marker
     double* ck_bl;    // [B][nblk]      its log scale
     float* ck_win;    // [B][nwin][Lp]  forward vector at each window's last marker
+    uint32_t* tmask;  // [ceil(M/16)]   typed markers of each load block (fast jobs)
+    uint32_t* tlist;  // [M]            the typed markers 1..M-1 in order (fast jobs)
+    uint32_t* tcount; // [1]            how many
+    URun ur;
 };
 
 // Emission tables shared by every job of a launch.
@@ -41,7 +58,13 @@ struct FbTables {
     const float* emax;     // [P][8]
     const float* palette;  // [P][3]   log-space kernel
     uint32_t P;
+    uint32_t ucode;        // code of an untyped marker (kNoUntyped: none)
 };
+constexpr uint32_t kNoUntyped = 0xffffffffu;
+// The untyped code a job asks for (SI_FB_UNTYPED), or kNoUntyped.
+inline uint32_t fb_untyped_code(const si_fb_job& j) {
+    return ((j.flags & SI_FB_UNTYPED) && j.untyped_code < j.n_palette) ? j.untyped_code : kNoUntyped;
+}
 
 // Forward-backward kernel variant: lanes per sample in the chain and block
 // passes (states per lane = Lp / lanes) and the checkpoint block length.
@@ -68,6 +91,30 @@ JobView fb_job_view(const si_fb_job& j, uint32_t cb);
 // Plain form: one job, one launch per pass.
 int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
               const FbVariant& var, hipStream_t stream);
+// Load blocks of a job's typed-marker mask per workgroup of its pass.
+constexpr uint32_t kTmaskBlocksPerGroup = 32;
+__host__ __device__ inline uint64_t fb_tmask_groups(uint32_t M) {
+    return (uint64_t((M + 15) / 16) + kTmaskBlocksPerGroup - 1) / kTmaskBlocksPerGroup;
+}
+
+// A batched class of fast jobs (untyped steps): class flag beside SI_FB_LOGSPACE.
+constexpr uint32_t kFbFastClass = 1u << 31;
+// Whether a job runs the untyped-step kernels (with 16-marker checkpoint
+// blocks): constant transitions the closed-form runs accept, no windows, an
+// untyped code in the palette (the kernels still check its likelihoods).
+URun fb_urun(const si_fb_job& j);
+inline bool fb_fast_job(const si_fb_job& j) {
+    return fb_untyped_code(j) < 256u && fb_window(j) == 0u && fb_urun(j).ok != 0u;
+}
+
+// Batched form: samples of a job padded to a multiple of 64 in both queues, so
+// a wavefront's round of groups never spans two jobs (its job descriptor and
+// untyped-marker mask are wavefront-uniform) and, in the block queue (samples
+// fastest within a checkpoint block), never spans two blocks.
+constexpr uint32_t kBatchSamplePad = 64;
+__host__ __device__ inline uint64_t fb_batch_padded_samples(uint32_t B) {
+    return (uint64_t(B) + kBatchSamplePad - 1) / kBatchSamplePad * kBatchSamplePad;
+}
 
 // Batched form: every job of one padded-state class in one persistent launch
 // per pass.  jobs: device array in launch order; item_off: device prefix of
@@ -77,11 +124,12 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
 struct BatchClass {
     uint32_t Lp;
     uint32_t n_jobs;
-    uint32_t flags;  // SI_FB_LOGSPACE
+    uint32_t flags;  // SI_FB_LOGSPACE, kFbFastClass
     const JobView* jobs;
     const uint64_t* chain_off;
     const uint64_t* block_off;
-    uint64_t chain_items, block_items;
+    const uint64_t* mask_off;  // workgroups of the typed-marker pass per job
+    uint64_t chain_items, block_items, mask_groups;
     unsigned long long* counters;  // [2] queue positions (chain, block)
     uint32_t grid_chain, grid_block;  // persistent grid sizes (workgroups)
 };
@@ -97,6 +145,7 @@ struct FbLaunch {
     int form;
     bool log;
     uint32_t cb;
+    bool fast;             // untyped-step kernels (16-marker checkpoint blocks)
     const JobView* job;    // plain forms
     uint32_t Bpad;         // plain chain: forward groups padded to whole wavefronts
     const BatchClass* bc;  // batched forms
@@ -108,7 +157,7 @@ struct FbImpl {
     uint32_t Lp, G;
     bool has_block;
     int (*launch)(const FbLaunch&);
-    int (*occupancy)(int form, bool log, uint32_t cb, uint32_t P);  // workgroups per CU
+    int (*occupancy)(int form, bool log, uint32_t cb, uint32_t P, bool fast);  // workgroups per CU
 };
 void fb_register(const FbImpl& impl);
 const FbImpl* fb_impl(uint32_t Lp, uint32_t G);

@@ -77,6 +77,13 @@ def step_coefs(Td: np.ndarray, To: np.ndarray) -> np.ndarray:
     return np.frombuffer(arr, np.float32).reshape(-1, 4)[:n].copy()
 
 
+def untyped_code(palette: np.ndarray):
+    """The first palette code whose likelihoods are all 0 (an untyped or
+    missing genotype: every state's emission is 1), or None."""
+    z = np.flatnonzero((np.asarray(palette, np.float32) == 0.0).all(axis=1))
+    return int(z[0]) if len(z) else None
+
+
 def emission_table(palette: np.ndarray):
     palette = np.ascontiguousarray(palette, np.float32)
     P = palette.shape[0]
@@ -226,6 +233,7 @@ class Job:
     argmax: bool = False
     logspace: bool = False
     window: int = 0              # the reference's windowed model when M > window > 0
+    untyped: bool = True         # untyped markers take the composed step (SI_FB_UNTYPED)
     out: JobOutputs = field(default_factory=JobOutputs)
     _keep: list = field(default_factory=list)
 
@@ -267,6 +275,10 @@ class Job:
         j.rng_stream = self.rng_stream
         j.sample_base = self.sample_base
         j.window = self.window
+        uc = untyped_code(self.palette) if self.untyped else None
+        if uc is not None:
+            j.flags |= _native.SI_FB_UNTYPED
+            j.untyped_code = uc
         j.seed = self.seed
         s = self.trans.step
         j.step = StepCoef(float(s[0]), float(s[1]), float(s[2]), float(s[3]))
@@ -316,7 +328,7 @@ def lane_candidates(L: int, pass_: int):
 
 def variants(L: int):
     """Every (chain lanes, block lanes, checkpoint block) built for L states."""
-    return [(gc, gb, cb) for cb in (16, 32) for gc in lane_candidates(L, 1)
+    return [(gc, gb, cb) for cb in (16,) for gc in lane_candidates(L, 1)
             for gb in lane_candidates(L, 2)]
 
 
@@ -395,8 +407,9 @@ class Batch:
 class SharedTables:
     """One emission table on the device, shared by the jobs of a batch."""
 
-    def __init__(self, palette, device=DEV):
+    def __init__(self, palette, device=DEV, untyped: bool = True):
         self.palette = np.ascontiguousarray(palette, np.float32)
+        self.ucode = untyped_code(self.palette) if untyped else None
         eps, emax = emission_table(self.palette)
         self.eps = torch.from_numpy(eps).to(device)
         self.emax = torch.from_numpy(emax).to(device)
@@ -405,6 +418,9 @@ class SharedTables:
     def apply(self, j: FbJob) -> FbJob:
         j.eps, j.emax, j.palette = _ptr(self.eps), _ptr(self.emax), _ptr(self.pal)
         j.n_palette = self.palette.shape[0]
+        if self.ucode is not None:
+            j.flags |= _native.SI_FB_UNTYPED
+            j.untyped_code = self.ucode
         return j
 
 

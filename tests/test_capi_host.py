@@ -259,7 +259,7 @@ def test_variant_queries():
     for L in (1, 4, 5, 8, 9, 16, 17, 32, 33, 64):
         vs = device.variants(L)
         assert device.default_variant(L) in vs
-        assert {cb for *_, cb in vs} == {16, 32}
+        assert {cb for *_, cb in vs} == {16}
 
 
 def test_fb_variant_argument_errors():

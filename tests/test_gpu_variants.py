@@ -1,9 +1,8 @@
 """Kernel variants and the autotuner.
 
 Every sum over states is the same binary tree over the padded states whatever
-the lane shape, and the checkpoint block length only changes where the chain
-pass saves vectors, so every variant (lanes per sample in each pass,
-checkpoint block 16 or 32) must give bit-identical outputs; the default
+the lane shape, so every variant (lanes per sample in each pass) must give
+bit-identical outputs; the default
 variant is itself checked against the oracle in test_gpu_parity.py.  The
 tuners must pick a built variant, leave the jobs' outputs untouched while
 timing, and a tuned batch must still equal the plain path bit for bit."""
@@ -47,7 +46,7 @@ def test_every_variant_gives_identical_outputs(world, L, logspace, per_marker, M
     rj = _job(world, L, logspace, per_marker, M=M)
     ref = _run(rj, None)
     vs = device.variants(L)
-    assert device.default_variant(L) in vs and len(vs) >= 8
+    assert device.default_variant(L) in vs and len(vs) >= 4
     for v in vs:
         print(f"L={L} log={logspace} pm={per_marker} M={M} variant={v}", flush=True)
         got = _run(rj, v)
@@ -59,7 +58,7 @@ def test_unbuilt_variant_is_refused(world):
     rj = _job(world, 16, False, True, M=64)
     cj = rj.cjob()
     arr = (_native.FbJob * 1)(cj)
-    for bad in [(3, 4, 16), (4, 4, 24), (4, 1, 16), (16, 4, 16)]:
+    for bad in [(3, 4, 16), (4, 4, 24), (4, 4, 32), (4, 1, 16), (16, 4, 16)]:
         rc = _native.hip().si_fb_run_variant(arr, 1, ctypes.byref(_native.FbVariant(*bad)), None, 0,
                                              3, None)
         assert rc == _native.SI_ERR_UNSUPPORTED, bad

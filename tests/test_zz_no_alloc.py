@@ -55,7 +55,7 @@ def test_noalloc_equals_caller_workspace_on_consecutive_calls(variant):
         var = None
         if variant is not None:
             lanes = device.lane_candidates(L, _native.SI_FB_PASS_CHAIN)
-            var = ctypes.byref(_native.FbVariant(lanes[-1], device.default_variant(L)[1], 32))
+            var = ctypes.byref(_native.FbVariant(lanes[-1], device.default_variant(L)[1], 16))
         ws = torch.empty(need, dtype=torch.uint8, device="cuda")
         _native.check(lib.si_fb_run_variant(arr, 1, var, ctypes.c_void_p(ws.data_ptr()), need, 3,
                                             sh), "si_fb_run_variant")
