@@ -14,39 +14,18 @@ namespace {
 constexpr int kLp = SI_FB_LP, kG = SI_FB_G, kR = kLp / kG;
 constexpr bool kHasBlock = kR <= 4;
 
-// Fast jobs (untyped steps, 16-marker checkpoint blocks only).
+// Chain pass of fast jobs (untyped steps, 16-marker checkpoint blocks only);
+// their block pass is the exact kernel's.
 template <bool LOG>
-int launch_fast(const FbLaunch& L) {
+int launch_fast_chain(const FbLaunch& L) {
     const size_t dyn = fb_table_lds_bytes(L.tb.P);
     const dim3 grid(L.grid), block(kFbThreads);
-    switch (L.form) {
-        case kFbPlainChain:
-            hipLaunchKernelGGL((fb_chain_fast_kernel<kG, kR, LOG, 16>), grid, block, dyn, L.stream, *L.job,
-                               L.tb, L.Bpad);
-            break;
-        case kFbBatchChain:
-            hipLaunchKernelGGL((fb_chain_fast_batch_kernel<kG, kR, LOG, 16>), grid, block, dyn, L.stream,
-                               *L.bc, L.tb);
-            break;
-        case kFbPlainBlock:
-            if constexpr (kHasBlock) {
-                hipLaunchKernelGGL((fb_block_fast_kernel<kG, kR, LOG, 16>), grid, block, dyn, L.stream,
-                                   *L.job, L.tb);
-                break;
-            } else {
-                return SI_ERR_UNSUPPORTED;
-            }
-        case kFbBatchBlock:
-            if constexpr (kHasBlock) {
-                hipLaunchKernelGGL((fb_block_fast_batch_kernel<kG, kR, LOG, 16>), grid, block, dyn, L.stream,
-                                   *L.bc, L.tb);
-                break;
-            } else {
-                return SI_ERR_UNSUPPORTED;
-            }
-        default:
-            return SI_ERR_ARG;
-    }
+    if (L.form == kFbPlainChain)
+        hipLaunchKernelGGL((fb_chain_fast_kernel<kG, kR, LOG, 16>), grid, block, dyn, L.stream, *L.job, L.tb,
+                           L.Bpad);
+    else
+        hipLaunchKernelGGL((fb_chain_fast_batch_kernel<kG, kR, LOG, 16>), grid, block, dyn, L.stream, *L.bc,
+                           L.tb);
     return hipGetLastError() == hipSuccess ? 0 : SI_ERR_DEVICE;
 }
 
@@ -86,38 +65,22 @@ int launch_t(const FbLaunch& L) {
 }
 
 int launch(const FbLaunch& L) {
-    if (L.fast) return L.cb == 16 ? (L.log ? launch_fast<true>(L) : launch_fast<false>(L)) : SI_ERR_UNSUPPORTED;
+    if (L.fast && (L.form == kFbPlainChain || L.form == kFbBatchChain))
+        return L.cb == 16 ? (L.log ? launch_fast_chain<true>(L) : launch_fast_chain<false>(L))
+                          : SI_ERR_UNSUPPORTED;
     if (L.cb == 16) return L.log ? launch_t<true, 16>(L) : launch_t<false, 16>(L);
     return SI_ERR_UNSUPPORTED;
 }
 
 template <bool LOG>
-int occupancy_fast(int form, uint32_t P) {
+int occupancy_fast_chain(int form, uint32_t P) {
     int n = 0;
     const size_t dyn = fb_table_lds_bytes(P);
-    hipError_t e = hipErrorInvalidValue;
-    switch (form) {
-        case kFbPlainChain:
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_chain_fast_kernel<kG, kR, LOG, 16>,
-                                                             kFbThreads, dyn);
-            break;
-        case kFbBatchChain:
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &n, fb_chain_fast_batch_kernel<kG, kR, LOG, 16>, kFbThreads, dyn);
-            break;
-        case kFbPlainBlock:
-            if constexpr (kHasBlock)
-                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_block_fast_kernel<kG, kR, LOG, 16>,
-                                                                 kFbThreads, dyn);
-            break;
-        case kFbBatchBlock:
-            if constexpr (kHasBlock)
-                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &n, fb_block_fast_batch_kernel<kG, kR, LOG, 16>, kFbThreads, dyn);
-            break;
-        default:
-            break;
-    }
+    const hipError_t e =
+        form == kFbPlainChain
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_chain_fast_kernel<kG, kR, LOG, 16>, kFbThreads, dyn)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_chain_fast_batch_kernel<kG, kR, LOG, 16>,
+                                                           kFbThreads, dyn);
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
@@ -152,7 +115,8 @@ int occupancy_t(int form, uint32_t P) {
 }
 
 int occupancy(int form, bool log, uint32_t cb, uint32_t P, bool fast) {
-    if (fast) return log ? occupancy_fast<true>(form, P) : occupancy_fast<false>(form, P);
+    if (fast && (form == kFbPlainChain || form == kFbBatchChain))
+        return log ? occupancy_fast_chain<true>(form, P) : occupancy_fast_chain<false>(form, P);
     return log ? occupancy_t<true, 16>(form, P) : occupancy_t<false, 16>(form, P);
 }
 

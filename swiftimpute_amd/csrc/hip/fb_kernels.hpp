@@ -493,15 +493,6 @@ __device__ __forceinline__ void posterior(const float (&a)[S::R], const float (&
 // where t_o >= 2^-30: its own step then adds the terms in linear space too
 // (the forward's -999 floor is below resolution).
 
-__device__ __forceinline__ bool fast_block(const JobView& J, uint32_t q) {
-    return J.ur.ok != 0u && J.tmask != nullptr && J.W == 0u && q >= 32u && (q | 31u) < J.M;
-}
-
-// Untyped markers of load block c of a job (0 unless fast), wavefront-uniform.
-__device__ __forceinline__ uint32_t block_untyped(const JobView& J, uint32_t c, bool fast) {
-    return fast ? __builtin_amdgcn_readfirstlane(~J.tmask[c] & 0xffffu) : 0u;
-}
-
 // Bit t set where code byte t of a load block equals uc (< 256).
 __device__ __forceinline__ uint32_t untyped_bits(const uint4& cw, uint32_t uc) {
     const uint32_t rep = uc * 0x01010101u;
@@ -654,31 +645,6 @@ __device__ __forceinline__ void bwd_run(float (&v)[S::R], double& Bl, const floa
 #pragma unroll
     for (int i = 0; i < S::R; ++i) v[i] = LOG ? fast_ln(fmaf(y, base[i], x)) : fmaf(y, base[i], x);
 }
-
-// Block pass: the backward vector at untyped markers below the anchor, for
-// posteriors only.  V = X 1 + Y base up to a scale: per step
-// V' = (u V + t_o W)/S + F with the floor term F growing by 1/S, all three
-// rescaled by a power of two whenever F passes 1.
-struct BwdShow {
-    float X, Y, F, W0;
-    __device__ __forceinline__ void reset(float W0_, float f0) {
-        X = 0.0f;
-        Y = 1.0f;
-        F = f0;
-        W0 = W0_;
-    }
-    __device__ __forceinline__ void step(const URun& q, float Lf) {
-        const float W = fmaf(X, Lf, Y * W0);
-        const float Fn = F * q.rS;
-        const float Xn = fmaf(q.c1, X, fmaf(q.c2, W, Fn));
-        const float Yn = q.c1 * Y;
-        const int e = __builtin_amdgcn_frexp_expf(Fn);
-        const int sh = e > 0 ? -e : 0;
-        X = __builtin_amdgcn_ldexpf(Xn, sh);
-        Y = __builtin_amdgcn_ldexpf(Yn, sh);
-        F = __builtin_amdgcn_ldexpf(Fn, sh);
-    }
-};
 
 // ------------------------------------------------------------ block inputs --
 // One 16-marker load block of a lane's inputs: the sample's 16 genotype
@@ -1446,16 +1412,17 @@ __device__ __forceinline__ void tlist_build(const uint32_t* tmask, uint32_t M, u
     if (threadIdx.x == 0) *tcount = base;
 }
 
-// A typed marker's inputs for one lane: the sample's code and the lane's
-// state-table words.
+// A typed marker's inputs for one lane: its index, the sample's code and the
+// lane's state-table words.
 template <class S>
 struct MarkIn {
-    uint32_t code;
+    uint32_t m, code;
     LaneWords<S> sw;
 };
 template <class S>
 __device__ __forceinline__ void mark_load(const JobView& J, const uint8_t* codes, uint32_t r, uint32_t m,
                                           MarkIn<S>& x) {
+    x.m = m;
     x.code = codes[m];
     const uint32_t* p =
         reinterpret_cast<const uint32_t*>(J.sa) + size_t(m) * (uint32_t(S::Lp) / 4u) + (r * uint32_t(S::R)) / 4u;
@@ -1488,11 +1455,19 @@ __device__ __forceinline__ void run_chain_fast(const JobView& J, const Tab& tab,
     const uint32_t T = __builtin_amdgcn_readfirstlane(*J.tcount);
     const uint32_t* tl = J.tlist;
     const uint32_t last = J.M - 1u;
+    // Typed marker i (in walking order) sits in slot i % K: ring holds the
+    // inputs of markers i..i+K-1, pos the indices of markers i+K..i+2K-1,
+    // both loaded by vector loads (wavefront-uniform addresses) so that no
+    // marker's index or inputs are waited for when its step comes.
     MarkIn<S> ring[K];
-    if (fwd) {
+    uint32_t pos[K];
+    auto ti = [&](uint32_t i) { return fwd ? i : T - 1u - i; };  // walking order -> list index
 #pragma unroll
-        for (uint32_t j = 0; j < K; ++j)
-            if (j < T) mark_load<S>(J, codes, r, tl[j], ring[j]);
+    for (uint32_t j = 0; j < K; ++j) {
+        if (j < T) mark_load<S>(J, codes, r, tl[ti(j)], ring[j]);
+        pos[j] = (K + j < T) ? tl[ti(K + j)] : 0u;
+    }
+    if (fwd) {
         float a[S::R];
         double ll;
         {
@@ -1502,14 +1477,14 @@ __device__ __forceinline__ void run_chain_fast(const JobView& J, const Tab& tab,
             const float off = lane_emission<S>(tab, P, x0.code, x0.sw, sub, e);
             ll = fwd_init<S, LOG>(a, e, off);
         }
-        store_vec<S::R>(J.ck_alpha + size_t(s) * S::Lp + vo, a);  // block 0
+        store_vec<S::R>(J.ck_alpha + size_t(s) * J.nblk * S::Lp + vo, a);  // block 0
         uint32_t pa = 0, q16 = kSub;
         auto anchors_below = [&](uint32_t lim) {
             while (q16 < lim) {
                 ll += fwd_run<S, LOG>(a, rm, q16 - pa, q);
                 pa = q16;
                 if (q16 % uint32_t(CB) == 0u)
-                    store_vec<S::R>(J.ck_alpha + (size_t(q16 / CB) * J.B + s) * S::Lp + vo, a);
+                    store_vec<S::R>(J.ck_alpha + (size_t(s) * J.nblk + q16 / CB) * S::Lp + vo, a);
                 q16 += kSub;
             }
         };
@@ -1518,14 +1493,15 @@ __device__ __forceinline__ void run_chain_fast(const JobView& J, const Tab& tab,
             for (uint32_t j = 0; j < K; ++j) {
                 const uint32_t i = i0 + j;
                 if (i < T) {
-                    const uint32_t mt = tl[i];
+                    const uint32_t mt = __builtin_amdgcn_readfirstlane(ring[j].m);
                     anchors_below(mt);
                     ll += fwd_run<S, LOG>(a, rm, mt - 1u - pa, q);
                     float e[S::R];
                     const float off = lane_emission<S>(tab, P, ring[j].code, ring[j].sw, sub, e);
                     ll += fwd_step<S, LOG>(a, e, off, k);
                     pa = mt;
-                    if (i + K < T) mark_load<S>(J, codes, r, tl[i + K], ring[j]);
+                    if (i + K < T) mark_load<S>(J, codes, r, pos[j], ring[j]);
+                    if (i + 2u * K < T) pos[j] = tl[i + 2u * K];
                 }
             }
         }
@@ -1534,14 +1510,11 @@ __device__ __forceinline__ void run_chain_fast(const JobView& J, const Tab& tab,
         if (LOG) ll += double(log_mass<S>(a));  // log mass of the last normalized vector
         if (r == 0 && J.loglik) J.loglik[s] = ll;
     } else {
-#pragma unroll
-        for (uint32_t j = 0; j < K; ++j)
-            if (j < T) mark_load<S>(J, codes, r, tl[T - 1u - j], ring[j]);
         float v[S::R];
         bwd_init<S, LOG>(v);
         double Bl = 0.0;
         {
-            const size_t b = size_t(J.nblk - 1u) * J.B + s;
+            const size_t b = size_t(s) * J.nblk + (J.nblk - 1u);
             store_vec<S::R>(J.ck_beta + b * S::Lp + vo, v);
             if (r == 0) J.ck_bl[b] = Bl;
         }
@@ -1552,7 +1525,7 @@ __device__ __forceinline__ void run_chain_fast(const JobView& J, const Tab& tab,
                 bwd_run<S, LOG>(v, Bl, rm, pb - (q16 - 1u), q);
                 pb = q16 - 1u;
                 if (q16 % uint32_t(CB) == 0u) {
-                    const size_t b = size_t(q16 / CB - 1u) * J.B + s;
+                    const size_t b = size_t(s) * J.nblk + (q16 / CB - 1u);
                     store_vec<S::R>(J.ck_beta + b * S::Lp + vo, v);
                     if (r == 0) J.ck_bl[b] = Bl;
                 }
@@ -1564,217 +1537,19 @@ __device__ __forceinline__ void run_chain_fast(const JobView& J, const Tab& tab,
             for (uint32_t j = 0; j < K; ++j) {
                 const uint32_t i = i0 + j;
                 if (i < T) {
-                    const uint32_t n = tl[T - 1u - i];
+                    const uint32_t n = __builtin_amdgcn_readfirstlane(ring[j].m);
                     anchors_above(n);
                     bwd_run<S, LOG>(v, Bl, rm, pb - n, q);
                     float e[S::R];
                     const float off = lane_emission<S>(tab, P, ring[j].code, ring[j].sw, sub, e);
                     bwd_step<S, LOG>(v, Bl, e, off, k);
                     pb = n - 1u;
-                    if (i + K < T) mark_load<S>(J, codes, r, tl[T - 1u - (i + K)], ring[j]);
+                    if (i + K < T) mark_load<S>(J, codes, r, pos[j], ring[j]);
+                    if (i + 2u * K < T) pos[j] = tl[T - 1u - (i + 2u * K)];
                 }
             }
         }
         anchors_above(0u);
-    }
-}
-
-// Checkpoint block c of sample s of a fast job, by one lane group: as
-// run_block, with the forward and backward vectors carried at anchors (the
-// same runs as the chain pass) and, at untyped markers past an anchor, formed
-// for the posterior by per-marker recurrences that are never carried.  Typed
-// markers read their emissions when they step.
-template <class S, bool LOG, int CB, bool SB>
-__device__ __forceinline__ void run_block_fast(const JobView& J, const Tab& tab, uint32_t P, bool active,
-                                               uint32_t s, uint32_t c, uint32_t r, float* pb, uint32_t* sb,
-                                               uint4* rng) {
-    using Lds = BlockLds<S>;
-    constexpr uint32_t NS = uint32_t(CB) / kSub;
-    float* const prow = pb + r * S::R;
-    constexpr uint32_t kPer = (kSub + uint32_t(S::G) - 1) / uint32_t(S::G);
-    const uint32_t m0 = c * CB;
-    const uint32_t nm = active ? min(uint32_t(CB), J.M - m0) : 0u;
-    const uint32_t sub = lane_sub<S>(r);
-    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
-    const size_t ck = size_t(c) * J.B + s;  // block-major
-    const uint32_t last = J.M - 1u;
-    const URun& q = J.ur;
-    const float Lf = float(J.L);
-    float rm[S::R];
-    real_mask<S>(r, J.L, rm);
-    const typename CoefT<LOG>::type k = const_coef<LOG>(J);
-    float v[S::R];
-    double Bl = 0.0;
-    if (active) {
-        load_vec<S::R>(J.ck_beta + ck * S::Lp + size_t(r) * S::R, v);
-        Bl = J.ck_bl[ck];
-    }
-    for (int j = int(NS) - 1; j >= 0; --j) {
-        const uint32_t base = uint32_t(j) * kSub;
-        float ap0r[kPer], ap1r[kPer];
-        BlockIn<S, LOG, false> cur;
-        auto post_out = [&](uint32_t t, const float (&p)[S::R]) {
-            float x0[S::R], x1[S::R];
-#pragma unroll
-            for (int i = 0; i < S::R; ++i) {
-                const uint32_t pi = __float_as_uint(p[i]);
-                x0[i] = __uint_as_float(pi & cur.sa[t].bitmask(i, sub, 0));
-                x1[i] = __uint_as_float(pi & cur.sa[t].bitmask(i, sub, 1));
-            }
-            float ap0 = tree_sum<S::R>(x0), ap1 = tree_sum<S::R>(x1);
-            gsum2<S::G>(ap0, ap1);
-            store_vec<S::R>(prow + t * Lds::T, p);
-            if (t % S::G == r) {
-                ap0r[t / S::G] = ap0;
-                ap1r[t / S::G] = ap1;
-            }
-        };
-        if (active) {
-            float a[S::R];
-            load_vec<S::R>(J.ck_alpha + ck * S::Lp + size_t(r) * S::R, a);
-            uint32_t pa = m0;
-            // forward through sub-blocks 0..j-1 (carried only), then j
-            cur.load(J, codes, r, m0);
-            for (uint32_t u = 0; u < uint32_t(j); ++u) {
-                BlockIn<S, LOG, false> nxt;
-                nxt.load(J, codes, r, m0 + (u + 1) * kSub);
-                const uint32_t tm = __builtin_amdgcn_readfirstlane(J.tmask[m0 / kSub + u]);
-#pragma unroll
-                for (uint32_t t = 0; t < kSub; ++t) {
-                    const uint32_t n = m0 + u * kSub + t;
-                    if (n == m0 || n > last) continue;  // the checkpoint itself; past the job
-                    if ((tm >> t) & 1u) {
-                        fwd_run<S, LOG>(a, rm, n - 1u - pa, q);
-                        float e[S::R];
-                        const float off = lane_emission<S>(tab, P, cur.code(t), cur.sa[t], sub, e);
-                        fwd_step<S, LOG>(a, e, off, k);
-                        pa = n;
-                    }
-                    if (t == 0) {
-                        fwd_run<S, LOG>(a, rm, n - pa, q);
-                        pa = n;
-                    }
-                }
-                cur = nxt;
-            }
-            const uint32_t tm = __builtin_amdgcn_readfirstlane(J.tmask[m0 / kSub + uint32_t(j)]);
-            // Sub-block j into LDS rows: the vector at an anchor; past one, the
-            // composition al a + be 1_L off it (log form: of e^a / sum e^a, in
-            // linear space, flagged in fpm).
-            float dal = 1.0f, dbe = 0.0f;
-            float dw[LOG ? S::R : 1];
-            uint32_t fpm = 0;
-#pragma unroll
-            for (uint32_t t = 0; t < kSub; ++t) {
-                if constexpr (!SB) {
-                } else if constexpr (S::R == 4) {
-                    reinterpret_cast<uint16_t*>(sb)[t * (S::Lp / 4) + r] = uint16_t(nib4(cur.sa[t].w[0]));
-                } else {
-                    static_assert(S::R == 2, "block pass lane shapes");
-                    reinterpret_cast<uint8_t*>(sb)[t * (S::Lp / 2) + r] =
-                        uint8_t(nib2(cur.sa[t].w[0] >> (8 * sub)));
-                }
-                const uint32_t n = m0 + base + t;
-                if (n > last) continue;
-                if (n != m0) {
-                    if ((tm >> t) & 1u) {
-                        fwd_run<S, LOG>(a, rm, n - 1u - pa, q);
-                        float e[S::R];
-                        const float off = lane_emission<S>(tab, P, cur.code(t), cur.sa[t], sub, e);
-                        fwd_step<S, LOG>(a, e, off, k);
-                        pa = n;
-                    }
-                    if (t == 0) {
-                        fwd_run<S, LOG>(a, rm, n - pa, q);
-                        pa = n;
-                    }
-                }
-                float row[S::R];
-                if (pa == n) {
-#pragma unroll
-                    for (int i = 0; i < S::R; ++i) row[i] = a[i];
-                } else {
-                    if (pa + 1u == n) {
-                        dal = 1.0f;
-                        dbe = 0.0f;
-                        if constexpr (LOG) {
-#pragma unroll
-                            for (int i = 0; i < S::R; ++i) dw[i] = fast_exp(a[i]);
-                            const float rw = __builtin_amdgcn_rcpf(vsum<S>(dw));
-#pragma unroll
-                            for (int i = 0; i < S::R; ++i) dw[i] *= rw;
-                        }
-                    }
-                    dal *= q.c1;
-                    dbe = fmaf(dbe, q.c1, q.c2);
-#pragma unroll
-                    for (int i = 0; i < S::R; ++i) row[i] = fmaf(dal, LOG ? dw[i] : a[i], dbe * rm[i]);
-                    if constexpr (LOG) fpm |= 1u << t;
-                }
-                store_vec<S::R>(prow + t * Lds::T, row);
-            }
-            // Backward walk down sub-block j: the anchor pb; at an untyped
-            // marker below it the posterior's vector comes from BwdShow.
-            uint32_t pbk = min(m0 + base + kSub - 1u, last);
-            BwdShow bs;
-            float bb[S::R];
-#pragma unroll
-            for (int t = int(kSub) - 1; t >= 0; --t) {
-                const uint32_t n = m0 + base + uint32_t(t);
-                if (n > last) continue;
-                const bool typed = (tm >> t) & 1u;
-                if (typed) {
-                    bwd_run<S, LOG>(v, Bl, rm, pbk - n, q);
-                    pbk = n;
-                }
-                float af[S::R], vb[S::R], p[S::R];
-                load_vec<S::R>(prow + t * Lds::T, af);
-                const bool bshow = pbk != n;
-                if constexpr (LOG) {
-                    const bool fshow = (fpm >> t) & 1u;
-                    if (bshow) {
-#pragma unroll
-                        for (int i = 0; i < S::R; ++i) vb[i] = fmaf(bs.Y, bb[i], bs.X);
-                    }
-                    if (fshow && bshow) {
-                        posterior<S, false>(af, vb, p);
-                    } else {
-                        if (fshow) {
-#pragma unroll
-                            for (int i = 0; i < S::R; ++i) af[i] = fast_ln(af[i]);
-                        }
-#pragma unroll
-                        for (int i = 0; i < S::R; ++i) vb[i] = bshow ? fast_ln(vb[i]) : v[i];
-                        posterior<S, true>(af, vb, p);
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < S::R; ++i) vb[i] = bshow ? fmaf(bs.Y, bb[i], bs.X) : v[i];
-                    posterior<S, false>(af, vb, p);
-                }
-                post_out(uint32_t(t), p);
-                if (n > m0) {  // the step out of the block's first marker belongs to the block before
-                    if (typed) {
-                        float e[S::R];
-                        const float off = lane_emission<S>(tab, P, cur.code(uint32_t(t)), cur.sa[t], sub, e);
-                        bwd_step<S, LOG>(v, Bl, e, off, k);
-                        pbk = n - 1u;
-                    } else {
-                        if (pbk == n) {
-                            double B;
-                            const float W0 = bwd_base<S, LOG>(v, Bl, rm, bb, B);
-                            bs.reset(W0, fast_exp(float(-999.0 - B)));
-                        }
-                        bs.step(q, Lf);
-                    }
-                    if (t == 0) {
-                        bwd_run<S, LOG>(v, Bl, rm, pbk - (n - 1u), q);
-                        pbk = n - 1u;
-                    }
-                }
-            }
-        }
-        block_sample<S, SB, kPer>(J, active, s, m0, base, nm, r, pb, sb, rng, ap0r, ap1r);
     }
 }
 
@@ -1789,28 +1564,6 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_fast_kernel(JobView J, Fb
     const uint32_t s = fwd ? grp : grp - Bpad;
     if (s >= J.B) return;  // whole lane groups leave together
     run_chain_fast<S, LOG, CB>(J, tab, tb.P, s, fwd, r);
-}
-
-template <int G, int R, bool LOG, int CB>
-__global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_fast_kernel(JobView J, FbTables tb) {
-    using S = Shape<G, R>;
-    using Lds = BlockLds<S>;
-    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
-    constexpr bool SB = Lds::kSbPlain;
-    __shared__ uint32_t sbuf[S::GPB * (SB ? Lds::SW : 1)];
-    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
-    const Tab tab = load_tables<LOG>(tb);
-    __syncthreads();
-    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
-    const uint32_t grp = gtid / G, r = gtid % G, gl = threadIdx.x / G;
-    // samples fastest, padded to whole wavefronts: the groups of a wavefront
-    // share a checkpoint block
-    const uint32_t Bp = (J.B + S::GPW - 1) / S::GPW * S::GPW;
-    const uint32_t s0 = grp % Bp, c0 = grp / Bp;
-    const bool active = s0 < J.B && c0 < J.nblk;
-    run_block_fast<S, LOG, CB, SB>(J, tab, tb.P, active, active ? s0 : 0u, c0 < J.nblk ? c0 : 0u, r,
-                                   pbuf + gl * Lds::PS, sbuf + (SB ? gl * Lds::SW : 0u),
-                                   rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u));
 }
 
 template <int G, int R, bool LOG, int CB>
@@ -1842,50 +1595,6 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_fast_batch_kernel(BatchCl
             const JobView J = uniform_job(bc.jobs + j);
             const uint32_t q = uint32_t(gs0 - (bc.chain_off[j] >> 1)) + gw;
             if (q < J.B) run_chain_fast<S, LOG, CB>(J, tab, tb.P, q, fwd, r);
-        }
-    }
-}
-
-template <int G, int R, bool LOG, int CB>
-__global__ __launch_bounds__(kFbThreads, kBlockWaves<R>) void fb_block_fast_batch_kernel(BatchClass bc,
-                                                                                         FbTables tb) {
-    using S = Shape<G, R>;
-    using Lds = BlockLds<S>;
-    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
-    __shared__ uint32_t sbuf[S::GPB * Lds::SW];
-    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
-    const Tab tab = load_tables<LOG>(tb);
-    __syncthreads();
-    const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
-    const uint32_t gl = threadIdx.x / G;
-    float* pb = pbuf + gl * Lds::PS;
-    uint32_t* sb = sbuf + gl * Lds::SW;
-    uint4* rg = rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u);
-    uint64_t lo = 0, hi = 0;  // item range of the job in J (wavefront-uniform)
-    JobView J{};
-    for (;;) {
-        const Chunk ch = wave_take(&bc.counters[1], bc.block_items, S::GPW, 32);
-        if (ch.base >= bc.block_items) break;
-        for (uint32_t kk = 0; kk < ch.n; kk += S::GPW) {
-            const uint64_t round = ch.base + kk;
-            if (round >= bc.block_items) break;
-            if (round < lo || round >= hi) {
-                const uint32_t j = find_job_uniform(bc.block_off, bc.n_jobs, round);
-                J = uniform_job(bc.jobs + j);
-                lo = bc.block_off[j];
-                hi = bc.block_off[j + 1];
-                lo = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(lo >> 32))) << 32) |
-                     __builtin_amdgcn_readfirstlane(uint32_t(lo));
-                hi = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(hi >> 32))) << 32) |
-                     __builtin_amdgcn_readfirstlane(uint32_t(hi));
-            }
-            // samples fastest within the job, padded to a multiple of 64
-            const uint64_t qq = round - lo + gw;
-            const uint32_t Bp = uint32_t(fb_batch_padded_samples(J.B));
-            const uint32_t s0 = uint32_t(qq % Bp), c0 = uint32_t(qq / Bp);
-            const bool active = s0 < J.B && c0 < J.nblk;
-            run_block_fast<S, LOG, CB, true>(J, tab, tb.P, active, active ? s0 : 0u, c0 < J.nblk ? c0 : 0u, r,
-                                             pb, sb, rg);
         }
     }
 }

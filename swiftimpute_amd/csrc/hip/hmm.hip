@@ -203,10 +203,7 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
     }
     if (passes & SI_FB_PASS_BLOCK) {
         const FbImpl* f = fb_impl(Lp, var.gb);
-        // fast: samples fastest, padded to whole wavefronts (fb_block_fast_kernel)
-        const uint32_t gpw = kWave / var.gb;
-        const size_t Bp = fast ? (size_t(v.B) + gpw - 1) / gpw * gpw : size_t(v.B);
-        const size_t threads = Bp * v.nblk * var.gb;
+        const size_t threads = size_t(v.B) * v.nblk * var.gb;
         const size_t grid = (threads + kFbThreads - 1) / kFbThreads;
         if (grid > 0x7fffffffu) return SI_ERR_UNSUPPORTED;
         FbLaunch L{kFbPlainBlock, log, var.cb, fast, &v, 0, nullptr, t, uint32_t(grid), stream};
