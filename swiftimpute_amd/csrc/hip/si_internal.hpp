@@ -109,8 +109,7 @@ inline bool fb_fast_job(const si_fb_job& j) {
 
 // Batched form: samples of a job padded to a multiple of 64 in both queues, so
 // a wavefront's round of groups never spans two jobs (its job descriptor and
-// untyped-marker mask are wavefront-uniform) and, in the block queue (samples
-// fastest within a checkpoint block), never spans two blocks.
+// untyped-marker mask are wavefront-uniform).
 constexpr uint32_t kBatchSamplePad = 64;
 __host__ __device__ inline uint64_t fb_batch_padded_samples(uint32_t B) {
     return (uint64_t(B) + kBatchSamplePad - 1) / kBatchSamplePad * kBatchSamplePad;
