@@ -283,6 +283,48 @@ def bench_batch(args):
 # --------------------------------------------------------------------------
 # serve: config-5 request stream against a resident panel
 # --------------------------------------------------------------------------
+def _pcie_rates(nbytes=1 << 30, reps=3):
+    """Page-locked host <-> device copy rates (GB/s): each direction alone,
+    and both at once on two streams (the link is full duplex)."""
+    import torch
+
+    dev = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dev2 = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    host2 = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    host.fill_(1)
+    host2.fill_(2)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+
+    def timed(fn):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    def d2h():
+        with torch.cuda.stream(s1):
+            host.copy_(dev, non_blocking=True)
+
+    def h2d():
+        with torch.cuda.stream(s2):
+            dev2.copy_(host2, non_blocking=True)
+
+    t_d2h, t_h2d = timed(d2h), timed(h2d)
+    t_both = timed(lambda: (d2h(), h2d()))
+    out = {"d2h_GBps": nbytes / t_d2h / 1e9, "h2d_GBps": nbytes / t_h2d / 1e9,
+           "both_GBps_each": nbytes / t_both / 1e9, "copy_bytes": nbytes,
+           "method": "1 GiB page-locked <-> device copies, best of 3, each direction alone "
+                     "and both at once on two streams"}
+    del dev, dev2, host, host2
+    return out
+
+
 def bench_serve(args):
     import numpy as np
     import torch
@@ -294,87 +336,167 @@ def bench_serve(args):
     pan = synth.make_panel(H, M, seed=12345)
     log(f"[serve] generating a pool of {pool_B} targets typed at every marker")
     pool = synth.make_targets(pan, pool_B, 1, seed=12346, with_truth=False)
-    reqs = synth.poisson_requests(args.rate, args.serve_seconds, M, pool_B, L=L, typed_every=te)
-    log(f"[serve] {len(reqs)} requests at {args.rate} req/s over {args.serve_seconds} s")
-    codes = [synth.region_codes(pool.codes, sp) for _, sp in reqs]
-    untyped = sum(int((c == synth.CODE_MISSING).sum()) for c in codes)
-    del pool
-    # the client's result arrays, page-locked (the server copies into them
-    # from the device directly), allocated and touched before the clock starts
-    outs, pinned = [], []
-    for _, sp in reqs:
-        ht = torch.empty((sp.B, 2, sp.M), dtype=torch.uint8, pin_memory=True)
-        dt = torch.empty((sp.B, sp.M), dtype=torch.float32, pin_memory=True)
-        h, d = ht.numpy(), dt.numpy()
-        h.fill(0)
-        d.fill(0)
-        pinned.append((ht, dt))
-        outs.append((h, d))
+
+    def request_set(rate, seconds, seed):
+        reqs = synth.poisson_requests(rate, seconds, M, pool_B, L=L, typed_every=te, seed=seed)
+        codes = [synth.region_codes(pool.codes, sp) for _, sp in reqs]
+        return reqs, codes
+
+    def pinned_outputs(reqs):
+        # the client's result arrays, page-locked (the server copies into them
+        # from the device directly), allocated and touched before the clock
+        outs, keep = [], []
+        for _, sp in reqs:
+            ht = torch.empty((sp.B, 2, sp.M), dtype=torch.uint8, pin_memory=True)
+            dt = torch.empty((sp.B, sp.M), dtype=torch.float32, pin_memory=True)
+            h, d = ht.numpy(), dt.numpy()
+            h.fill(0)
+            d.fill(0)
+            keep.append((ht, dt))
+            outs.append((h, d))
+        return outs, keep
+
+    def online(srv, reqs, codes, outs):
+        lat = [None] * len(reqs)
+        ids = [None] * len(reqs)
+        done = threading.Event()
+
+        def waiter():
+            for i in range(len(reqs)):
+                while ids[i] is None:
+                    time.sleep(0.0005)
+                lat[i] = srv.wait(ids[i]).latency_s
+            done.set()
+
+        th = threading.Thread(target=waiter, daemon=True)
+        th.start()
+        t_start = time.perf_counter()
+        for i, ((t_arr, sp), c) in enumerate(zip(reqs, codes)):
+            now = time.perf_counter() - t_start
+            if t_arr > now:
+                time.sleep(t_arr - now)
+            ids[i] = srv.submit_into(sp.m0, sp.m0 + sp.M, c, outs[i][0], outs[i][1], seed=sp.seed)
+        done.wait()
+        t_total = time.perf_counter() - t_start
+        lat_ms = np.array(lat) * 1e3
+        untyped = sum(int((c == synth.CODE_MISSING).sum()) for c in codes)
+        busy_s = t_total - reqs[0][0] if reqs else t_total
+        return {"requests": len(reqs), "p50_ms": float(np.percentile(lat_ms, 50)),
+                "p99_ms": float(np.percentile(lat_ms, 99)), "mean_ms": float(lat_ms.mean()),
+                "genotypes_per_s": untyped / busy_s}
+
+    def offline(srv, reqs, codes, outs):
+        before = srv.stats()
+        t1 = time.perf_counter()
+        ids = [srv.submit_into(sp.m0, sp.m0 + sp.M, c, o[0], o[1], seed=sp.seed)
+               for (_, sp), c, o in zip(reqs, codes, outs)]
+        t_sub = time.perf_counter() - t1
+        for i in ids:
+            srv.wait(i)
+        wall = time.perf_counter() - t1
+        after = srv.stats()
+        d = {k: after[k] - before[k] for k in after
+             if k not in ("max_batch_requests", "peak_bytes")}
+        d["submit_s"] = t_sub
+        return wall, d
+
+    def make_server(plain=False):
+        return serving.Server(pan.cols, H, pan.pos, pan.genetic_cM(), L=L, ne=ne, max_streams=4,
+                              max_batch=64, hbm_budget_gb=args.serve_budget_gb,
+                              max_queue_delay_us=args.queue_delay_us, plain=plain)
+
+    pcie = _pcie_rates()
+    log(f"[serve] PCIe: D2H {pcie['d2h_GBps']:.1f} GB/s, H2D {pcie['h2d_GBps']:.1f} GB/s, "
+        f"both at once {pcie['both_GBps_each']:.1f} GB/s each")
+
+    main_reqs, main_codes = request_set(args.rate, args.serve_seconds, 12345)
+    untyped_main = sum(int((c == synth.CODE_MISSING).sum()) for c in main_codes)
+    log(f"[serve] {len(main_reqs)} requests at {args.rate} req/s over {args.serve_seconds} s")
     log("[serve] starting the server (PBWT index build on the device)")
     t0 = time.perf_counter()
-    srv = serving.Server(pan.cols, H, pan.pos, pan.genetic_cM(), L=L, ne=ne, max_streams=4,
-                         max_batch=64, hbm_budget_gb=args.serve_budget_gb,
-                         max_queue_delay_us=args.queue_delay_us)
+    srv = make_server()
     startup_s = time.perf_counter() - t0
     log(f"[serve] server ready in {startup_s:.1f} s")
+    srv.wait(srv.submit(0, 2000, np.full((4, 2000), 3, np.uint8)))  # kernels warmed once
 
-    # warm up the kernels once
-    srv.wait(srv.submit(0, 2000, np.full((4, 2000), 3, np.uint8)))
+    # arrival-rate sweep (online latency and throughput per rate)
+    sweep = {}
+    for k, rate in enumerate(args.serve_rates):
+        reqs, codes = request_set(rate, args.sweep_seconds, 777 + k)
+        if not reqs:
+            continue
+        outs, keep = pinned_outputs(reqs)
+        r = online(srv, reqs, codes, outs)
+        sweep[str(rate)] = r
+        log(f"[serve] {rate} req/s: {r['requests']} requests, p50 {r['p50_ms']:.1f} ms, "
+            f"p99 {r['p99_ms']:.1f} ms, {r['genotypes_per_s']:.3g} genotypes/s")
+        del outs, keep
 
-    lat = [None] * len(reqs)
-    ids = [None] * len(reqs)
-    done = threading.Event()
-
-    def waiter():
-        for i in range(len(reqs)):
-            while ids[i] is None:
-                time.sleep(0.0005)
-            lat[i] = srv.wait(ids[i]).latency_s
-        done.set()
-
-    th = threading.Thread(target=waiter, daemon=True)
-    th.start()
-    t_start = time.perf_counter()
-    for i, ((t_arr, sp), c) in enumerate(zip(reqs, codes)):
-        now = time.perf_counter() - t_start
-        if t_arr > now:
-            time.sleep(t_arr - now)
-        ids[i] = srv.submit_into(sp.m0, sp.m0 + sp.M, c, outs[i][0], outs[i][1], seed=sp.seed)
-    done.wait()
-    t_online = time.perf_counter() - t_start
-    st_online = srv.stats()
-    lat_ms = np.array(lat) * 1e3
-    log(f"[serve] online: p50 {np.percentile(lat_ms, 50):.1f} ms, p99 {np.percentile(lat_ms, 99):.1f} ms")
-
-    # the same requests, all queued at once (batched-offline)
-    t1 = time.perf_counter()
-    ids2 = [srv.submit_into(sp.m0, sp.m0 + sp.M, c, o[0], o[1], seed=sp.seed)
-            for (_, sp), c, o in zip(reqs, codes, outs)]
-    for i in ids2:
-        srv.wait(i)
-    t_offline = time.perf_counter() - t1
-    st_all = srv.stats()
+    outs, keep = pinned_outputs(main_reqs)
+    main_online = online(srv, main_reqs, main_codes, outs)
+    log(f"[serve] online at {args.rate} req/s: p50 {main_online['p50_ms']:.1f} ms, "
+        f"p99 {main_online['p99_ms']:.1f} ms")
+    t_off, st_off = offline(srv, main_reqs, main_codes, outs)
+    peak = srv.stats()["peak_bytes"]
     srv.close()
-    log(f"[serve] offline: {t_offline:.2f} s for {len(reqs)} requests")
-    busy_s = t_online - reqs[0][0] if reqs else t_online
+    log(f"[serve] offline: {t_off:.2f} s for {len(main_reqs)} requests")
+    # the plain form on the same requests: each request its own launches
+    srv = make_server(plain=True)
+    srv.wait(srv.submit(0, 2000, np.full((4, 2000), 3, np.uint8)))
+    t_plain, st_plain = offline(srv, main_reqs, main_codes, outs)
+    srv.close()
+    log(f"[serve] offline, plain form: {t_plain:.2f} s")
+    del outs, keep
+
+    out_bytes = st_off["d2h_bytes"]
+    in_bytes = st_off["h2d_bytes"]
+    # the link's copy rate with both directions busy is its rate for their sum
+    pcie_s = max(out_bytes / (pcie["d2h_GBps"] * 1e9), in_bytes / (pcie["h2d_GBps"] * 1e9),
+                 (in_bytes + out_bytes) / (2.0 * pcie["both_GBps_each"] * 1e9))
+
+    def split(st, wall):
+        return {"wall_s": wall, "submit_s": st["submit_s"], "kernel_busy_s": st["kernel_s"],
+                "h2d_busy_s": st["h2d_s"],
+                "d2h_busy_s": st["d2h_s"], "launch_host_s": st["launch_host_s"],
+                "copy_host_s": st["copy_host_s"], "batches": st["batches"],
+                "h2d_bytes": st["h2d_bytes"], "d2h_bytes": st["d2h_bytes"]}
+
     return {
-        "workload": f"{H} x {M} resident panel; Poisson arrivals at {args.rate} req/s over "
-                    f"{args.serve_seconds} s; cohorts log-uniform 1-5000, regions log-uniform "
-                    f"1000-100000 markers; L={L}, Ne={ne}, typed every {te}th marker",
-        "requests": len(reqs),
+        "workload": f"{H} x {M} resident panel; Poisson arrivals, cohorts log-uniform 1-5000, "
+                    f"regions log-uniform 1000-100000 markers; L={L}, Ne={ne}, typed every "
+                    f"{te}th marker; 4 streams, up to 64 requests per batch",
+        "rate_sweep": {"seconds_per_rate": args.sweep_seconds, "per_rate": sweep},
+        "requests": len(main_reqs),
         "rate_req_per_s": args.rate,
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
-        "mean_ms": float(lat_ms.mean()),
-        "genotypes_per_s": untyped / busy_s,
-        "offline_genotypes_per_s": untyped / t_offline,
-        "untyped_genotypes": untyped,
-        "batches_online": st_online["batches"] - 1,
-        "max_batch_requests": st_online["max_batch_requests"],
-        "peak_admitted_bytes": st_all["peak_bytes"],
+        "seconds": args.serve_seconds,
+        "p50_ms": main_online["p50_ms"],
+        "p99_ms": main_online["p99_ms"],
+        "mean_ms": main_online["mean_ms"],
+        "genotypes_per_s": main_online["genotypes_per_s"],
+        "untyped_genotypes": untyped_main,
+        "offline_genotypes_per_s": untyped_main / t_off,
+        "offline_plain_genotypes_per_s": untyped_main / t_plain,
+        "offline_batched_vs_plain": t_plain / t_off,
+        "offline_split": split(st_off, t_off),
+        "offline_plain_split": split(st_plain, t_plain),
+        "pcie": pcie,
+        "pcie_bound": {
+            "s": pcie_s,
+            "genotypes_per_s": untyped_main / pcie_s,
+            "frac": pcie_s / t_off,
+            "in_bytes": in_bytes,
+            "out_bytes": out_bytes,
+            "note": "every request's results (2 haplotype bytes and a 4-byte dosage per "
+                    "(sample, marker)) cross to the host and its codes (1 byte per (sample, "
+                    "marker), rows padded to 16) cross to the device; bound = the slowest of "
+                    "results alone at the D2H rate, codes alone at the H2D rate, and both at "
+                    "the rate the link gives both directions at once (their sum)",
+        },
+        "peak_admitted_bytes": peak,
         "server_startup_s": startup_s,
         "note": "online genotypes/s is bounded by the arrival rate; offline = every request "
-                "queued at once on the same server",
+                "queued at once on the same server; busy times are sums over batches on 4 "
+                "streams, so they can exceed the wall time",
     }
 
 
@@ -844,6 +966,9 @@ def main():
                     help="timed steps of the one-job-per-launch comparison")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=15.0)
+    ap.add_argument("--serve-rates", type=lambda x: [float(v) for v in x.split(",")],
+                    default=[2.0, 5.0, 10.0, 20.0, 50.0])
+    ap.add_argument("--sweep-seconds", type=float, default=4.0)
     ap.add_argument("--serve-H", type=int, default=40000)
     ap.add_argument("--serve-M", type=int, default=500000)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)

@@ -49,6 +49,14 @@ typedef struct si_server_cfg {
                                    at m0, as the reference's --region run does
                                    (main.cpp:103, imputer.cpp:40-44); costs a sweep and
                                    M*H*4 bytes per request */
+    uint32_t plain;             /* 0: a slot's requests run as one batched launch per pass;
+                                   1: each request alone through si_fb_run (one job per
+                                   launch), the same staging and copies otherwise */
+    uint64_t staging_bytes;     /* page-locked host pool for queued requests' codes (4 GiB):
+                                   submit copies the codes into it and the upload reads
+                                   them from it; a request that finds it full keeps its
+                                   codes in pageable memory, uploaded through a staging
+                                   copy (0: no pool) */
 } si_server_cfg;
 
 void si_server_defaults(si_server_cfg* cfg);
@@ -83,10 +91,18 @@ int si_server_wait(si_server* s, uint64_t request_id, uint8_t* hap, float* ds, d
 /* Counters since creation: completed requests, batches, imputed genotypes
  * (untyped (sample, marker) pairs), largest batch, peak admitted bytes,
  * admissions deferred for memory, and requests whose results the device
- * copied straight into page-locked caller buffers (si_server_submit_into). */
+ * copied straight into page-locked caller buffers (si_server_submit_into).
+ * Where the time goes, summed over retired batches: bytes copied to and
+ * from the device; the scheduler's host time staging and enqueueing them;
+ * each batch's stream spans for its code upload, its forward-backward
+ * launches and its result copies (batches on different streams overlap, so
+ * these can add up to more than the wall time); the copier threads' host
+ * time moving results out of staging. */
 typedef struct si_server_stats {
     uint64_t requests, batches, genotypes, untyped_genotypes, max_batch_requests;
     uint64_t peak_bytes, deferred_admissions, direct_copies;
+    uint64_t h2d_bytes, d2h_bytes;
+    double launch_host_s, h2d_s, kernel_s, d2h_s, copy_host_s;
 } si_server_stats;
 int si_server_get_stats(si_server* s, si_server_stats* out);
 

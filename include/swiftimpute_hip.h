@@ -295,13 +295,22 @@ int si_fb_tune(const si_fb_job* jobs, uint32_t n_jobs, void* workspace, size_t w
  * (src/api/imputer.cpp:765-777).
  *
  * All jobs share one emission table (eps/emax/palette pointers and
- * n_palette of jobs[0]; the others must match).  si_fb_batch_create copies
- * the job descriptors to the device and allocates the batch's queues (set-up,
- * not a hot call); si_fb_batch_run enqueues the batch on a stream and
- * allocates nothing.  The workspace holds every job's checkpoints at once.
+ * n_palette of jobs[0]; the others must match).  si_fb_batch_create plans the
+ * batch and allocates its device tables (set-up, not a hot call);
+ * si_fb_batch_run copies the tables on its stream, from page-locked staging,
+ * ahead of the launches, and waits on the host only for the previous run's
+ * table copies.  The workspace holds every job's checkpoints at once.
  */
 typedef struct si_fb_batch si_fb_batch;
 int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out);
+/* Re-plans an existing batch for other jobs, keeping its device tables,
+ * staging, streams and events where they have room (it allocates only when a
+ * class outgrows every table the batch holds), so a serving loop can cycle a
+ * few batches without allocating per request.  Variants go back to the
+ * defaults, untuned.  The batch's previous runs must have completed, or be
+ * ordered before its next run on that run's stream.  No counterpart in the
+ * reference, which has no batched form. */
+int si_fb_batch_reset(si_fb_batch* batch, const si_fb_job* jobs, uint32_t n_jobs);
 size_t si_fb_batch_workspace_bytes(const si_fb_batch* batch);
 /* use_graph: capture the launches into a hipGraph on first run for this
  * (workspace, stream) and replay it afterwards.  A class without a variant

@@ -111,6 +111,7 @@ HIP_SYMBOLS = [
                                         ctypes.c_size_t, ctypes.c_uint32, _vp]),
     ("si_fb_batch_create", ctypes.c_int, [ctypes.POINTER(FbJob), ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_void_p)]),
+    ("si_fb_batch_reset", ctypes.c_int, [_vp, ctypes.POINTER(FbJob), ctypes.c_uint32]),
     ("si_fb_batch_workspace_bytes", ctypes.c_size_t, [_vp]),
     ("si_fb_batch_run", ctypes.c_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_int, _vp]),
     ("si_fb_batch_destroy", ctypes.c_int, [_vp]),

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -51,6 +52,82 @@ void host_parallel(size_t n, size_t bytes, F&& fn) {
     fn(size_t(0), n / nt);
     for (auto& x : th) x.join();
 }
+
+// A fixed set of host threads for the request path's byte work (copying and
+// scanning request codes): run(n, fn) calls fn(k) for k in [0, n) on the
+// pool's threads and the caller, and returns when every call has finished
+// and every thread is idle again.  One job at a time (callers queue on
+// run_mu); no thread is started per request.
+class WorkPool {
+  public:
+    explicit WorkPool(unsigned n) {
+        for (unsigned i = 0; i < n; ++i) th_.emplace_back([this] { loop(); });
+    }
+    ~WorkPool() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    size_t threads() const { return th_.size() + 1; }
+    void run(size_t n, const std::function<void(size_t)>& fn) {
+        if (n == 0) return;
+        std::lock_guard<std::mutex> one(run_mu_);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            job_ = &fn;
+            n_ = n;
+            next_.store(0);
+            remaining_.store(n);
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> g(mu_);
+        done_.wait(g, [&] { return remaining_.load() == 0 && active_ == 0; });
+        job_ = nullptr;
+        n_ = 0;
+    }
+
+  private:
+    void work() {
+        for (;;) {
+            const size_t k = next_.fetch_add(1);
+            if (k >= n_) return;
+            (*job_)(k);
+            if (remaining_.fetch_sub(1) == 1) {
+                std::lock_guard<std::mutex> g(mu_);
+                done_.notify_all();
+            }
+        }
+    }
+    void loop() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> g(mu_);
+        for (;;) {
+            cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            if (!job_) continue;
+            ++active_;
+            g.unlock();
+            work();
+            g.lock();
+            if (--active_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex run_mu_, mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(size_t)>* job_ = nullptr;
+    size_t n_ = 0;
+    std::atomic<size_t> next_{0}, remaining_{0};
+    uint64_t gen_ = 0;
+    unsigned active_ = 0;
+    bool stop_ = false;
+};
 
 // Page-locked host memory (hipHostMalloc, torch's pin_memory): results are
 // copied from the device into it directly, without the staging copy.
@@ -127,7 +204,12 @@ struct Request {
     uint64_t id = 0;
     uint32_t m0 = 0, M = 0, B = 0, pitch = 0;
     uint64_t seed = 0;
-    std::vector<uint8_t> codes;  // [B][pitch]
+    // codes [B][pitch]: in the server's page-locked pool (uploaded from there)
+    // or, when the pool is full, in codes_heap (uploaded through slot staging)
+    uint8_t* codes = nullptr;
+    std::unique_ptr<uint8_t[]> codes_heap;
+    size_t codes_bytes = 0, pool_off = 0;
+    bool in_pool = false;
     uint64_t untyped = 0;
     Clock::time_point arrival, done_at;
     size_t off_codes = 0, off_hap = 0, off_ds = 0, own_bytes = 0;
@@ -147,13 +229,16 @@ enum class SlotState { Free, Running, Copying };
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    // stream marks: before the code upload, before the forward-backward,
+    // before the result copies, after them
+    hipEvent_t t0 = nullptr, k0 = nullptr, k1 = nullptr, done = nullptr;
     SlotState state = SlotState::Free;
     std::vector<std::shared_ptr<Request>> reqs;
-    si_fb_batch* batch = nullptr;
+    si_fb_batch* batch = nullptr;  // kept across launches (si_fb_batch_reset)
     size_t ws_off = 0, ws_bytes = 0;
-    char* pinned = nullptr;  // staging for codes in and results out
+    char* pinned = nullptr;  // staging for codes in (pool misses) and results out
     size_t pinned_bytes = 0;
+    size_t out_base = 0;     // where the results start in `pinned`
 };
 
 }  // namespace
@@ -175,6 +260,12 @@ struct si_server {
     float* d_palette = nullptr;
     char* d_arena = nullptr;
     Arena arena;
+    // page-locked pool for request codes: submit writes them there once and
+    // the upload reads them from there (no second host copy, no page faults
+    // of fresh pageable memory in the request path)
+    char* h_codes = nullptr;
+    Arena codes_pool;
+    std::unique_ptr<WorkPool> pool;  // host threads copying and scanning codes in submit
     // scheduling
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -190,9 +281,6 @@ struct si_server {
     bool stop = false;
     bool copiers_stop = false;
     uint64_t next_id = 1;
-    // finished batch descriptors; released when no batch is in flight, since
-    // freeing device memory can wait on every stream of the device
-    std::vector<si_fb_batch*> finished;
     si_server_stats stats{};
     std::string worker_error;
     // Testing aid: when nonzero, the launch of this batch (1-based count)
@@ -204,13 +292,14 @@ struct si_server {
     std::atomic<uint64_t> n_direct{0};
 
     ~si_server() {
-        for (auto* b : finished) si_fb_batch_destroy(b);
         for (auto& sl : slots) {
             if (sl.batch) si_fb_batch_destroy(sl.batch);
             if (sl.pinned) (void)hipHostFree(sl.pinned);
-            if (sl.done) (void)hipEventDestroy(sl.done);
+            for (hipEvent_t e : {sl.t0, sl.k0, sl.k1, sl.done})
+                if (e) (void)hipEventDestroy(e);
             if (sl.stream) (void)hipStreamDestroy(sl.stream);
         }
+        if (h_codes) (void)hipHostFree(h_codes);
         for (void* p : {static_cast<void*>(d_cols), static_cast<void*>(d_sa), static_cast<void*>(d_mmask),
                         static_cast<void*>(d_steps), static_cast<void*>(d_eps), static_cast<void*>(d_emax),
                         static_cast<void*>(d_palette), static_cast<void*>(d_arena)})
@@ -284,10 +373,18 @@ struct si_server {
     // The batch of a slot finished on the device: its device memory is free
     // again, and its results wait in the slot's pinned staging for a copier.
     void retire(Slot& sl) {
-        for (auto& r : sl.reqs) arena.release(r->off_codes, r->own_bytes);
+        for (auto& r : sl.reqs) {
+            arena.release(r->off_codes, r->own_bytes);
+            if (r->in_pool) codes_pool.release(r->pool_off, r->codes_bytes);
+            r->in_pool = false;
+            r->codes = nullptr;
+            r->codes_heap.reset();
+        }
         arena.release(sl.ws_off, sl.ws_bytes);
-        finished.push_back(sl.batch);
-        sl.batch = nullptr;
+        float a = 0.0f, b = 0.0f, c = 0.0f;
+        if (hipEventElapsedTime(&a, sl.t0, sl.k0) == hipSuccess) stats.h2d_s += 1e-3 * a;
+        if (hipEventElapsedTime(&b, sl.k0, sl.k1) == hipSuccess) stats.kernel_s += 1e-3 * b;
+        if (hipEventElapsedTime(&c, sl.k1, sl.done) == hipSuccess) stats.d2h_s += 1e-3 * c;
         sl.state = SlotState::Copying;
         copy_queue.push_back(&sl);
         cv_copy.notify_one();
@@ -303,24 +400,29 @@ struct si_server {
             Slot* sl = copy_queue.front();
             copy_queue.pop_front();
             lk.unlock();
-            size_t o = 0;
-            for (auto& r : sl->reqs) o += size_t(r->B) * r->pitch;
+            const auto c0 = Clock::now();
+            // the staged parts (destinations not in page-locked memory), in
+            // the order launch() laid them out
+            size_t o = sl->out_base;
             for (auto& r : sl->reqs) {
-                const size_t hb = size_t(r->B) * 2 * r->M, n = size_t(r->B) * r->M;
+                const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
+                const char* sh = r->hap_pinned ? nullptr : sl->pinned + o;
+                if (!r->hap_pinned) o += hb;
+                const char* sd = r->ds_pinned ? nullptr : sl->pinned + o;
+                if (!r->ds_pinned) o += db;
                 if (r->into) {
-                    if (r->dst_hap && !r->hap_pinned) par_memcpy(r->dst_hap, sl->pinned + o, hb);
-                    if (r->dst_ds && !r->ds_pinned)
-                        par_memcpy(r->dst_ds, sl->pinned + o + hb, n * sizeof(float));
+                    if (r->dst_hap && sh) par_memcpy(r->dst_hap, sh, hb);
+                    if (r->dst_ds && sd) par_memcpy(r->dst_ds, sd, db);
                 } else {
                     r->hap.resize(hb);
-                    r->ds.resize(n);
-                    par_memcpy(r->hap.data(), sl->pinned + o, hb);
-                    par_memcpy(r->ds.data(), sl->pinned + o + hb, n * sizeof(float));
+                    r->ds.resize(db / sizeof(float));
+                    par_memcpy(r->hap.data(), sh, hb);
+                    par_memcpy(r->ds.data(), sd, db);
                 }
-                o += hb + n * sizeof(float);
             }
             const auto now = Clock::now();
             lk.lock();
+            stats.copy_host_s += std::chrono::duration<double>(now - c0).count();
             for (auto& r : sl->reqs) {
                 r->done = true;
                 r->done_at = now;
@@ -376,29 +478,51 @@ struct si_server {
         std::vector<si_fb_job> jobs;
         for (auto& r : take_ref) jobs.push_back(job_of(*r));
         lk.unlock();  // the slot is ours; staging and launch need no shared state
-        struct Relock {
-            std::unique_lock<std::mutex>& l;
-            ~Relock() { l.lock(); }
-        } relock{lk};
-        size_t in_bytes = 0, out_bytes = 0;
+        const auto h0 = Clock::now();
+        // page-locked staging holds the codes of pool misses and the results
+        // bound for pageable destinations; the rest moves by DMA directly
+        size_t in_bytes = 0, out_bytes = 0, stage_in = 0, stage_out = 0;
         for (auto& r : take_ref) {
-            in_bytes += size_t(r->B) * r->pitch;
-            out_bytes += size_t(r->B) * r->M * (2 + sizeof(float));
+            const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
+            in_bytes += r->codes_bytes;
+            if (!r->in_pool) stage_in += r->codes_bytes;
+            out_bytes += hb + db;
+            stage_out += (r->hap_pinned ? 0 : hb) + (r->ds_pinned ? 0 : db);
         }
-        if (sl.pinned_bytes < in_bytes + out_bytes) {
+        struct Relock {
+            si_server* s;
+            std::unique_lock<std::mutex>& l;
+            Clock::time_point h0;
+            size_t in, out;
+            ~Relock() {
+                const double dt = std::chrono::duration<double>(Clock::now() - h0).count();
+                l.lock();
+                s->stats.launch_host_s += dt;
+                s->stats.h2d_bytes += in;
+                s->stats.d2h_bytes += out;
+            }
+        } relock{this, lk, h0, in_bytes, out_bytes};
+        if (sl.pinned_bytes < stage_in + stage_out) {
             if (sl.pinned) (void)hipHostFree(sl.pinned);
-            sl.pinned_bytes = std::max<size_t>(in_bytes + out_bytes, sl.pinned_bytes * 2);
+            sl.pinned = nullptr;
+            sl.pinned_bytes = std::max<size_t>(stage_in + stage_out, sl.pinned_bytes * 2);
             if (hipHostMalloc(reinterpret_cast<void**>(&sl.pinned), sl.pinned_bytes, 0) != hipSuccess)
                 throw std::runtime_error("hipHostMalloc failed");
         }
+        if (hipEventRecord(sl.t0, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
         size_t o = 0;
         for (auto& r : take_ref) {
-            par_memcpy(sl.pinned + o, r->codes.data(), r->codes.size());
-            if (hipMemcpyAsync(arena.ptr(r->off_codes), sl.pinned + o, r->codes.size(),
-                               hipMemcpyHostToDevice, sl.stream) != hipSuccess)
+            const uint8_t* src = r->codes;
+            if (!r->in_pool) {
+                par_memcpy(sl.pinned + o, r->codes, r->codes_bytes);
+                src = reinterpret_cast<const uint8_t*>(sl.pinned + o);
+                o += r->codes_bytes;
+            }
+            if (hipMemcpyAsync(arena.ptr(r->off_codes), src, r->codes_bytes, hipMemcpyHostToDevice,
+                               sl.stream) != hipSuccess)
                 throw std::runtime_error("codes upload failed");
-            o += r->codes.size();
         }
+        sl.out_base = o;
         if (cfg.region_resweep) {
             // each request's region swept from the identity order at its first marker
             const size_t W = (H + 63) / 64;
@@ -416,32 +540,50 @@ struct si_server {
         }
         if (fail_at_batch && stats.batches == fail_at_batch)
             throw std::runtime_error("injected launch failure (SI_SERVER_FAIL_AT_BATCH)");
-        if (si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch) != SI_OK)
-            throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
-        // Default lanes per sample, not the autotuner: a batch of a cohort-size
-        // bucket not seen before would otherwise time every candidate inside
-        // its requests' latency, and on these shapes the tuner keeps the
-        // defaults anyway.  Outputs do not depend on the variant.
-        for (uint32_t k = 0; k < si_fb_batch_n_classes(sl.batch); ++k) {
-            uint32_t lp = 0, flags = 0;
-            int tuned = 0;
-            si_fb_variant v{};
-            if (si_fb_batch_class_info(sl.batch, k, &lp, &flags, &v, &tuned) != SI_OK ||
-                si_fb_default_variant(lp, &v) != SI_OK || si_fb_batch_set_variant(sl.batch, k, &v) != SI_OK)
-                throw std::runtime_error(std::string("batch variant failed: ") + si_last_error());
+        if (hipEventRecord(sl.k0, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
+        if (cfg.plain) {
+            // one job per launch, each with its own part of the slot's workspace
+            size_t wo = 0;
+            for (size_t k = 0; k < jobs.size(); ++k) {
+                const size_t wb = ws_bytes_of(*take_ref[k]);
+                if (si_fb_run(&jobs[k], 1, arena.ptr(ws_off + wo), wb, sl.stream) != SI_OK)
+                    throw std::runtime_error(std::string("plain launch failed: ") + si_last_error());
+                wo += wb;
+            }
+        } else {
+            // the slot's batch is planned again in place: its device tables and
+            // staging are reused, so a launch allocates nothing once the
+            // batches have reached their working size
+            const int rc = sl.batch ? si_fb_batch_reset(sl.batch, jobs.data(), uint32_t(jobs.size()))
+                                    : si_fb_batch_create(jobs.data(), uint32_t(jobs.size()), &sl.batch);
+            if (rc != SI_OK) throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
+            // Default lanes per sample, not the autotuner: a batch of a
+            // cohort-size bucket not seen before would otherwise time every
+            // candidate inside its requests' latency, and on these shapes the
+            // tuner keeps the defaults anyway.  Outputs do not depend on the
+            // variant.
+            for (uint32_t k = 0; k < si_fb_batch_n_classes(sl.batch); ++k) {
+                uint32_t lp = 0, flags = 0;
+                int tuned = 0;
+                si_fb_variant v{};
+                if (si_fb_batch_class_info(sl.batch, k, &lp, &flags, &v, &tuned) != SI_OK ||
+                    si_fb_default_variant(lp, &v) != SI_OK || si_fb_batch_set_variant(sl.batch, k, &v) != SI_OK)
+                    throw std::runtime_error(std::string("batch variant failed: ") + si_last_error());
+            }
+            if (si_fb_batch_run(sl.batch, arena.ptr(ws_off), ws_total, 0, sl.stream) != SI_OK)
+                throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
         }
-        if (si_fb_batch_run(sl.batch, arena.ptr(ws_off), ws_total, 0, sl.stream) != SI_OK)
-            throw std::runtime_error(std::string("batch launch failed: ") + si_last_error());
+        if (hipEventRecord(sl.k1, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
         for (auto& r : take_ref) {
             const size_t hb = size_t(r->B) * 2 * r->M, db = size_t(r->B) * r->M * sizeof(float);
             if (r->hap_pinned || r->ds_pinned) n_direct.fetch_add(1, std::memory_order_relaxed);
             // into the staging, or straight into page-locked caller buffers
             void* dh = r->hap_pinned ? static_cast<void*>(r->dst_hap) : static_cast<void*>(sl.pinned + o);
             (void)hipMemcpyAsync(dh, arena.ptr(r->off_codes + r->off_hap), hb, hipMemcpyDeviceToHost, sl.stream);
-            o += hb;
+            if (!r->hap_pinned) o += hb;
             void* dd = r->ds_pinned ? static_cast<void*>(r->dst_ds) : static_cast<void*>(sl.pinned + o);
             (void)hipMemcpyAsync(dd, arena.ptr(r->off_codes + r->off_ds), db, hipMemcpyDeviceToHost, sl.stream);
-            o += db;
+            if (!r->ds_pinned) o += db;
         }
         if (hipEventRecord(sl.done, sl.stream) != hipSuccess) throw std::runtime_error("event record failed");
         return true;
@@ -471,10 +613,6 @@ struct si_server {
                     if (ready && sl.state == SlotState::Free && !queue.empty() && launch(sl, lk)) {
                         launched = inflight = true;
                     }
-                if (!inflight && !finished.empty()) {
-                    for (auto* b : finished) si_fb_batch_destroy(b);
-                    finished.clear();
-                }
                 if (stop && queue.empty() && !inflight && !copying) break;
                 if (launched) continue;
                 if (inflight) {
@@ -525,6 +663,8 @@ void si_server_defaults(si_server_cfg* c) {
     c->max_queue_delay_us = 0;
     c->selection = 0;
     c->region_resweep = 0;
+    c->plain = 0;
+    c->staging_bytes = 4ull << 30;
 }
 
 int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H, uint32_t M,
@@ -534,6 +674,7 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
     if (cfg->n_states == 0 || cfg->n_states > 64) return fail(SI_ERR_ARG, "n_states must be in 1..64");
     if (cfg->selection > 1) return fail(SI_ERR_ARG, "selection must be 0 (CPU path) or 1 (GPU kernels)");
     if (cfg->region_resweep > 1) return fail(SI_ERR_ARG, "region_resweep must be 0 or 1");
+    if (cfg->plain > 1) return fail(SI_ERR_ARG, "plain must be 0 or 1");
     auto s = std::make_unique<si_server>();
     s->cfg = *cfg;
     s->cfg.max_streams = std::max(1u, cfg->max_streams);
@@ -598,12 +739,24 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
         hipMalloc(&s->d_arena, cfg->hbm_budget_bytes) != hipSuccess)
         return fail(SI_ERR_DEVICE, "si_server_create: device allocation failed");
     s->arena.init(s->d_arena, cfg->hbm_budget_bytes);
+    if (cfg->staging_bytes) {
+        if (hipHostMalloc(reinterpret_cast<void**>(&s->h_codes), cfg->staging_bytes, 0) != hipSuccess) {
+            s->h_codes = nullptr;
+            return fail(SI_ERR_DEVICE, "si_server_create: page-locked staging allocation failed");
+        }
+        s->codes_pool.init(s->h_codes, cfg->staging_bytes);
+    }
     s->slots.resize(s->cfg.max_streams);
     for (auto& sl : s->slots)
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+            hipEventCreate(&sl.t0) != hipSuccess || hipEventCreate(&sl.k0) != hipSuccess ||
+            hipEventCreate(&sl.k1) != hipSuccess || hipEventCreate(&sl.done) != hipSuccess)
             return fail(SI_ERR_DEVICE, "si_server_create: stream creation failed");
     if (const char* f = std::getenv("SI_SERVER_FAIL_AT_BATCH")) s->fail_at_batch = std::strtoull(f, nullptr, 10);
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        s->pool = std::make_unique<WorkPool>(std::min(hw, 16u) - 1u);
+    }
     si_server* raw = s.release();
     raw->worker = std::thread([raw] { raw->run(); });
     for (uint32_t i = 0; i < raw->cfg.max_streams; ++i)
@@ -629,19 +782,41 @@ int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
     r->dst_ds = ds;
     r->hap_pinned = hap && host_pinned(hap);
     r->ds_pinned = ds && host_pinned(ds);
-    r->codes.resize(size_t(B) * r->pitch);
+    const size_t need = s->own_bytes(*r) + s->ws_bytes_of(*r) + 8192;
+    if (need > s->cfg.hbm_budget_bytes)
+        return fail(SI_ERR_ARG, "si_server_submit: request needs " + std::to_string(need) +
+                                    " bytes, above the server's HBM budget");
+    r->codes_bytes = size_t(B) * r->pitch;
+    if (s->h_codes) {
+        std::lock_guard<std::mutex> g(s->mu);
+        r->in_pool = s->codes_pool.alloc(r->codes_bytes, &r->pool_off);
+    }
+    if (r->in_pool) {
+        r->codes = reinterpret_cast<uint8_t*>(s->codes_pool.ptr(r->pool_off));
+    } else {
+        r->codes_heap.reset(new uint8_t[r->codes_bytes]);
+        r->codes = r->codes_heap.get();
+    }
+    auto unreserve = [&] {
+        if (!r->in_pool) return;
+        std::lock_guard<std::mutex> g(s->mu);
+        s->codes_pool.release(r->pool_off, r->codes_bytes);
+        r->in_pool = false;
+    };
     // rows copied into the padded layout while checking the codes and counting
-    // the untyped ones (branch-free, so the scan vectorizes)
-    std::vector<uint64_t> part_untyped(8, 0);
-    std::vector<uint8_t> part_max(8, 0);
-    std::atomic<uint32_t> slot{0};
-    host_parallel(B, size_t(B) * r->M, [&](size_t lo, size_t hi) {
-        const uint32_t me = slot.fetch_add(1);
+    // the untyped ones (branch-free, so the scan vectorizes), in tasks of
+    // about 1 MiB of codes over the server's host threads
+    const size_t total = size_t(B) * r->M;
+    const size_t ntask = std::min<size_t>(B, std::max<size_t>(1, total >> 20));
+    std::vector<uint64_t> part_untyped(ntask, 0);
+    std::vector<uint8_t> part_max(ntask, 0);
+    const std::function<void(size_t)> task = [&](size_t me) {
+        const size_t lo = size_t(B) * me / ntask, hi = size_t(B) * (me + 1) / ntask;
         uint64_t un = 0;
         uint8_t mx = 0;
         for (size_t b = lo; b < hi; ++b) {
             const uint8_t* src = codes + b * r->M;
-            uint8_t* dst = &r->codes[b * r->pitch];
+            uint8_t* dst = r->codes + b * r->pitch;
             std::memcpy(dst, src, r->M);
             std::memset(dst + r->M, 3, r->pitch - r->M);
             uint32_t row_un = 0;
@@ -655,17 +830,25 @@ int si_server_submit_into(si_server* s, uint32_t m0, uint32_t m1, uint32_t B,
         }
         part_untyped[me] = un;
         part_max[me] = mx;
-    });
+    };
+    if (ntask == 1 || !s->pool) {
+        for (size_t k = 0; k < ntask; ++k) task(k);
+    } else {
+        s->pool->run(ntask, task);
+    }
     for (size_t t = 0; t < part_untyped.size(); ++t) {
-        if (part_max[t] > 3) return fail(SI_ERR_ARG, "si_server_submit: code outside 0..3");
+        if (part_max[t] > 3) {
+            unreserve();
+            return fail(SI_ERR_ARG, "si_server_submit: code outside 0..3");
+        }
         r->untyped += part_untyped[t];
     }
-    const size_t need = s->own_bytes(*r) + s->ws_bytes_of(*r) + 8192;
-    if (need > s->cfg.hbm_budget_bytes)
-        return fail(SI_ERR_ARG, "si_server_submit: request needs " + std::to_string(need) +
-                                    " bytes, above the server's HBM budget");
     std::lock_guard<std::mutex> g(s->mu);
-    if (!s->worker_error.empty()) return fail(SI_ERR_DEVICE, s->worker_error);
+    if (!s->worker_error.empty()) {
+        if (r->in_pool) s->codes_pool.release(r->pool_off, r->codes_bytes);
+        r->in_pool = false;
+        return fail(SI_ERR_DEVICE, s->worker_error);
+    }
     r->id = s->next_id++;
     s->reqs[r->id] = r;
     s->queue.push_back(r);

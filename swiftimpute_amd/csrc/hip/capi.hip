@@ -597,11 +597,14 @@ struct si_fb_batch {
     struct Class {
         uint32_t Lp = 0, flags = 0, pm = 0;
         std::vector<uint32_t> order;  // job indices, longest first
+        // device tables (room for `cap` jobs; kept and reused by si_fb_batch_reset)
         si::JobView* d_jobs = nullptr;
         uint64_t* d_off = nullptr;    // chain, block and mask-pass offsets, [n+1] each
         si::JobView* d_tjobs = nullptr;  // tuning copies
         uint64_t* d_toff = nullptr;
         unsigned long long* d_counters = nullptr;
+        size_t cap = 0;
+        std::vector<uint64_t> h_off;  // offsets for d_off, uploaded by the next bind
         uint64_t chain_items = 0, block_items = 0, mask_groups = 0;
         uint64_t units = 0;           // sum of samples x markers over the class's jobs
         uint32_t off_cb = 0;          // checkpoint block the offsets were made for
@@ -621,17 +624,35 @@ struct si_fb_batch {
     void* bound_ws = nullptr;
     hipGraphExec_t exec = nullptr;
     void* graph_ws = nullptr;
+    // Page-locked staging of the device tables: a bind writes them here and
+    // copies them on the run's stream; `staged` marks the copies' end, which
+    // the next bind waits for before writing the staging again.
+    char* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    hipEvent_t staged = nullptr;
+    // device tables of classes a reset dropped, reused by later classes
+    std::vector<Class> spare;
 };
+
+static void class_mem_free(si_fb_batch::Class& c) {
+    if (c.d_jobs) (void)hipFree(c.d_jobs);
+    if (c.d_off) (void)hipFree(c.d_off);
+    if (c.d_tjobs) (void)hipFree(c.d_tjobs);
+    if (c.d_toff) (void)hipFree(c.d_toff);
+    if (c.d_counters) (void)hipFree(c.d_counters);
+    c.d_jobs = c.d_tjobs = nullptr;
+    c.d_off = c.d_toff = nullptr;
+    c.d_counters = nullptr;
+    c.cap = 0;
+}
 
 static void batch_free(si_fb_batch* b) {
     if (!b) return;
-    for (auto& c : b->classes) {
-        if (c.d_jobs) (void)hipFree(c.d_jobs);
-        if (c.d_off) (void)hipFree(c.d_off);
-        if (c.d_tjobs) (void)hipFree(c.d_tjobs);
-        if (c.d_toff) (void)hipFree(c.d_toff);
-        if (c.d_counters) (void)hipFree(c.d_counters);
-    }
+    if (b->staged) (void)hipEventSynchronize(b->staged);
+    for (auto& c : b->classes) class_mem_free(c);
+    for (auto& c : b->spare) class_mem_free(c);
+    if (b->h_stage) (void)hipHostFree(b->h_stage);
+    if (b->staged) (void)hipEventDestroy(b->staged);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     for (hipStream_t s : b->side)
         if (s) (void)hipStreamDestroy(s);
@@ -662,17 +683,14 @@ static void class_offsets(const si_fb_batch& b, const si_fb_batch::Class& c, uin
 // Offsets and persistent grids of a class for its current variant.
 static int class_prepare(si_fb_batch* b, si_fb_batch::Class& c) {
     if (c.off_cb != c.var.cb) {
-        std::vector<uint64_t> off;
+        std::vector<uint64_t>& off = c.h_off;
         class_offsets(*b, c, c.var.cb, 0xffffffffu, off);
-        if (hipMemcpy(c.d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice) !=
-            hipSuccess)
-            return SI_ERR_DEVICE;
         const size_t n = c.order.size();
         c.chain_items = off[n];
         c.block_items = off[2 * n + 1];
         c.mask_groups = off[3 * n + 2];
         c.off_cb = c.var.cb;
-        b->bound_ws = nullptr;  // job views carry the block count: rebind
+        b->bound_ws = nullptr;  // new offsets, and job views carry the block count: rebind
         if (b->exec) {
             (void)hipGraphExecDestroy(b->exec);
             b->exec = nullptr;
@@ -690,23 +708,79 @@ static int class_prepare(si_fb_batch* b, si_fb_batch::Class& c) {
 }
 }  // extern "C++"
 
-int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out) {
-    if (!out || (n_jobs && !jobs)) return fail(SI_ERR_ARG, "si_fb_batch_create: bad arguments");
-    *out = nullptr;
-    if (n_jobs == 0) return fail(SI_ERR_ARG, "si_fb_batch_create: no jobs");
+extern "C++" {
+static int check_batch_jobs(const si_fb_job* jobs, uint32_t n_jobs, const char* what) {
+    if (n_jobs == 0) return fail(SI_ERR_ARG, "%s: no jobs", what);
     for (uint32_t i = 0; i < n_jobs; ++i) {
         const int rc = check_job(jobs[i], i);
         if (rc) return rc;
         if (jobs[i].eps != jobs[0].eps || jobs[i].emax != jobs[0].emax ||
             jobs[i].palette != jobs[0].palette || jobs[i].n_palette != jobs[0].n_palette ||
             si::fb_untyped_code(jobs[i]) != si::fb_untyped_code(jobs[0]))
-            return fail(SI_ERR_ARG, "si_fb_batch_create: job %u uses a different emission table", i);
+            return fail(SI_ERR_ARG, "%s: job %u uses a different emission table", what, i);
     }
-    auto* b = new si_fb_batch();
+    return SI_OK;
+}
+
+// Device tables for a class of n jobs: the smallest spare tables with room,
+// else new ones (1.5x the need and at least 64 jobs, so a few resets of
+// growing batches settle).  Tables too small stay spare until the batch is
+// destroyed: freeing device memory can wait for every stream of the device,
+// which a serving loop planning its next batch must not.
+static int class_mem(si_fb_batch* b, si_fb_batch::Class& c, size_t n) {
+    size_t best = b->spare.size();
+    for (size_t k = 0; k < b->spare.size(); ++k)
+        if (b->spare[k].cap >= n && (best == b->spare.size() || b->spare[k].cap < b->spare[best].cap)) best = k;
+    if (best < b->spare.size()) {
+        si_fb_batch::Class& sp = b->spare[best];
+        c.d_jobs = sp.d_jobs;
+        c.d_off = sp.d_off;
+        c.d_tjobs = sp.d_tjobs;
+        c.d_toff = sp.d_toff;
+        c.d_counters = sp.d_counters;
+        c.cap = sp.cap;
+        b->spare.erase(b->spare.begin() + long(best));
+        return SI_OK;
+    }
+    const size_t cap = std::max<size_t>({n + n / 2, n, 64});
+    if (hipMalloc(&c.d_jobs, cap * sizeof(si::JobView)) != hipSuccess ||
+        hipMalloc(&c.d_off, 3 * (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&c.d_tjobs, cap * sizeof(si::JobView)) != hipSuccess ||
+        hipMalloc(&c.d_toff, 3 * (cap + 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&c.d_counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        class_mem_free(c);
+        return SI_ERR_DEVICE;
+    }
+    c.cap = cap;
+    return SI_OK;
+}
+
+// Plans `b` for `jobs`: per-job workspace offsets, classes by (padded states,
+// numeric space, untyped-step kernels) with jobs longest chain first, device
+// tables (the batch's own where they have room), side streams and events.
+static int batch_plan(si_fb_batch* b, const si_fb_job* jobs, uint32_t n_jobs, const char* what) {
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    b->graph_ws = nullptr;
+    b->bound_ws = nullptr;
+    for (auto& c : b->classes) {
+        si_fb_batch::Class sp;
+        sp.d_jobs = c.d_jobs;
+        sp.d_off = c.d_off;
+        sp.d_tjobs = c.d_tjobs;
+        sp.d_toff = c.d_toff;
+        sp.d_counters = c.d_counters;
+        sp.cap = c.cap;
+        b->spare.push_back(std::move(sp));
+    }
+    b->classes.clear();
     b->jobs.assign(jobs, jobs + n_jobs);
     b->tables = si::FbTables{jobs[0].eps, jobs[0].emax, jobs[0].palette, jobs[0].n_palette,
                              si::fb_untyped_code(jobs[0])};
-    b->ws_off.resize(n_jobs);
+    b->ws_off.assign(n_jobs, 0);
+    b->ws_bytes = 0;
     for (uint32_t i = 0; i < n_jobs; ++i) {
         b->ws_off[i] = b->ws_bytes;
         const size_t w = si::fb_workspace_bytes_one(jobs[i].n_samples, jobs[i].n_markers,
@@ -714,8 +788,6 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
                                                     si::fb_window(jobs[i]));
         b->ws_bytes += (w + 255) & ~size_t(255);
     }
-    // classes by (padded states, numeric space, untyped-step kernels); jobs
-    // longest chain first
     for (uint32_t i = 0; i < n_jobs; ++i) {
         if (jobs[i].n_samples == 0 || jobs[i].n_markers == 0) continue;
         const uint32_t Lp = si_padded_states(jobs[i].n_states);
@@ -737,37 +809,47 @@ int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out
             return jobs[x].n_markers > jobs[y].n_markers;
         });
         for (uint32_t i : c.order) c.units += uint64_t(jobs[i].n_samples) * jobs[i].n_markers;
-        const size_t n = c.order.size();
-        if (hipMalloc(&c.d_jobs, n * sizeof(si::JobView)) != hipSuccess ||
-            hipMalloc(&c.d_off, 3 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
-            hipMalloc(&c.d_tjobs, n * sizeof(si::JobView)) != hipSuccess ||
-            hipMalloc(&c.d_toff, 3 * (n + 1) * sizeof(uint64_t)) != hipSuccess ||
-            hipMalloc(&c.d_counters, 2 * sizeof(unsigned long long)) != hipSuccess ||
-            class_prepare(b, c) != SI_OK) {
-            batch_free(b);
-            return device_status(SI_ERR_DEVICE, "si_fb_batch_create");
-        }
+        if (class_mem(b, c, c.order.size()) != SI_OK || class_prepare(b, c) != SI_OK)
+            return device_status(SI_ERR_DEVICE, what);
     }
     if (b->classes.size() > 1) {
-        b->side.assign(b->classes.size(), nullptr);
-        b->ev.assign(b->classes.size() + 1, nullptr);
-        for (auto& s : b->side)
-            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
-        for (auto& e : b->ev)
-            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
-        for (auto s : b->side)
-            if (!s) {
-                batch_free(b);
-                return device_status(SI_ERR_DEVICE, "si_fb_batch_create (streams)");
-            }
-        for (auto e : b->ev)
-            if (!e) {
-                batch_free(b);
-                return device_status(SI_ERR_DEVICE, "si_fb_batch_create (events)");
-            }
+        while (b->side.size() < b->classes.size()) {
+            hipStream_t s = nullptr;
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+                return device_status(SI_ERR_DEVICE, "batch side streams");
+            b->side.push_back(s);
+        }
+        while (b->ev.size() < b->classes.size() + 1) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+                return device_status(SI_ERR_DEVICE, "batch side events");
+            b->ev.push_back(e);
+        }
+    }
+    return SI_OK;
+}
+}  // extern "C++"
+
+int si_fb_batch_create(const si_fb_job* jobs, uint32_t n_jobs, si_fb_batch** out) {
+    if (!out || (n_jobs && !jobs)) return fail(SI_ERR_ARG, "si_fb_batch_create: bad arguments");
+    *out = nullptr;
+    int rc = check_batch_jobs(jobs, n_jobs, "si_fb_batch_create");
+    if (rc) return rc;
+    auto* b = new si_fb_batch();
+    rc = batch_plan(b, jobs, n_jobs, "si_fb_batch_create");
+    if (rc) {
+        batch_free(b);
+        return rc;
     }
     *out = b;
     return SI_OK;
+}
+
+int si_fb_batch_reset(si_fb_batch* b, const si_fb_job* jobs, uint32_t n_jobs) {
+    if (!b || (n_jobs && !jobs)) return fail(SI_ERR_ARG, "si_fb_batch_reset: bad arguments");
+    const int rc = check_batch_jobs(jobs, n_jobs, "si_fb_batch_reset");
+    if (rc) return rc;
+    return batch_plan(b, jobs, n_jobs, "si_fb_batch_reset");
 }
 
 size_t si_fb_batch_workspace_bytes(const si_fb_batch* b) { return b ? b->ws_bytes : 0; }
@@ -937,20 +1019,46 @@ int si_fb_batch_tune(si_fb_batch* b, void* workspace, size_t workspace_bytes, ui
     return SI_OK;
 }
 
-// Writes the device job tables with checkpoint pointers into `ws`.
+// Writes the device job tables (checkpoint pointers into `ws`) and item
+// offsets: staged in page-locked memory and copied on `stream`, ahead of the
+// launches that read them; the host waits for nothing but the previous
+// bind's copies out of the same staging.
 static int batch_bind(si_fb_batch* b, void* ws, hipStream_t stream) {
+    size_t need = 0;
+    for (const auto& c : b->classes)
+        need += c.order.size() * sizeof(si::JobView) + c.h_off.size() * sizeof(uint64_t);
+    if (b->staged && hipEventSynchronize(b->staged) != hipSuccess)
+        return device_status(SI_ERR_DEVICE, "si_fb_batch_run (bind)");
+    if (need > b->h_stage_bytes) {
+        if (b->h_stage) (void)hipHostFree(b->h_stage);
+        b->h_stage = nullptr;
+        b->h_stage_bytes = 0;
+        const size_t sz = std::max<size_t>(need + need / 2, 256u << 10);
+        if (hipHostMalloc(reinterpret_cast<void**>(&b->h_stage), sz, 0) != hipSuccess)
+            return device_status(SI_ERR_DEVICE, "si_fb_batch_run (bind staging)");
+        b->h_stage_bytes = sz;
+    }
+    if (!b->staged && hipEventCreateWithFlags(&b->staged, hipEventDisableTiming) != hipSuccess) {
+        b->staged = nullptr;
+        return device_status(SI_ERR_DEVICE, "si_fb_batch_run (bind event)");
+    }
+    char* p = b->h_stage;
     for (auto& c : b->classes) {
-        std::vector<si::JobView> v(c.order.size());
+        auto* v = reinterpret_cast<si::JobView*>(p);
         for (size_t k = 0; k < c.order.size(); ++k) {
             const uint32_t i = c.order[k];
             v[k] = si::fb_job_view(b->jobs[i], c.var.cb);
             si::fb_bind_workspace(v[k], c.Lp, static_cast<char*>(ws) + b->ws_off[i]);
         }
-        if (hipMemcpyAsync(c.d_jobs, v.data(), v.size() * sizeof(si::JobView),
-                           hipMemcpyHostToDevice, stream) != hipSuccess ||
-            hipStreamSynchronize(stream) != hipSuccess)
+        const size_t jb = c.order.size() * sizeof(si::JobView), ob = c.h_off.size() * sizeof(uint64_t);
+        std::memcpy(p + jb, c.h_off.data(), ob);
+        if (hipMemcpyAsync(c.d_jobs, p, jb, hipMemcpyHostToDevice, stream) != hipSuccess ||
+            hipMemcpyAsync(c.d_off, p + jb, ob, hipMemcpyHostToDevice, stream) != hipSuccess)
             return device_status(SI_ERR_DEVICE, "si_fb_batch_run (bind)");
+        p += jb + ob;
     }
+    if (hipEventRecord(b->staged, stream) != hipSuccess)
+        return device_status(SI_ERR_DEVICE, "si_fb_batch_run (bind)");
     b->bound_ws = ws;
     return SI_OK;
 }

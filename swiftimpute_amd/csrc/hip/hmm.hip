@@ -3,7 +3,9 @@
 // views, variants and the plain and batched launch sequences.
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "fb_kernels.hpp"
@@ -223,17 +225,46 @@ static bool fast_class(const BatchClass& c, const FbTables& t, const FbVariant& 
     return (c.flags & kFbFastClass) && t.ucode < t.P && t.ucode < 256u && var.cb == 16u;
 }
 
+// Workgroups of a batched kernel resident per CU, per device; the runtime's
+// occupancy query is kept, since a serving loop plans a batch per launch.
+static int batch_occupancy(int dev, uint32_t Lp, uint32_t lanes, int form, bool log, uint32_t cb, uint32_t P,
+                           bool fast) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, uint32_t, uint32_t, int, bool, uint32_t, uint32_t, bool>, int> memo;
+    const auto key = std::make_tuple(dev, Lp, lanes, form, log, cb, P, fast);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        const auto it = memo.find(key);
+        if (it != memo.end()) return it->second;
+    }
+    const FbImpl* f = fb_impl(Lp, lanes);
+    const int n = f ? f->occupancy(form, log, cb, P, fast) : 1;
+    std::lock_guard<std::mutex> g(mu);
+    memo[key] = n;
+    return n;
+}
+
 void fb_batch_grid(BatchClass& c, const FbVariant& var, uint32_t P) {
-    int dev = 0, cus = 0;
+    int dev = 0;
     (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    static std::mutex cu_mu;
+    static std::map<int, int> cu_count;
+    int cus = 0;
+    {
+        std::lock_guard<std::mutex> g(cu_mu);
+        auto it = cu_count.find(dev);
+        if (it == cu_count.end()) {
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            cu_count[dev] = cus;
+        } else {
+            cus = it->second;
+        }
+    }
     const uint64_t ncu = uint64_t(cus > 0 ? cus : 1);
     const bool log = (c.flags & SI_FB_LOGSPACE) != 0;
     const bool fast = (c.flags & kFbFastClass) && var.cb == 16u;
-    const FbImpl* fc = fb_impl(c.Lp, var.gc);
-    const FbImpl* fbk = fb_impl(c.Lp, var.gb);
-    const uint64_t occ_c = fc ? uint64_t(fc->occupancy(kFbBatchChain, log, var.cb, P, fast)) : 1;
-    const uint64_t occ_b = fbk ? uint64_t(fbk->occupancy(kFbBatchBlock, log, var.cb, P, fast)) : 1;
+    const uint64_t occ_c = uint64_t(batch_occupancy(dev, c.Lp, var.gc, kFbBatchChain, log, var.cb, P, fast));
+    const uint64_t occ_b = uint64_t(batch_occupancy(dev, c.Lp, var.gb, kFbBatchBlock, log, var.cb, P, fast));
     const uint64_t gpc = kFbThreads / var.gc, gpb = kFbThreads / var.gb;  // groups per workgroup
     const uint64_t need_c = (c.chain_items + gpc - 1) / gpc, need_b = (c.block_items + gpb - 1) / gpb;
     c.grid_chain = uint32_t(std::max<uint64_t>(1, std::min(need_c, ncu * occ_c)));

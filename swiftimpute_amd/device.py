@@ -358,6 +358,13 @@ class Batch:
         self.ws_bytes = int(hip().si_fb_batch_workspace_bytes(h))
         self.workspace = None
 
+    def reset(self, cjobs):
+        """Plan the batch again for other jobs, reusing its device tables
+        (si_fb_batch_reset); variants go back to untuned defaults."""
+        self._arr = (FbJob * len(cjobs))(*cjobs)
+        check(hip().si_fb_batch_reset(self.handle, self._arr, len(cjobs)), "si_fb_batch_reset")
+        self.ws_bytes = int(hip().si_fb_batch_workspace_bytes(self.handle))
+
     def run(self, use_graph=False, stream=None, workspace=None):
         if workspace is not None:
             self.workspace = workspace

@@ -20,13 +20,17 @@ class ServerCfg(ctypes.Structure):
                 ("per_marker", ctypes.c_uint32), ("max_streams", ctypes.c_uint32),
                 ("max_batch", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
                 ("device_id", ctypes.c_int), ("max_queue_delay_us", ctypes.c_uint32),
-                ("selection", ctypes.c_uint32), ("region_resweep", ctypes.c_uint32)]
+                ("selection", ctypes.c_uint32), ("region_resweep", ctypes.c_uint32),
+                ("plain", ctypes.c_uint32), ("staging_bytes", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_uint64) for n in ("requests", "batches", "genotypes", "untyped_genotypes",
-                                               "max_batch_requests", "peak_bytes",
-                                               "deferred_admissions", "direct_copies")]
+    _fields_ = ([(n, ctypes.c_uint64) for n in ("requests", "batches", "genotypes", "untyped_genotypes",
+                                                "max_batch_requests", "peak_bytes",
+                                                "deferred_admissions", "direct_copies",
+                                                "h2d_bytes", "d2h_bytes")] +
+                [(n, ctypes.c_double) for n in ("launch_host_s", "h2d_s", "kernel_s", "d2h_s",
+                                                "copy_host_s")])
 
 
 _SYMBOLS = [
@@ -81,7 +85,7 @@ class Server:
                  L: int = 8, ne: int = 10000, per_marker: bool = False, max_streams: int = 4,
                  max_batch: int = 256, hbm_budget_gb: float = 32.0, rho_rate: float = 4.0,
                  max_queue_delay_us: int = 0, gpu_compat_selection: bool = False,
-                 region_resweep: bool = False):
+                 region_resweep: bool = False, plain: bool = False, staging_gb: float = 4.0):
         lib = _lib()
         cfg = ServerCfg()
         lib.si_server_defaults(ctypes.byref(cfg))
@@ -92,6 +96,8 @@ class Server:
         cfg.max_queue_delay_us = max_queue_delay_us
         cfg.selection = int(gpu_compat_selection)
         cfg.region_resweep = int(region_resweep)
+        cfg.plain = int(plain)
+        cfg.staging_bytes = int(staging_gb * (1 << 30))
         cols = np.ascontiguousarray(cols, np.uint64)
         pos = np.ascontiguousarray(pos, np.uint64)
         cmv = None if cM is None else np.ascontiguousarray(cM, np.float64)
@@ -152,7 +158,7 @@ class Server:
     def stats(self) -> dict:
         st = ServerStats()
         _check(_lib().si_server_get_stats(self.h, ctypes.byref(st)), "si_server_get_stats")
-        return {n: int(getattr(st, n)) for n, _ in ServerStats._fields_}
+        return {n: (float if t is ctypes.c_double else int)(getattr(st, n)) for n, t in ServerStats._fields_}
 
     def close(self):
         if self.h:

@@ -103,6 +103,27 @@ def test_batch_equals_plain_eager():
     b.close()
 
 
+def test_batch_reset_equals_plain():
+    # one batch re-planned for other job sets: more jobs and classes (its
+    # tables grow), then fewer (spare tables reused), eager and from a graph
+    sets = [make_jobs(seed=5, n_jobs=6)[1], make_jobs(seed=9, n_jobs=16)[1],
+            make_jobs(seed=13, n_jobs=3)[1]]
+    refs = [plain_results(js) for js in sets]
+    b = device.Batch([rj.cjob() for rj in sets[0]])
+    stream = torch.cuda.Stream()
+    for rep, (js, ref) in enumerate(zip(sets + sets[:1], refs + refs[:1])):
+        if rep:
+            b.reset([rj.cjob() for rj in js])
+        with torch.cuda.stream(stream):
+            b.run(use_graph=bool(rep % 2), stream=stream)
+        stream.synchronize()
+        for rj, r in zip(js, ref):
+            got = rj.result()
+            for k in r:
+                assert np.array_equal(got[k], r[k]), (rep, k, rj.B, rj.M, rj.idx.L)
+    b.close()
+
+
 def test_batch_graph_replays():
     pan, jobs = make_jobs(seed=9, n_jobs=10)
     ref = plain_results(jobs)

@@ -124,6 +124,35 @@ def test_alone_equals_mix_equals_plain(world):
         assert np.array_equal(a.ds, ds), r[:2]
 
 
+def test_plain_mode_equals_batched_and_splits_time(world):
+    # the plain form (one job per launch) on the same requests gives the
+    # batched form's results, and so do servers whose page-locked codes pool
+    # is too small for some requests or absent (codes in pageable memory,
+    # uploaded through slot staging); all report bytes moved and where time
+    # went
+    pan, tg = world
+    reqs = _requests(pan, tg, 10, seed=8)
+    cm = pan.genetic_cM()
+    got = {}
+    for key, plain, staging_gb in (("batched", False, 4.0), ("plain", True, 4.0),
+                                   ("small pool", False, 2.0 ** -16), ("no pool", False, 0.0)):
+        with serving.Server(pan.cols, pan.H, pan.pos, cm, L=L, ne=NE, max_streams=2, max_batch=4,
+                            max_queue_delay_us=100000, plain=plain, staging_gb=staging_gb) as srv:
+            ids = [srv.submit(*r) for r in reqs]
+            got[key] = [srv.wait(i) for i in ids]
+            st = srv.stats()
+        assert st["requests"] == len(reqs)
+        assert st["h2d_bytes"] == sum(r[2].shape[0] * ((r[1] - r[0] + 15) // 16 * 16) for r in reqs)
+        assert st["d2h_bytes"] == sum(r[2].shape[0] * (r[1] - r[0]) * 6 for r in reqs)
+        for k in ("launch_host_s", "h2d_s", "kernel_s", "d2h_s", "copy_host_s"):
+            assert st[k] >= 0.0, (k, st)
+        assert st["kernel_s"] > 0.0 and st["launch_host_s"] > 0.0, st
+    for key in ("plain", "small pool", "no pool"):
+        for r, a, b in zip(reqs, got["batched"], got[key]):
+            assert np.array_equal(a.hap, b.hap), (key, r[:2])
+            assert np.array_equal(a.ds, b.ds), (key, r[:2])
+
+
 def test_admission_under_tight_budget(world):
     pan, tg = world
     reqs = _requests(pan, tg, 10, seed=8)
