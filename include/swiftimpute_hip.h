@@ -179,7 +179,15 @@ enum si_fb_flags {
                                    over runs of such markers in O(1) per
                                    marker instead of O(L).  Same model, FP32
                                    rounding only; ignored when that palette
-                                   entry is not (0, 0, 0)                     */
+                                   entry is not (0, 0, 0)                     */,
+    SI_FB_BANDED = 1u << 4      /* log-space jobs whose switch log-probability
+                                   can fall to the -999 floor (coincident
+                                   markers): every value carried as a residual
+                                   beside a count of -999 floors, so values on
+                                   the floor keep FP32 precision relative to
+                                   it (posteriors within 1e-5 of FP64 where
+                                   the plain log form gives 5e-5).  Ignored
+                                   for linear and windowed jobs              */
 };
 
 typedef struct si_fb_job {

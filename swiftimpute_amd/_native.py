@@ -31,6 +31,7 @@ SI_FB_PER_MARKER = 1
 SI_FB_ARGMAX = 2
 SI_FB_LOGSPACE = 4
 SI_FB_UNTYPED = 8
+SI_FB_BANDED = 16
 SI_FB_PASS_CHAIN = 1
 SI_FB_PASS_BLOCK = 2
 SI_PERM_FIXED = 0x80000000

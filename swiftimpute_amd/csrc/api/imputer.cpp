@@ -322,7 +322,9 @@ std::unique_ptr<ImputationResult> Imputer::impute_with_progress(
     job.n_states = L;
     job.flags = (per_marker ? uint32_t(SI_FB_PER_MARKER) : 0u) |
                 (config_.sampling == SamplingMode::Argmax ? uint32_t(SI_FB_ARGMAX) : 0u) |
-                (logspace ? uint32_t(SI_FB_LOGSPACE) : 0u);
+                (logspace ? uint32_t(SI_FB_LOGSPACE) : 0u) |
+                // switch probabilities at the floor's scale: carry it exactly
+                ((logspace && !(min_switch >= 0x1p-30f)) ? uint32_t(SI_FB_BANDED) : 0u);
     job.code_pitch = pitch;
     job.n_palette = P;
     for (uint32_t p = 0; p < P; ++p)  // untyped markers: likelihoods (0, 0, 0)

@@ -348,8 +348,10 @@ struct si_server {
         j.n_samples = r.B;
         j.n_markers = r.M;
         j.n_states = cfg.n_states;
+        // the log form is chosen for switch probabilities below 1e-30, which
+        // the banded kernels carry exactly against the -999 floor
         j.flags = (cfg.per_marker ? uint32_t(SI_FB_PER_MARKER) : 0u) |
-                  (uses_logspace(r.m0, r.m0 + r.M) ? uint32_t(SI_FB_LOGSPACE) : 0u);
+                  (uses_logspace(r.m0, r.m0 + r.M) ? uint32_t(SI_FB_LOGSPACE | SI_FB_BANDED) : 0u);
         j.code_pitch = r.pitch;
         j.n_palette = 4;
         j.flags |= SI_FB_UNTYPED;  // code 3 of the hard-call palette: (0, 0, 0)

@@ -258,7 +258,7 @@ size_t si_fb_workspace_bytes(const si_fb_job* jobs, uint32_t n_jobs) {
     for (uint32_t i = 0; jobs && i < n_jobs; ++i)
         mx = std::max(mx, si::fb_workspace_bytes_one(jobs[i].n_samples, jobs[i].n_markers,
                                                      si_padded_states(jobs[i].n_states),
-                                                     si::fb_window(jobs[i])));
+                                                     si::fb_window(jobs[i]), si::fb_band_job(jobs[i])));
     return mx;
 }
 
@@ -785,13 +785,14 @@ static int batch_plan(si_fb_batch* b, const si_fb_job* jobs, uint32_t n_jobs, co
         b->ws_off[i] = b->ws_bytes;
         const size_t w = si::fb_workspace_bytes_one(jobs[i].n_samples, jobs[i].n_markers,
                                                     si_padded_states(jobs[i].n_states),
-                                                    si::fb_window(jobs[i]));
+                                                    si::fb_window(jobs[i]), si::fb_band_job(jobs[i]));
         b->ws_bytes += (w + 255) & ~size_t(255);
     }
     for (uint32_t i = 0; i < n_jobs; ++i) {
         if (jobs[i].n_samples == 0 || jobs[i].n_markers == 0) continue;
         const uint32_t Lp = si_padded_states(jobs[i].n_states);
-        const uint32_t fl = (jobs[i].flags & SI_FB_LOGSPACE) | (si::fb_fast_job(jobs[i]) ? si::kFbFastClass : 0u);
+        const uint32_t fl = (jobs[i].flags & SI_FB_LOGSPACE) | (si::fb_band_job(jobs[i]) ? SI_FB_BANDED : 0u) |
+                            (si::fb_fast_job(jobs[i]) ? si::kFbFastClass : 0u);
         auto it = std::find_if(b->classes.begin(), b->classes.end(),
                                [&](const si_fb_batch::Class& c) { return c.Lp == Lp && c.flags == fl; });
         if (it == b->classes.end()) {

@@ -64,7 +64,40 @@ int launch_t(const FbLaunch& L) {
     return hipGetLastError() == hipSuccess ? 0 : SI_ERR_DEVICE;
 }
 
+// Banded log-space kernels (SI_FB_BANDED jobs, 16-marker checkpoint blocks).
+int launch_band(const FbLaunch& L) {
+    const size_t dyn = fb_table_lds_bytes_band(L.tb.P);
+    const dim3 grid(L.grid), block(kFbThreads);
+    switch (L.form) {
+        case kFbPlainChain:
+            hipLaunchKernelGGL((fb_chain_band_kernel<kG, kR>), grid, block, dyn, L.stream, *L.job, L.tb, L.Bpad);
+            break;
+        case kFbBatchChain:
+            hipLaunchKernelGGL((fb_chain_band_batch_kernel<kG, kR>), grid, block, dyn, L.stream, *L.bc, L.tb);
+            break;
+        case kFbPlainBlock:
+            if constexpr (kHasBlock) {
+                hipLaunchKernelGGL((fb_block_band_kernel<kG, kR>), grid, block, dyn, L.stream, *L.job, L.tb);
+                break;
+            } else {
+                return SI_ERR_UNSUPPORTED;
+            }
+        case kFbBatchBlock:
+            if constexpr (kHasBlock) {
+                hipLaunchKernelGGL((fb_block_band_batch_kernel<kG, kR>), grid, block, dyn, L.stream, *L.bc,
+                                   L.tb);
+                break;
+            } else {
+                return SI_ERR_UNSUPPORTED;
+            }
+        default:
+            return SI_ERR_ARG;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : SI_ERR_DEVICE;
+}
+
 int launch(const FbLaunch& L) {
+    if (L.band) return (L.cb == 16 && L.log) ? launch_band(L) : SI_ERR_UNSUPPORTED;
     if (L.fast && (L.form == kFbPlainChain || L.form == kFbBatchChain))
         return L.cb == 16 ? (L.log ? launch_fast_chain<true>(L) : launch_fast_chain<false>(L))
                           : SI_ERR_UNSUPPORTED;
@@ -114,7 +147,35 @@ int occupancy_t(int form, uint32_t P) {
     return (e == hipSuccess && n > 0) ? n : 1;
 }
 
-int occupancy(int form, bool log, uint32_t cb, uint32_t P, bool fast) {
+int occupancy_band(int form, uint32_t P) {
+    int n = 0;
+    const size_t dyn = fb_table_lds_bytes_band(P);
+    hipError_t e = hipErrorInvalidValue;
+    switch (form) {
+        case kFbPlainChain:
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_chain_band_kernel<kG, kR>, kFbThreads, dyn);
+            break;
+        case kFbBatchChain:
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_chain_band_batch_kernel<kG, kR>, kFbThreads,
+                                                             dyn);
+            break;
+        case kFbPlainBlock:
+            if constexpr (kHasBlock)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_block_band_kernel<kG, kR>, kFbThreads, dyn);
+            break;
+        case kFbBatchBlock:
+            if constexpr (kHasBlock)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fb_block_band_batch_kernel<kG, kR>, kFbThreads,
+                                                                 dyn);
+            break;
+        default:
+            break;
+    }
+    return (e == hipSuccess && n > 0) ? n : 1;
+}
+
+int occupancy(int form, bool log, uint32_t cb, uint32_t P, bool fast, bool band) {
+    if (band) return occupancy_band(form, P);
     if (fast && (form == kFbPlainChain || form == kFbBatchChain))
         return log ? occupancy_fast_chain<true>(form, P) : occupancy_fast_chain<false>(form, P);
     return log ? occupancy_t<true, 16>(form, P) : occupancy_t<false, 16>(form, P);

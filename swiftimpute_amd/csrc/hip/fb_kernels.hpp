@@ -1599,4 +1599,583 @@ __global__ __launch_bounds__(kFbThreads) void fb_chain_fast_batch_kernel(BatchCl
     }
 }
 
+// ---------------------------------------------------------------------------
+// Banded log space (jobs with SI_FB_BANDED: log-space steps whose switch
+// log-probability falls to the -999 floor, as at coincident markers, where
+// the posteriors are set by values e^-999 apart).  A value is carried as the
+// pair (r, n) = r - 999 n: an FP32 residual r and a small integer n per state
+// (a byte), so values resting on the floor, or on a multiple of it, keep FP32
+// precision relative to it instead of its 6.1e-5 spacing at |x| ~ 999.  A
+// log-sum-exp takes its terms by value: the largest sets the band of the
+// result and the others enter as e^d, d = (r_t - r_m) - 999 (n_t - n_m),
+// which FP32 gives exactly for terms of one band (the ones that matter near
+// a floor).  Vectors are normalized to a pivot within 0.5 of their largest
+// value; the -999 floor is the term (0, 1).  Log-likelihood increments and
+// the backward scale stay in FP64.  Padding states carry band kPadBand and
+// residual -inf.  Reference semantics: the same quantities as the log form
+// above (forward_backward.cu:41-317).  No windows (SI_FB_BANDED is ignored
+// for windowed jobs), 16-marker checkpoint blocks.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPadBand = 255u;
+constexpr uint32_t kMaxBand = 254u;
+constexpr float kBandStep = 999.0f;
+
+template <class S>
+struct Bands {
+    static constexpr int NW = (S::R + 3) / 4;
+    uint32_t w[NW];
+    __device__ __forceinline__ uint32_t get(int i) const { return (w[i >> 2] >> (8 * (i & 3))) & 0xffu; }
+    __device__ __forceinline__ void set(int i, uint32_t n) {
+        const uint32_t sh = 8u * uint32_t(i & 3);
+        w[i >> 2] = (w[i >> 2] & ~(0xffu << sh)) | (n << sh);
+    }
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = 0u;
+    }
+};
+
+struct BT {
+    float r;
+    uint32_t n;
+};
+// Approximate value, for comparisons only.
+__device__ __forceinline__ float bval(const BT& t) { return t.r - kBandStep * float(t.n); }
+// Band form of a log value given in FP32 (x + 999 n is exact for |x| < 2048).
+__device__ __forceinline__ BT band_f(float x) {
+    const uint32_t n = x <= -499.5f ? (x <= -1498.5f ? 2u : 1u) : 0u;
+    return BT{x + kBandStep * float(n), n};
+}
+// Band form of a log value given in FP64 (one rounding, of the residual).
+__host__ __device__ inline BT band_d(double x) {
+    double q = floor((499.5 - x) / 999.0);
+    q = q < 0.0 ? 0.0 : (q > 254.0 ? 254.0 : q);
+    return BT{float(x + 999.0 * q), uint32_t(q)};
+}
+// LSE of three band terms (absent terms: residual -inf).
+__device__ __forceinline__ BT lse3b(const BT& a, const BT& b, const BT& c) {
+    BT m = a;
+    float vm = bval(a);
+    const float vb = bval(b), vc = bval(c);
+    if (vb > vm) {
+        m = b;
+        vm = vb;
+    }
+    if (vc > vm) m = c;
+    if (!(m.r > -INFINITY)) return BT{-INFINITY, 0u};
+    const float mn = float(m.n);
+    const float s = fast_exp((a.r - m.r) - kBandStep * (float(a.n) - mn)) +
+                    fast_exp((b.r - m.r) - kBandStep * (float(b.n) - mn)) +
+                    fast_exp((c.r - m.r) - kBandStep * (float(c.n) - mn));
+    return BT{m.r + fast_ln(s), m.n};
+}
+
+// Pivot of a band vector over the lane group: among the states within 0.5 of
+// the largest value, the smallest band and, in it, the largest residual.
+template <class S>
+__device__ __forceinline__ BT band_pivot(const float (&x)[S::R], const Bands<S>& xb) {
+    float v[S::R];
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) v[i] = x[i] - kBandStep * float(xb.get(i));
+    const float vm = vmax<S>(v);
+    float c[S::R];
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) c[i] = v[i] >= vm - 0.5f ? -float(xb.get(i)) : -1.0e9f;
+    const float nb = -vmax<S>(c);
+    float rr[S::R];
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) rr[i] = (v[i] >= vm - 0.5f && float(xb.get(i)) == nb) ? x[i] : -INFINITY;
+    return BT{vmax<S>(rr), uint32_t(nb)};
+}
+
+// x relative to the pivot p: band n - p.n; a state below the pivot's band
+// (a residual far under its band's floor) folds into band 0.
+template <class S>
+__device__ __forceinline__ void band_shift(float (&x)[S::R], Bands<S>& xb, const BT& p) {
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t n = xb.get(i);
+        if (n == kPadBand) continue;
+        const float d = x[i] - p.r;
+        if (n >= p.n) {
+            x[i] = d;
+            xb.set(i, n - p.n);
+        } else {
+            x[i] = d + kBandStep * float(p.n - n);
+            xb.set(i, 0u);
+        }
+    }
+}
+
+template <class S>
+__device__ __forceinline__ double band_normalize(float (&x)[S::R], Bands<S>& xb) {
+    const BT p = band_pivot<S>(x, xb);
+    band_shift<S>(x, xb, p);
+    return double(p.r) - 999.0 * double(p.n);
+}
+
+// Emission table for the banded kernels (dynamic LDS): per (code, class
+// mask) row, the band residuals of the four classes (gl_c - off, exact in
+// FP64 before one rounding), the row's offset and the classes' bands in the
+// bytes of a word (class 3, the padding state, and absent classes:
+// kPadBand).
+constexpr uint32_t kTabRowBand = 8;
+__host__ __device__ constexpr size_t fb_table_lds_bytes_band(uint32_t P) {
+    return size_t(P) * 8 * kTabRowBand * sizeof(float);
+}
+__device__ __forceinline__ const float* load_tables_band(const FbTables& tb) {
+    extern __shared__ float4 fb_dyn_lds[];
+    float* f = reinterpret_cast<float*>(fb_dyn_lds);
+    const uint32_t n = tb.P * 8;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        float* row = f + i * kTabRowBand;
+        const float o = tb.emax[i];
+        const float* g = tb.palette + 3 * (i / 8);
+        const uint32_t cm = i % 8;
+        uint32_t bb = kPadBand << 24;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (cm & (1u << c)) {
+                const BT t = band_d(double(g[c]) - double(o));
+                row[c] = t.r;
+                bb |= t.n << (8 * c);
+            } else {
+                row[c] = -INFINITY;
+                bb |= kPadBand << (8 * c);
+            }
+        }
+        row[3] = -INFINITY;
+        row[4] = o;
+        row[5] = __uint_as_float(bb);
+        row[6] = 0.0f;
+        row[7] = 0.0f;
+    }
+    return f;
+}
+
+template <class S>
+__device__ __forceinline__ float lane_emission_band(const float* tab, uint32_t P, uint32_t code,
+                                                    const LaneWords<S>& sw, uint32_t sub, float (&e)[S::R],
+                                                    Bands<S>& eb) {
+    const float* rp = tab + (min(code, P - 1u) * 8u + class_mask(sw.w[0])) * kTabRowBand;
+    const uint32_t bb = __float_as_uint(rp[5]);
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t c = sw.cls(i, sub);
+        e[i] = rp[c];
+        eb.set(i, (bb >> (8u * c)) & 0xffu);
+    }
+    return rp[4];
+}
+
+// Marker 0: a = e, normalized.  Returns the log-likelihood increment.
+template <class S>
+__device__ __forceinline__ double fwd_init_band(float (&a)[S::R], Bands<S>& ab, const float (&e)[S::R],
+                                                const Bands<S>& eb, float off) {
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) a[i] = e[i];
+#pragma unroll
+    for (int k = 0; k < Bands<S>::NW; ++k) ab.w[k] = eb.w[k];
+    return double(off) + band_normalize<S>(a, ab);
+}
+
+// ln sum_i e^{a_i} of a normalized band vector (states below band 0 add
+// e^-999 or less, zero in FP32 as in FP64).
+template <class S>
+__device__ __forceinline__ float log_mass_band(const float (&a)[S::R], const Bands<S>& ab) {
+    float w[S::R];
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) w[i] = fast_exp(a[i] - kBandStep * float(ab.get(i)));
+    return fast_ln(vsum<S>(w));
+}
+
+// Step into a marker: a_j <- LSE(-999, To + LSE(a), U + a_j) + e_j, with
+// k = (t_o, u, To, U).
+template <class S>
+__device__ __forceinline__ double fwd_step_band(float (&a)[S::R], Bands<S>& ab, const float (&e)[S::R],
+                                                const Bands<S>& eb, float off, float4 k) {
+    const float lm = log_mass_band<S>(a, ab);
+    const BT tfl{0.0f, 1u};
+    BT tto = band_f(k.z);
+    tto.r += lm;
+    const BT tu = band_f(k.w);
+    float x[S::R];
+    Bands<S> xb;
+    xb.clear();
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t n = ab.get(i), ne = eb.get(i);
+        const bool pad = n == kPadBand || ne == kPadBand;
+        const BT u{tu.r + a[i], min(tu.n + n, kMaxBand)};
+        const BT t = lse3b(tfl, tto, pad ? BT{-INFINITY, 0u} : u);
+        x[i] = pad ? -INFINITY : t.r + e[i];
+        xb.set(i, pad ? kPadBand : min(t.n + ne, kMaxBand));
+    }
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) a[i] = x[i];
+#pragma unroll
+    for (int w = 0; w < Bands<S>::NW; ++w) ab.w[w] = xb.w[w];
+    return double(off) + band_normalize<S>(a, ab);
+}
+
+// From the vector at marker n (emission e, offset off) to n - 1: the true
+// vector is Bl + (v, vb); b_i <- LSE(-999, To + nat, U + Y_i) with
+// Y_i = Bl + off + e_i + v_i and nat = LSE_i Y_i, and the new scale
+// Bl' = max(nat, -999) (the floor pins it, as in the log form).
+template <class S>
+__device__ __forceinline__ void bwd_step_band(float (&v)[S::R], Bands<S>& vb, double& Bl, const float (&e)[S::R],
+                                              const Bands<S>& eb, float off, float4 k) {
+    float y[S::R];
+    Bands<S> yb;
+    yb.clear();
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t n = vb.get(i), ne = eb.get(i);
+        const bool pad = n == kPadBand || ne == kPadBand;
+        y[i] = pad ? -INFINITY : v[i] + e[i];
+        yb.set(i, pad ? kPadBand : min(n + ne, kMaxBand));
+    }
+    const BT p = band_pivot<S>(y, yb);
+    float w[S::R];
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t n = yb.get(i);
+        w[i] = n == kPadBand ? 0.0f : fast_exp((y[i] - p.r) - kBandStep * (float(n) - float(p.n)));
+    }
+    const float lw = p.r + fast_ln(vsum<S>(w));  // LSE_i y_i = lw - 999 p.n
+    const double nat = Bl + double(off) + double(lw) - 999.0 * double(p.n);
+    const double Bn = nat > -999.0 ? nat : -999.0;
+    const double sh = nat - Bn;  // 0, or below 0 where the floor dominates
+    const BT tfl = band_d(-999.0 - Bn);
+    const BT tto = band_d(double(k.z) + sh);
+    const BT tus = band_d(double(k.w) + sh);
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t n = yb.get(i);
+        const bool pad = n == kPadBand;
+        // Y_i - Bl' = sh + (y_i - LSE y), as a band term
+        float d = y[i] - lw;
+        uint32_t nd = 0u;
+        if (n >= p.n) nd = n - p.n;
+        else d += kBandStep * float(p.n - n);
+        const BT u{tus.r + d, min(tus.n + nd, kMaxBand)};
+        const BT t = lse3b(tfl, tto, pad ? BT{-INFINITY, 0u} : u);
+        v[i] = pad ? -INFINITY : t.r;
+        vb.set(i, pad ? kPadBand : t.n);
+    }
+    Bl = Bn;
+}
+
+template <class S>
+__device__ __forceinline__ void posterior_band(const float (&a)[S::R], const Bands<S>& ab, const float (&v)[S::R],
+                                               const Bands<S>& vb, float (&p)[S::R]) {
+    float q[S::R];
+    Bands<S> qb;
+    qb.clear();
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t na = ab.get(i), nv = vb.get(i);
+        const bool pad = na == kPadBand || nv == kPadBand;
+        q[i] = pad ? -INFINITY : a[i] + v[i];
+        qb.set(i, pad ? kPadBand : min(na + nv, kMaxBand));
+    }
+    const BT pv = band_pivot<S>(q, qb);
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) {
+        const uint32_t n = qb.get(i);
+        p[i] = n == kPadBand ? 0.0f : fast_exp((q[i] - pv.r) - kBandStep * (float(n) - float(pv.n)));
+    }
+    const float rq = __builtin_amdgcn_rcpf(vsum<S>(p));
+#pragma unroll
+    for (int i = 0; i < S::R; ++i) p[i] *= rq;
+}
+
+// A lane's band bytes of a vector in memory (R consecutive bytes).
+template <class S>
+__device__ __forceinline__ void store_bands(uint8_t* p, const Bands<S>& b) {
+    if constexpr (S::R == 2) {
+        *reinterpret_cast<uint16_t*>(p) = uint16_t(b.w[0]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < Bands<S>::NW; ++k) reinterpret_cast<uint32_t*>(p)[k] = b.w[k];
+    }
+}
+template <class S>
+__device__ __forceinline__ void load_bands(const uint8_t* p, Bands<S>& b) {
+    if constexpr (S::R == 2) {
+        b.w[0] = *reinterpret_cast<const uint16_t*>(p);
+    } else {
+#pragma unroll
+        for (int k = 0; k < Bands<S>::NW; ++k) b.w[k] = reinterpret_cast<const uint32_t*>(p)[k];
+    }
+}
+
+// Chain pass of a banded job (no windows; checkpoints at every 16-marker
+// block), one lane group per (sample, direction), as run_chain.
+template <class S, bool PM>
+__device__ __forceinline__ void run_chain_band(const JobView& J, const float* tab, uint32_t P, uint32_t s,
+                                               bool fwd, uint32_t r) {
+    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
+    const uint32_t nlb = (J.M + kSub - 1) / kSub;
+    const uint32_t sub = lane_sub<S>(r);
+    const size_t vo = size_t(r) * S::R;
+    BlockIn<S, true, PM> cur, nx;
+    cur.load(J, codes, r, 0);
+    if (fwd) {
+        float a[S::R];
+        Bands<S> ab;
+        double ll;
+        {
+            float e[S::R];
+            Bands<S> eb;
+            const float off = lane_emission_band<S>(tab, P, cur.code(0), cur.sa[0], sub, e, eb);
+            ll = fwd_init_band<S>(a, ab, e, eb, off);
+        }
+        for (uint32_t c = 0; c < nlb; ++c) {
+            if (c + 1 < nlb) nx.load(J, codes, r, (c + 1) * kSub);
+            const uint32_t m0 = c * kSub;
+#pragma unroll
+            for (uint32_t t = 0; t < kSub; ++t) {
+                const uint32_t m = m0 + t;
+                if (m >= 1 && m < J.M) {
+                    float e[S::R];
+                    Bands<S> eb;
+                    const float off = lane_emission_band<S>(tab, P, cur.code(t), cur.sa[t], sub, e, eb);
+                    ll += fwd_step_band<S>(a, ab, e, eb, off, cur.k[PM ? t : 0]);
+                }
+                if (t == 0) {
+                    const size_t ck = size_t(s) * J.nblk + c;
+                    store_vec<S::R>(J.ck_alpha + ck * S::Lp + vo, a);
+                    store_bands<S>(J.ck_aband + ck * S::Lp + vo, ab);
+                }
+            }
+            cur = nx;
+        }
+        ll += double(log_mass_band<S>(a, ab));
+        if (r == 0 && J.loglik) J.loglik[s] = ll;
+    } else {
+        float v[S::R];
+        Bands<S> vb;
+#pragma unroll
+        for (int i = 0; i < S::R; ++i) v[i] = 0.0f;
+        vb.clear();
+        double Bl = 0.0;
+        const size_t last = size_t(s) * J.nblk + (J.nblk - 1);
+        store_vec<S::R>(J.ck_beta + last * S::Lp + vo, v);
+        store_bands<S>(J.ck_bband + last * S::Lp + vo, vb);
+        if (r == 0) J.ck_bl[last] = Bl;
+        cur.load(J, codes, r, (nlb - 1) * kSub);
+        for (int c = int(nlb) - 1; c >= 0; --c) {
+            if (c > 0) nx.load(J, codes, r, uint32_t(c - 1) * kSub);
+            const uint32_t m0 = uint32_t(c) * kSub;
+#pragma unroll
+            for (int t = int(kSub) - 1; t >= 0; --t) {
+                const uint32_t n = m0 + uint32_t(t);
+                if (n >= 1 && n < J.M) {
+                    float e[S::R];
+                    Bands<S> eb;
+                    const float off = lane_emission_band<S>(tab, P, cur.code(uint32_t(t)), cur.sa[t], sub, e, eb);
+                    bwd_step_band<S>(v, vb, Bl, e, eb, off, cur.k[PM ? t : 0]);
+                }
+                if (t == 0 && c > 0) {  // v at m0 - 1 closes block c - 1
+                    const size_t b = size_t(s) * J.nblk + uint32_t(c) - 1u;
+                    store_vec<S::R>(J.ck_beta + b * S::Lp + vo, v);
+                    store_bands<S>(J.ck_bband + b * S::Lp + vo, vb);
+                    if (r == 0) J.ck_bl[b] = Bl;
+                }
+            }
+            cur = nx;
+        }
+    }
+}
+
+// Block pass of a banded job: one lane group per (sample, 16-marker block),
+// as run_block with one sub-block; the forward rows' bands sit beside them in
+// LDS (bb: 16 rows of Lp bytes per group).
+template <class S, bool PM, bool SB>
+__device__ __forceinline__ void run_block_band(const JobView& J, const float* tab, uint32_t P, bool active,
+                                               uint32_t s, uint32_t c, uint32_t r, float* pb, uint32_t* sb,
+                                               uint8_t* bb, uint4* rng) {
+    using Lds = BlockLds<S>;
+    static_assert(S::R <= 4, "block pass lane shapes");
+    float* const prow = pb + r * S::R;
+    uint8_t* const brow = bb + r * S::R;
+    constexpr uint32_t kPer = (kSub + uint32_t(S::G) - 1) / uint32_t(S::G);
+    const uint32_t m0 = c * kSub;
+    const uint32_t nm = active ? min(kSub, J.M - m0) : 0u;
+    const uint32_t sub = lane_sub<S>(r);
+    const uint8_t* codes = J.codes + size_t(s) * J.pitch;
+    const size_t ck = size_t(s) * J.nblk + c;
+    float ap0r[kPer], ap1r[kPer];
+    if (active) {
+        float a[S::R], v[S::R];
+        Bands<S> ab, vb;
+        load_vec<S::R>(J.ck_alpha + ck * S::Lp + size_t(r) * S::R, a);
+        load_bands<S>(J.ck_aband + ck * S::Lp + size_t(r) * S::R, ab);
+        load_vec<S::R>(J.ck_beta + ck * S::Lp + size_t(r) * S::R, v);
+        load_bands<S>(J.ck_bband + ck * S::Lp + size_t(r) * S::R, vb);
+        double Bl = J.ck_bl[ck];
+        BlockIn<S, true, PM> in;
+        in.load(J, codes, r, m0);
+#pragma unroll
+        for (uint32_t t = 0; t < kSub; ++t) {
+            if constexpr (!SB) {
+            } else if constexpr (S::R == 4) {
+                reinterpret_cast<uint16_t*>(sb)[t * (S::Lp / 4) + r] = uint16_t(nib4(in.sa[t].w[0]));
+            } else {
+                reinterpret_cast<uint8_t*>(sb)[t * (S::Lp / 2) + r] = uint8_t(nib2(in.sa[t].w[0] >> (8 * sub)));
+            }
+            if (t > 0 && t < nm) {
+                float e[S::R];
+                Bands<S> eb;
+                const float off = lane_emission_band<S>(tab, P, in.code(t), in.sa[t], sub, e, eb);
+                fwd_step_band<S>(a, ab, e, eb, off, in.k[PM ? t : 0]);
+            }
+            store_vec<S::R>(prow + t * Lds::T, a);
+            store_bands<S>(brow + t * S::Lp, ab);
+        }
+#pragma unroll
+        for (int t = int(kSub) - 1; t >= 0; --t) {
+            float af[S::R], p[S::R];
+            Bands<S> afb;
+            load_vec<S::R>(prow + t * Lds::T, af);
+            load_bands<S>(brow + t * S::Lp, afb);
+            posterior_band<S>(af, afb, v, vb, p);
+            float x0[S::R], x1[S::R];
+#pragma unroll
+            for (int i = 0; i < S::R; ++i) {
+                const uint32_t pi = __float_as_uint(p[i]);
+                x0[i] = __uint_as_float(pi & in.sa[t].bitmask(i, sub, 0));
+                x1[i] = __uint_as_float(pi & in.sa[t].bitmask(i, sub, 1));
+            }
+            float ap0 = tree_sum<S::R>(x0), ap1 = tree_sum<S::R>(x1);
+            gsum2<S::G>(ap0, ap1);
+            if (uint32_t(t) < nm) store_vec<S::R>(prow + t * Lds::T, p);
+            if (uint32_t(t) % S::G == r) {
+                ap0r[uint32_t(t) / S::G] = ap0;
+                ap1r[uint32_t(t) / S::G] = ap1;
+            }
+            if (t > 0 && uint32_t(t) < nm) {
+                float e[S::R];
+                Bands<S> eb;
+                const float off = lane_emission_band<S>(tab, P, in.code(uint32_t(t)), in.sa[t], sub, e, eb);
+                bwd_step_band<S>(v, vb, Bl, e, eb, off, in.k[PM ? t : 0]);
+            }
+        }
+    }
+    block_sample<S, SB, kPer>(J, active, s, m0, 0u, nm, r, pb, sb, rng, ap0r, ap1r);
+}
+
+template <int G, int R>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_band_kernel(JobView J, FbTables tb, uint32_t Bpad) {
+    using S = Shape<G, R>;
+    const float* tab = load_tables_band(tb);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / G, r = gtid % G;
+    const bool fwd = grp < Bpad;
+    const uint32_t s = fwd ? grp : grp - Bpad;
+    if (s >= J.B) return;  // whole lane groups leave together
+    if (J.flags & SI_FB_PER_MARKER) run_chain_band<S, true>(J, tab, tb.P, s, fwd, r);
+    else run_chain_band<S, false>(J, tab, tb.P, s, fwd, r);
+}
+
+template <int G, int R>
+__global__ __launch_bounds__(kFbThreads) void fb_block_band_kernel(JobView J, FbTables tb) {
+    using S = Shape<G, R>;
+    using Lds = BlockLds<S>;
+    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
+    constexpr bool SB = Lds::kSbPlain;
+    __shared__ uint32_t sbuf[S::GPB * (SB ? Lds::SW : 1)];
+    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
+    __shared__ __align__(16) uint8_t bbuf[S::GPB * kSub * S::Lp];
+    const float* tab = load_tables_band(tb);
+    __syncthreads();
+    const uint32_t gtid = blockIdx.x * kFbThreads + threadIdx.x;
+    const uint32_t grp = gtid / G, r = gtid % G, gl = threadIdx.x / G;
+    const bool active = grp < J.B * J.nblk;
+    const uint32_t s = active ? grp / J.nblk : 0u;
+    const uint32_t c = active ? grp % J.nblk : 0u;
+    float* pb = pbuf + gl * Lds::PS;
+    uint32_t* sb = sbuf + (SB ? gl * Lds::SW : 0u);
+    uint8_t* bb = bbuf + gl * kSub * S::Lp;
+    uint4* rg = rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u);
+    if (J.flags & SI_FB_PER_MARKER) run_block_band<S, true, SB>(J, tab, tb.P, active, s, c, r, pb, sb, bb, rg);
+    else run_block_band<S, false, SB>(J, tab, tb.P, active, s, c, r, pb, sb, bb, rg);
+}
+
+template <int G, int R>
+__global__ __launch_bounds__(kFbThreads) void fb_chain_band_batch_kernel(BatchClass bc, FbTables tb) {
+    using S = Shape<G, R>;
+    const float* tab = load_tables_band(tb);
+    __syncthreads();
+    const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
+    // rounds of GPW samples in one direction, as fb_chain_batch_kernel
+    const uint64_t n_samples = bc.chain_items >> 1;
+    const uint64_t total = 2 * ((n_samples + S::GPW - 1) / S::GPW) * S::GPW;
+    for (;;) {
+        const Chunk ch = wave_take(&bc.counters[0], total, S::GPW, 1);
+        if (ch.base >= total) break;
+        for (uint32_t k = 0; k < ch.n; k += S::GPW) {
+            const uint64_t round = (ch.base + k) / S::GPW;
+            const uint64_t gs = (round >> 1) * S::GPW + gw;
+            const bool fwd = (round & 1) == 0;
+            if (gs < n_samples) {
+                const uint32_t j = find_job(bc.chain_off, bc.n_jobs, 2 * gs);
+                const JobView J = bc.jobs[j];
+                const uint32_t q = uint32_t(gs - (bc.chain_off[j] >> 1));
+                if (q >= J.B) continue;
+                if (J.flags & SI_FB_PER_MARKER) run_chain_band<S, true>(J, tab, tb.P, q, fwd, r);
+                else run_chain_band<S, false>(J, tab, tb.P, q, fwd, r);
+            }
+        }
+    }
+}
+
+template <int G, int R>
+__global__ __launch_bounds__(kFbThreads) void fb_block_band_batch_kernel(BatchClass bc, FbTables tb) {
+    using S = Shape<G, R>;
+    using Lds = BlockLds<S>;
+    __shared__ __align__(16) float pbuf[S::GPB * Lds::PS];
+    __shared__ uint32_t sbuf[S::GPB * Lds::SW];
+    __shared__ uint4 rng[Lds::kRngLds ? S::GPB * (kSub / 4) : 1];
+    __shared__ __align__(16) uint8_t bbuf[S::GPB * kSub * S::Lp];
+    const float* tab = load_tables_band(tb);
+    __syncthreads();
+    const uint32_t lane = __lane_id(), gw = lane / G, r = lane % G;
+    const uint32_t gl = threadIdx.x / G;
+    float* pb = pbuf + gl * Lds::PS;
+    uint32_t* sb = sbuf + gl * Lds::SW;
+    uint8_t* bb = bbuf + gl * kSub * S::Lp;
+    uint4* rg = rng + (Lds::kRngLds ? gl * (kSub / 4) : 0u);
+    uint64_t lo = 0, hi = 0;  // item range of the job in J (wavefront-uniform)
+    JobView J{};
+    for (;;) {
+        const Chunk ch = wave_take(&bc.counters[1], bc.block_items, S::GPW, 32);
+        if (ch.base >= bc.block_items) break;
+        for (uint32_t kk = 0; kk < ch.n; kk += S::GPW) {
+            const uint64_t round = ch.base + kk;
+            if (round >= bc.block_items) break;
+            if (round < lo || round >= hi) {
+                const uint32_t j = find_job_uniform(bc.block_off, bc.n_jobs, round);
+                J = uniform_job(bc.jobs + j);
+                lo = bc.block_off[j];
+                hi = bc.block_off[j + 1];
+                lo = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(lo >> 32))) << 32) |
+                     __builtin_amdgcn_readfirstlane(uint32_t(lo));
+                hi = (uint64_t(__builtin_amdgcn_readfirstlane(uint32_t(hi >> 32))) << 32) |
+                     __builtin_amdgcn_readfirstlane(uint32_t(hi));
+            }
+            const uint64_t q = round - lo + gw;
+            const bool active = q < uint64_t(J.B) * J.nblk;
+            const uint32_t s = active ? uint32_t(q / J.nblk) : 0u;
+            const uint32_t c = active ? uint32_t(q % J.nblk) : 0u;
+            if (J.flags & SI_FB_PER_MARKER)
+                run_block_band<S, true, true>(J, tab, tb.P, active, s, c, r, pb, sb, bb, rg);
+            else
+                run_block_band<S, false, true>(J, tab, tb.P, active, s, c, r, pb, sb, bb, rg);
+        }
+    }
+}
+
 }  // namespace si

@@ -68,21 +68,20 @@ namespace {
 size_t align256(size_t n) { return (n + 255) & ~size_t(255); }
 }  // namespace
 
-// Checkpoint vectors (forward, backward), backward scales and, with windows,
-// one forward vector per window, each region 256-byte aligned.  Sized at the
-// smallest checkpoint block, so one workspace serves every variant.
 // Checkpoint vectors (forward, backward), backward scales, with windows one
-// forward vector per window, and the typed-marker mask, each region 256-byte
-// aligned.  Sized at the smallest checkpoint block, so one workspace serves
-// every variant.
-size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp, uint32_t W) {
+// forward vector per window, the typed-marker mask and list and, for banded
+// jobs, the bands of the checkpoint vectors, each region 256-byte aligned.
+// Sized at the smallest checkpoint block, so one workspace serves every
+// variant.
+size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp, uint32_t W, bool band) {
     const size_t nblk = (size_t(M) + kMinCkptBlock - 1) / kMinCkptBlock;
     const size_t vec = align256(size_t(B) * nblk * Lp * sizeof(float));
     const size_t bl = align256(size_t(B) * nblk * sizeof(double));
     const size_t nwin = W ? (size_t(M) + W - 1) / W : 0;
     const size_t mask = align256((size_t(M) + 15) / 16 * sizeof(uint32_t));
     const size_t list = align256((size_t(M) + 1) * sizeof(uint32_t));
-    return 2 * vec + bl + align256(size_t(B) * nwin * Lp * sizeof(float)) + mask + list;
+    const size_t bands = band ? 2 * align256(size_t(B) * nblk * Lp) : 0;
+    return 2 * vec + bl + align256(size_t(B) * nwin * Lp * sizeof(float)) + mask + list + bands;
 }
 
 void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws) {
@@ -98,6 +97,14 @@ void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws) {
     v.tmask = reinterpret_cast<uint32_t*>(ws + 2 * vec + bl + win);
     v.tcount = reinterpret_cast<uint32_t*>(ws + 2 * vec + bl + win + mask);
     v.tlist = v.tcount + 1;
+    if ((v.flags & SI_FB_BANDED) && (v.flags & SI_FB_LOGSPACE) && v.W == 0u) {
+        const size_t list = align256((size_t(v.M) + 1) * sizeof(uint32_t));
+        const size_t bnd = align256(size_t(v.B) * nblk * Lp);
+        v.ck_aband = reinterpret_cast<uint8_t*>(ws + 2 * vec + bl + win + mask + list);
+        v.ck_bband = v.ck_aband + bnd;
+    } else {
+        v.ck_aband = v.ck_bband = nullptr;
+    }
 }
 
 // ------------------------------------------------------ typed-marker mask --
@@ -175,8 +182,11 @@ JobView fb_job_view(const si_fb_job& j, uint32_t cb) {
 // Whether a launch runs the untyped-step kernels: a fast job (untyped code
 // in the tables, constant transitions, no windows) at 16-marker checkpoint
 // blocks; otherwise the exact kernels.
+static bool band_launch(const JobView& v) {
+    return (v.flags & SI_FB_BANDED) && (v.flags & SI_FB_LOGSPACE) && v.W == 0u;
+}
 static bool fast_launch(const JobView& v, const FbTables& t, const FbVariant& var) {
-    return v.ur.ok != 0u && v.W == 0u && t.ucode < t.P && t.ucode < 256u && var.cb == 16u;
+    return v.ur.ok != 0u && v.W == 0u && t.ucode < t.P && t.ucode < 256u && var.cb == 16u && !band_launch(v);
 }
 
 // ----------------------------------------------------------- plain form --
@@ -186,6 +196,7 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
     if (!fb_variant_valid(Lp, var)) return SI_ERR_UNSUPPORTED;
     const bool log = (v.flags & SI_FB_LOGSPACE) != 0;
     const bool fast = fast_launch(v, t, var);
+    const bool band = band_launch(v);
     if ((passes & SI_FB_PASS_CHAIN) && fast) {
         hipLaunchKernelGGL(fb_tmask_kernel, dim3(uint32_t(fb_tmask_groups(v.M))), dim3(kMaskThreads), 0,
                            stream, v, t.ucode);
@@ -199,7 +210,7 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
         const size_t threads = size_t(Bpad + v.B) * var.gc;
         const size_t grid = (threads + kFbThreads - 1) / kFbThreads;
         if (grid > 0x7fffffffu) return SI_ERR_UNSUPPORTED;
-        FbLaunch L{kFbPlainChain, log, var.cb, fast, &v, Bpad, nullptr, t, uint32_t(grid), stream};
+        FbLaunch L{kFbPlainChain, log, var.cb, fast, band, &v, Bpad, nullptr, t, uint32_t(grid), stream};
         const int rc = f->launch(L);
         if (rc) return rc;
     }
@@ -208,7 +219,7 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
         const size_t threads = size_t(v.B) * v.nblk * var.gb;
         const size_t grid = (threads + kFbThreads - 1) / kFbThreads;
         if (grid > 0x7fffffffu) return SI_ERR_UNSUPPORTED;
-        FbLaunch L{kFbPlainBlock, log, var.cb, fast, &v, 0, nullptr, t, uint32_t(grid), stream};
+        FbLaunch L{kFbPlainBlock, log, var.cb, fast, band, &v, 0, nullptr, t, uint32_t(grid), stream};
         const int rc = f->launch(L);
         if (rc) return rc;
     }
@@ -222,23 +233,24 @@ __global__ void zero_words_kernel(uint32_t* p, uint32_t n) {
 }
 
 static bool fast_class(const BatchClass& c, const FbTables& t, const FbVariant& var) {
-    return (c.flags & kFbFastClass) && t.ucode < t.P && t.ucode < 256u && var.cb == 16u;
+    return (c.flags & kFbFastClass) && !(c.flags & SI_FB_BANDED) && t.ucode < t.P && t.ucode < 256u &&
+           var.cb == 16u;
 }
 
 // Workgroups of a batched kernel resident per CU, per device; the runtime's
 // occupancy query is kept, since a serving loop plans a batch per launch.
 static int batch_occupancy(int dev, uint32_t Lp, uint32_t lanes, int form, bool log, uint32_t cb, uint32_t P,
-                           bool fast) {
+                           bool fast, bool band) {
     static std::mutex mu;
-    static std::map<std::tuple<int, uint32_t, uint32_t, int, bool, uint32_t, uint32_t, bool>, int> memo;
-    const auto key = std::make_tuple(dev, Lp, lanes, form, log, cb, P, fast);
+    static std::map<std::tuple<int, uint32_t, uint32_t, int, bool, uint32_t, uint32_t, bool, bool>, int> memo;
+    const auto key = std::make_tuple(dev, Lp, lanes, form, log, cb, P, fast, band);
     {
         std::lock_guard<std::mutex> g(mu);
         const auto it = memo.find(key);
         if (it != memo.end()) return it->second;
     }
     const FbImpl* f = fb_impl(Lp, lanes);
-    const int n = f ? f->occupancy(form, log, cb, P, fast) : 1;
+    const int n = f ? f->occupancy(form, log, cb, P, fast, band) : 1;
     std::lock_guard<std::mutex> g(mu);
     memo[key] = n;
     return n;
@@ -262,9 +274,10 @@ void fb_batch_grid(BatchClass& c, const FbVariant& var, uint32_t P) {
     }
     const uint64_t ncu = uint64_t(cus > 0 ? cus : 1);
     const bool log = (c.flags & SI_FB_LOGSPACE) != 0;
-    const bool fast = (c.flags & kFbFastClass) && var.cb == 16u;
-    const uint64_t occ_c = uint64_t(batch_occupancy(dev, c.Lp, var.gc, kFbBatchChain, log, var.cb, P, fast));
-    const uint64_t occ_b = uint64_t(batch_occupancy(dev, c.Lp, var.gb, kFbBatchBlock, log, var.cb, P, fast));
+    const bool band = log && (c.flags & SI_FB_BANDED);
+    const bool fast = (c.flags & kFbFastClass) && var.cb == 16u && !band;
+    const uint64_t occ_c = uint64_t(batch_occupancy(dev, c.Lp, var.gc, kFbBatchChain, log, var.cb, P, fast, band));
+    const uint64_t occ_b = uint64_t(batch_occupancy(dev, c.Lp, var.gb, kFbBatchBlock, log, var.cb, P, fast, band));
     const uint64_t gpc = kFbThreads / var.gc, gpb = kFbThreads / var.gb;  // groups per workgroup
     const uint64_t need_c = (c.chain_items + gpc - 1) / gpc, need_b = (c.block_items + gpb - 1) / gpb;
     c.grid_chain = uint32_t(std::max<uint64_t>(1, std::min(need_c, ncu * occ_c)));
@@ -276,6 +289,7 @@ int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, con
     if (!fb_variant_valid(c.Lp, var)) return SI_ERR_UNSUPPORTED;
     const bool log = (c.flags & SI_FB_LOGSPACE) != 0;
     const bool fast = fast_class(c, t, var);
+    const bool band = log && (c.flags & SI_FB_BANDED);
     hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, stream,
                        reinterpret_cast<uint32_t*>(c.counters), 4u);
     if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
@@ -286,12 +300,12 @@ int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, con
         if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     }
     if ((passes & SI_FB_PASS_CHAIN) && c.chain_items) {
-        FbLaunch L{kFbBatchChain, log, var.cb, fast, nullptr, 0, &c, t, c.grid_chain, stream};
+        FbLaunch L{kFbBatchChain, log, var.cb, fast, band, nullptr, 0, &c, t, c.grid_chain, stream};
         const int rc = fb_impl(c.Lp, var.gc)->launch(L);
         if (rc) return rc;
     }
     if ((passes & SI_FB_PASS_BLOCK) && c.block_items) {
-        FbLaunch L{kFbBatchBlock, log, var.cb, fast, nullptr, 0, &c, t, c.grid_block, stream};
+        FbLaunch L{kFbBatchBlock, log, var.cb, fast, band, nullptr, 0, &c, t, c.grid_block, stream};
         const int rc = fb_impl(c.Lp, var.gb)->launch(L);
         if (rc) return rc;
     }

@@ -46,6 +46,8 @@ struct JobView {
     float* ck_beta;   // [B][nblk][Lp]  backward vector at each block's last marker
     double* ck_bl;    // [B][nblk]      its log scale
     float* ck_win;    // [B][nwin][Lp]  forward vector at each window's last marker
+    uint8_t* ck_aband;  // [B][nblk][Lp] bands of ck_alpha (banded jobs)
+    uint8_t* ck_bband;  // [B][nblk][Lp] bands of ck_beta (banded jobs)
     uint32_t* tmask;  // [ceil(M/16)]   typed markers of each load block (fast jobs)
     uint32_t* tlist;  // [M]            the typed markers 1..M-1 in order (fast jobs)
     uint32_t* tcount; // [1]            how many
@@ -84,7 +86,12 @@ inline uint32_t fb_window(const si_fb_job& j) {
     return (j.window > 0 && j.n_markers > j.window) ? j.window : 0u;
 }
 constexpr uint32_t kMinWindow = 32;
-size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp, uint32_t W);
+// Banded jobs (SI_FB_BANDED on a log-space job without windows) run the
+// banded kernels and keep the bands of their checkpoints in the workspace.
+inline bool fb_band_job(const si_fb_job& j) {
+    return (j.flags & SI_FB_BANDED) && (j.flags & SI_FB_LOGSPACE) && fb_window(j) == 0u;
+}
+size_t fb_workspace_bytes_one(uint32_t B, uint32_t M, uint32_t Lp, uint32_t W, bool band);
 void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws);
 JobView fb_job_view(const si_fb_job& j, uint32_t cb);
 
@@ -104,7 +111,7 @@ constexpr uint32_t kFbFastClass = 1u << 31;
 // untyped code in the palette (the kernels still check its likelihoods).
 URun fb_urun(const si_fb_job& j);
 inline bool fb_fast_job(const si_fb_job& j) {
-    return fb_untyped_code(j) < 256u && fb_window(j) == 0u && fb_urun(j).ok != 0u;
+    return fb_untyped_code(j) < 256u && fb_window(j) == 0u && fb_urun(j).ok != 0u && !fb_band_job(j);
 }
 
 // Batched form: samples of a job padded to a multiple of 64 in both queues, so
@@ -123,7 +130,7 @@ __host__ __device__ inline uint64_t fb_batch_padded_samples(uint32_t B) {
 struct BatchClass {
     uint32_t Lp;
     uint32_t n_jobs;
-    uint32_t flags;  // SI_FB_LOGSPACE, kFbFastClass
+    uint32_t flags;  // SI_FB_LOGSPACE, SI_FB_BANDED, kFbFastClass
     const JobView* jobs;
     const uint64_t* chain_off;
     const uint64_t* block_off;
@@ -145,6 +152,7 @@ struct FbLaunch {
     bool log;
     uint32_t cb;
     bool fast;             // untyped-step kernels (16-marker checkpoint blocks)
+    bool band;             // banded log-space kernels
     const JobView* job;    // plain forms
     uint32_t Bpad;         // plain chain: forward groups padded to whole wavefronts
     const BatchClass* bc;  // batched forms
@@ -156,7 +164,7 @@ struct FbImpl {
     uint32_t Lp, G;
     bool has_block;
     int (*launch)(const FbLaunch&);
-    int (*occupancy)(int form, bool log, uint32_t cb, uint32_t P, bool fast);  // workgroups per CU
+    int (*occupancy)(int form, bool log, uint32_t cb, uint32_t P, bool fast, bool band);  // workgroups per CU
 };
 void fb_register(const FbImpl& impl);
 const FbImpl* fb_impl(uint32_t Lp, uint32_t G);

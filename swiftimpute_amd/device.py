@@ -129,6 +129,12 @@ class Transitions:
         to = self.To if self.per_marker else self.To[:1]
         return float(np.exp(to.astype(np.float64)).min())
 
+    def floor_scale(self) -> bool:
+        """Whether a switch probability the kernel uses is at the -999 floor's
+        scale (below 2^-30, coincident markers): a log-space job then carries
+        the floor exactly (SI_FB_BANDED)."""
+        return self.min_switch() < 2.0 ** -30
+
 
 # --------------------------------------------------------------------- index
 def pbwt_sweep(cols: np.ndarray, H: int, chunk: int | None = None, with_div: bool = True,
@@ -234,6 +240,7 @@ class Job:
     logspace: bool = False
     window: int = 0              # the reference's windowed model when M > window > 0
     untyped: bool = True         # untyped markers take the composed step (SI_FB_UNTYPED)
+    banded: bool | None = None   # SI_FB_BANDED (None: for log-space jobs at the floor's scale)
     out: JobOutputs = field(default_factory=JobOutputs)
     _keep: list = field(default_factory=list)
 
@@ -267,9 +274,11 @@ class Job:
         j.n_samples = self.B
         j.n_markers = self.index.M
         j.n_states = self.index.L
+        banded = self.trans.floor_scale() if self.banded is None else self.banded
         j.flags = ((_native.SI_FB_PER_MARKER if self.trans.per_marker else 0)
                    | (_native.SI_FB_ARGMAX if self.argmax else 0)
-                   | (_native.SI_FB_LOGSPACE if self.logspace else 0))
+                   | (_native.SI_FB_LOGSPACE if self.logspace else 0)
+                   | (_native.SI_FB_BANDED if self.logspace and banded else 0))
         j.code_pitch = self.codes.shape[1]
         j.n_palette = self.palette.shape[0]
         j.rng_stream = self.rng_stream
@@ -465,7 +474,8 @@ class RegionJob:
         j = FbJob()
         j.n_samples, j.n_markers, j.n_states = self.B, self.M, ix.L
         j.flags = ((_native.SI_FB_PER_MARKER if self.trans.per_marker else 0)
-                   | (_native.SI_FB_LOGSPACE if self.logspace else 0))
+                   | (_native.SI_FB_LOGSPACE if self.logspace else 0)
+                   | (_native.SI_FB_BANDED if self.logspace and self.trans.floor_scale() else 0))
         j.code_pitch = self.codes.shape[1]
         j.rng_stream, j.sample_base, j.seed = self.rng_stream, self.sample_base, self.seed
         s = self.trans.step

@@ -292,16 +292,66 @@ def test_fb_coincident_markers_logspace():
             p2[1] = p2[0]          # the first interval drives every step
         tr = device.Transitions.build(p2, None, 10000, per_marker=per_marker)
         assert (tr.To == -999.0).any()
+        assert tr.floor_scale()  # the job runs the banded kernels
         res = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=True)
         # Reference-compatible mode with a zero-length first interval has
         # To = -999 at every step: no switching at all, and the posteriors are
-        # set by terms e^-999 apart.  FP32 resolves |x| ~ 999 to 6.1e-5, which
-        # bounds the posterior error by about 2 ulp * p(1 - p) < 5e-5 (an FP32
-        # emulation of the kernel's arithmetic gives 1.29e-5 on this input).
-        check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, per_marker, fold=True,
-                 tol=DS_TOL if per_marker else 5e-5)
+        # set by terms e^-999 apart, which FP32 resolves to 6.1e-5 at |x| ~ 999
+        # unless the floor is carried apart from the residual (SI_FB_BANDED).
+        check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, res, per_marker, fold=True, tol=DS_TOL)
     with pytest.raises(Exception):
         run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=False)
+
+
+@pytest.mark.parametrize("L", [3, 8, 16, 33])
+def test_fb_banded_variants_and_batch_agree(L):
+    # Banded jobs: every lane shape gives the same bits (the same trees over
+    # the states), and the batched form equals the plain one.
+    rng = np.random.default_rng(L + 70)
+    pan = synth.make_panel(160, 150, seed=L + 3)
+    pos = pan.pos.copy()
+    pos[30:70] = pos[30]
+    pos[1] = pos[0]
+    idx = device.DeviceIndex(pan.cols, pan.H, L)
+    codes = hardcall_codes(rng, 11, pan.M, typed_every=3)
+    out = {}
+    for per_marker in (False, True):
+        tr = device.Transitions.build(pos, None, 10000, per_marker=per_marker)
+        assert tr.floor_scale()
+        ref = run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=True)
+        check_fb(idx, codes, synth.HARDCALL_PALETTE, tr, ref, per_marker, fold=True, tol=DS_TOL)
+        out[per_marker] = ref
+        B, M = codes.shape[0], idx.M
+        pitch = (M + 15) // 16 * 16
+        c = np.full((B, pitch), 3, np.uint8)
+        c[:, :M] = codes
+        keys = ("ds", "ap", "hap", "prob", "loglik")
+        for v in device.variants(idx.Lp):
+            job = device.Job(idx, torch.from_numpy(c).cuda(), synth.HARDCALL_PALETTE, tr, seed=12345,
+                             logspace=True)
+            job.alloc_outputs(ds=True, ap=True, hap=True, prob=True, loglik=True)
+            device.run_jobs([job.c_job()], variant=v)
+            torch.cuda.synchronize()
+            for k in keys:
+                assert np.array_equal(getattr(job.out, k).cpu().numpy(), ref[k]), (v, k)
+        # two cohorts of one region in one batch (a shared emission table)
+        tables = device.SharedTables(synth.HARDCALL_PALETTE)
+        rjs = [device.RegionJob(idx, 0, M, torch.from_numpy(c[lo:hi]).cuda(), tr, seed=5, tables=tables,
+                                logspace=True, outputs=keys) for lo, hi in ((0, 4), (4, B))]
+        plain = []
+        for rj in rjs:
+            device.run_jobs([rj.c_job()])
+            torch.cuda.synchronize()
+            plain.append({k: rj.out[k].cpu().numpy() for k in keys})
+            rj.out["ds"].zero_()
+            rj.out["hap"].zero_()
+        b = device.Batch([rj.c_job() for rj in rjs])
+        b.run(use_graph=False)
+        torch.cuda.synchronize()
+        for rj, pr in zip(rjs, plain):
+            for k in keys:
+                assert np.array_equal(rj.out[k].cpu().numpy(), pr[k]), ("batch", k)
+        b.close()
 
 
 @pytest.mark.parametrize("L", [4, 8, 16])

@@ -247,10 +247,14 @@ def emission_matrix(a0, a1, gl_s):
     return E
 
 
-def brute_posterior(a0, a1, gl_s, Td, To, per_marker):
-    """Path enumeration: forward = sum over all paths ending in state k;
-    backward = sum over partial paths from k, the ones stopping before the
-    last marker weighted by the -999 floor (e^-999), the full ones by 1."""
+def brute_posterior(a0, a1, gl_s, Td, To, per_marker, forward_floor=True):
+    """Path enumeration.  Forward: every path ending in state k, plus the
+    paths the -999 floor starts at each marker r >= 1 (the reference adds the
+    floor to the forward vector rescaled to maximum 0, so in unscaled terms a
+    path starting at r carries -999 + max_i A_{r-1}[i]); backward: the partial
+    paths from k, the ones stopping before the last marker weighted by the
+    -999 floor (e^-999), the full ones by 1.  forward_floor=False drops the
+    floor-started forward paths (a planted mistake for the tests)."""
     M, L = a0.shape
     E = emission_matrix(a0, a1, gl_s)
 
@@ -268,6 +272,13 @@ def brute_posterior(a0, a1, gl_s, Td, To, per_marker):
                 s = list(path) + [k]
                 w = E[0, s[0]] + sum(T(r, s[r - 1], s[r]) + E[r, s[r]] for r in range(1, m + 1))
                 terms.append(w)
+            for r0 in range(1, m + 1) if forward_floor else ():
+                start = -999.0 + max(fwd_all[r0 - 1])
+                for path in itertools.product(range(L), repeat=m - r0):
+                    s = list(path) + [k]  # states at markers r0..m
+                    w = start + E[r0, s[0]] + sum(T(q, s[q - r0 - 1], s[q - r0]) + E[q, s[q - r0]]
+                                                  for q in range(r0 + 1, m + 1))
+                    terms.append(w)
             lf.append(lse(terms))
         lb = []
         for k in range(L):
@@ -301,6 +312,59 @@ def random_instance(rng, M, L, with_floor=True):
     if rng.random() < 0.3:
         Td = To.copy()
     return a0, a1, gl, Td, To
+
+
+def coincident_instance(rng, M, L):
+    """Hard calls with missing markers over coincident intervals (rho = 0:
+    stay 0, switch -999, the oracle's transitions of equal positions), some
+    intervals ordinary: no switching across the coincident ones, so the
+    -999 floor carries the forward and backward vectors there."""
+    a0 = rng.integers(0, 2, (M, L)).astype(np.uint8)
+    a1 = rng.integers(0, 2, (M, L)).astype(np.uint8)
+    pal = np.array([[0.0, -999.0, -999.0], [-999.0, 0.0, -999.0], [-999.0, -999.0, 0.0],
+                    [0.0, 0.0, 0.0]], np.float32)
+    gl = pal[rng.integers(0, 4, M)][None]
+    step = rng.integers(0, 3, M) * 1000  # repeated positions, at least one pair
+    step[1 + int(rng.integers(0, M - 1))] = 0
+    pos = np.cumsum(step).astype(np.uint64)
+    Td, To = oracle.transitions(oracle.genetic_map(pos), 10000)
+    return a0, a1, gl, Td, To
+
+
+@pytest.mark.parametrize("seed", range(10))
+@pytest.mark.parametrize("per_marker", [False, True])
+def test_fb_f64_matches_brute_force_at_coincident_markers(seed, per_marker):
+    rng = np.random.default_rng(300 + seed)
+    M, L = int(rng.integers(2, 6)), int(rng.integers(2, 4))
+    a0, a1, gl, Td, To = coincident_instance(rng, M, L)
+    if not per_marker:
+        Td[0], To[0] = 0.0, -999.0  # the first interval drives every step
+    assert (To == -999.0).any()
+    ref, ll = brute_posterior(a0, a1, gl[0], Td, To, per_marker)
+    out = oracle.fb_f64(a0, a1, gl, Td, To, per_marker=per_marker)
+    assert np.abs(out["prob"][0] - ref).max() < 1e-10
+    assert abs(out["loglik"][0] - ll) < 1e-9 * max(1.0, abs(ll))
+    fast = oracle.fb_f64_fast(a0, a1, gl, Td, To, per_marker=per_marker)
+    assert np.abs(fast["prob"][0] - ref).max() < 1e-10
+    assert abs(fast["loglik"][0] - ll) < 1e-9 * max(1.0, abs(ll))
+
+
+def test_forward_floor_changes_the_answer_at_coincident_markers():
+    # The forward floor is what carries a coincident run whose states all
+    # mismatch: enumerating without it (a planted mistake) must move the
+    # posteriors, so the brute force above pins the oracle's forward floor.
+    hit = 0
+    for seed in range(40):
+        rng = np.random.default_rng(500 + seed)
+        a0, a1, gl, Td, To = coincident_instance(rng, 5, 3)
+        Td[0], To[0] = 0.0, -999.0
+        good, _ = brute_posterior(a0, a1, gl[0], Td, To, False)
+        bad, _ = brute_posterior(a0, a1, gl[0], Td, To, False, forward_floor=False)
+        out = oracle.fb_f64(a0, a1, gl, Td, To, per_marker=False)
+        assert np.abs(out["prob"][0] - good).max() < 1e-10
+        if np.isfinite(bad).all() and np.abs(bad - good).max() > 1e-3:
+            hit += 1
+    assert hit >= 3, hit
 
 
 @pytest.mark.parametrize("seed", range(8))
