@@ -54,6 +54,9 @@ struct PbwtArgs {
     uint32_t* nseg;          // [nchunk]   number of those groups
     uint32_t SL;             // chunks per composition segment
     uint32_t rows_div;       // phase C also writes the divergence rows of its chunk
+    uint16_t* ord0;          // [nchunk][H] each chunk's order sorted by its key (radix form)
+    uint16_t* ord1;          // [nchunk][H] the scan's second order buffer
+    uint32_t* gcnt;          // [2][nchunk] group counts of the scan's two levels in flight
 };
 
 // Divergence row m (every entry m), streaming stores by the whole workgroup.
@@ -500,9 +503,22 @@ struct LdsGeom {
     static constexpr uint32_t NW32 = NPOS / 32u;
 };
 
+// Phase C with the order in LDS (H <= 49152).  Positions past H hold padding
+// ids (the position itself) whose allele bits read as 1, so every round of
+// every wave is whole: the padding sorts after every haplotype and keeps the
+// positions past H.  A step keeps each round's zero ballot in scalar
+// registers and each lane's haplotype and allele in vector registers, so the
+// scatter needs only the zero count before the lane (mbcnt) and a select
+// between the zero and one destinations of its round.
+__device__ __forceinline__ void store_col_padded(uint32_t* col, uint32_t i, uint32_t w, uint32_t nw,
+                                                 uint32_t padm) {
+    col[i] = i + 1u == nw ? (w | padm) : w;
+}
+
 template <int K>
 __global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
     using G = LdsGeom<K>;
+    constexpr int NZ = (K + 31) / 32;  // words of per-round zero bits
     __shared__ uint16_t ord[G::NPOS];
     __shared__ uint32_t col[2][G::NW32];
     __shared__ uint32_t wz[kPW];
@@ -511,12 +527,18 @@ __global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
     const bool divs = a.rows_div && a.div;
     if (divs) div_row(a, m_end - 1u);  // every divergence row of the chunk, its last one too
     if (m_end - m_first < 2) return;
-    const WaveCtx q(a);
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t base = w * uint32_t(64 * K);
+    const uint32_t nw = (a.H + 31u) / 32u;
+    const uint32_t padm = (a.H & 31u) ? ~((1u << (a.H & 31u)) - 1u) : 0u;
 
     const uint32_t* start = (c == 0) ? nullptr : a.prefix + size_t(m_first - 1) * a.H;
-    for (uint32_t i = q.t; i < G::NPOS; i += kPT)
-        ord[i] = uint16_t(i < a.H ? (start ? start[i] : i) : 0u);
-    load_column(a, m_first, col[0]);
+    for (uint32_t i = t; i < G::NPOS; i += kPT) ord[i] = uint16_t(i < a.H ? (start ? start[i] : i) : i);
+    for (uint32_t i = nw + t; i < G::NW32; i += kPT) {
+        col[0][i] = ~0u;
+        col[1][i] = ~0u;
+    }
+    for (uint32_t i = t; i < nw; i += kPT) store_col_padded(col[0], i, a.cols32[size_t(m_first) * a.W2 + i], nw, padm);
     __syncthreads();
     for (uint32_t m = m_first; m + 1 < m_end; ++m) {
         const uint32_t* cb = col[(m - m_first) & 1u];
@@ -524,37 +546,44 @@ __global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
         ColPrefetch nx;
         if (pre) nx.fetch(a, m + 1);
 
-        uint32_t pk[K];
+        uint32_t hv[K];
 #pragma unroll
-        for (int j = 0; j < K; ++j) pk[j] = ord[q.base + uint32_t(j) * 64u + q.lane];
-        uint32_t xb[K];
+        for (int j = 0; j < K; ++j) hv[j] = ord[base + uint32_t(j) * 64u + lane];
+        uint64_t Zs[K];
+        uint32_t zbits[NZ], zsum = 0;
 #pragma unroll
-        for (int j = 0; j < K; ++j) xb[j] = allele_bit(cb, pk[j]);
-        uint32_t zsum = 0;
+        for (int k = 0; k < NZ; ++k) zbits[k] = 0u;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const uint32_t x = xb[j] | uint32_t(!q.valid(uint32_t(j)));
-            const uint64_t Z = __ballot(x == 0u);
-            pk[j] |= ((zsum + mbcnt64(Z)) << 16) | (x << 31);
-            zsum += uint32_t(__popcll(Z));
+            const uint32_t x = (cb[hv[j] >> 5] >> (hv[j] & 31u)) & 1u;
+            Zs[j] = __ballot(x == 0u);
+            zbits[j >> 5] |= (x ^ 1u) << (j & 31);
+            zsum += uint32_t(__popcll(Zs[j]));
         }
-        if (q.lane == 0) wz[q.w] = zsum;
+        if (lane == 0) wz[w] = zsum;
         __syncthreads();
         uint32_t zbefore, ztot;
-        zero_offsets(wz, q.w, zbefore, ztot);
-        uint32_t* row = a.prefix + size_t(m) * a.H;
+        zero_offsets(wz, w, zbefore, ztot);
+        zbefore = __builtin_amdgcn_readfirstlane(zbefore);
+        ztot = __builtin_amdgcn_readfirstlane(ztot);
+        char* row = reinterpret_cast<char*>(a.prefix + size_t(m) * a.H);
+        uint32_t zrun = 0;
 #pragma unroll
         for (int j = 0; j < K; ++j) {
-            const uint32_t e = uint32_t(j) * 64u + q.lane;
-            const uint32_t v = pk[j], zb = (v >> 16) & 0x1fffu, h = v & 0xffffu;
-            const uint32_t dst = (v >> 31) ? ztot + (q.base - zbefore) + (e - zb) : zbefore + zb;
-            if (e < q.nvalid) {
-                ord[dst] = uint16_t(h);
-                row[dst] = h;
-            }
+            const uint32_t mb = mbcnt64(Zs[j]);                            // zeros before the lane
+            const uint32_t z0 = zbefore + zrun;                           // the round's first zero
+            const uint32_t o0 = ztot + (base - zbefore) + uint32_t(j) * 64u - zrun;  // and one
+            const uint32_t dst = ((zbits[j >> 5] >> (j & 31)) & 1u) ? z0 + mb : o0 + (lane - mb);
+            ord[dst] = uint16_t(hv[j]);
+            if (dst < a.H) *reinterpret_cast<uint32_t*>(row + dst * 4u) = hv[j];
+            zrun += uint32_t(__popcll(Zs[j]));
         }
         if (divs) div_row(a, m);
-        if (pre) nx.store(a, col[((m - m_first) & 1u) ^ 1u]);
+        if (pre) {
+            const uint32_t i0 = t, i1 = t + kPT;
+            if (i0 < nw) store_col_padded(col[((m - m_first) & 1u) ^ 1u], i0, nx.w0, nw, padm);
+            if (i1 < nw) store_col_padded(col[((m - m_first) & 1u) ^ 1u], i1, nx.w1, nw, padm);
+        }
         __syncthreads();
     }
 }
@@ -888,6 +917,10 @@ __global__ __launch_bounds__(kPT) void pbwt_chunk_sort_kernel(PbwtArgs a) {
         uint32_t* lastrow = a.prefix + size_t(m_first + len - 1u) * H;
         for (uint32_t i = t; i < H; i += kPT) lastrow[i] = X[i];
     }
+    if (a.ord0) {  // the chunk's sorted order, for the composition scan
+        uint16_t* o = a.ord0 + size_t(c) * H;
+        for (uint32_t i = t; i < H; i += kPT) o[i] = X[i];
+    }
     // group starts: a key differs from its predecessor's (exact, all bytes)
     const uint8_t* thc = a.th + size_t(c) * H * a.TS;
     uint32_t flags = 0, cntf = 0;
@@ -1092,6 +1125,88 @@ __global__ __launch_bounds__(kPT) void pbwt_seg_fill_kernel(PbwtArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Phase B as a prefix scan over the chunks (radix form).  With R_k[c] the
+// dense rank of each haplotype's key over chunks [c - 2^k + 1, c] (clamped
+// at 0) and O_k[c] the haplotypes sorted by (R_k[c], index), level k with
+// d = 2^k composes, for every chunk c >= d at once,
+//   O_{k+1}[c] = O_k[c - d] stably sorted by R_k[c],
+//   R_{k+1}[c] = dense rank of (R_k[c], R_k[c - d]) along O_{k+1}[c],
+// since the later chunks' key is the more significant; chunks c < d are
+// complete and are copied.  After ceil(log2 n) levels every chunk's range
+// starts at marker 0, and O[c] is the true order at its last marker, which
+// the level that completes it (d <= c < 2d) writes to that prefix row.  The
+// serial depth is log2(n) sorts, each level one launch of n workgroups.
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(kPT) void pbwt_scan_kernel(PbwtArgs a, uint32_t d, const uint32_t* Rin,
+                                                        uint32_t* Rout, const uint16_t* Oin, uint16_t* Oout,
+                                                        const uint32_t* Gin, uint32_t* Gout) {
+    using G = LdsGeom<K>;
+    __shared__ __align__(16) uint16_t X[G::NPOS];
+    __shared__ __align__(16) uint16_t cnt[16 * kPT];
+    __shared__ __align__(4) uint16_t gl[G::NPOS];
+    const R4Lds L{cnt, X};
+    uint32_t* s_b = reinterpret_cast<uint32_t*>(cnt);  // free after the sort
+    const uint32_t c = blockIdx.x, t = threadIdx.x, lane = t & 63u, w = t >> 6, H = a.H;
+    const uint32_t p0 = t * uint32_t(K);
+    if (c < d) {  // complete: carried to the next level's buffers
+        for (uint32_t i = t; i < H; i += kPT) {
+            Rout[size_t(c) * H + i] = Rin[size_t(c) * H + i];
+            Oout[size_t(c) * H + i] = Oin[size_t(c) * H + i];
+        }
+        if (t == 0) Gout[c] = Gin[c];
+        return;
+    }
+    BOrd<K> e;
+    const uint16_t* lo_ord = Oin + size_t(c - d) * H;
+#pragma unroll
+    for (int j = 0; j < K; ++j) e.set(j, p0 + uint32_t(j) < H ? uint32_t(lo_ord[p0 + uint32_t(j)]) : 0u);
+    const bool done = c < 2u * d;
+    sort_by_ids<K>(e, Rin + size_t(c) * H, Gin[c], H, L, gl, done ? chunk_end_row(a, c) : nullptr);
+    // X: the new order by position; gl: R_k[c] of each haplotype
+    const uint32_t* lo = Rin + size_t(c - d) * H;
+    uint32_t flags = 0, cntf = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t pos = p0 + uint32_t(j);
+        if (pos >= H) continue;
+        const uint32_t h = e.get(j);
+        bool start = pos == 0u;
+        if (!start) {
+            const uint32_t hp = X[pos - 1u];
+            start = gl[h] != gl[hp] || lo[h] != lo[hp];
+        }
+        flags |= uint32_t(start) << j;
+        cntf += uint32_t(start);
+    }
+    uint32_t incl = cntf;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t n = __shfl_up(incl, o, kWave);
+        if (lane >= uint32_t(o)) incl += n;
+    }
+    if (lane == 63u) s_b[w] = incl;
+    __syncthreads();
+    uint32_t rk = incl - cntf, Gn = 0;
+#pragma unroll
+    for (int v = 0; v < kPW; ++v) {
+        rk += uint32_t(v) < w ? s_b[v] : 0u;
+        Gn += s_b[v];
+    }
+    uint32_t* out = Rout + size_t(c) * H;
+    uint16_t* oo = Oout + size_t(c) * H;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t pos = p0 + uint32_t(j);
+        if (pos >= H) continue;
+        rk += (flags >> j) & 1u;
+        out[e.get(j)] = rk - 1u;
+        oo[pos] = uint16_t(e.get(j));
+    }
+    if (t == 0) Gout[c] = Gn;
+}
+
+// ---------------------------------------------------------------------------
 // Divergence rows alone (row m = m everywhere), for the radix form, which
 // runs it on a second stream beside the sort and composition launches: those
 // leave most of the HBM bandwidth idle.  Workgroups stride over rows; 16-byte
@@ -1202,12 +1317,22 @@ void launch_radix(PbwtArgs a, hipStream_t st) {
     // the end of the key launch on (fill in the row re-sweep if that fails)
     if (a.div) a.rows_div = div_fill_beside(a, st, &join) ? 0u : 1u;
     hipLaunchKernelGGL(pbwt_chunk_sort_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
-    if (a.nchunk > 1) {
-        const uint32_t nseg = (a.nchunk - 1u + a.SL - 1u) / a.SL;
-        hipLaunchKernelGGL(pbwt_seg_ranks_kernel<K>, dim3(nseg), dim3(kPT), 0, st, a);
-        hipLaunchKernelGGL(pbwt_seg_chain_kernel<K>, dim3(1), dim3(kPT), 0, st, a);
-        if (a.nchunk > 2)
-            hipLaunchKernelGGL(pbwt_seg_fill_kernel<K>, dim3(a.nchunk - 1u), dim3(kPT), 0, st, a);
+    // the composition scan: rank and order buffers alternate between levels
+    // (level 0 reads the chunk sort's ids, orders and group counts)
+    const size_t nh = size_t(a.nchunk) * a.H;
+    const uint32_t* Rin = a.gid;
+    const uint16_t* Oin = a.ord0;
+    const uint32_t* Gin = a.ngroups;
+    uint32_t* Rb[2] = {a.xa, a.xa + nh};
+    uint16_t* Ob[2] = {a.ord1, a.ord0};
+    uint32_t* Gb[2] = {a.gcnt, a.gcnt + a.nchunk};
+    uint32_t lv = 0;
+    for (uint32_t d = 1; d < a.nchunk; d *= 2, ++lv) {
+        hipLaunchKernelGGL(pbwt_scan_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a, d, Rin, Rb[lv & 1u], Oin,
+                           Ob[lv & 1u], Gin, Gb[lv & 1u]);
+        Rin = Rb[lv & 1u];
+        Oin = Ob[lv & 1u];
+        Gin = Gb[lv & 1u];
     }
     hipLaunchKernelGGL(pbwt_rows_lds_kernel<K>, dim3(a.nchunk), dim3(kPT), 0, st, a);
     if (join) {
@@ -1237,7 +1362,9 @@ size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C) {
     const size_t npos = size_t(kPW) * 64u * K;  // padded exchange length
     const size_t base = align256(n * 2 * h * 4) + align256(n * h * 4) + align256(n * 4) +
                         align256(2 * npos * 4);
-    return base + (K != 0 && K <= kRadixMaxK ? key_bytes(H, M, C) + align256(n * 4) : 0);
+    return base + (K != 0 && K <= kRadixMaxK ? key_bytes(H, M, C) + align256(n * 4) + align256(2 * n * h * 2) +
+                                                   align256(2 * n * 4)
+                                             : 0);
 }
 
 int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,
@@ -1274,6 +1401,11 @@ int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* pr
         a.th = reinterpret_cast<uint8_t*>(p);
         p += align256(n * h * a.TS);
         a.nseg = reinterpret_cast<uint32_t*>(p);
+        p += align256(n * 4);
+        a.ord0 = reinterpret_cast<uint16_t*>(p);
+        a.ord1 = a.ord0 + n * h;
+        p += align256(2 * n * h * 2);
+        a.gcnt = reinterpret_cast<uint32_t*>(p);
         a.gseg = a.xa;  // the partition form's exchange, unused by the radix form
         uint32_t sl = 1;
         while (sl * sl < a.nchunk - 1u) ++sl;
