@@ -57,6 +57,10 @@ typedef struct si_server_cfg {
                                    them from it; a request that finds it full keeps its
                                    codes in pageable memory, uploaded through a staging
                                    copy (0: no pool) */
+    uint32_t window_size;       /* the reference's windowed model for requests longer than
+                                   this many markers (imputer.cpp:965-971; its default
+                                   configuration uses 10,000); 0: the exact model for every
+                                   request.  At least 32 when set. */
 } si_server_cfg;
 
 void si_server_defaults(si_server_cfg* cfg);

@@ -335,15 +335,9 @@ struct si_server {
         }
         return o;
     }
-    size_t ws_bytes_of(const Request& r) const {
-        si_fb_job j{};
-        j.n_samples = r.B;
-        j.n_markers = r.M;
-        j.n_states = cfg.n_states;
-        return align256(si_fb_workspace_bytes(&j, 1));
-    }
-
-    si_fb_job job_of(const Request& r) const {
+    // A request's job as far as its shape goes: sizes, numeric form, window
+    // (what its workspace depends on).
+    si_fb_job job_shape(const Request& r) const {
         si_fb_job j{};
         j.n_samples = r.B;
         j.n_markers = r.M;
@@ -352,6 +346,16 @@ struct si_server {
         // the banded kernels carry exactly against the -999 floor
         j.flags = (cfg.per_marker ? uint32_t(SI_FB_PER_MARKER) : 0u) |
                   (uses_logspace(r.m0, r.m0 + r.M) ? uint32_t(SI_FB_LOGSPACE | SI_FB_BANDED) : 0u);
+        j.window = cfg.window_size;
+        return j;
+    }
+    size_t ws_bytes_of(const Request& r) const {
+        const si_fb_job j = job_shape(r);
+        return align256(si_fb_workspace_bytes(&j, 1));
+    }
+
+    si_fb_job job_of(const Request& r) const {
+        si_fb_job j = job_shape(r);
         j.code_pitch = r.pitch;
         j.n_palette = 4;
         j.flags |= SI_FB_UNTYPED;  // code 3 of the hard-call palette: (0, 0, 0)
@@ -667,6 +671,7 @@ void si_server_defaults(si_server_cfg* c) {
     c->region_resweep = 0;
     c->plain = 0;
     c->staging_bytes = 4ull << 30;
+    c->window_size = 0;
 }
 
 int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H, uint32_t M,
@@ -677,6 +682,8 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
     if (cfg->selection > 1) return fail(SI_ERR_ARG, "selection must be 0 (CPU path) or 1 (GPU kernels)");
     if (cfg->region_resweep > 1) return fail(SI_ERR_ARG, "region_resweep must be 0 or 1");
     if (cfg->plain > 1) return fail(SI_ERR_ARG, "plain must be 0 or 1");
+    if (cfg->window_size != 0 && cfg->window_size < 32)
+        return fail(SI_ERR_ARG, "window_size must be 0 or at least 32 markers");
     auto s = std::make_unique<si_server>();
     s->cfg = *cfg;
     s->cfg.max_streams = std::max(1u, cfg->max_streams);

@@ -21,7 +21,8 @@ class ServerCfg(ctypes.Structure):
                 ("max_batch", ctypes.c_uint32), ("hbm_budget_bytes", ctypes.c_uint64),
                 ("device_id", ctypes.c_int), ("max_queue_delay_us", ctypes.c_uint32),
                 ("selection", ctypes.c_uint32), ("region_resweep", ctypes.c_uint32),
-                ("plain", ctypes.c_uint32), ("staging_bytes", ctypes.c_uint64)]
+                ("plain", ctypes.c_uint32), ("staging_bytes", ctypes.c_uint64),
+                ("window_size", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -85,7 +86,8 @@ class Server:
                  L: int = 8, ne: int = 10000, per_marker: bool = False, max_streams: int = 4,
                  max_batch: int = 256, hbm_budget_gb: float = 32.0, rho_rate: float = 4.0,
                  max_queue_delay_us: int = 0, gpu_compat_selection: bool = False,
-                 region_resweep: bool = False, plain: bool = False, staging_gb: float = 4.0):
+                 region_resweep: bool = False, plain: bool = False, staging_gb: float = 4.0,
+                 window_size: int = 0):
         lib = _lib()
         cfg = ServerCfg()
         lib.si_server_defaults(ctypes.byref(cfg))
@@ -98,6 +100,7 @@ class Server:
         cfg.region_resweep = int(region_resweep)
         cfg.plain = int(plain)
         cfg.staging_bytes = int(staging_gb * (1 << 30))
+        cfg.window_size = window_size
         cols = np.ascontiguousarray(cols, np.uint64)
         pos = np.ascontiguousarray(pos, np.uint64)
         cmv = None if cM is None else np.ascontiguousarray(cM, np.float64)

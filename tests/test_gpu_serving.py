@@ -36,9 +36,10 @@ def world():
     return pan, tg
 
 
-def _plain(pan, idx, tables, m0, m1, codes, seed, idx_m0=0):
+def _plain(pan, idx, tables, m0, m1, codes, seed, idx_m0=0, window=0, pos=None):
     """The plain path over [m0, m1); idx holds the markers from idx_m0 on
-    (0: a panel-wide index)."""
+    (0: a panel-wide index); window: the windowed model; pos: positions in
+    place of the panel's."""
     base = m0 - idx_m0
     M = m1 - m0
     B = codes.shape[0]
@@ -46,13 +47,15 @@ def _plain(pan, idx, tables, m0, m1, codes, seed, idx_m0=0):
     c = np.full((B, pitch), 3, np.uint8)
     c[:, :M] = codes
     cm = pan.genetic_cM()
-    tr = device.Transitions.build(pan.pos[m0:m1], None if cm is None else cm[m0:m1], NE)
+    p = pan.pos if pos is None else pos
+    tr = device.Transitions.build(p[m0:m1], None if cm is None else cm[m0:m1], NE)
     cd = torch.from_numpy(c).cuda()
     ds = torch.full((B, M), -1.0, device="cuda")
     hap = torch.full((B, 2, M), 7, dtype=torch.uint8, device="cuda")
     j = _native.FbJob()
     j.n_samples, j.n_markers, j.n_states = B, M, L
-    j.flags = _native.SI_FB_LOGSPACE if tr.min_switch() < 1e-30 else 0
+    j.flags = (_native.SI_FB_LOGSPACE | _native.SI_FB_BANDED) if tr.min_switch() < 1e-30 else 0
+    j.window = window
     j.code_pitch = pitch
     j.seed = seed
     s = tr.step
@@ -151,6 +154,34 @@ def test_plain_mode_equals_batched_and_splits_time(world):
         for r, a, b in zip(reqs, got["batched"], got[key]):
             assert np.array_equal(a.hap, b.hap), (key, r[:2])
             assert np.array_equal(a.ds, b.ds), (key, r[:2])
+
+
+def test_windowed_and_coincident_requests_equal_plain(world):
+    # The reference's windowed model on long requests (window_size), and
+    # requests over coincident markers (switch probability at the -999
+    # floor: the banded log form), served in a mix: each equals the plain
+    # path with the same model.
+    pan, tg = world
+    cm = pan.genetic_cM()
+    idx = device.DeviceIndex(pan.cols, pan.H, L, keep_prefix=False)
+    tables = device.SharedTables(synth.HARDCALL_PALETTE)
+    reqs = _requests(pan, tg, 8, seed=21)
+    reqs.append((100, 1500, np.ascontiguousarray(tg.codes[:9, 100:1500]), 900))
+    with serving.Server(pan.cols, pan.H, pan.pos, cm, L=L, ne=NE, max_streams=2, max_batch=4,
+                        max_queue_delay_us=100000, window_size=400) as srv:
+        got = [srv.wait(srv.submit(*r)) for r in reqs]
+    for r, g in zip(reqs, got):
+        hap, ds = _plain(pan, idx, tables, *r, window=400)
+        assert np.array_equal(g.hap, hap) and np.array_equal(g.ds, ds), r[:2]
+    pos = pan.pos.copy()
+    pos[300:700] = pos[300]  # a run of coincident markers
+    with serving.Server(pan.cols, pan.H, pos, None, L=L, ne=NE, max_streams=2, max_batch=4,
+                        max_queue_delay_us=100000) as srv:
+        ids = [srv.submit(*r) for r in reqs]
+        got = [srv.wait(i) for i in ids]
+    for r, g in zip(reqs, got):
+        hap, ds = _plain(pan, idx, tables, *r, pos=pos)
+        assert np.array_equal(g.hap, hap) and np.array_equal(g.ds, ds), r[:2]
 
 
 def test_admission_under_tight_budget(world):
