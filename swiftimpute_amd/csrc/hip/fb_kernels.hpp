@@ -978,9 +978,11 @@ __device__ __forceinline__ void block_sample(const JobView& J, bool active, uint
                     const uint32_t c = t & 3u;
                     rv = c == 0u ? rb.x : c == 1u ? rb.y : c == 2u ? rb.z : rb.w;
                 }
+                // the cumulative sums never decrease (p >= 0), so the first k
+                // with u < cs_k is the number of k with cs_k <= u
                 const float uu = u24(rv);
                 float cs = 0.0f;
-                uint32_t kd = 0xffffffffu;
+                uint32_t below = 0;
 #pragma unroll
                 for (int k4 = 0; k4 < S::Lp; k4 += 4) {
                     const float4 x = *reinterpret_cast<const float4*>(pt + k4);
@@ -988,10 +990,10 @@ __device__ __forceinline__ void block_sample(const JobView& J, bool active, uint
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         cs += xs[i];
-                        kd = (kd == 0xffffffffu && uu < cs) ? uint32_t(k4 + i) : kd;
+                        below += (uu < cs) ? 0u : 1u;
                     }
                 }
-                ks = kd == 0xffffffffu ? 0u : kd;
+                ks = below < uint32_t(S::Lp) ? below : 0u;
             }
             const size_t o = size_t(s) * J.M + m;
             if (J.ds) J.ds[o] = ap0r[q] + ap1r[q];
