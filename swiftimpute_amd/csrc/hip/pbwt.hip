@@ -7,20 +7,20 @@
 // sorts the haplotypes by (x_m, x_{m-1}, ..., x_0) lexicographically, ties by
 // haplotype index.
 //
-// The markers are cut into chunks of C.  Three launches:
-//   A  one workgroup per chunk, all chunks at once: sweep the chunk from the
-//      identity order, carrying per position a flag "starts a group", a group
-//      being the haplotypes with equal alleles over the chunk so far.  At the
-//      chunk end a haplotype's group id is the dense rank of its chunk key.
-//      Chunk 0's final order is the true order at its last marker and goes to
-//      that prefix row.  A also streams out the constant divergence rows.
-//   B  one workgroup, serial over chunk boundaries: the order at the end of
-//      chunk c is the stable sort, by chunk-c group id, of the order at the end
-//      of chunk c - 1 (LSD radix over 8-bit digits of the id), written to the
-//      chunk's last prefix row.
+// The markers are cut into chunks of C, and the rows come out of three phases:
+//   A  one workgroup per chunk, all chunks at once: the chunk's order at its
+//      end when swept from the identity order, and each haplotype's group id
+//      (the dense rank of its allele string over the chunk).  For H <= 32768
+//      this is an LSD radix sort of per-chunk key bytes (4-bit passes); above,
+//      a sweep carrying per-position group-start flags.
+//   B  the true order at every chunk end: for H <= 32768 a log-depth prefix
+//      scan of rank compositions (one launch per level over every chunk);
+//      above, one workgroup serial over the chunk boundaries.
 //   C  one workgroup per chunk: re-sweep the chunk from its true starting
 //      order (identity for chunk 0, else the previous chunk's last row),
 //      writing every row of the chunk but its last.
+// The constant divergence rows go out beside A and B on a second stream (or
+// inside A or C where that stream is not available).
 //
 // Inside a workgroup (16 wavefronts) wave w owns the positions
 // [w*S, (w+1)*S), S = 64*K; in round j < K lane l holds position
@@ -31,6 +31,7 @@
 // H <= 65536, so a haplotype fits in 16 bits of an element word.
 #include "si_device.hpp"
 #include "si_internal.hpp"
+#include <atomic>
 
 namespace si {
 
@@ -50,9 +51,6 @@ struct PbwtArgs {
     uint8_t* tp;             // [nchunk][NP][H] key byte p of each haplotype (radix digit planes)
     uint8_t* th;             // [nchunk][H][TS] the same bytes, haplotype-major (key comparison)
     uint32_t NP, TS;         // key bytes per chunk (C / 8), row stride of th (NP rounded to 16)
-    uint32_t* gseg;          // [nchunk][H] group id of the key over chunks [segment start, c]
-    uint32_t* nseg;          // [nchunk]   number of those groups
-    uint32_t SL;             // chunks per composition segment
     uint32_t rows_div;       // phase C also writes the divergence rows of its chunk
     uint16_t* ord0;          // [nchunk][H] each chunk's order sorted by its key (radix form)
     uint16_t* ord1;          // [nchunk][H] the scan's second order buffer
@@ -67,6 +65,19 @@ __device__ __forceinline__ void div_row(const PbwtArgs& a, uint32_t m) {
 
 __device__ __forceinline__ uint32_t mbcnt64(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+// Inclusive prefix sum over the 64 lanes of a wavefront: shifts within each
+// 16-lane row, then row 15's and row 31's last lanes broadcast to the rows
+// after them (DPP, no LDS round trips).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xf, 0xf, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xf, 0xf, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xf, 0xf, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xf, 0xf, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return x;
 }
 
 __device__ __forceinline__ uint32_t allele_bit(const uint32_t* col, uint32_t h) {
@@ -786,30 +797,58 @@ struct R4Lds {
     uint16_t* X;
 };
 
-// One stable counting-sort pass of the workgroup's H elements by the 4-bit
-// digit(h).  The first barrier also publishes LDS data the digit function
-// reads, written before the call.  If row is set, the new order is also
-// written there (coalesced, from the exchange buffer).
+// One stable counting-sort pass of the workgroup's elements by the 4-bit
+// digit(h).  Every thread holds K elements: positions past H hold padding
+// ids (the position itself) whose digits read as 15, so they sort after
+// every haplotype, keep the positions past H and no element needs a range
+// check.  The first barrier publishes LDS data the digit function reads,
+// written before the call.  If row is set, the new order of positions
+// [0, H) is also written there (coalesced, from the exchange buffer).
+//
+// Each thread counts its own elements' digits in registers: 16 four-bit
+// counters in a 64-bit word per group of at most 15 elements (so no counter
+// carries into the next), and keeps the digits as packed nibbles.  The
+// counts go to LDS once (digit-major), one scan turns them into each
+// (digit, thread) pair's first destination, and the scatter adds the
+// element's rank among the thread's earlier elements of its digit, read from
+// the same counters: no LDS read-modify-write chains and no second digit
+// lookup.  Destinations are read in batches before their exchange writes.
 template <int K, class Digit>
 __device__ __forceinline__ void radix4_pass(BOrd<K>& e, uint32_t H, const R4Lds& L, Digit digit,
                                             uint32_t* row) {
+    constexpr int GS = 15;                  // elements per counter group
+    constexpr int NA = (K + GS - 1) / GS;   // counter groups
+    constexpr int ND = (K + 7) / 8;         // words of packed digits
+    constexpr int BT = K > 24 ? 4 : 8;      // scatter batch (fewer registers for long orders)
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
     const uint32_t p0 = t * uint32_t(K);
-    uint4* c4 = reinterpret_cast<uint4*>(L.cnt);  // 2048 uint4
-    c4[t] = make_uint4(0u, 0u, 0u, 0u);
-    c4[t + kPT] = make_uint4(0u, 0u, 0u, 0u);
+    uint16_t* const cnt = L.cnt;
+    uint16_t* const X = L.X;
+    uint4* c4 = reinterpret_cast<uint4*>(cnt);  // 2048 uint4
     __syncthreads();
+    uint32_t dg[ND];
+    uint64_t acc[NA];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) dg[k] = 0u;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0u;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        if (p0 + uint32_t(j) < H) {
-            uint16_t* cp = L.cnt + digit(e.get(j)) * kPT + t;
-            *cp = uint16_t(*cp + 1u);
-        }
+        const uint32_t d = digit(e.get(j));
+        dg[j >> 3] |= d << (4 * (j & 7));
+        acc[j / GS] += uint64_t(1) << (4u * d);
+    }
+#pragma unroll
+    for (int d = 0; d < 16; ++d) {
+        uint32_t n = 0;
+#pragma unroll
+        for (int k = 0; k < NA; ++k) n += uint32_t(acc[k] >> (4 * d)) & 15u;
+        cnt[d * kPT + t] = uint16_t(n);
     }
     __syncthreads();
     // exclusive scan of the digit-major counters: thread t owns entries
     // [16t, 16t + 16) = digit w, threads 16*lane .. 16*lane + 15
-    uint32_t* wsum = reinterpret_cast<uint32_t*>(L.X);
+    uint32_t* wsum = reinterpret_cast<uint32_t*>(X);
     const uint4 A = c4[2 * t], B = c4[2 * t + 1];
     const uint32_t wd[8] = {A.x, A.y, A.z, A.w, B.x, B.y, B.z, B.w};
     uint32_t ex[16];
@@ -819,38 +858,48 @@ __device__ __forceinline__ void radix4_pass(BOrd<K>& e, uint32_t H, const R4Lds&
         ex[i] = s;
         s += (wd[i >> 1] >> (16 * (i & 1))) & 0xffffu;
     }
-    uint32_t incl = s;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t n = __shfl_up(incl, o, kWave);
-        if (lane >= uint32_t(o)) incl += n;
-    }
+    const uint32_t incl = wave_incl_scan(s);
     if (lane == 63u) wsum[w] = incl;
     __syncthreads();
     uint32_t base = incl - s;
-    for (uint32_t v = 0; v < w; ++v) base += wsum[v];
+#pragma unroll
+    for (int v = 0; v < kPW; ++v) {
+        const uint32_t x = wsum[v];
+        base += uint32_t(v) < w ? x : 0u;
+    }
     uint32_t o[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) o[i] = (base + ex[2 * i]) | ((base + ex[2 * i + 1]) << 16);
     c4[2 * t] = make_uint4(o[0], o[1], o[2], o[3]);
     c4[2 * t + 1] = make_uint4(o[4], o[5], o[6], o[7]);
     __syncthreads();
+    uint64_t run = 0;  // digits of the current group's earlier elements
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-        if (p0 + uint32_t(j) < H) {
-            const uint32_t h = e.get(j);
-            uint16_t* cp = L.cnt + digit(h) * kPT + t;
-            const uint32_t dst = *cp;
-            *cp = uint16_t(dst + 1u);
-            L.X[dst] = uint16_t(h);
+    for (int j0 = 0; j0 < K; j0 += BT) {
+        uint32_t dst[BT];
+#pragma unroll
+        for (int i = 0; i < BT && j0 + i < K; ++i) {
+            const int j = j0 + i;
+            const uint32_t d = (dg[j >> 3] >> (4 * (j & 7))) & 15u;
+            dst[i] = cnt[d * kPT + t];
+        }
+#pragma unroll
+        for (int i = 0; i < BT && j0 + i < K; ++i) {
+            const int j = j0 + i;
+            if (j % GS == 0) run = 0u;
+            const uint32_t sh = 4u * ((dg[j >> 3] >> (4 * (j & 7))) & 15u);
+            uint32_t rk = uint32_t(run >> sh) & 15u;
+#pragma unroll
+            for (int k = 0; k < j / GS; ++k) rk += uint32_t(acc[k] >> sh) & 15u;
+            run += uint64_t(1) << sh;
+            X[dst[i] + rk] = uint16_t(e.get(j));
         }
     }
     __syncthreads();
     if (row)
-        for (uint32_t i = t; i < H; i += kPT) row[i] = L.X[i];
+        for (uint32_t i = t; i < H; i += kPT) row[i] = X[i];
 #pragma unroll
-    for (int j = 0; j < K; ++j)
-        if (p0 + uint32_t(j) < H) e.set(j, L.X[p0 + uint32_t(j)]);
+    for (int j = 0; j < K; ++j) e.set(j, X[p0 + uint32_t(j)]);
 }
 
 // Phase A, radix form: one workgroup per chunk.
@@ -874,14 +923,17 @@ __global__ __launch_bounds__(kPT) void pbwt_chunk_sort_kernel(PbwtArgs a) {
     const uint32_t nw = (H + 3u) / 4u;  // 32-bit words of a plane
     constexpr uint32_t kPF = (G::NPOS / 4u + kPT - 1) / kPT;
     auto plane_word = [&](uint32_t p, uint32_t i) -> uint32_t {
-        // four key bytes; a plane is H bytes, so an unaligned or last word is assembled bytewise
+        // four key bytes; a plane is H bytes, so an unaligned or last word is
+        // assembled bytewise, bytes past H reading as 0xff (padding digits)
         const uint8_t* src = tpc + size_t(p) * H + 4u * i;
         if (4u * i + 4u <= H && ((reinterpret_cast<uintptr_t>(src) & 3u) == 0u))
             return *reinterpret_cast<const uint32_t*>(src);
         uint32_t v = 0;
-        for (uint32_t k = 0; k < 4u && 4u * i + k < H; ++k) v |= uint32_t(src[k]) << (8u * k);
+        for (uint32_t k = 0; k < 4u; ++k) v |= (4u * i + k < H ? uint32_t(src[k]) : 0xffu) << (8u * k);
         return v;
     };
+    // padding ids' key bytes (words the plane loads below never write)
+    for (uint32_t i = nw + t; i < G::NPOS / 4u; i += kPT) reinterpret_cast<uint32_t*>(dig)[i] = ~0u;
     uint32_t pf[kPF];
 #pragma unroll
     for (uint32_t k = 0; k < kPF; ++k) {
@@ -901,8 +953,8 @@ __global__ __launch_bounds__(kPT) void pbwt_chunk_sort_kernel(PbwtArgs a) {
                 pf[k] = i < nw ? plane_word(p + 1, i) : 0u;
             }
         }
-        // low nibble, then high nibble (the dig buffer is rewritten only after
-        // the second pass's scatter barrier)
+        // low nibble, then high nibble (a pass reads its digits before its
+        // second barrier, so the buffer is free once the second pass returns)
         radix4_pass<K>(e, H, L, [&](uint32_t h) { return uint32_t(dig[h]) & 15u; }, nullptr);
         radix4_pass<K>(e, H, L, [&](uint32_t h) { return uint32_t(dig[h]) >> 4; }, nullptr);
     }
@@ -965,19 +1017,6 @@ __global__ __launch_bounds__(kPT) void pbwt_chunk_sort_kernel(PbwtArgs a) {
     if (t == 0) a.ngroups[c] = Gn;
 }
 
-// Phase B, radix form, as a two-level scan over chunk segments.  With
-// g_c the chunk-c group ids and G[a, c] the dense rank of the key over chunks
-// a..c (later chunk more significant), G[a, c] = rank of (g_c, G[a, c-1]),
-// and the order at the end of chunk c is the stable sort of the order at the
-// end of chunk a - 1 by G[a, c].  Chunks 1..n-1 are cut into segments of SL:
-//   B1  one workgroup per segment: G[a, c] for each chunk c of the segment
-//       (a = its first chunk), from the identity order, by stable sorts by g_c
-//       and the adjacent comparison of (g_c, G[a, c-1]);
-//   B2  one workgroup, segments in sequence: the order at each segment end;
-//   B3  one workgroup per other chunk: its end order from its segment's
-//       starting order.
-// The serial depth is about 2 sqrt(n) sorts instead of n.
-
 // Radix passes (4 bits each) needed for ids below Gn.
 __device__ __forceinline__ uint32_t id_passes(uint32_t Gn) {
     uint32_t n = 0;
@@ -1003,7 +1042,8 @@ __device__ __forceinline__ void load_order(BOrd<K>& e, const uint32_t* row, uint
 template <int K>
 __device__ __forceinline__ void sort_by_ids(BOrd<K>& e, const uint32_t* ids, uint32_t Gn, uint32_t H,
                                             const R4Lds& L, uint16_t* gl, uint32_t* row) {
-    for (uint32_t i = threadIdx.x; i < H; i += kPT) gl[i] = uint16_t(ids[i]);
+    // ids of the padding past H read as 0xffff (digit 15 in every pass)
+    for (uint32_t i = threadIdx.x; i < LdsGeom<K>::NPOS; i += kPT) gl[i] = i < H ? uint16_t(ids[i]) : uint16_t(0xffffu);
     const uint32_t np = id_passes(Gn);
     for (uint32_t pass = 0; pass < np; ++pass) {
         const uint32_t sh = 4u * pass;
@@ -1025,104 +1065,6 @@ __device__ __forceinline__ void sort_by_ids(BOrd<K>& e, const uint32_t* ids, uin
     }
 }
 
-// B1: one workgroup per segment.
-template <int K>
-__global__ __launch_bounds__(kPT) void pbwt_seg_ranks_kernel(PbwtArgs a) {
-    using G = LdsGeom<K>;
-    __shared__ __align__(16) uint16_t X[G::NPOS];
-    __shared__ __align__(16) uint16_t cnt[16 * kPT];
-    __shared__ __align__(4) uint16_t gl[G::NPOS];
-    const R4Lds L{cnt, X};
-    uint32_t* s_b = reinterpret_cast<uint32_t*>(cnt);  // free between sorts
-    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6, H = a.H, p0 = t * uint32_t(K);
-    const uint32_t c0 = 1u + blockIdx.x * a.SL, c1 = min(a.nchunk, c0 + a.SL);
-    BOrd<K> e;
-#pragma unroll
-    for (int j = 0; j < K; ++j) e.set(j, p0 + uint32_t(j));
-    for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t* g = a.gid + size_t(c) * H;
-        uint32_t* out = a.gseg + size_t(c) * H;
-        sort_by_ids<K>(e, g, a.ngroups[c], H, L, gl, nullptr);
-        if (c == c0) {  // G[a, a] = g_a
-            for (uint32_t i = t; i < H; i += kPT) out[i] = g[i];
-            if (t == 0) a.nseg[c] = a.ngroups[c];
-        } else {
-            // X holds the new order by position; gl the g_c of each haplotype
-            const uint32_t* prev = a.gseg + size_t(c - 1u) * H;
-            uint32_t flags = 0, cntf = 0;
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                const uint32_t pos = p0 + uint32_t(j);
-                if (pos >= H) continue;
-                const uint32_t h = e.get(j);
-                bool start = pos == 0u;
-                if (!start) {
-                    const uint32_t hp = X[pos - 1u];
-                    start = gl[h] != gl[hp] || prev[h] != prev[hp];
-                }
-                flags |= uint32_t(start) << j;
-                cntf += uint32_t(start);
-            }
-            uint32_t incl = cntf;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint32_t n = __shfl_up(incl, o, kWave);
-                if (lane >= uint32_t(o)) incl += n;
-            }
-            if (lane == 63u) s_b[w] = incl;
-            __syncthreads();
-            uint32_t rk = incl - cntf, Gn = 0;
-#pragma unroll
-            for (int v = 0; v < kPW; ++v) {
-                rk += uint32_t(v) < w ? s_b[v] : 0u;
-                Gn += s_b[v];
-            }
-#pragma unroll
-            for (int j = 0; j < K; ++j) {
-                if (p0 + uint32_t(j) >= H) continue;
-                rk += (flags >> j) & 1u;
-                out[e.get(j)] = rk - 1u;
-            }
-            if (t == 0) a.nseg[c] = Gn;
-        }
-        // this step's ranks are read (from global memory) by the next step
-        __threadfence_block();
-        __syncthreads();
-    }
-}
-
-// B2: the order at each segment end, segments in sequence.
-template <int K>
-__global__ __launch_bounds__(kPT) void pbwt_seg_chain_kernel(PbwtArgs a) {
-    using G = LdsGeom<K>;
-    __shared__ __align__(16) uint16_t X[G::NPOS];
-    __shared__ __align__(16) uint16_t cnt[16 * kPT];
-    __shared__ __align__(4) uint16_t gl[G::NPOS];
-    const R4Lds L{cnt, X};
-    BOrd<K> e;
-    load_order<K>(e, chunk_end_row(a, 0), a.H);
-    for (uint32_t c0 = 1; c0 < a.nchunk; c0 += a.SL) {
-        const uint32_t ce = min(a.nchunk, c0 + a.SL) - 1u;
-        sort_by_ids<K>(e, a.gseg + size_t(ce) * a.H, a.nseg[ce], a.H, L, gl, chunk_end_row(a, ce));
-    }
-}
-
-// B3: one workgroup per chunk that does not end a segment.
-template <int K>
-__global__ __launch_bounds__(kPT) void pbwt_seg_fill_kernel(PbwtArgs a) {
-    using G = LdsGeom<K>;
-    __shared__ __align__(16) uint16_t X[G::NPOS];
-    __shared__ __align__(16) uint16_t cnt[16 * kPT];
-    __shared__ __align__(4) uint16_t gl[G::NPOS];
-    const R4Lds L{cnt, X};
-    const uint32_t c = 1u + blockIdx.x;
-    const uint32_t c0 = 1u + ((c - 1u) / a.SL) * a.SL;
-    const uint32_t ce = min(a.nchunk, c0 + a.SL) - 1u;
-    if (c == ce) return;  // done by B2 (the whole workgroup leaves)
-    BOrd<K> e;
-    load_order<K>(e, chunk_end_row(a, c0 - 1u), a.H);
-    sort_by_ids<K>(e, a.gseg + size_t(c) * a.H, a.nseg[c], a.H, L, gl, chunk_end_row(a, c));
-}
 
 // ---------------------------------------------------------------------------
 // Phase B as a prefix scan over the chunks (radix form).  With R_k[c] the
@@ -1160,7 +1102,8 @@ __global__ __launch_bounds__(kPT) void pbwt_scan_kernel(PbwtArgs a, uint32_t d, 
     BOrd<K> e;
     const uint16_t* lo_ord = Oin + size_t(c - d) * H;
 #pragma unroll
-    for (int j = 0; j < K; ++j) e.set(j, p0 + uint32_t(j) < H ? uint32_t(lo_ord[p0 + uint32_t(j)]) : 0u);
+    for (int j = 0; j < K; ++j)
+        e.set(j, p0 + uint32_t(j) < H ? uint32_t(lo_ord[p0 + uint32_t(j)]) : p0 + uint32_t(j));  // padding ids past H
     const bool done = c < 2u * d;
     sort_by_ids<K>(e, Rin + size_t(c) * H, Gin[c], H, L, gl, done ? chunk_end_row(a, c) : nullptr);
     // X: the new order by position; gl: R_k[c] of each haplotype
@@ -1255,6 +1198,21 @@ hipStream_t side_stream() {
 // Runs the divergence fill on the side stream, forked from and joined back
 // into `st` with events (so it also works under graph capture).  Returns
 // false if the fork could not be set up (the caller then fills in line).
+// Compute units of the current device (memoized per device).
+uint32_t device_cus() {
+    static std::atomic<uint32_t> cache[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+    if (dev < 64) {
+        const uint32_t c = cache[dev].load(std::memory_order_relaxed);
+        if (c) return c;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (dev < 64) cache[dev].store(uint32_t(cus), std::memory_order_relaxed);
+    return uint32_t(cus);
+}
+
 bool div_fill_beside(const PbwtArgs& a, hipStream_t st, hipEvent_t* join) {
     hipStream_t side = side_stream();
     hipEvent_t fork = nullptr;
@@ -1265,13 +1223,12 @@ bool div_fill_beside(const PbwtArgs& a, hipStream_t st, hipEvent_t* join) {
     }
     bool ok = hipEventRecord(fork, st) == hipSuccess && hipStreamWaitEvent(side, fork, 0) == hipSuccess;
     if (ok) {
-        // a quarter of the CUs: enough stores in flight to finish during the
-        // sort, while the sort's one-workgroup-per-chunk launches still find
-        // room (a fill over every CU makes them wait and adds its whole time)
-        int dev = 0, cus = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        const uint32_t gmax = cus >= 8 ? uint32_t(cus) / 4u : 2u;
+        // an eighth of the CUs, the ones the chunk launches leave free (the
+        // default chunk length keeps them to seven eighths): enough stores in
+        // flight to finish during the sort and composition, while those
+        // launches still find room (a fill over every CU makes them wait)
+        const uint32_t cus = device_cus();
+        const uint32_t gmax = cus >= 16 ? cus / 8u : 2u;
         const uint32_t grid = a.M < gmax ? a.M : gmax;
         hipLaunchKernelGGL(pbwt_div_fill_kernel, dim3(grid), dim3(256), 0, side, a.div, a.H, a.M);
         ok = hipGetLastError() == hipSuccess && hipEventRecord(*join, side) == hipSuccess;
@@ -1350,9 +1307,15 @@ size_t key_bytes(uint32_t H, uint32_t M, uint32_t C) {
 }  // namespace
 
 uint32_t pbwt_default_chunk(uint32_t H, uint32_t M) {
-    (void)H;
-    (void)M;
-    return 512;
+    const uint32_t K = pbwt_rounds(H);
+    if (K == 0 || K > kRadixMaxK) return 512;  // partition form: its composition walks the chunks in turn
+    // radix form: one round of chunk workgroups (one per CU) on seven eighths
+    // of the CUs, the rest left to the divergence fill; every phase-A and
+    // phase-C step is serial within a chunk, so chunks are as short as that allows
+    const uint32_t cus = device_cus();
+    const uint32_t slots = cus >= 16 ? cus - cus / 8u : 1u;
+    const uint32_t c = (M + slots - 1u) / slots;
+    return c < 16u ? 16u : c;
 }
 
 size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C) {
@@ -1362,8 +1325,7 @@ size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C) {
     const size_t npos = size_t(kPW) * 64u * K;  // padded exchange length
     const size_t base = align256(n * 2 * h * 4) + align256(n * h * 4) + align256(n * 4) +
                         align256(2 * npos * 4);
-    return base + (K != 0 && K <= kRadixMaxK ? key_bytes(H, M, C) + align256(n * 4) + align256(2 * n * h * 2) +
-                                                   align256(2 * n * 4)
+    return base + (K != 0 && K <= kRadixMaxK ? key_bytes(H, M, C) + align256(2 * n * h * 2) + align256(2 * n * 4)
                                              : 0);
 }
 
@@ -1400,16 +1362,10 @@ int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* pr
         p += align256(n * a.NP * h);
         a.th = reinterpret_cast<uint8_t*>(p);
         p += align256(n * h * a.TS);
-        a.nseg = reinterpret_cast<uint32_t*>(p);
-        p += align256(n * 4);
         a.ord0 = reinterpret_cast<uint16_t*>(p);
         a.ord1 = a.ord0 + n * h;
         p += align256(2 * n * h * 2);
         a.gcnt = reinterpret_cast<uint32_t*>(p);
-        a.gseg = a.xa;  // the partition form's exchange, unused by the radix form
-        uint32_t sl = 1;
-        while (sl * sl < a.nchunk - 1u) ++sl;
-        a.SL = sl;
         a.rows_div = 1;
     }
     switch (K) {

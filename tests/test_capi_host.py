@@ -80,8 +80,12 @@ def test_pbwt_argument_errors():
     # workspace: three phases' buffers, growing with the number of chunks
     w512 = lib.si_pbwt_chunked_workspace_bytes(20000, 100000, 512)
     w64 = lib.si_pbwt_chunked_workspace_bytes(20000, 100000, 64)
-    assert lib.si_pbwt_workspace_bytes(20000, 100000) == w512
     assert w64 > w512 >= 196 * 3 * 20000 * 4
+    # default chunk: one round of chunks on 7/8 of the compute units (256 when
+    # no device answers, as here: 224 chunks of 447), 512 in the partition form
+    assert lib.si_pbwt_workspace_bytes(20000, 100000) == lib.si_pbwt_chunked_workspace_bytes(20000, 100000, 447)
+    assert lib.si_pbwt_workspace_bytes(20000, 1000) == lib.si_pbwt_chunked_workspace_bytes(20000, 1000, 16)
+    assert lib.si_pbwt_workspace_bytes(40000, 100000) == lib.si_pbwt_chunked_workspace_bytes(40000, 100000, 512)
 
 
 def test_abi_version_and_padding():
