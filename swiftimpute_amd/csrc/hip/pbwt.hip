@@ -32,6 +32,7 @@
 #include "si_device.hpp"
 #include "si_internal.hpp"
 #include <atomic>
+#include <type_traits>
 
 namespace si {
 
@@ -526,76 +527,161 @@ __device__ __forceinline__ void store_col_padded(uint32_t* col, uint32_t i, uint
     col[i] = i + 1u == nw ? (w | padm) : w;
 }
 
+// The 32 allele bits of a column word as 32 bytes (0 or 1), for the byte
+// columns of the row re-sweep: each nibble spreads to the low bits of four
+// bytes through one multiply (its shifted copies land on disjoint bits).
+__device__ __forceinline__ void store_col_bytes(uint8_t* colb, uint32_t i, uint32_t w) {
+    uint32_t d[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) d[k] = (__umul24((w >> (4 * k)) & 15u, 0x204081u)) & 0x01010101u;
+    uint4* p = reinterpret_cast<uint4*>(colb + 32u * i);
+    p[0] = make_uint4(d[0], d[1], d[2], d[3]);
+    p[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+template <int K>
+struct RowsGeom {
+    // K <= 32: the column as one byte per position (a haplotype's allele is
+    // one LDS byte load at its id); above, bits (the order and two byte
+    // columns would not fit in LDS).  K <= 24: the order as 32-bit ids (no
+    // widening of each loaded id, one address for the LDS and row stores).
+    static constexpr bool BYTES = K <= 32;
+    static constexpr bool WIDE = K <= 24;
+    using Ord = typename std::conditional<WIDE, uint32_t, uint16_t>::type;
+    static constexpr uint32_t NPOS = LdsGeom<K>::NPOS;
+    static constexpr uint32_t COLW = BYTES ? NPOS / 4u : LdsGeom<K>::NW32;  // words per column buffer
+};
+
+// One step of phase C: the order after marker m (stable partition of the
+// order in LDS by the allele of column buffer PAR), written to LDS and to
+// prefix row m, with the next column (prefetched) stored to the other buffer.
+template <int K, int PAR>
+__device__ __forceinline__ void rows_step(const PbwtArgs& a, uint32_t m, bool pre, typename RowsGeom<K>::Ord* ord,
+                                          uint32_t (*col)[RowsGeom<K>::COLW], uint32_t* wz, uint32_t nw,
+                                          uint32_t padm, bool divs) {
+    using R = RowsGeom<K>;
+    constexpr int NZ = (K + 31) / 32;  // words of per-round zero bits (bit columns)
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t base = w * uint32_t(64 * K);
+    const uint32_t* cb = col[PAR];
+    ColPrefetch nx;
+    if (pre) nx.fetch(a, m + 1);
+
+    uint32_t hv[K], xs[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) hv[j] = ord[base + uint32_t(j) * 64u + lane];
+    // every round's allele load issued before the first is used
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if constexpr (R::BYTES) xs[j] = reinterpret_cast<const uint8_t*>(cb)[hv[j]];
+        else xs[j] = (cb[hv[j] >> 5] >> (hv[j] & 31u)) & 1u;
+    }
+    if constexpr (R::BYTES) {
+        __builtin_amdgcn_sched_group_barrier(0x0100, K, 0);  // the order loads
+        __builtin_amdgcn_sched_group_barrier(0x0100, K, 0);  // the allele loads
+    }
+    uint64_t Zs[K];
+    uint32_t zbits[NZ], zsum = 0;
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) zbits[k] = 0u;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        Zs[j] = __ballot(xs[j] == 0u);
+        if constexpr (!R::BYTES) zbits[j >> 5] |= (xs[j] ^ 1u) << (j & 31);
+        zsum += uint32_t(__popcll(Zs[j]));
+    }
+    if (lane == 0) wz[w] = zsum;
+    __syncthreads();
+    uint32_t zbefore, ztot;
+    zero_offsets(wz, w, zbefore, ztot);
+    zbefore = __builtin_amdgcn_readfirstlane(zbefore);
+    ztot = __builtin_amdgcn_readfirstlane(ztot);
+    uint32_t* const rowp = a.prefix + size_t(m) * a.H;
+    // stores past H (padding) fall outside the row's buffer range and are dropped
+    const __amdgpu_buffer_rsrc_t rrow = __builtin_amdgcn_make_buffer_rsrc(rowp, 0, int(a.H * 4u), 0x00020000);
+    uint32_t zrun = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t mb = mbcnt64(Zs[j]);                            // zeros before the lane
+        const uint32_t z0 = zbefore + zrun;                           // the round's first zero
+        const uint32_t o0 = ztot + (base - zbefore) + uint32_t(j) * 64u - zrun;  // and one
+        uint32_t dst;
+        if constexpr (R::BYTES) {
+            // the ballot itself as the lane condition (no per-lane bit test)
+            dst = __builtin_amdgcn_inverse_ballot_w64(Zs[j]) ? z0 + mb : o0 + (lane - mb);
+        } else {
+            dst = ((zbits[j >> 5] >> (j & 31)) & 1u) ? z0 + mb : o0 + (lane - mb);
+        }
+        ord[dst] = typename R::Ord(hv[j]);
+        if constexpr (R::BYTES) {
+            __builtin_amdgcn_raw_buffer_store_b32(hv[j], rrow, int(dst * 4u), 0, 0);
+        } else {
+            if (dst < a.H) rowp[dst] = hv[j];
+        }
+        zrun += uint32_t(__popcll(Zs[j]));
+    }
+    if (divs) div_row(a, m);
+    if (pre) {
+        uint32_t* nb = col[PAR ^ 1];
+        if constexpr (R::BYTES) {
+            if (t < nw) store_col_bytes(reinterpret_cast<uint8_t*>(nb), t, t + 1u == nw ? (nx.w0 | padm) : nx.w0);
+        } else {
+            const uint32_t i0 = t, i1 = t + kPT;
+            if (i0 < nw) store_col_padded(nb, i0, nx.w0, nw, padm);
+            if (i1 < nw) store_col_padded(nb, i1, nx.w1, nw, padm);
+        }
+    }
+    __syncthreads();
+}
+
 template <int K>
 __global__ __launch_bounds__(kPT) void pbwt_rows_lds_kernel(PbwtArgs a) {
-    using G = LdsGeom<K>;
-    constexpr int NZ = (K + 31) / 32;  // words of per-round zero bits
-    __shared__ uint16_t ord[G::NPOS];
-    __shared__ uint32_t col[2][G::NW32];
-    __shared__ uint32_t wz[kPW];
+    using R = RowsGeom<K>;
+    // the column buffers first, so the allele loads reach them by immediate offsets
+    __shared__ __align__(16) struct {
+        uint32_t col[2][R::COLW];
+        typename R::Ord ord[R::NPOS];
+        uint32_t wz[kPW];
+    } sm;
+    auto& col = sm.col;
+    auto& ord = sm.ord;
+    auto& wz = sm.wz;
     const uint32_t c = blockIdx.x;
     const uint32_t m_first = c * a.C, m_end = min(a.M, m_first + a.C);
     const bool divs = a.rows_div && a.div;
     if (divs) div_row(a, m_end - 1u);  // every divergence row of the chunk, its last one too
     if (m_end - m_first < 2) return;
-    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    const uint32_t base = w * uint32_t(64 * K);
+    const uint32_t t = threadIdx.x;
     const uint32_t nw = (a.H + 31u) / 32u;
     const uint32_t padm = (a.H & 31u) ? ~((1u << (a.H & 31u)) - 1u) : 0u;
 
     const uint32_t* start = (c == 0) ? nullptr : a.prefix + size_t(m_first - 1) * a.H;
-    for (uint32_t i = t; i < G::NPOS; i += kPT) ord[i] = uint16_t(i < a.H ? (start ? start[i] : i) : i);
-    for (uint32_t i = nw + t; i < G::NW32; i += kPT) {
-        col[0][i] = ~0u;
-        col[1][i] = ~0u;
+    for (uint32_t i = t; i < R::NPOS; i += kPT) ord[i] = typename R::Ord(i < a.H ? (start ? start[i] : i) : i);
+    if constexpr (R::BYTES) {
+        // padding positions read as allele 1 in both buffers; the last word's
+        // padding bits are set by the padded column store
+        for (uint32_t i = nw * 8u + t; i < R::COLW; i += kPT) {
+            col[0][i] = 0x01010101u;
+            col[1][i] = 0x01010101u;
+        }
+        for (uint32_t i = t; i < nw; i += kPT)
+            store_col_bytes(reinterpret_cast<uint8_t*>(col[0]), i,
+                            i + 1u == nw ? (a.cols32[size_t(m_first) * a.W2 + i] | padm)
+                                         : a.cols32[size_t(m_first) * a.W2 + i]);
+    } else {
+        for (uint32_t i = nw + t; i < R::COLW; i += kPT) {
+            col[0][i] = ~0u;
+            col[1][i] = ~0u;
+        }
+        for (uint32_t i = t; i < nw; i += kPT)
+            store_col_padded(col[0], i, a.cols32[size_t(m_first) * a.W2 + i], nw, padm);
     }
-    for (uint32_t i = t; i < nw; i += kPT) store_col_padded(col[0], i, a.cols32[size_t(m_first) * a.W2 + i], nw, padm);
     __syncthreads();
-    for (uint32_t m = m_first; m + 1 < m_end; ++m) {
-        const uint32_t* cb = col[(m - m_first) & 1u];
-        const bool pre = m + 2 < m_end;
-        ColPrefetch nx;
-        if (pre) nx.fetch(a, m + 1);
-
-        uint32_t hv[K];
-#pragma unroll
-        for (int j = 0; j < K; ++j) hv[j] = ord[base + uint32_t(j) * 64u + lane];
-        uint64_t Zs[K];
-        uint32_t zbits[NZ], zsum = 0;
-#pragma unroll
-        for (int k = 0; k < NZ; ++k) zbits[k] = 0u;
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const uint32_t x = (cb[hv[j] >> 5] >> (hv[j] & 31u)) & 1u;
-            Zs[j] = __ballot(x == 0u);
-            zbits[j >> 5] |= (x ^ 1u) << (j & 31);
-            zsum += uint32_t(__popcll(Zs[j]));
-        }
-        if (lane == 0) wz[w] = zsum;
-        __syncthreads();
-        uint32_t zbefore, ztot;
-        zero_offsets(wz, w, zbefore, ztot);
-        zbefore = __builtin_amdgcn_readfirstlane(zbefore);
-        ztot = __builtin_amdgcn_readfirstlane(ztot);
-        char* row = reinterpret_cast<char*>(a.prefix + size_t(m) * a.H);
-        uint32_t zrun = 0;
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            const uint32_t mb = mbcnt64(Zs[j]);                            // zeros before the lane
-            const uint32_t z0 = zbefore + zrun;                           // the round's first zero
-            const uint32_t o0 = ztot + (base - zbefore) + uint32_t(j) * 64u - zrun;  // and one
-            const uint32_t dst = ((zbits[j >> 5] >> (j & 31)) & 1u) ? z0 + mb : o0 + (lane - mb);
-            ord[dst] = uint16_t(hv[j]);
-            if (dst < a.H) *reinterpret_cast<uint32_t*>(row + dst * 4u) = hv[j];
-            zrun += uint32_t(__popcll(Zs[j]));
-        }
-        if (divs) div_row(a, m);
-        if (pre) {
-            const uint32_t i0 = t, i1 = t + kPT;
-            if (i0 < nw) store_col_padded(col[((m - m_first) & 1u) ^ 1u], i0, nx.w0, nw, padm);
-            if (i1 < nw) store_col_padded(col[((m - m_first) & 1u) ^ 1u], i1, nx.w1, nw, padm);
-        }
-        __syncthreads();
+    // steps alternate between the column buffers (compile-time, so the allele
+    // loads address them by immediate offsets)
+    for (uint32_t m = m_first; m + 1 < m_end; m += 2) {
+        rows_step<K, 0>(a, m, m + 2 < m_end, ord, col, wz, nw, padm, divs);
+        if (m + 2 < m_end) rows_step<K, 1>(a, m + 1, m + 3 < m_end, ord, col, wz, nw, padm, divs);
     }
 }
 
@@ -1309,12 +1395,15 @@ size_t key_bytes(uint32_t H, uint32_t M, uint32_t C) {
 uint32_t pbwt_default_chunk(uint32_t H, uint32_t M) {
     const uint32_t K = pbwt_rounds(H);
     if (K == 0 || K > kRadixMaxK) return 512;  // partition form: its composition walks the chunks in turn
-    // radix form: one round of chunk workgroups (one per CU) on seven eighths
-    // of the CUs, the rest left to the divergence fill; every phase-A and
-    // phase-C step is serial within a chunk, so chunks are as short as that allows
+    // radix form: one round of chunk workgroups (one per CU) on at most seven
+    // eighths of the CUs, the rest left to the divergence fill; every phase-A
+    // and phase-C step is serial within a chunk, so chunks are as short as
+    // that allows, rounded up to a multiple of 128 markers beyond 64 (a
+    // chunk's key is then a whole number of 16-byte words)
     const uint32_t cus = device_cus();
     const uint32_t slots = cus >= 16 ? cus - cus / 8u : 1u;
-    const uint32_t c = (M + slots - 1u) / slots;
+    uint32_t c = (M + slots - 1u) / slots;
+    if (c > 64u) c = (c + 127u) / 128u * 128u;
     return c < 16u ? 16u : c;
 }
 

@@ -82,9 +82,14 @@ def test_pbwt_argument_errors():
     w64 = lib.si_pbwt_chunked_workspace_bytes(20000, 100000, 64)
     assert w64 > w512 >= 196 * 3 * 20000 * 4
     # default chunk: one round of chunks on 7/8 of the compute units (256 when
-    # no device answers, as here: 224 chunks of 447), 512 in the partition form
-    assert lib.si_pbwt_workspace_bytes(20000, 100000) == lib.si_pbwt_chunked_workspace_bytes(20000, 100000, 447)
-    assert lib.si_pbwt_workspace_bytes(20000, 1000) == lib.si_pbwt_chunked_workspace_bytes(20000, 1000, 16)
+    # no device answers, as here: 224 slots), rounded up to a multiple of 128
+    # above 64 markers; 512 in the partition form
+    def ws(H, M, C):
+        return lib.si_pbwt_chunked_workspace_bytes(H, M, C)
+    assert lib.si_pbwt_workspace_bytes(20000, 100000) == ws(20000, 100000, 512)   # 447 -> 512
+    assert lib.si_pbwt_workspace_bytes(20000, 200000) == ws(20000, 200000, 896)   # 893 -> 896
+    assert lib.si_pbwt_workspace_bytes(20000, 10000) == ws(20000, 10000, 45)
+    assert lib.si_pbwt_workspace_bytes(20000, 1000) == ws(20000, 1000, 16)
     assert lib.si_pbwt_workspace_bytes(40000, 100000) == lib.si_pbwt_chunked_workspace_bytes(40000, 100000, 512)
 
 
