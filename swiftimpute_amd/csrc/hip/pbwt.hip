@@ -1079,12 +1079,7 @@ __global__ __launch_bounds__(kPT) void pbwt_chunk_sort_kernel(PbwtArgs a) {
         cntf += uint32_t(start);
     }
     // dense rank of each group in position order: thread counts, block scan
-    uint32_t incl = cntf;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t n = __shfl_up(incl, o, kWave);
-        if (lane >= uint32_t(o)) incl += n;
-    }
+    const uint32_t incl = wave_incl_scan(cntf);
     if (lane == 63u) s_b[w] = incl;
     __syncthreads();
     uint32_t rk = incl - cntf, Gn = 0;
@@ -1170,9 +1165,14 @@ __global__ __launch_bounds__(kPT) void pbwt_scan_kernel(PbwtArgs a, uint32_t d, 
                                                         uint32_t* Rout, const uint16_t* Oin, uint16_t* Oout,
                                                         const uint32_t* Gin, uint32_t* Gout) {
     using G = LdsGeom<K>;
+    // K <= 20: the earlier chunk's ranks staged in LDS (coalesced, beside the
+    // sort) and the new ranks gathered there before one coalesced store,
+    // instead of random global loads and stores; larger orders do not fit
+    constexpr bool STAGE = K <= 20;
     __shared__ __align__(16) uint16_t X[G::NPOS];
     __shared__ __align__(16) uint16_t cnt[16 * kPT];
     __shared__ __align__(4) uint16_t gl[G::NPOS];
+    __shared__ __align__(4) uint16_t lo16[STAGE ? G::NPOS : 2];
     const R4Lds L{cnt, X};
     uint32_t* s_b = reinterpret_cast<uint32_t*>(cnt);  // free after the sort
     const uint32_t c = blockIdx.x, t = threadIdx.x, lane = t & 63u, w = t >> 6, H = a.H;
@@ -1187,13 +1187,15 @@ __global__ __launch_bounds__(kPT) void pbwt_scan_kernel(PbwtArgs a, uint32_t d, 
     }
     BOrd<K> e;
     const uint16_t* lo_ord = Oin + size_t(c - d) * H;
+    const uint32_t* lo = Rin + size_t(c - d) * H;
 #pragma unroll
     for (int j = 0; j < K; ++j)
         e.set(j, p0 + uint32_t(j) < H ? uint32_t(lo_ord[p0 + uint32_t(j)]) : p0 + uint32_t(j));  // padding ids past H
+    if constexpr (STAGE)
+        for (uint32_t i = t; i < H; i += kPT) lo16[i] = uint16_t(lo[i]);  // published by the sort's barriers
     const bool done = c < 2u * d;
     sort_by_ids<K>(e, Rin + size_t(c) * H, Gin[c], H, L, gl, done ? chunk_end_row(a, c) : nullptr);
     // X: the new order by position; gl: R_k[c] of each haplotype
-    const uint32_t* lo = Rin + size_t(c - d) * H;
     uint32_t flags = 0, cntf = 0;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -1203,17 +1205,13 @@ __global__ __launch_bounds__(kPT) void pbwt_scan_kernel(PbwtArgs a, uint32_t d, 
         bool start = pos == 0u;
         if (!start) {
             const uint32_t hp = X[pos - 1u];
-            start = gl[h] != gl[hp] || lo[h] != lo[hp];
+            if constexpr (STAGE) start = gl[h] != gl[hp] || lo16[h] != lo16[hp];
+            else start = gl[h] != gl[hp] || lo[h] != lo[hp];
         }
         flags |= uint32_t(start) << j;
         cntf += uint32_t(start);
     }
-    uint32_t incl = cntf;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t n = __shfl_up(incl, o, kWave);
-        if (lane >= uint32_t(o)) incl += n;
-    }
+    const uint32_t incl = wave_incl_scan(cntf);
     if (lane == 63u) s_b[w] = incl;
     __syncthreads();
     uint32_t rk = incl - cntf, Gn = 0;
@@ -1229,8 +1227,13 @@ __global__ __launch_bounds__(kPT) void pbwt_scan_kernel(PbwtArgs a, uint32_t d, 
         const uint32_t pos = p0 + uint32_t(j);
         if (pos >= H) continue;
         rk += (flags >> j) & 1u;
-        out[e.get(j)] = rk - 1u;
+        if constexpr (STAGE) gl[e.get(j)] = uint16_t(rk - 1u);  // every read of gl ended at the scan's barrier
+        else out[e.get(j)] = rk - 1u;
         oo[pos] = uint16_t(e.get(j));
+    }
+    if constexpr (STAGE) {
+        __syncthreads();
+        for (uint32_t i = t; i < H; i += kPT) out[i] = gl[i];
     }
     if (t == 0) Gout[c] = Gn;
 }
