@@ -817,16 +817,47 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_lds_kernel(PbwtArgs a) {
 // [t*K, t*K + K) (two 16-bit ids per register); a pass counts each thread's
 // digits in its own LDS counters (so the count is stable without any
 // cross-lane matching), scans the digit-major counter array, scatters
-// through an LDS exchange buffer and reads the new order back.
+// through an LDS exchange buffer and reads the new order back (see the pass
+// itself for how the counts are kept).
 // ---------------------------------------------------------------------------
+
+// Transpose of a 64 x 64 bit block held one 64-bit row per lane (lane i,
+// bit j = entry (i, j)): afterwards lane i holds column i.  Six stages, each
+// swapping the off-diagonal sub-blocks of one size between lanes i and i ^ s
+// (row index bit s against column index bit s).
+__device__ __forceinline__ void wave_transpose64(uint32_t& lo, uint32_t& hi, uint32_t lane) {
+    {   // s = 32: lanes below 32 take the partner's low word as their high word
+        const bool up = lane >= 32u;
+        const uint32_t r = __shfl_xor(up ? lo : hi, 32, kWave);
+        if (up) lo = r;
+        else hi = r;
+    }
+#pragma unroll
+    for (int k = 4; k >= 0; --k) {
+        const uint32_t sh = 1u << k;
+        // the columns of each 2s-wide group kept by this lane (low half for a
+        // lane with row bit s clear, high half otherwise)
+        const uint32_t m = k == 4 ? 0x0000ffffu : k == 3 ? 0x00ff00ffu : k == 2 ? 0x0f0f0f0fu
+                         : k == 1 ? 0x33333333u : 0x55555555u;
+        const bool up = (lane & sh) != 0u;
+        const uint32_t keep = up ? ~m : m;
+        const uint32_t ylo = __shfl_xor(lo, int(sh), kWave), yhi = __shfl_xor(hi, int(sh), kWave);
+        lo = (lo & keep) | ((up ? (ylo >> sh) : (ylo << sh)) & ~keep);
+        hi = (hi & keep) | ((up ? (yhi >> sh) : (yhi << sh)) & ~keep);
+    }
+}
 
 // Key bytes of a chunk: workgroup (chunk c, 16 column words = 1024
 // haplotypes), wave w the 64 haplotypes of word b0 + w.  The chunk's columns
-// are staged in LDS 512 markers at a time with coalesced loads; both output
-// layouts are written with coalesced stores; markers past M count as 0.
+// are staged in LDS 512 markers at a time with coalesced loads (rows padded
+// to 17 words, so a lane's row read meets few bank conflicts); a wave then
+// transposes each 64-marker block of its word in registers, lane i ending
+// with haplotype b*64 + i's 8 key bytes, stored as one 8-byte word of its key
+// row and one byte of each of 8 key planes; markers past M count as 0.
 constexpr uint32_t kKeyBlock = 512;  // markers staged per round (64 key bytes)
 __global__ __launch_bounds__(kPT) void pbwt_keys_kernel(PbwtArgs a) {
-    __shared__ uint64_t colw[kKeyBlock][kPW];
+    constexpr uint32_t kRow = kPW + 1;  // padded LDS row, in 64-bit words
+    __shared__ uint64_t colw[kKeyBlock * kRow];
     const uint32_t c = blockIdx.x;
     const uint32_t t = threadIdx.x, lane = t & 63u;
     const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -835,29 +866,26 @@ __global__ __launch_bounds__(kPT) void pbwt_keys_kernel(PbwtArgs a) {
     const uint32_t h = b * 64u + lane;
     const bool valid = b < W && h < a.H;
     const uint32_t m0 = c * a.C, len = min(a.M, m0 + a.C) - m0;
-    uint8_t* tp = a.tp + size_t(c) * a.NP * a.H;
+    uint8_t* tp = a.tp + size_t(c) * a.NP * a.H + (valid ? h : 0u);
     uint8_t* th = a.th + (size_t(c) * a.H + (valid ? h : 0u)) * a.TS;
     for (uint32_t mb = 0; mb < a.TS * 8u; mb += kKeyBlock) {
         __syncthreads();
         for (uint32_t i = t; i < kKeyBlock * kPW; i += kPT) {
             const uint32_t mr = mb + i / kPW, bb = b0 + i % kPW;
-            colw[i / kPW][i % kPW] = (mr < len && bb < W) ? cols[size_t(m0 + mr) * W + bb] : 0ull;
+            colw[(i / kPW) * kRow + i % kPW] = (mr < len && bb < W) ? cols[size_t(m0 + mr) * W + bb] : 0ull;
         }
         __syncthreads();
-        for (uint32_t p0 = 0; p0 < kKeyBlock / 8u && (mb / 8u + p0) < a.TS; p0 += 16) {
-            uint32_t v[4] = {0u, 0u, 0u, 0u};
+        for (uint32_t sb = 0; sb < kKeyBlock / 64u && mb / 8u + 8u * sb < a.TS; ++sb) {
+            const uint64_t x = colw[(64u * sb + lane) * kRow + w];  // marker mb + 64 sb + lane
+            uint32_t lo = uint32_t(x), hi = uint32_t(x >> 32);
+            wave_transpose64(lo, hi, lane);
+            const uint32_t p0 = mb / 8u + 8u * sb;  // first key byte of the block
+            if (valid) {
+                *reinterpret_cast<uint2*>(th + p0) = make_uint2(lo, hi);
 #pragma unroll
-            for (uint32_t q = 0; q < 16; ++q) {
-                const uint32_t pr = p0 + q;  // byte within the staged block
-                uint32_t byte = 0;
-#pragma unroll
-                for (uint32_t k = 0; k < 8; ++k)
-                    byte |= uint32_t((colw[8u * pr + k][w] >> lane) & 1u) << k;
-                const uint32_t p = mb / 8u + pr;
-                if (valid && p < a.NP) tp[size_t(p) * a.H + h] = uint8_t(byte);
-                v[q >> 2] |= byte << (8u * (q & 3u));
+                for (uint32_t q = 0; q < 8; ++q)
+                    if (p0 + q < a.NP) tp[size_t(p0 + q) * a.H] = uint8_t((q < 4 ? lo : hi) >> (8u * (q & 3u)));
             }
-            if (valid) *reinterpret_cast<uint4*>(th + mb / 8u + p0) = make_uint4(v[0], v[1], v[2], v[3]);
         }
     }
 }
