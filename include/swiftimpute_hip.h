@@ -59,7 +59,7 @@ enum si_status {
 const char* si_last_error(void);
 int si_abi_version(void);
 
-/* Padded state count the kernels use: the next of 4, 8, 16, 32, 64 >= L. */
+/* Padded state count the kernels use: the next of 4, 8, 16, 32, 64, 128 >= L. */
 uint32_t si_padded_states(uint32_t L);
 
 /* ------------------------------------------------------------------ PBWT --
@@ -102,11 +102,14 @@ int si_select_perm(uint32_t H, uint32_t L, uint32_t* perm);
  *   L <= 32 and H <= 10000 (select_states_warp_kernel, :212-312): prefix
  *     positions 0, 32, 64, 96 (as many as lane 0 holds, min(4, ceil(H/32))),
  *     then haplotypes 4, 5, ..., flagged SI_PERM_FIXED;
- *   otherwise (select_states_bitonic_kernel, :320-394): positions 0..L-1.
+ *   otherwise, L <= 64 (select_states_bitonic_kernel, :320-394) and L > 64
+ *     (select_states_kernel, :47-204): positions 0..L-1.
  * For H < 128 the reference's warp kernel shuffles across lanes that have
- * left its loop (undefined); this function takes lane 0's entries.
- * perm receives min(L, H) entries; L <= 64 (the reference's third kernel,
- * for L > 64, is outside the supported state counts). */
+ * left its loop (undefined); this function takes lane 0's entries.  Where a
+ * thread's range of the reference's L > 64 kernel holds fewer than L
+ * haplotypes, its merge also reads slots that thread never wrote (undefined);
+ * this function takes them as empty.  perm receives min(L, H) entries;
+ * L <= 128. */
 int si_select_perm_gpu_compat(uint32_t H, uint32_t L, uint32_t* perm);
 
 /* Device: L-best state selection + pairing + allele lookup, replacing the
@@ -197,7 +200,7 @@ enum si_fb_flags {
 typedef struct si_fb_job {
     uint32_t n_samples;   /* B */
     uint32_t n_markers;   /* M */
-    uint32_t n_states;    /* L (1..64) */
+    uint32_t n_states;    /* L (1..128) */
     uint32_t flags;       /* si_fb_flags */
     uint32_t code_pitch;  /* row pitch of codes, >= M, multiple of 16 */
     uint32_t n_palette;   /* P (1..64) */

@@ -50,7 +50,7 @@ def lib():
         L.or_state_table.argtypes = [_u8p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
                                      ctypes.c_uint32, ctypes.c_int, _u32p, _u32p, _u8p, _u8p]
         for name in ("or_select_states_gpu_warp", "or_select_states_gpu_insert",
-                     "or_select_states_gpu"):
+                     "or_select_states_gpu_block", "or_select_states_gpu"):
             getattr(L, name).argtypes = [_u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32, _u32p]
         L.or_state_table_gpu.argtypes = [_u8p, _u32p, _u32p, ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, _u32p, _u32p, _u8p, _u8p]
@@ -176,13 +176,14 @@ def state_table(ref: np.ndarray, prefix: np.ndarray, L: int, simple_mode: bool =
 
 def select_states_gpu(prefix_row: np.ndarray, div_row: np.ndarray, L: int,
                       kernel: str = "dispatch") -> np.ndarray:
-    """The reference's GPU selection at one marker (pbwt_selector.cu:212-463):
-    ``kernel`` is "warp", "insert" or "dispatch" (launch_select_states' pick)."""
+    """The reference's GPU selection at one marker (pbwt_selector.cu:47-463):
+    ``kernel`` is "warp", "insert", "block" or "dispatch" (launch_select_states'
+    pick)."""
     prefix_row = np.ascontiguousarray(prefix_row, np.uint32)
     div_row = np.ascontiguousarray(div_row, np.uint32)
     out = np.empty(L, np.uint32)
     fn = {"warp": "or_select_states_gpu_warp", "insert": "or_select_states_gpu_insert",
-          "dispatch": "or_select_states_gpu"}[kernel]
+          "block": "or_select_states_gpu_block", "dispatch": "or_select_states_gpu"}[kernel]
     _chk(getattr(lib(), fn)(_p(prefix_row, _u32p), _p(div_row, _u32p), prefix_row.size, L,
                             _p(out, _u32p)), fn)
     return out

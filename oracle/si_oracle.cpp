@@ -267,15 +267,65 @@ int or_select_states_gpu_insert(const uint32_t* prefix_row, const uint32_t* div_
 }
 
 // ---------------------------------------------------------------------------
+// A3: select_states_kernel (src/pbwt/pbwt_selector.cu:47-204) for one marker,
+// L > 64.  256 threads; thread t scans positions [t*c, min(t*c + c, H)),
+// c = ceil(H/256): its first L entries go into a local list with the same
+// insertion (before the first strictly smaller divergence, :84-105), later
+// ones only when strictly better than the list's last (:108-128).  Thread 0
+// then merges the lists in thread order with the same rule (:139-186) and
+// fills slots past the count with i % H (:189-199).  A thread whose range
+// holds fewer than L entries leaves the rest of its slots unwritten and the
+// merge reads them (undefined in the reference); here they hold nothing.
+// The reference keeps its merge list in a MAX_STATES_PER_BLOCK = 256 array.
+// ---------------------------------------------------------------------------
+namespace {
+void top_insert(std::vector<uint32_t>& td, std::vector<uint32_t>& th, uint32_t L, uint32_t dv,
+                uint32_t hap) {
+    const uint32_t n = uint32_t(td.size());
+    if (n < L) {
+        uint32_t pos = n;
+        for (uint32_t j = 0; j < n; ++j)
+            if (dv > td[j]) { pos = j; break; }
+        td.insert(td.begin() + pos, dv);
+        th.insert(th.begin() + pos, hap);
+    } else if (dv > td[L - 1]) {
+        uint32_t pos = L - 1;
+        for (uint32_t j = 0; j + 1 < L; ++j)
+            if (dv > td[j]) { pos = j; break; }
+        td.insert(td.begin() + pos, dv);
+        th.insert(th.begin() + pos, hap);
+        td.pop_back();
+        th.pop_back();
+    }
+}
+}  // namespace
+
+int or_select_states_gpu_block(const uint32_t* prefix_row, const uint32_t* div_row, uint32_t H,
+                               uint32_t L, uint32_t* out) {
+    if (H == 0 || L == 0 || L > 256) return -1;
+    constexpr uint32_t kThreads = 256;
+    const uint32_t c = (H + kThreads - 1) / kThreads;
+    std::vector<uint32_t> gd, gh;
+    for (uint32_t t = 0; t < kThreads; ++t) {
+        std::vector<uint32_t> td, th;
+        const uint32_t b = t * c, e = std::min(H, b + c);
+        for (uint32_t i = b; i < e; ++i) top_insert(td, th, L, div_row[i], prefix_row[i]);
+        for (size_t k = 0; k < td.size(); ++k) top_insert(gd, gh, L, td[k], th[k]);
+    }
+    for (uint32_t i = 0; i < L; ++i) out[i] = i < gh.size() ? gh[i] : i % H;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // A3 dispatch: launch_select_states (src/pbwt/pbwt_selector.cu:400-463) picks
-// the warp kernel for L <= 32 and H <= 10000, else the insertion kernel for
-// L <= 64 (the L > 64 kernel is not restated: -1).
+// the warp kernel for L <= 32 and H <= 10000, the insertion kernel for L <= 64
+// and the block kernel above.
 // ---------------------------------------------------------------------------
 int or_select_states_gpu(const uint32_t* prefix_row, const uint32_t* div_row, uint32_t H,
                          uint32_t L, uint32_t* out) {
     if (L <= 32 && H <= 10000) return or_select_states_gpu_warp(prefix_row, div_row, H, L, out);
     if (L <= 64) return or_select_states_gpu_insert(prefix_row, div_row, H, L, out);
-    return -1;
+    return or_select_states_gpu_block(prefix_row, div_row, H, L, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -286,7 +336,7 @@ int or_select_states_gpu(const uint32_t* prefix_row, const uint32_t* div_row, ui
 int or_state_table_gpu(const uint8_t* ref, const uint32_t* prefix, const uint32_t* div,
                        uint32_t M, uint32_t H, uint32_t L, uint32_t* h0_out, uint32_t* h1_out,
                        uint8_t* a0_out, uint8_t* a1_out) {
-    if (H == 0 || L == 0 || L > 64) return -1;
+    if (H == 0 || L == 0 || L > 256) return -1;
     std::vector<uint32_t> sel(L);
     for (uint32_t m = 0; m < M; ++m) {
         const int rc = or_select_states_gpu(prefix + size_t(m) * H, div + size_t(m) * H, H, L,

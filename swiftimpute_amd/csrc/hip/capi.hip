@@ -113,16 +113,21 @@ int si_select_perm(uint32_t H, uint32_t L, uint32_t* perm) {
 }
 
 int si_select_perm_gpu_compat(uint32_t H, uint32_t L, uint32_t* perm) {
-    if (H == 0 || L == 0 || L > 64 || !perm)
-        return fail(SI_ERR_ARG, "si_select_perm_gpu_compat: need H > 0, 1 <= L <= 64, perm");
+    if (H == 0 || L == 0 || L > si::kMaxStates || !perm)
+        return fail(SI_ERR_ARG, "si_select_perm_gpu_compat: need H > 0, 1 <= L <= %u, perm", si::kMaxStates);
     // launch_select_states (src/pbwt/pbwt_selector.cu:400-463) with every
     // divergence in a row equal.  Warp kernel (L <= 32, H <= 10000,
     // :212-312): each lane keeps its first four strided entries, every round's
     // maximum is tied and lane 0 wins it, so round r emits lane 0's r-th entry,
     // prefix position 32r, for r below lane 0's count min(4, ceil(H/32)); the
     // rest is filled with haplotype k (k % H, :305-310).  Insertion kernel
-    // (:320-394): ties never displace, so the first min(L, H) positions in
-    // prefix order.
+    // (:320-394, 32 < L <= 64 or H > 10000): ties never displace, so the
+    // first min(L, H) positions in prefix order.  Block kernel (L > 64,
+    // :47-204): thread t keeps the first min(L, n_t) positions of its range
+    // [t*ceil(H/256), ...) in order (ties never displace), and the merge in
+    // thread order appends them until L: again the first min(L, H) positions
+    // (a thread whose range holds fewer than L leaves the rest of its slots
+    // unwritten there and the merge reads them; taken here as empty).
     const uint32_t n = std::min(L, H);
     if (L <= 32 && H <= 10000) {
         const uint32_t c0 = std::min(4u, (H + 31) / 32);
@@ -136,8 +141,9 @@ int si_select_perm_gpu_compat(uint32_t H, uint32_t L, uint32_t* perm) {
 int si_state_table(const uint32_t* prefix, const uint64_t* cols, const uint64_t* miss, uint32_t H,
                    uint32_t M, uint32_t L, uint32_t mode, const uint32_t* perm_host,
                    uint32_t* states, uint8_t* sa, uint8_t* mmask, void* stream) {
-    if (H == 0 || L == 0 || L > 64)
-        return fail(SI_ERR_ARG, "si_state_table: need H > 0 and 1 <= L <= 64 (got H=%u L=%u)", H, L);
+    if (H == 0 || L == 0 || L > si::kMaxStates)
+        return fail(SI_ERR_ARG, "si_state_table: need H > 0 and 1 <= L <= %u (got H=%u L=%u)", si::kMaxStates,
+                    H, L);
     if (mode > 1) return fail(SI_ERR_ARG, "si_state_table: unknown mode %u", mode);
     if (mode == 1 && 2ull * L > H)
         return fail(SI_ERR_ARG, "si_state_table: simple selection needs 2L <= H");
@@ -220,8 +226,8 @@ int si_emission_table(const float* palette, uint32_t P, float* eps, float* emax)
 
 // -------------------------------------------------------- forward-backward --
 static int check_job(const si_fb_job& j, uint32_t idx) {
-    if (j.n_states == 0 || j.n_states > 64)
-        return fail(SI_ERR_ARG, "job %u: n_states must be in 1..64 (got %u)", idx, j.n_states);
+    if (j.n_states == 0 || j.n_states > si::kMaxStates)
+        return fail(SI_ERR_ARG, "job %u: n_states must be in 1..%u (got %u)", idx, si::kMaxStates, j.n_states);
     if (j.n_palette == 0 || j.n_palette > si::kMaxPalette)
         return fail(SI_ERR_UNSUPPORTED, "job %u: palette size %u outside 1..%u", idx, j.n_palette,
                     si::kMaxPalette);
@@ -263,15 +269,15 @@ size_t si_fb_workspace_bytes(const si_fb_job* jobs, uint32_t n_jobs) {
 }
 
 int si_fb_default_variant(uint32_t n_states, si_fb_variant* out) {
-    if (!out || n_states == 0 || n_states > 64)
-        return fail(SI_ERR_ARG, "si_fb_default_variant: n_states must be in 1..64");
+    if (!out || n_states == 0 || n_states > si::kMaxStates)
+        return fail(SI_ERR_ARG, "si_fb_default_variant: n_states must be in 1..%u", si::kMaxStates);
     const si::FbVariant v = si::fb_default_variant(si_padded_states(n_states));
     *out = si_fb_variant{v.gc, v.gb, v.cb};
     return SI_OK;
 }
 
 uint32_t si_fb_lane_candidates(uint32_t n_states, uint32_t pass, uint32_t* lanes, uint32_t max) {
-    if (n_states == 0 || n_states > 64 || (pass != SI_FB_PASS_CHAIN && pass != SI_FB_PASS_BLOCK))
+    if (n_states == 0 || n_states > si::kMaxStates || (pass != SI_FB_PASS_CHAIN && pass != SI_FB_PASS_BLOCK))
         return 0;
     return si::fb_candidates(si_padded_states(n_states), pass == SI_FB_PASS_CHAIN ? 0 : 1, lanes,
                              lanes ? max : 0);

@@ -14,8 +14,9 @@ namespace si {
 // State table: one thread per marker.
 // ---------------------------------------------------------------------------
 struct PermArg {
-    uint32_t v[64];
+    uint32_t v[kMaxStates];  // a kernel argument: every selected slot's prefix position
 };
+static_assert(sizeof(PermArg) <= 1024, "state permutation fits the kernel arguments");
 
 __device__ __forceinline__ uint32_t allele_raw(const uint64_t* cols, const uint64_t* miss,
                                                uint32_t W, uint32_t m, uint32_t h) {
@@ -75,6 +76,7 @@ int launch_state_table(const uint32_t* prefix, const uint64_t* cols, const uint6
                        const uint32_t* perm_host, uint32_t* states, uint8_t* sa, uint8_t* mmask,
                        hipStream_t stream) {
     if (M == 0) return 0;
+    if (L > kMaxStates) return SI_ERR_ARG;  // the permutation argument holds kMaxStates slots
     PermArg p{};
     if (mode == 0) {
         const uint32_t n = L < H ? L : H;

@@ -106,7 +106,7 @@ def read_outputs(d, pan, B, L):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("per_marker,spacing,L,batch", [(False, 0, 8, 100), (True, 1, 16, 7),
-                                                         (True, 1, 5, 1000)])
+                                                         (True, 1, 5, 1000), (False, 1, 100, 9)])
 def test_cpp_api_against_oracle(tmp_path, per_marker, spacing, L, batch):
     pan = synth.make_panel(1000, 600, seed=31, spacing_mode=spacing)
     tg = synth.make_targets(pan, 23, 4, seed=32)

@@ -64,18 +64,18 @@ class RegionJob:
         return {k: v.cpu().numpy() for k, v in self.out.items()}
 
 
-def make_jobs(seed=5, n_jobs=14):
+def make_jobs(seed=5, n_jobs=14, states=(4, 5, 8, 16), max_M=2500, max_B=300):
     rng = np.random.default_rng(seed)
     pan = synth.make_panel(1500, 2600, seed=seed, spacing_mode=1)
     tg = synth.make_targets(pan, 300, 10, seed=seed + 1)
     tables = device.SharedTables(synth.HARDCALL_PALETTE)
-    idx = {L: device.DeviceIndex(pan.cols, pan.H, L, keep_prefix=False) for L in (4, 5, 8, 16)}
+    idx = {L: device.DeviceIndex(pan.cols, pan.H, L, keep_prefix=False) for L in states}
     jobs = []
     for n in range(n_jobs):
-        L = int(rng.choice([4, 5, 8, 16]))
-        M = int(np.exp(rng.uniform(np.log(1), np.log(2500))))
+        L = int(rng.choice(list(states)))
+        M = int(np.exp(rng.uniform(np.log(1), np.log(max_M))))
         m0 = int(rng.integers(0, pan.M - M + 1))
-        B = int(np.exp(rng.uniform(0, np.log(300))))
+        B = int(np.exp(rng.uniform(0, np.log(max_B))))
         s0 = int(rng.integers(0, tg.B - B + 1))
         jobs.append(RegionJob(idx[L], pan, m0, m0 + M, tg.codes[s0:s0 + B], 10000, tables,
                               seed=1000 + n, logspace=bool(n % 5 == 4)))
@@ -100,6 +100,25 @@ def test_batch_equals_plain_eager():
         got = rj.result()
         for k in r:
             assert np.array_equal(got[k], r[k]), (k, rj.B, rj.M, rj.idx.L)
+    b.close()
+
+
+def test_batch_large_state_classes_equal_plain():
+    # 65-128 states (padded to 128: 16 or 32 lanes of 8 or 4 states) beside
+    # a 64-state class, eager and from a graph
+    pan, jobs = make_jobs(seed=21, n_jobs=7, states=(40, 65, 100, 128), max_M=900, max_B=70)
+    ref = plain_results(jobs)
+    assert {rj.idx.Lp for rj in jobs} >= {64, 128}
+    b = device.Batch([rj.cjob() for rj in jobs])
+    stream = torch.cuda.Stream()
+    for graph in (False, True):
+        with torch.cuda.stream(stream):
+            b.run(use_graph=graph, stream=stream)
+        stream.synchronize()
+        for rj, r in zip(jobs, ref):
+            got = rj.result()
+            for k in r:
+                assert np.array_equal(got[k], r[k]), (graph, k, rj.B, rj.M, rj.idx.L)
     b.close()
 
 

@@ -160,6 +160,30 @@ def test_gpu_insert_selection_closed_form(H, L):
         assert oracle.select_states_gpu(p, div[m], L).tolist() == want
 
 
+@pytest.mark.parametrize("H,L", [(300, 65), (20000, 100), (40, 128), (33000, 128)])
+def test_gpu_block_selection_closed_form(H, L):
+    # L > 64 (the block kernel): ties never displace, so again the first
+    # min(L, H) prefix positions, then i % H
+    _, prefix, div = _pbwt_rows(H)
+    for m in range(prefix.shape[0]):
+        p = prefix[m]
+        want = [int(p[i]) if i < H else i % H for i in range(L)]
+        assert oracle.select_states_gpu(p, div[m], L, "block").tolist() == want
+        assert oracle.select_states_gpu(p, div[m], L).tolist() == want
+
+
+@pytest.mark.parametrize("H,L", [(256 * 80, 80), (256 * 100 + 17, 100)])
+def test_gpu_block_selection_distinct_divergences_is_top_l(H, L):
+    # with every thread's range holding at least L entries and distinct
+    # divergences, the per-thread top-L lists and their merge give the global
+    # top L by divergence, largest first (brute force: a full sort)
+    rng = np.random.default_rng(H)
+    div = rng.permutation(H * 3)[:H].astype(np.uint32)
+    prefix = rng.permutation(H).astype(np.uint32)
+    order = np.argsort(-div.astype(np.int64), kind="stable")[:L]
+    assert oracle.select_states_gpu(prefix, div, L, "block").tolist() == prefix[order].tolist()
+
+
 def test_gpu_warp_selection_hand_traced_rounds():
     p = np.arange(1000, 1160, dtype=np.uint32)  # prefix row: position i holds haplotype 1000+i
     # a larger divergence at lane 5's first entry wins only the first selection pass
