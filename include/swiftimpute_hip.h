@@ -59,7 +59,7 @@ enum si_status {
 const char* si_last_error(void);
 int si_abi_version(void);
 
-/* Padded state count the kernels use: the next of 4, 8, 16, 32, 64, 128 >= L. */
+/* Padded state count the kernels use: the next of 4, 8, 16, ..., 256 >= L. */
 uint32_t si_padded_states(uint32_t L);
 
 /* ------------------------------------------------------------------ PBWT --
@@ -109,7 +109,7 @@ int si_select_perm(uint32_t H, uint32_t L, uint32_t* perm);
  * thread's range of the reference's L > 64 kernel holds fewer than L
  * haplotypes, its merge also reads slots that thread never wrote (undefined);
  * this function takes them as empty.  perm receives min(L, H) entries;
- * L <= 128. */
+ * L <= 256. */
 int si_select_perm_gpu_compat(uint32_t H, uint32_t L, uint32_t* perm);
 
 /* Device: L-best state selection + pairing + allele lookup, replacing the
@@ -200,7 +200,7 @@ enum si_fb_flags {
 typedef struct si_fb_job {
     uint32_t n_samples;   /* B */
     uint32_t n_markers;   /* M */
-    uint32_t n_states;    /* L (1..128) */
+    uint32_t n_states;    /* L (1..256) */
     uint32_t flags;       /* si_fb_flags */
     uint32_t code_pitch;  /* row pitch of codes, >= M, multiple of 16 */
     uint32_t n_palette;   /* P (1..64) */

@@ -41,9 +41,10 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{INCLUDE}",
 HIP_SOURCES = ["hip/capi.hip", "hip/index.hip", "hip/pbwt.hip", "hip/hmm.hip"]
 HIP_HEADERS = ["hip/si_device.hpp", "hip/si_internal.hpp", "hip/fb_kernels.hpp"]
 # Forward-backward lane shapes (padded states Lp, lanes per sample G), one
-# object each from hip/fb_inst.hip: 8, 4 and 2 states per lane, at most 32 lanes.
+# object each from hip/fb_inst.hip: 8, 4 and 2 states per lane, at most 64 lanes.
 FB_SHAPES = [(4, 1), (4, 2), (8, 1), (8, 2), (8, 4), (16, 2), (16, 4), (16, 8),
-             (32, 4), (32, 8), (32, 16), (64, 8), (64, 16), (64, 32), (128, 16), (128, 32)]
+             (32, 4), (32, 8), (32, 16), (64, 8), (64, 16), (64, 32), (128, 16), (128, 32),
+             (256, 32), (256, 64)]
 API_SOURCES = ["api/panel.cpp", "api/imputer.cpp", "api/synth.cpp", "api/server.cpp"]
 
 

@@ -143,7 +143,7 @@ struct Imputer::Device {
 Imputer::Imputer(const ReferencePanel& reference, const ImputationConfig& config)
     : reference_(reference), config_(config), dev_(std::make_unique<Device>()) {
     const uint32_t L = config_.hmm_params.num_states;
-    if (L == 0 || L > 128) throw ImputationError("num_states must be in 1..128");
+    if (L == 0 || L > 256) throw ImputationError("num_states must be in 1..256");
     if (reference_.num_haplotypes() == 0) throw ImputationError("reference panel has no haplotypes");
     int n = 0;
     hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");

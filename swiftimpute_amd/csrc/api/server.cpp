@@ -678,7 +678,7 @@ int si_server_create(const si_server_cfg* cfg, const uint64_t* cols, uint32_t H,
                      const uint64_t* pos, const double* cM, si_server** out) {
     if (!cfg || !cols || !pos || !out || H == 0 || M == 0)
         return fail(SI_ERR_ARG, "si_server_create: bad arguments");
-    if (cfg->n_states == 0 || cfg->n_states > 128) return fail(SI_ERR_ARG, "n_states must be in 1..128");
+    if (cfg->n_states == 0 || cfg->n_states > 256) return fail(SI_ERR_ARG, "n_states must be in 1..256");
     if (cfg->selection > 1) return fail(SI_ERR_ARG, "selection must be 0 (CPU path) or 1 (GPU kernels)");
     if (cfg->region_resweep > 1) return fail(SI_ERR_ARG, "region_resweep must be 0 or 1");
     if (cfg->plain > 1) return fail(SI_ERR_ARG, "plain must be 0 or 1");

@@ -71,7 +71,7 @@ struct Shape {
     static constexpr int NW = R_ >= 4 ? R_ / 4 : 1;  // state-table words a lane loads per marker
     static constexpr int GPW = kWave / G_;           // lane groups per wavefront
     static constexpr int GPB = kFbThreads / G_;      // lane groups per workgroup
-    static_assert(G_ >= 1 && G_ <= 32 && R_ >= 2 && R_ <= 8, "lane shape");
+    static_assert(G_ >= 1 && G_ <= 64 && R_ >= 2 && R_ <= 8, "lane shape");
 };
 
 // Block-pass LDS of one lane group: posteriors [16][T] floats (row t at
@@ -235,7 +235,9 @@ __device__ __forceinline__ float gsum(float v) {
         return group_sum<G>(v);
     } else {
         v = group_sum<16>(v);
-        return v + __shfl_xor(v, 16, kWave);
+        v += __shfl_xor(v, 16, kWave);
+        if constexpr (G >= 64) v += __shfl_xor(v, 32, kWave);
+        return v;
     }
 }
 template <int G>
@@ -244,7 +246,9 @@ __device__ __forceinline__ float gmax(float v) {
         return group_max<G>(v);
     } else {
         v = group_max<16>(v);
-        return fmaxf(v, __shfl_xor(v, 16, kWave));
+        v = fmaxf(v, __shfl_xor(v, 16, kWave));
+        if constexpr (G >= 64) v = fmaxf(v, __shfl_xor(v, 32, kWave));
+        return v;
     }
 }
 
@@ -263,6 +267,10 @@ __device__ __forceinline__ void gsum2(float& a, float& b) {
     if constexpr (G >= 32) {
         a += __shfl_xor(a, 16, kWave);
         b += __shfl_xor(b, 16, kWave);
+    }
+    if constexpr (G >= 64) {
+        a += __shfl_xor(a, 32, kWave);
+        b += __shfl_xor(b, 32, kWave);
     }
 }
 template <class S>

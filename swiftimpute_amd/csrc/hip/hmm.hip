@@ -47,7 +47,7 @@ uint32_t fb_candidates(uint32_t Lp, int pass, uint32_t* lanes, uint32_t max) {
     const uint32_t Rs[3] = {8, 4, 2};
     for (uint32_t R : Rs) {
         if (pass == 1 && R > 4) continue;
-        if (R > Lp || Lp / R > 32) continue;
+        if (R > Lp || Lp / R > 64) continue;
         const FbImpl* f = fb_impl(Lp, Lp / R);
         if (!f || (pass == 1 && !f->has_block)) continue;
         if (n < max) lanes[n] = Lp / R;

@@ -16,7 +16,7 @@ namespace si {
 struct PermArg {
     uint32_t v[kMaxStates];  // a kernel argument: every selected slot's prefix position
 };
-static_assert(sizeof(PermArg) <= 1024, "state permutation fits the kernel arguments");
+static_assert(sizeof(PermArg) <= 1024, "state permutation fits the kernel arguments (4 KiB)");
 
 __device__ __forceinline__ uint32_t allele_raw(const uint64_t* cols, const uint64_t* miss,
                                                uint32_t W, uint32_t m, uint32_t h) {

@@ -11,8 +11,8 @@ namespace si {
 
 // Smallest checkpoint block of the forward-backward passes (markers).
 constexpr uint32_t kMinCkptBlock = 16;
-// Largest state count L (padded to 128: 16 or 32 lanes of 8 or 4 states).
-constexpr uint32_t kMaxStates = 128;
+// Largest state count L (padded to 256: 32 or 64 lanes of 8 or 4 states).
+constexpr uint32_t kMaxStates = 256;
 // Largest genotype-likelihood palette the kernels stage in LDS.
 constexpr uint32_t kMaxPalette = 64;
 

@@ -96,8 +96,8 @@ def test_pbwt_argument_errors():
 def test_abi_version_and_padding():
     lib = _native.hip()
     assert lib.si_abi_version() == 1
-    assert [lib.si_padded_states(L) for L in (1, 4, 5, 8, 9, 16, 17, 33, 64, 65, 128)] == \
-        [4, 4, 8, 8, 16, 16, 32, 64, 64, 128, 128]
+    assert [lib.si_padded_states(L) for L in (1, 4, 5, 8, 9, 16, 17, 33, 64, 65, 128, 129, 256)] == \
+        [4, 4, 8, 8, 16, 16, 32, 64, 64, 128, 128, 256, 256]
 
 
 @pytest.mark.parametrize("H,L", [(1, 8), (16, 16), (4432, 8), (20000, 16), (8000, 8), (40000, 8)])
@@ -116,7 +116,7 @@ def _gather(prefix_row, perm, L):
 
 @pytest.mark.parametrize("H,L", [(128, 1), (128, 8), (200, 32), (4432, 8), (10000, 16), (10001, 8),
                                  (20000, 16), (300, 33), (300, 64), (40, 8), (70, 12), (5, 8),
-                                 (1, 3), (300, 65), (20000, 100), (90, 128)])
+                                 (1, 3), (300, 65), (20000, 100), (90, 128), (70000, 256)])
 def test_gpu_compat_perm_matches_the_reference_kernels(H, L):
     # the permutation gathered from each PBWT row equals the oracle's lane-by-lane
     # run of the reference's GPU selection kernels on that row and its divergences
@@ -197,7 +197,7 @@ def _job(**kw):
 
 
 @pytest.mark.parametrize("kw,code", [
-    (dict(n_states=0), -1), (dict(n_states=129), -1), (dict(n_palette=0), -3),
+    (dict(n_states=0), -1), (dict(n_states=257), -1), (dict(n_palette=0), -3),
     (dict(n_palette=65), -3), (dict(code_pitch=8), -1), (dict(code_pitch=17), -1),
     (dict(flags=4), -1),  # log-space needs the palette
 ])
@@ -230,8 +230,8 @@ def test_state_table_argument_errors():
         assert b"outside the panel" in lib.si_last_error()
     q = np.zeros(4, np.uint32)
     assert lib.si_select_perm_gpu_compat(0, 8, q.ctypes.data_as(_native._u32p)) == -1
-    assert lib.si_select_perm_gpu_compat(10, 129, q.ctypes.data_as(_native._u32p)) == -1
-    assert lib.si_state_table(fake, fake, None, 10, 10, 129, 0, p32, None, fake, fake, None) == -1
+    assert lib.si_select_perm_gpu_compat(10, 257, q.ctypes.data_as(_native._u32p)) == -1
+    assert lib.si_state_table(fake, fake, None, 10, 10, 257, 0, p32, None, fake, fake, None) == -1
 
 
 def test_short_window_is_refused():
@@ -260,7 +260,9 @@ def test_variant_queries():
     assert lib.si_fb_default_variant(16, ctypes.byref(v)) == 0
     assert v.as_tuple() == (4, 4, 16)
     assert lib.si_fb_default_variant(0, ctypes.byref(v)) == _native.SI_ERR_ARG
-    assert lib.si_fb_default_variant(129, ctypes.byref(v)) == _native.SI_ERR_ARG
+    assert lib.si_fb_default_variant(257, ctypes.byref(v)) == _native.SI_ERR_ARG
+    assert lib.si_fb_default_variant(200, ctypes.byref(v)) == 0
+    assert v.as_tuple() == (64, 64, 16)
     assert lib.si_fb_default_variant(100, ctypes.byref(v)) == 0
     assert v.as_tuple() == (32, 32, 16)
     assert device.lane_candidates(16, _native.SI_FB_PASS_CHAIN) == [2, 4, 8]
@@ -268,9 +270,11 @@ def test_variant_queries():
     assert device.lane_candidates(64, _native.SI_FB_PASS_CHAIN) == [8, 16, 32]
     assert device.lane_candidates(128, _native.SI_FB_PASS_CHAIN) == [16, 32]
     assert device.lane_candidates(128, _native.SI_FB_PASS_BLOCK) == [32]
+    assert device.lane_candidates(256, _native.SI_FB_PASS_CHAIN) == [32, 64]
+    assert device.lane_candidates(256, _native.SI_FB_PASS_BLOCK) == [64]
     assert device.lane_candidates(3, _native.SI_FB_PASS_BLOCK) == [1, 2]
     assert lib.si_fb_lane_candidates(16, 3, None, 0) == 0  # not a single pass
-    for L in (1, 4, 5, 8, 9, 16, 17, 32, 33, 64, 65, 128):
+    for L in (1, 4, 5, 8, 9, 16, 17, 32, 33, 64, 65, 128, 129, 256):
         vs = device.variants(L)
         assert device.default_variant(L) in vs
         assert {cb for *_, cb in vs} == {16}

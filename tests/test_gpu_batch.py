@@ -104,11 +104,11 @@ def test_batch_equals_plain_eager():
 
 
 def test_batch_large_state_classes_equal_plain():
-    # 65-128 states (padded to 128: 16 or 32 lanes of 8 or 4 states) beside
-    # a 64-state class, eager and from a graph
-    pan, jobs = make_jobs(seed=21, n_jobs=7, states=(40, 65, 100, 128), max_M=900, max_B=70)
+    # 65-256 states (padded to 128 or 256: 16 to 64 lanes of 8 or 4 states)
+    # beside a 64-state class, eager and from a graph
+    pan, jobs = make_jobs(seed=21, n_jobs=9, states=(40, 65, 100, 128, 200, 256), max_M=900, max_B=70)
     ref = plain_results(jobs)
-    assert {rj.idx.Lp for rj in jobs} >= {64, 128}
+    assert {rj.idx.Lp for rj in jobs} >= {64, 128, 256}
     b = device.Batch([rj.cjob() for rj in jobs])
     stream = torch.cuda.Stream()
     for graph in (False, True):

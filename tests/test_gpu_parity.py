@@ -127,7 +127,7 @@ def test_pbwt_config1_panel():
 
 # ---------------------------------------------------------------- state table
 @pytest.mark.parametrize("H,L", [(4432, 8), (300, 16), (100, 64), (5, 8), (3, 3), (1, 4),
-                                 (77, 5), (200, 12), (300, 100), (200, 128), (90, 128)])
+                                 (77, 5), (200, 12), (300, 100), (200, 128), (90, 128), (600, 256)])
 def test_state_table(H, L):
     rng = np.random.default_rng(H + L)
     M = 40
@@ -154,7 +154,7 @@ def test_state_table(H, L):
 
 
 @pytest.mark.parametrize("H,L", [(4432, 8), (200, 32), (128, 3), (10001, 8), (300, 40), (300, 64),
-                                 (70, 12), (5, 8), (300, 65), (20000, 100), (90, 128)])
+                                 (70, 12), (5, 8), (300, 65), (20000, 100), (90, 128), (600, 200)])
 def test_state_table_gpu_compat(H, L):
     # the reference's GPU selection kernels, run lane by lane in the oracle on
     # the PBWT rows and divergences
@@ -216,7 +216,7 @@ def hardcall_codes(rng, B, M, typed_every=4):
     return codes
 
 
-@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 8, 12, 16, 32, 64, 65, 100, 128])
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 8, 12, 16, 32, 64, 65, 100, 128, 200, 256])
 @pytest.mark.parametrize("per_marker", [False, True])
 @pytest.mark.parametrize("logspace", [False, True])
 def test_fb_states_and_modes(L, per_marker, logspace):
@@ -303,7 +303,7 @@ def test_fb_coincident_markers_logspace():
         run_fb(idx, codes, synth.HARDCALL_PALETTE, tr, logspace=False)
 
 
-@pytest.mark.parametrize("L", [3, 8, 16, 33, 100])
+@pytest.mark.parametrize("L", [3, 8, 16, 33, 100, 256])
 def test_fb_banded_variants_and_batch_agree(L):
     # Banded jobs: every lane shape gives the same bits (the same trees over
     # the states), and the batched form equals the plain one.

@@ -41,7 +41,7 @@ def _run(rj, variant):
 @pytest.mark.parametrize("L,logspace,per_marker,M", [
     (4, False, True, 430), (5, True, True, 430), (8, False, False, 417), (16, False, True, 430),
     (16, True, False, 33), (33, False, True, 430), (64, True, True, 200), (64, False, False, 17),
-    (100, False, True, 200), (128, True, False, 33),
+    (100, False, True, 200), (128, True, False, 33), (256, False, True, 64),
 ])
 def test_every_variant_gives_identical_outputs(world, L, logspace, per_marker, M):
     rj = _job(world, L, logspace, per_marker, M=M)
