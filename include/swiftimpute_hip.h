@@ -68,7 +68,7 @@ uint32_t si_padded_states(uint32_t L);
  * which the reference runs on the host.
  *   cols   [M][ceil(H/64)] device, bit h of word h/64 set iff allele != 0
  *   prefix [M][H] uint32 device; div [M][H] uint32 device or NULL (not written)
- * H <= 65536.  The sweep works in chunks of markers: by default, for
+ * H <= 524288 (2^19).  The sweep works in chunks of markers: by default, for
  * H <= 32768, c = ceil(M / (7/8 of the current device's compute units)),
  * rounded up to a multiple of 128 when above 64 and at least 16 (256 units
  * assumed when no device answers), and 512 above.  The

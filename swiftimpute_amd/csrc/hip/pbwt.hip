@@ -28,7 +28,9 @@
 // 256-byte run.  A stable partition needs, per element, the count of zeros
 // before it: a ballot and mbcnt within the round, a running count across
 // rounds and one LDS exchange of the 16 per-wave totals across waves.
-// H <= 65536, so a haplotype fits in 16 bits of an element word.
+// Haplotype ids: 16 bits in the LDS forms (H <= 49152); the global-memory
+// form (49152 < H <= 2^19) keeps 31 bits beside a flag bit, and 64-bit
+// element words where an id travels with its group id.
 #include "si_device.hpp"
 #include "si_internal.hpp"
 #include <atomic>
@@ -48,7 +50,7 @@ struct PbwtArgs {
     uint32_t* xa;            // [nchunk][2][H] phase A exchange
     uint32_t* gid;           // [nchunk][H] group id of each haplotype at the chunk end
     uint32_t* ngroups;       // [nchunk]
-    uint32_t* xb;            // [2][H] phase B exchange
+    uint32_t* xb;            // [2][16 * 64 * K] 64-bit words: the boundary composition's exchange
     uint8_t* tp;             // [nchunk][NP][H] key byte p of each haplotype (radix digit planes)
     uint8_t* th;             // [nchunk][H][TS] the same bytes, haplotype-major (key comparison)
     uint32_t NP, TS;         // key bytes per chunk (C / 8), row stride of th (NP rounded to 16)
@@ -99,6 +101,23 @@ __device__ __forceinline__ void load_column(const PbwtArgs& a, uint32_t m, uint3
 
 // The next column is fetched into registers at the start of a step and stored
 // to the other LDS buffer at its end (n <= 2 * kPT words).
+// The global-memory form's columns: up to kPbwtMaxW32 words, kColWide a thread.
+constexpr uint32_t kColWide = kPbwtMaxW32 / kPT;
+struct ColPrefetchWide {
+    uint32_t w[kColWide];
+    __device__ __forceinline__ void fetch(const PbwtArgs& a, uint32_t m) {
+        const uint32_t n = (a.H + 31u) / 32u, t = threadIdx.x;
+#pragma unroll
+        for (uint32_t k = 0; k < kColWide; ++k)
+            if (t + k * kPT < n) w[k] = a.cols32[size_t(m) * a.W2 + t + k * kPT];
+    }
+    __device__ __forceinline__ void store(const PbwtArgs& a, uint32_t* col) const {
+        const uint32_t n = (a.H + 31u) / 32u, t = threadIdx.x;
+#pragma unroll
+        for (uint32_t k = 0; k < kColWide; ++k)
+            if (t + k * kPT < n) col[t + k * kPT] = w[k];
+    }
+};
 struct ColPrefetch {
     uint32_t w0 = 0, w1 = 0;
     __device__ __forceinline__ void fetch(const PbwtArgs& a, uint32_t m) {
@@ -169,7 +188,7 @@ __global__ __launch_bounds__(kPT) void pbwt_rows_kernel(PbwtArgs a) {
     for (uint32_t m = m_first; m + 1 < m_end; ++m) {
         const uint32_t* cb = col[(m - m_first) & 1u];
         const bool pre = m + 2 < m_end;
-        ColPrefetch nx;
+        ColPrefetchWide nx;
         if (pre) nx.fetch(a, m + 1);
 
         uint32_t zsum = 0;
@@ -210,6 +229,7 @@ __global__ __launch_bounds__(kPT) void pbwt_rows_kernel(PbwtArgs a) {
 // divergence rows of the chunk).  Element word: haplotype | 1 << 31 where a
 // group starts.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kIdMask = 0x7fffffffu;
 
 // One round of group tracking.  Z/O: lanes with allele 0 / 1, Bm: group
 // starts.  Returns, for this lane, whether it is the first of its allele in
@@ -255,7 +275,7 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_kernel(PbwtArgs a) {
         const uint32_t k = m - m_first;
         const uint32_t* cb = col[k & 1u];
         const bool pre = m + 1 < m_end;
-        ColPrefetch nx;
+        ColPrefetchWide nx;
         if (pre) nx.fetch(a, m + 1);
         const uint32_t* Xin = (k & 1u) ? X0 : X1;
         uint32_t* Xout = (k & 1u) ? X1 : X0;
@@ -270,7 +290,7 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_kernel(PbwtArgs a) {
         for (uint32_t j = 0; j < K; ++j) {
             const uint32_t v = elem(j);
             const bool valid = q.valid(j);
-            const uint32_t x = allele_bit(cb, v & 0xffffu);
+            const uint32_t x = allele_bit(cb, v & kIdMask);
             const uint64_t Z = __ballot(valid && x == 0u);
             const uint64_t O = __ballot(valid && x != 0u);
             const uint64_t Bm = __ballot(valid && (v >> 31) != 0u);
@@ -309,7 +329,7 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_kernel(PbwtArgs a) {
         for (uint32_t j = 0; j < K; ++j) {
             const uint32_t v = elem(j);
             const bool valid = q.valid(j);
-            const uint32_t x = allele_bit(cb, v & 0xffffu);
+            const uint32_t x = allele_bit(cb, v & kIdMask);
             const uint64_t Z = __ballot(valid && x == 0u);
             const uint64_t O = __ballot(valid && x != 0u);
             const uint64_t Bm = __ballot(valid && (v >> 31) != 0u);
@@ -320,7 +340,7 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_kernel(PbwtArgs a) {
             const uint32_t first = (fl && (!op || (x ? co : cz) == 0u)) ? 1u : 0u;
             const uint32_t e = j * 64u + q.lane;
             const uint32_t dst = x ? ztot + (q.base - zbefore) + (e - zb) : zbefore + zb;
-            if (valid) Xout[dst] = (v & 0xffffu) | (first << 31);
+            if (valid) Xout[dst] = (v & kIdMask) | (first << 31);
         }
         if (a.div) {
             uint32_t* drow = a.div + size_t(m) * a.H;
@@ -353,8 +373,8 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_kernel(PbwtArgs a) {
         const uint32_t inc = brun + uint32_t(__popcll(Bm & q.le));
         brun += uint32_t(__popcll(Bm));
         if (valid) {
-            gid[v & 0xffffu] = bbefore + inc - 1u;
-            if (c == 0) lastrow[q.base + j * 64u + q.lane] = v & 0xffffu;
+            gid[v & kIdMask] = bbefore + inc - 1u;
+            if (c == 0) lastrow[q.base + j * 64u + q.lane] = v & kIdMask;
         }
     }
     if (q.t == 0) a.ngroups[c] = G;
@@ -362,8 +382,8 @@ __global__ __launch_bounds__(kPT) void pbwt_groups_kernel(PbwtArgs a) {
 
 // ---------------------------------------------------------------------------
 // Phase B: chunk-boundary orders, one workgroup, boundaries in sequence.
-// Element word: haplotype | group id << 16; LSD radix over 8-bit digits of
-// the id.  Lanes of a round with the same digit agree on all 8 digit-bit
+// Element word (64-bit): haplotype | group id << 32; LSD radix over 8-bit
+// digits of the id.  Lanes of a round with the same digit agree on all 8 digit-bit
 // ballots; the wave's running count per digit lives in LDS and is advanced by
 // the highest lane of each digit.  The exchange buffers hold every position a
 // wave owns (16 * S >= H), so loads and stores need no validity branch; the
@@ -389,6 +409,7 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
     const WaveCtx q(a);
     const uint32_t K = a.S / 64u;
     const uint32_t npos = kPW * a.S;    // padded buffer length
+    uint64_t* const xb = reinterpret_cast<uint64_t*>(a.xb);
     const uint32_t own = q.base + q.lane;
 
     for (uint32_t c = 1; c < a.nchunk; ++c) {
@@ -405,35 +426,36 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
         }
         // element words, gathered once into the buffer pass 0 reads
         {
-            uint32_t* X0 = a.xb + npos;
+            uint64_t* X0 = xb + npos;
             for (uint32_t j0 = 0; j0 < K; j0 += kBatch) {
                 uint32_t h[kBatch];
+                uint64_t e[kBatch];
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u) h[u] = prev[min(own + (j0 + u) * 64u, q.last)];
 #pragma unroll
-                for (uint32_t u = 0; u < kBatch; ++u) h[u] |= gid[h[u] & 0xffffu] << 16;
+                for (uint32_t u = 0; u < kBatch; ++u) e[u] = uint64_t(h[u]) | (uint64_t(gid[h[u]]) << 32);
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u)
-                    if (j0 + u < K) X0[own + (j0 + u) * 64u] = h[u];
+                    if (j0 + u < K) X0[own + (j0 + u) * 64u] = e[u];
             }
             __syncthreads();
         }
         for (uint32_t pass = 0; pass < npass; ++pass) {
-            const uint32_t sh = 16u + 8u * pass;
-            const uint32_t* X = a.xb + ((pass + 1u) & 1u) * npos;  // this pass's input
+            const uint32_t sh = 32u + 8u * pass;
+            const uint64_t* X = xb + ((pass + 1u) & 1u) * npos;  // this pass's input
             const bool last = pass + 1 == npass;
-            uint32_t* Y = a.xb + (pass & 1u) * npos;
+            uint64_t* Y = xb + (pass & 1u) * npos;
             for (uint32_t i = q.t; i < uint32_t(kPW) * 256u; i += kPT) (&cnt[0][0])[i] = 0u;
             __syncthreads();
             for (uint32_t j0 = 0; j0 < K; j0 += kBatch) {
-                uint32_t v[kBatch];
+                uint64_t v[kBatch];
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u) v[u] = X[own + min(j0 + u, K - 1u) * 64u];
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u) {
                     if (j0 + u < K) {
                         const bool valid = q.valid(j0 + u);
-                        const uint32_t d = (v[u] >> sh) & 255u;
+                        const uint32_t d = uint32_t(v[u] >> sh) & 255u;
                         const uint64_t match = digit_match(d, valid);
                         if (valid && (match >> q.lane) == 1ull)
                             cnt[q.w][d] += uint32_t(__popcll(match));
@@ -464,19 +486,19 @@ __global__ __launch_bounds__(kPT) void pbwt_compose_kernel(PbwtArgs a) {
             }
             __syncthreads();
             for (uint32_t j0 = 0; j0 < K; j0 += kBatch) {
-                uint32_t v[kBatch];
+                uint64_t v[kBatch];
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u) v[u] = X[own + min(j0 + u, K - 1u) * 64u];
 #pragma unroll
                 for (uint32_t u = 0; u < kBatch; ++u) {
                     if (j0 + u < K) {
                         const bool valid = q.valid(j0 + u);
-                        const uint32_t d = (v[u] >> sh) & 255u;
+                        const uint32_t d = uint32_t(v[u] >> sh) & 255u;
                         const uint64_t match = digit_match(d, valid);
                         const uint32_t c0 = cnt[q.w][d];
                         const uint32_t dst = dbase[d] + c0 + mbcnt64(match);
                         if (valid) {
-                            if (last) out[dst] = v[u] & 0xffffu;
+                            if (last) out[dst] = uint32_t(v[u]);
                             else Y[dst] = v[u];
                             if ((match >> q.lane) == 1ull) cnt[q.w][d] = c0 + uint32_t(__popcll(match));
                         }
@@ -1367,6 +1389,11 @@ uint32_t pbwt_rounds(uint32_t H) {
         if (k >= need) return k;
     return 0;
 }
+// Rounds of the global-memory form, which takes any K (H above 65536 too).
+uint32_t pbwt_rounds_any(uint32_t H) {
+    const uint32_t k = pbwt_rounds(H);
+    return k ? k : (H + kPT - 1) / kPT;
+}
 
 size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -1441,10 +1468,10 @@ uint32_t pbwt_default_chunk(uint32_t H, uint32_t M) {
 size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C) {
     if (H == 0 || M == 0 || C == 0) return 0;
     const size_t n = (size_t(M) + C - 1) / C, h = H;
-    const uint32_t K = pbwt_rounds(H);
-    const size_t npos = size_t(kPW) * 64u * K;  // padded exchange length
+    const uint32_t K = pbwt_rounds_any(H);
+    const size_t npos = size_t(kPW) * 64u * K;  // padded exchange length (64-bit words)
     const size_t base = align256(n * 2 * h * 4) + align256(n * h * 4) + align256(n * 4) +
-                        align256(2 * npos * 4);
+                        align256(2 * npos * 8);
     return base + (K != 0 && K <= kRadixMaxK ? key_bytes(H, M, C) + align256(2 * n * h * 2) + align256(2 * n * 4)
                                              : 0);
 }
@@ -1454,7 +1481,7 @@ int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* pr
     if (H == 0 || M == 0) return 0;
     if (H > kPbwtMaxH || C == 0) return SI_ERR_UNSUPPORTED;
     if (ws_bytes < pbwt_workspace_bytes(H, M, C)) return SI_ERR_WORKSPACE;
-    const uint32_t K = pbwt_rounds(H);
+    const uint32_t K = pbwt_rounds_any(H);
     PbwtArgs a{};
     a.cols32 = reinterpret_cast<const uint32_t*>(cols);
     a.H = H;
@@ -1473,8 +1500,8 @@ int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* pr
     p += align256(n * h * 4);
     a.ngroups = reinterpret_cast<uint32_t*>(p);
     p += align256(n * 4);
-    a.xb = reinterpret_cast<uint32_t*>(p);
-    p += align256(2 * size_t(kPW) * 64u * K * 4);
+    a.xb = reinterpret_cast<uint32_t*>(p);  // 64-bit element words in the boundary composition
+    p += align256(2 * size_t(kPW) * 64u * K * 8);
     a.NP = (C + 7u) / 8u;
     a.TS = (a.NP + 15u) & ~15u;
     if (K <= kRadixMaxK) {
@@ -1501,7 +1528,7 @@ int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* pr
         case 32: launch_radix<32>(a, stream); break;
         case 40: launch_lds<40>(a, stream); break;
         case 48: launch_lds<48>(a, stream); break;
-        default:  // H > 49152: the order exchanges through global memory
+        default:  // H > 49152: the order exchanges through global memory (K = ceil(H / 1024))
             hipLaunchKernelGGL(pbwt_groups_kernel, dim3(a.nchunk), dim3(kPT), 0, stream, a);
             if (a.nchunk > 1)
                 hipLaunchKernelGGL(pbwt_compose_kernel, dim3(1), dim3(kPT), 0, stream, a);

@@ -172,7 +172,7 @@ void fb_register(const FbImpl& impl);
 const FbImpl* fb_impl(uint32_t Lp, uint32_t G);
 
 // PBWT prefix/divergence construction (pbwt.hip): chunks of C markers.
-constexpr uint32_t kPbwtMaxH = 65536;
+constexpr uint32_t kPbwtMaxH = 1u << 19;  // the global-memory form's two LDS column buffers: 128 KiB
 uint32_t pbwt_default_chunk(uint32_t H, uint32_t M);
 size_t pbwt_workspace_bytes(uint32_t H, uint32_t M, uint32_t C);
 int launch_pbwt_sweep(const uint64_t* cols, uint32_t H, uint32_t M, uint32_t* prefix,

@@ -72,8 +72,10 @@ def test_pbwt_argument_errors():
     lib = _native.hip()
     one = ctypes.c_void_p(8)  # never dereferenced: the checks come first
     assert lib.si_pbwt_sweep(one, 0, 5, one, None, None, 0, None) == _native.SI_ERR_ARG
-    assert lib.si_pbwt_sweep(one, 70000, 5, one, None, None, 0, None) == _native.SI_ERR_UNSUPPORTED
-    assert b"65536" in lib.si_last_error()
+    assert lib.si_pbwt_sweep(one, 524289, 5, one, None, None, 0, None) == _native.SI_ERR_UNSUPPORTED
+    assert b"524288" in lib.si_last_error()
+    # above 65536 haplotypes the global-memory form with 512-marker chunks
+    assert lib.si_pbwt_workspace_bytes(100000, 1000) == lib.si_pbwt_chunked_workspace_bytes(100000, 1000, 512)
     assert lib.si_pbwt_sweep_chunked(one, 10, 5, one, None, 0, None, 0, None) == _native.SI_ERR_ARG
     assert lib.si_pbwt_sweep(None, 10, 5, one, None, None, 0, None) == _native.SI_ERR_ARG
     assert lib.si_pbwt_sweep(one, 10, 0, one, None, None, 0, None) == 0  # nothing to do

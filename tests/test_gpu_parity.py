@@ -89,7 +89,8 @@ def _dup_panel(rng, M, H, n_distinct=6, flip=0.01):
                                        (50, 65, 5), (64, 300, 3), (100, 1100, 16), (37, 1025, 2),
                                        (300, 4432, 64), (30, 40000, 7), (9, 65536, 4),
                                        (700, 2000, 512), (1200, 500, 100), (1100, 20000, 512),
-                                       (900, 24000, 200), (300, 32000, 96)])
+                                       (900, 24000, 200), (300, 32000, 96), (30, 65537, 7),
+                                       (23, 100000, 5), (12, 300000, 3), (6, 524288, 2)])
 @pytest.mark.parametrize("dup", [False, True])
 def test_pbwt_chunked(M, H, chunk, dup):
     rng = np.random.default_rng(M * 7 + H + chunk + dup)
