@@ -518,25 +518,44 @@ __device__ __forceinline__ uint32_t untyped_bits(const uint4& cw, uint32_t uc) {
     return m;
 }
 
+// Whether the untyped code's row is a missing genotype (likelihoods 0, 0, 0:
+// emission 1 in every class, offset 0), the predicate load_tables records in
+// the table's last word.  If not, the code is ignored: every marker counts
+// as typed.
+__device__ __forceinline__ bool untyped_row_ok(const FbTables& tb, bool log) {
+    if (tb.ucode >= tb.P) return false;
+    const uint32_t i = tb.ucode * 8u + 7u;  // the row with every class present
+    const float o = tb.emax[i];
+    if (log) {
+        const float* g = tb.palette + 3u * tb.ucode;
+        return g[0] == 0.0f && g[1] == 0.0f && g[2] == 0.0f && o == 0.0f;
+    }
+    return tb.eps[4u * i] == 1.0f && tb.eps[4u * i + 1u] == 1.0f && tb.eps[4u * i + 2u] == 1.0f && o == 0.0f;
+}
+
 // The job's typed-marker mask: bit t of tmask[c] set where marker 16c + t
-// (< M) has a code other than uc in some sample, so a marker is untyped for
-// the job when no sample has a genotype there; the decision is the same for
-// every sample of the job (wavefront-uniform control in both passes) and
-// does not depend on which samples share a wavefront.  A workgroup covers
+// (< M) has a code other than uc in some sample (every marker when ok is
+// false), so a marker is untyped for the job when no sample has a genotype
+// there; the decision is the same for every sample of the job (wavefront-
+// uniform control in both passes) and does not depend on which samples share
+// a wavefront.  A workgroup covers
 // kMaskBlocks load blocks with kMaskLanes lanes each striding over the
 // samples (consecutive lanes read consecutive 16-byte code words of a row).
 constexpr uint32_t kMaskBlocks = kTmaskBlocksPerGroup, kMaskLanes = 8,
                    kMaskThreads = kMaskBlocks * kMaskLanes;
 __device__ __forceinline__ void tmask_tile(const uint8_t* codes, uint32_t B, uint32_t M, uint32_t pitch,
-                                           uint32_t uc, uint32_t c0, uint32_t* tmask) {
+                                           uint32_t uc, bool ok, uint32_t c0, uint32_t* tmask) {
     __shared__ uint32_t red[kMaskThreads];
     const uint32_t ci = threadIdx.x % kMaskBlocks, sl = threadIdx.x / kMaskBlocks;
     const uint32_t c = c0 + ci;
     const uint32_t nlb = (M + kSub - 1) / kSub;
     uint32_t m = 0;
     if (c < nlb) {
-        for (uint32_t s = sl; s < B; s += kMaskLanes)
-            m |= untyped_bits(*reinterpret_cast<const uint4*>(codes + size_t(s) * pitch + kSub * c), uc) ^ 0xffffu;
+        if (!ok) m = 0xffffu;
+        else
+            for (uint32_t s = sl; s < B; s += kMaskLanes)
+                m |= untyped_bits(*reinterpret_cast<const uint4*>(codes + size_t(s) * pitch + kSub * c), uc) ^
+                     0xffffu;
         if (kSub * c + kSub > M) m &= (1u << (M - kSub * c)) - 1u;  // markers past the job
     }
     red[threadIdx.x] = m;

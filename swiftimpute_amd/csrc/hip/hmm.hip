@@ -110,8 +110,8 @@ void fb_bind_workspace(JobView& v, uint32_t Lp, char* ws) {
 // ------------------------------------------------------ typed-marker mask --
 // Before a fast job's chain pass: its mask of typed markers per load block,
 // then the list of those markers.
-__global__ __launch_bounds__(kMaskThreads) void fb_tmask_kernel(JobView J, uint32_t uc) {
-    tmask_tile(J.codes, J.B, J.M, J.pitch, uc, blockIdx.x * kMaskBlocks, J.tmask);
+__global__ __launch_bounds__(kMaskThreads) void fb_tmask_kernel(JobView J, FbTables tb, bool log) {
+    tmask_tile(J.codes, J.B, J.M, J.pitch, tb.ucode, untyped_row_ok(tb, log), blockIdx.x * kMaskBlocks, J.tmask);
 }
 
 __global__ __launch_bounds__(kListThreads) void fb_tlist_kernel(JobView J) {
@@ -120,10 +120,11 @@ __global__ __launch_bounds__(kListThreads) void fb_tlist_kernel(JobView J) {
 
 __global__ __launch_bounds__(kMaskThreads) void fb_tmask_batch_kernel(const JobView* jobs,
                                                                      const uint64_t* off, uint32_t n,
-                                                                     uint32_t uc) {
+                                                                     FbTables tb, bool log) {
     const uint32_t j = find_job_uniform(off, n, blockIdx.x);
     const JobView* J = jobs + j;
-    tmask_tile(J->codes, J->B, J->M, J->pitch, uc, uint32_t(blockIdx.x - off[j]) * kMaskBlocks, J->tmask);
+    tmask_tile(J->codes, J->B, J->M, J->pitch, tb.ucode, untyped_row_ok(tb, log),
+               uint32_t(blockIdx.x - off[j]) * kMaskBlocks, J->tmask);
 }
 
 __global__ __launch_bounds__(kListThreads) void fb_tlist_batch_kernel(const JobView* jobs) {
@@ -199,7 +200,7 @@ int launch_fb(const JobView& v, const FbTables& t, uint32_t Lp, uint32_t passes,
     const bool band = band_launch(v);
     if ((passes & SI_FB_PASS_CHAIN) && fast) {
         hipLaunchKernelGGL(fb_tmask_kernel, dim3(uint32_t(fb_tmask_groups(v.M))), dim3(kMaskThreads), 0,
-                           stream, v, t.ucode);
+                           stream, v, t, log);
         hipLaunchKernelGGL(fb_tlist_kernel, dim3(1), dim3(kListThreads), 0, stream, v);
         if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     }
@@ -295,7 +296,7 @@ int launch_fb_batch(const BatchClass& c, const FbTables& t, uint32_t passes, con
     if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     if ((passes & SI_FB_PASS_CHAIN) && c.chain_items && fast) {
         hipLaunchKernelGGL(fb_tmask_batch_kernel, dim3(uint32_t(c.mask_groups)), dim3(kMaskThreads), 0,
-                           stream, c.jobs, c.mask_off, c.n_jobs, t.ucode);
+                           stream, c.jobs, c.mask_off, c.n_jobs, t, log);
         hipLaunchKernelGGL(fb_tlist_batch_kernel, dim3(c.n_jobs), dim3(kListThreads), 0, stream, c.jobs);
         if (hipGetLastError() != hipSuccess) return SI_ERR_DEVICE;
     }

@@ -12,7 +12,7 @@ import pytest
 import torch
 
 import oracle
-from swiftimpute_amd import device, synth
+from swiftimpute_amd import _native, device, synth
 from test_oracle import fixture_panel
 
 pytestmark = pytest.mark.gpu
@@ -215,6 +215,35 @@ def hardcall_codes(rng, B, M, typed_every=4):
     codes = np.full((B, M), 3, np.uint8)
     codes[:, ::typed_every] = rng.integers(0, 3, (B, (M + typed_every - 1) // typed_every))
     return codes
+
+
+@pytest.mark.parametrize("logspace", [False, True])
+def test_untyped_code_with_nonzero_likelihoods_is_ignored(logspace):
+    # SI_FB_UNTYPED naming a palette entry that is not (0, 0, 0) (here the
+    # hom-ref call): the code is ignored and every marker takes the exact
+    # step, so the outputs equal the same job without the flag
+    rng = np.random.default_rng(808)
+    pan = synth.make_panel(300, 250, seed=41)
+    idx = device.DeviceIndex(pan.cols, pan.H, 8)
+    codes = hardcall_codes(rng, 13, pan.M, typed_every=5)
+    pitch = (pan.M + 15) // 16 * 16
+    c = np.full((13, pitch), 3, np.uint8)
+    c[:, :pan.M] = codes
+    tr = device.Transitions.build(pan.pos, pan.genetic_cM(), 10000)
+    outs = []
+    for code in (None, 0):
+        job = device.Job(idx, torch.from_numpy(c).cuda(), synth.HARDCALL_PALETTE, tr, seed=5,
+                         logspace=logspace, untyped=False)
+        job.alloc_outputs(ds=True, ap=True, hap=True, loglik=True)
+        cj = job.c_job()
+        if code is not None:
+            cj.flags |= _native.SI_FB_UNTYPED
+            cj.untyped_code = code
+        device.run_jobs([cj])
+        torch.cuda.synchronize()
+        outs.append({k: getattr(job.out, k).cpu().numpy() for k in ("ds", "ap", "hap", "loglik")})
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
 
 
 @pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 8, 12, 16, 32, 64, 65, 100, 128, 200, 256])
