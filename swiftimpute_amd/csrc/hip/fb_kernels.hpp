@@ -540,8 +540,10 @@ __device__ __forceinline__ bool untyped_row_ok(const FbTables& tb, bool log) {
 // uniform control in both passes) and does not depend on which samples share
 // a wavefront.  A workgroup covers
 // kMaskBlocks load blocks with kMaskLanes lanes each striding over the
-// samples (consecutive lanes read consecutive 16-byte code words of a row).
-constexpr uint32_t kMaskBlocks = kTmaskBlocksPerGroup, kMaskLanes = 8,
+// samples (consecutive lanes read consecutive 16-byte code words of a row);
+// 32 lanes give a 100,000-marker job 196 workgroups of 16 wavefronts (8
+// lanes: 4), enough loads in flight to stream the codes near the HBM rate.
+constexpr uint32_t kMaskBlocks = kTmaskBlocksPerGroup, kMaskLanes = 32,
                    kMaskThreads = kMaskBlocks * kMaskLanes;
 __device__ __forceinline__ void tmask_tile(const uint8_t* codes, uint32_t B, uint32_t M, uint32_t pitch,
                                            uint32_t uc, bool ok, uint32_t c0, uint32_t* tmask) {
@@ -553,6 +555,7 @@ __device__ __forceinline__ void tmask_tile(const uint8_t* codes, uint32_t B, uin
     if (c < nlb) {
         if (!ok) m = 0xffffu;
         else
+#pragma unroll 4
             for (uint32_t s = sl; s < B; s += kMaskLanes)
                 m |= untyped_bits(*reinterpret_cast<const uint4*>(codes + size_t(s) * pitch + kSub * c), uc) ^
                      0xffffu;
